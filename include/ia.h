@@ -1,0 +1,213 @@
+/*
+ * inverseacoustics_amd C ABI
+ *
+ * Drop-in boundary for the frequency-domain inverse solve of
+ * LuisCorralR/InverseAcoustics (CUDA/esm.cuh, hels.cuh, ibem.cuh, invr.cuh,
+ * tsvd.cuh, utils.cuh), implemented with HIP kernels for MI355X (gfx950).
+ *
+ * Conventions (identical to the reference):
+ *   - complex values are {double re, im} pairs, layout-compatible with
+ *     cuDoubleComplex, hipDoubleComplex, std::complex<double> and C99
+ *     double _Complex;
+ *   - matrices are column-major;
+ *   - point sets are structure-of-arrays: x[0..n) y[0..n) z[0..n);
+ *   - connectivity is structure-of-arrays and 0-based;
+ *   - e^{+ikR} outgoing waves with e^{-i w t} time dependence;
+ *   - the legacy entry points return host buffers allocated with malloc();
+ *     the caller frees them with free().
+ *
+ * Differences from the reference, all deliberate:
+ *   - no C++ exception crosses this boundary: the ia_* variants return an
+ *     ia_status, the legacy void wrappers print the message and leave the
+ *     outputs NULL (ia_last_error() holds the text);
+ *   - c and rho are runtime parameters (ia_set_air); the default is the
+ *     reference's compile-time C_AIR 343.5, RHO_AIR 1.19
+ *     (CUDA/constants.cuh:2,5);
+ *   - esm works for l != m (the reference's CUDA layout needs l == m,
+ *     CUDA/esm.cuh:34,142,165-168);
+ *   - the *gpu variants accept host or device pointers for their inputs.
+ */
+#ifndef IA_H
+#define IA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } ia_z;
+typedef struct { float re, im; } ia_c;
+
+typedef enum {
+    IA_OK = 0,
+    IA_ERR_ARG = 1,        /* invalid argument (size, range, NULL)          */
+    IA_ERR_DEVICE = 2,     /* HIP runtime error                             */
+    IA_ERR_NOMEM = 3,      /* host or device allocation failed              */
+    IA_ERR_SINGULAR = 4,   /* exact zero pivot in an LU factorisation       */
+    IA_ERR_NONFINITE = 5,  /* NaN/Inf in inputs or results                  */
+    IA_ERR_IO = 6,         /* file missing or malformed                     */
+    IA_ERR_NOCONV = 7      /* iteration cap reached                         */
+} ia_status;
+
+/* ------------------------------------------------------------ runtime */
+const char *ia_last_error(void);
+const char *ia_version(void);
+/* Speed of sound and air density used by every entry point of this thread.
+ * Replaces the compile-time C_AIR / RHO_AIR of CUDA/constants.cuh:1-6. */
+ia_status ia_set_air(double c, double rho);
+ia_status ia_get_air(double *c, double *rho);
+/* Number of visible HIP devices (0 when no GPU). */
+int ia_device_count(void);
+/* Release cached device state of this library on the current device. */
+void ia_release(void);
+
+/* -------------------------------------------------- ESM (esm.cuh) */
+/* replaces esm(), CUDA/esm.cuh:112.  M_e = M_sq D^-1 (m x m),
+ * H_e = M_hq D^-1 (l x m); r_ms (3m) surface points, r_lh (3l) field
+ * points, sources at r_i * r_ms with radial normals (esm.cuh:9-15). */
+void esm(int m, int l, double *r_ms, double *r_lh, ia_z **m_e, ia_z **h,
+         double k, double r_i);
+/* replaces esmgpu(), CUDA/esm.cuh:38 (same outputs; inputs host or device). */
+void esmgpu(int m, int l, double *r_msg, double *r_lhg, ia_z **me, ia_z **he,
+            double k, double r_i);
+/* Status-returning general form: explicit source points src (3m) and unit
+ * normals nrm (3m, NULL = radial).  Any output pointer may be NULL.
+ * d, m_sq (m x m) and m_hq (l x m) return the assembled matrices. */
+ia_status ia_esm(int m, int l, const double *r_ms, const double *r_lh,
+                 const double *src, const double *nrm, double k,
+                 ia_z **m_e, ia_z **h_e, ia_z **d, ia_z **m_sq, ia_z **m_hq);
+
+/* --------------------------------------------- SVD + filters (tsvd.cuh) */
+/* replaces csvd(), CUDA/tsvd.cuh:10: compact SVD a = u diag(s) v^H.
+ * m >= n: u m x n, s n, v n x n;  m < n: u m x m, s m, v n x m. */
+void csvd(int m, int n, ia_z *a, ia_z **u, double **s, ia_z **v);
+ia_status ia_csvd(int m, int n, const ia_z *a, ia_z **u, double **s, ia_z **v);
+/* replaces tsvd(), CUDA/tsvd.cuh:122: x = V(:,1:k) diag(1/s(1:k)) U(:,1:p)^H b
+ * (u m x p, v n x p); 1 <= k <= p as the reference requires (tsvd.cuh:124). */
+void tsvd(int m, int n, int p, int k, ia_z *u, double *s, ia_z *v, ia_z *b,
+          ia_z **x);
+/* Tikhonov (tikhonovG.m:26-30) and damped SVD (dsvdG.m:23-26) solutions
+ * from an SVD, lambda >= 0. */
+ia_status ia_tikhonov(int m, int n, int p, const ia_z *u, const double *s,
+                      const ia_z *v, const ia_z *b, double lambda, ia_z **x);
+ia_status ia_dsvd(int m, int n, int p, const ia_z *u, const double *s,
+                  const ia_z *v, const ia_z *b, double lambda, ia_z **x);
+
+/* ------------------------------------------------------ dense kernels */
+/* C = alpha op(A) op(B) + beta C, op in {'N','C'}, column-major complex128;
+ * host or device pointers, synchronous.  Replaces the cublasZgemm call
+ * sites of CUDA/ibem.cuh:398 and CUDA/invr.cuh:402-433. */
+ia_status ia_zgemm(char opa, char opb, int m, int n, int k, ia_z alpha,
+                   const ia_z *a, int lda, const ia_z *b, int ldb, ia_z beta,
+                   ia_z *c, int ldc);
+/* Same product in complex64 (FP32 matrix cores, FP32 accumulation). */
+ia_status ia_cgemm(char opa, char opb, int m, int n, int k, ia_c alpha,
+                   const ia_c *a, int lda, const ia_c *b, int ldb, ia_c beta,
+                   ia_c *c, int ldc);
+/* Solve D X = B (op 'N') or D^H X = B (op 'C') for square D (n x n) with
+ * partial-pivoting LU; B (n x nrhs) is overwritten by X.  Replaces the
+ * square cusolverDnZZgels / zgels_ solves of CUDA/esm.cuh:73-85,153-168. */
+ia_status ia_zgesv(char op, int n, int nrhs, const ia_z *d, ia_z *b);
+
+/* ------------------------------------------------ utils (utils.cuh) */
+/* replaces to_pressure(), to_velocity(), sound_power() (utils.cuh:199,232,265) */
+void to_pressure(int m, int n, ia_z *me, ia_z *x, ia_z **p);
+void to_velocity(int m, int n, ia_z *qe, ia_z *x, ia_z **v);
+void sound_power(int m, ia_z *p, ia_z *v, double *w, double s);
+/* replaces sphere(), utils.cuh:384: pulsating sphere of radius a, velocity u. */
+void sphere(double *r_ms, double *r_lh, ia_z **p_s, ia_z **p_h, ia_z **v_s,
+            double k, double u, double a, int m, int l);
+/* replaces geometry(), utils.cuh:291 (Gmsh MSH 2.2, triangles only,
+ * degenerate triangles dropped, field points r_ms (1 + 2 pi t / k)). */
+ia_status ia_geometry(const char *file, double k, double t, double **r_ms,
+                      double **r_lh, int **e_n, int *m, int *l, int *n);
+void geometry(double **r_ms, double **r_lh, int **e_n, double k, double t,
+              int *m_out, int *l_out, int *n_out, char *file);
+/* replaces cartesian_2_spherical(), utils.cuh:414. */
+void cartesian_2_spherical(int m, double *r_xx, double **azelr);
+
+/* ---------------------------------------- parameter choice (regtools) */
+typedef enum {
+    IA_FILTER_TIKHONOV = 0,  /* f = s^2/(s^2+lam^2)                */
+    IA_FILTER_TSVD = 1,      /* f = 1 for i < k                    */
+    IA_FILTER_DSVD = 2       /* f = s/(s+lam)                      */
+} ia_filter;
+
+typedef enum {
+    IA_SELECT_GCV = 0,        /* argmin of GCV on the lambda grid (Tikhonov,
+                                 dsvd) / discrete GCV over k (TSVD)         */
+    IA_SELECT_LCURVE = 1,     /* argmin of lcfun (negative curvature)       */
+    IA_SELECT_FIXED = 2,      /* use lambda / k given by the caller         */
+    IA_SELECT_GCV_REGTOOLS = 3,   /* literal gcvG.m: 200-point grid + fminbnd */
+    IA_SELECT_LCURVE_REGTOOLS = 4,/* literal l_curveG.m: 200 points + fminbnd */
+    IA_SELECT_QUASIOPT = 5        /* quasioptG.m: 200 points + fminbnd        */
+} ia_select;
+
+typedef struct {
+    ia_filter filter;
+    ia_select select;
+    int n_lambda;        /* grid points (default 50)                    */
+    double decades;      /* grid spans sigma_max 10^-decades .. sigma_max */
+    double lambda;       /* IA_SELECT_FIXED: lambda                     */
+    int k;               /* IA_SELECT_FIXED + TSVD: truncation index    */
+} ia_reg_params;
+
+/* One regularised solve from an SVD: selects lambda (or k) and returns
+ * x (n), the selected lambda, its grid index and the TSVD index. */
+ia_status ia_regularize(int m, int n, int p, const ia_z *u, const double *s,
+                        const ia_z *v, const ia_z *b, const ia_reg_params *prm,
+                        ia_z **x, double *lambda, int *index);
+
+/* -------------------------------------------------- batched ESM sweep */
+typedef enum { IA_PREC_FP64 = 0, IA_PREC_MIXED = 1 } ia_precision;
+
+typedef struct {
+    int m, l;                 /* surface points (= sources), microphones   */
+    int n_freq;               /* frequencies in the sweep                  */
+    ia_precision precision;
+    ia_reg_params reg;
+    double area;              /* S in W = 1/2 Re(v^H p) S/m                */
+    int chunk;                /* frequencies per device batch (0 = auto)   */
+} ia_sweep_desc;
+
+typedef struct {
+    double *lambda;           /* [n_freq] selected lambda                  */
+    int *lambda_index;        /* [n_freq] grid index (Tikhonov/dsvd)       */
+    int *k_tsvd;              /* [n_freq] discrete-GCV TSVD index          */
+    double *power;            /* [n_freq] W                                */
+    double *sigma_max;        /* [n_freq]                                  */
+    ia_z *velocity;           /* [n_freq * m] surface normal velocity      */
+    ia_z *pressure;           /* [n_freq * m] surface pressure             */
+    int *status;              /* [n_freq] ia_status per frequency          */
+} ia_sweep_out;
+
+/* Solve an ESM sweep: for each frequency f, assemble D, M_hq at k[f],
+ * H_e = M_hq D^-1, SVD, select the regularisation parameter, solve
+ * H_e v = b_f, and reconstruct p_s = M_sq D^-1 v and W.  Inputs are
+ * host or device pointers (r_s, src, nrm: 3m; r_h: 3l; b: n_freq*l
+ * column per frequency); outputs are host pointers, any may be NULL. */
+ia_status ia_esm_sweep(const ia_sweep_desc *desc, const double *k,
+                       const double *r_s, const double *src, const double *nrm,
+                       const double *r_h, const ia_z *b, ia_sweep_out *out);
+
+/* Same, with every pointer (inputs and outputs) on the device and the work
+ * enqueued on `stream` (a hipStream_t; NULL = default stream).  Returns
+ * after the work is enqueued; scratch is owned by the library. */
+ia_status ia_esm_sweep_device(const ia_sweep_desc *desc, const double *k,
+                              const double *r_s, const double *src,
+                              const double *nrm, const double *r_h,
+                              const ia_z *b, ia_sweep_out *out, void *stream);
+
+/* Phase timings (ms) of the last sweep on this thread. */
+typedef struct {
+    double assemble, factor, solve_he, bidiag, bisect, vectors, select,
+           post, total;
+} ia_stats;
+ia_status ia_last_stats(ia_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IA_H */

@@ -1,0 +1,920 @@
+// Host orchestration and the C ABI (include/ia.h).
+//
+// Every entry point keeps the name, argument order and output convention
+// (callee-malloc'd host buffers, caller frees) of the reference function it
+// replaces; the ia_* variants return an ia_status instead of throwing
+// (the reference throws through extern "C", CUDA/cusolver_utils.cuh:30-57).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cctype>
+#include <cstring>
+#include <algorithm>
+#include <vector>
+
+#include "ia_internal.hpp"
+
+namespace ia {
+
+// ------------------------------------------------------------- errors
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+ia_status fail(ia_status st, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return st;
+}
+
+Air &air() {
+    static thread_local Air a;
+    return a;
+}
+
+bool is_device_ptr(const void *p) {
+    if (!p) return false;
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes(&attr, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+ia_status to_device(DevBuf &dst, const void *src, size_t n, hipStream_t s) {
+    IA_TRY(dst.alloc(n));
+    if (n == 0) return IA_OK;
+    IA_HIP(hipMemcpyAsync(dst.p, src, n, hipMemcpyDefault, s));
+    return IA_OK;
+}
+
+static thread_local ia_stats g_stats;
+
+static bool finite_all(const double *p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+static ia_status check_device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return fail(IA_ERR_DEVICE, "no HIP device available (this library has no CPU fallback)");
+    }
+    return IA_OK;
+}
+
+template <typename T>
+static ia_status host_out(T **dst, const void *dsrc, size_t count) {
+    if (!dst) return IA_OK;
+    *dst = (T *)malloc(sizeof(T) * (count ? count : 1));
+    if (!*dst) return fail(IA_ERR_NOMEM, "malloc(%zu)", sizeof(T) * count);
+    if (count) IA_HIP(hipMemcpy(*dst, dsrc, sizeof(T) * count, hipMemcpyDeviceToHost));
+    return IA_OK;
+}
+
+// ------------------------------------------------------ SVD driver
+// Factor Y (R x C, R >= C, ld R, batch stride R*C, complex T) in place:
+// bidiagonal reflectors in Y/Vr/tau, singular values sig (descending),
+// bidiagonal singular vectors QBt/PBt (vector i in row i).
+template <typename T>
+struct SvdWork {
+    DevBuf d, e, sig, tauL, tauR, Vr, QBt, PBt, work;
+    int R = 0, C = 0, batch = 0, sub = 1;
+    ia_status alloc(int R_, int C_, int batch_) {
+        R = R_;
+        C = C_;
+        batch = batch_;
+        size_t bc = (size_t)batch * C;
+        IA_TRY(d.alloc(bc * sizeof(double)));
+        IA_TRY(e.alloc(bc * sizeof(double)));
+        IA_TRY(sig.alloc(bc * sizeof(double)));
+        IA_TRY(tauL.alloc(bc * sizeof(cx<T>)));
+        IA_TRY(tauR.alloc(bc * sizeof(cx<T>)));
+        IA_TRY(Vr.alloc(bc * C * sizeof(cx<T>)));
+        IA_TRY(QBt.alloc(bc * C * sizeof(double)));
+        IA_TRY(PBt.alloc(bc * C * sizeof(double)));
+        // inverse-iteration scratch bounded to ~1 GiB
+        size_t per = inviter_work_bytes(C, 1);
+        sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)batch, ((size_t)1 << 30) / per));
+        IA_TRY(work.alloc(inviter_work_bytes(C, sub)));
+        return IA_OK;
+    }
+};
+
+template <typename T>
+static ia_status svd_factor(SvdWork<T> &w, cx<T> *Y, hipStream_t s) {
+    const int R = w.R, C = w.C, batch = w.batch;
+    size_t lds = bidiag_lds_bytes(R, C, sizeof(cx<T>));
+    if (lds > 160 * 1024)
+        return fail(IA_ERR_ARG, "SVD of %d x %d exceeds the on-chip bidiagonalisation limit", R, C);
+    launch_bidiag<T>(R, C, Y, R, (size_t)R * C, w.d.template as<double>(), w.e.template as<double>(),
+                     w.tauL.template as<cx<T>>(), w.tauR.template as<cx<T>>(),
+                     w.Vr.template as<cx<T>>(), batch, s);
+    IA_HIP(hipGetLastError());
+    launch_bisect(C, w.d.template as<double>(), w.e.template as<double>(), w.sig.template as<double>(),
+                  batch, s);
+    IA_HIP(hipGetLastError());
+    launch_inviter(C, w.d.template as<double>(), w.e.template as<double>(),
+                   w.sig.template as<double>(), w.QBt.template as<double>(),
+                   w.PBt.template as<double>(), w.work.template as<double>(), w.sub, batch, s);
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
+// ------------------------------------------------------ sweep engine
+struct Timer {
+    hipEvent_t ev[10];
+    int n = 0;
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st) {
+        for (auto &e : ev) (void)hipEventCreate(&e);
+    }
+    ~Timer() {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    void mark() {
+        if (n < 10) (void)hipEventRecord(ev[n++], s);
+    }
+    double ms(int a, int b) {
+        float t = 0;
+        if (a < n && b < n) (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+        return t;
+    }
+};
+
+template <typename T>
+static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const double *r_s,
+                                const double *src, const double *nrm, const double *r_h,
+                                const cx<double> *b, double *lam, int *lidx, int *ktsvd,
+                                double *power, double *smax, cx<double> *vel, cx<double> *pres,
+                                int *status, hipStream_t s) {
+    const int m = D.m, l = D.l, F = D.n_freq;
+    if (l > m) return fail(IA_ERR_ARG, "ia_esm_sweep: l (%d) > m (%d) is not supported", l, m);
+    int chunk = D.chunk > 0 ? D.chunk : std::min(F, 64);
+    chunk = std::min(chunk, F);
+    const int R = m, C = l;  // Y = H_e^H (m x l), A = Y^H
+    const double rho_c = air().rho * air().c;
+    RegDev prm{(int)D.reg.filter, (int)D.reg.select, D.reg.n_lambda > 0 ? D.reg.n_lambda : 50,
+               D.reg.k, D.reg.decades > 0 ? D.reg.decades : 8.0, D.reg.lambda};
+    if (prm.n_lambda > 512) return fail(IA_ERR_ARG, "n_lambda must be <= 512");
+
+    DevBuf bD, bY, bipiv, binfo, bc, bbeta, bphi, bx, bq, bps, bpsd, bn2, blam, bidx, bk, bW;
+    IA_TRY(bD.alloc(sizeof(cx<T>) * (size_t)chunk * m * m));
+    IA_TRY(bY.alloc(sizeof(cx<T>) * (size_t)chunk * m * l));
+    IA_TRY(bipiv.alloc(sizeof(int) * (size_t)chunk * m));
+    IA_TRY(binfo.alloc(sizeof(int) * (size_t)chunk));
+    IA_TRY(bc.alloc(sizeof(cx<double>) * (size_t)chunk * C));
+    IA_TRY(bbeta.alloc(sizeof(cx<double>) * (size_t)chunk * C));
+    IA_TRY(bphi.alloc(sizeof(cx<double>) * (size_t)chunk * C));
+    IA_TRY(bx.alloc(sizeof(cx<double>) * (size_t)chunk * R));
+    IA_TRY(bq.alloc(sizeof(cx<T>) * (size_t)chunk * m));
+    IA_TRY(bps.alloc(sizeof(cx<T>) * (size_t)chunk * m));
+    IA_TRY(bpsd.alloc(sizeof(cx<double>) * (size_t)chunk * m));
+    IA_TRY(bn2.alloc(sizeof(double) * (size_t)chunk));
+    SvdWork<T> sw;
+    IA_TRY(sw.alloc(R, C, chunk));
+
+    Timer tm(s);
+    ia_stats st{};
+    for (int f0 = 0; f0 < F; f0 += chunk) {
+        const int nf = std::min(chunk, F - f0);
+        const double *kf = k + f0;
+        cx<T> *Dm = bD.as<cx<T>>();
+        cx<T> *Y = bY.as<cx<T>>();
+        tm.n = 0;
+        tm.mark();  // 0
+        launch_esm_D<T>(m, nf, kf, r_s, src, nrm, Dm, (size_t)m * m, s);
+        launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s);
+        IA_HIP(hipGetLastError());
+        tm.mark();  // 1
+        IA_HIP(hipMemsetAsync(binfo.p, 0, sizeof(int) * nf, s));
+        IA_TRY(getrf_batched<T>(m, Dm, m, (size_t)m * m, bipiv.as<int>(), binfo.as<int>(), nf, s));
+        tm.mark();  // 2
+        IA_TRY(getrs_batched<T>('C', m, l, Dm, m, (size_t)m * m, bipiv.as<int>(), Y, m,
+                                (size_t)m * l, nf, s));
+        tm.mark();  // 3
+        sw.batch = nf;
+        {
+            size_t lds = bidiag_lds_bytes(R, C, sizeof(cx<T>));
+            if (lds > 160 * 1024)
+                return fail(IA_ERR_ARG, "ESM sweep %d x %d exceeds the on-chip limit", R, C);
+            launch_bidiag<T>(R, C, Y, R, (size_t)R * C, sw.d.template as<double>(),
+                             sw.e.template as<double>(), sw.tauL.template as<cx<T>>(),
+                             sw.tauR.template as<cx<T>>(), sw.Vr.template as<cx<T>>(), nf, s);
+        }
+        tm.mark();  // 4
+        launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(), nf, s);
+        tm.mark();  // 5
+        launch_inviter(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(),
+                       sw.QBt.template as<double>(), sw.PBt.template as<double>(), sw.work.template as<double>(), sw.sub, nf, s);
+        tm.mark();  // 6
+        // beta = Q_B^T V_b^H b
+        const cx<double> *bf = b + (size_t)f0 * l;
+        IA_HIP(hipMemcpyAsync(bc.p, bf, sizeof(cx<double>) * (size_t)nf * l, hipMemcpyDefault, s));
+        launch_norm2<double>(l, bc.as<cx<double>>(), l, bn2.as<double>(), nf, s);
+        launch_apply_right<T>(true, C, sw.Vr.template as<cx<T>>(), sw.tauR.template as<cx<T>>(), bc.as<cx<double>>(),
+                              C, 1, nf, s);
+        launch_real_matvec(C, sw.QBt.template as<double>(), 0, bc.as<cx<double>>(), C,
+                           bbeta.as<cx<double>>(), C, nf, s);
+        launch_select(C, l, sw.sig.template as<double>(), bbeta.as<cx<double>>(), bn2.as<double>(), prm,
+                      lam ? lam + f0 : nullptr, lidx ? lidx + f0 : nullptr,
+                      ktsvd ? ktsvd + f0 : nullptr, bphi.as<cx<double>>(), nf, s);
+        // x = U_b [P_B phi; 0]
+        launch_real_matvec(C, sw.PBt.template as<double>(), 1, bphi.as<cx<double>>(), C,
+                           bc.as<cx<double>>(), C, nf, s);
+        launch_pad_copy(C, R, bc.as<cx<double>>(), C, bx.as<cx<double>>(), R, nf, s);
+        launch_apply_left<T>(false, R, C, Y, R, (size_t)R * C, sw.tauL.template as<cx<T>>(),
+                             bx.as<cx<double>>(), R, 1, nf, s);
+        IA_HIP(hipGetLastError());
+        tm.mark();  // 7
+        // post: q = D^-1 x, p_s = M_sq q, W = 1/2 Re(x^H p_s) S/m
+        if (sizeof(T) == sizeof(double))
+            IA_HIP(hipMemcpyAsync(bq.p, bx.p, sizeof(cx<double>) * (size_t)nf * m,
+                                  hipMemcpyDeviceToDevice, s));
+        else
+            launch_convert<float, double>((size_t)nf * m, bx.as<cx<double>>(),
+                                          reinterpret_cast<cx<float> *>(bq.p), s);
+        IA_TRY(getrs_batched<T>('N', m, 1, Dm, m, (size_t)m * m, bipiv.as<int>(), bq.as<cx<T>>(),
+                                m, (size_t)m, nf, s));
+        launch_esm_msq_apply<T>(m, nf, kf, rho_c, r_s, src, bq.as<cx<T>>(), m, bps.as<cx<T>>(), m,
+                                s);
+        if (sizeof(T) == sizeof(double))
+            IA_HIP(hipMemcpyAsync(bpsd.p, bps.p, sizeof(cx<double>) * (size_t)nf * m,
+                                  hipMemcpyDeviceToDevice, s));
+        else
+            launch_convert<double, float>((size_t)nf * m, reinterpret_cast<cx<float> *>(bps.p),
+                                          bpsd.as<cx<double>>(), s);
+        if (power)
+            launch_power<double>(m, bx.as<cx<double>>(), m, bpsd.as<cx<double>>(), m, D.area / m,
+                                 power + f0, nf, s);
+        if (vel)
+            IA_HIP(hipMemcpyAsync(vel + (size_t)f0 * m, bx.p, sizeof(cx<double>) * (size_t)nf * m,
+                                  hipMemcpyDefault, s));
+        if (pres)
+            IA_HIP(hipMemcpyAsync(pres + (size_t)f0 * m, bpsd.p,
+                                  sizeof(cx<double>) * (size_t)nf * m, hipMemcpyDefault, s));
+        if (smax)
+            IA_HIP(hipMemcpy2DAsync(smax + f0, sizeof(double), sw.sig.p, sizeof(double) * C,
+                                    sizeof(double), nf, hipMemcpyDefault, s));
+        if (status)
+            IA_HIP(hipMemcpyAsync(status + f0, binfo.p, sizeof(int) * nf, hipMemcpyDefault, s));
+        IA_HIP(hipGetLastError());
+        tm.mark();  // 8
+        IA_HIP(hipStreamSynchronize(s));
+        st.assemble += tm.ms(0, 1);
+        st.factor += tm.ms(1, 2);
+        st.solve_he += tm.ms(2, 3);
+        st.bidiag += tm.ms(3, 4);
+        st.bisect += tm.ms(4, 5);
+        st.vectors += tm.ms(5, 6);
+        st.select += tm.ms(6, 7);
+        st.post += tm.ms(7, 8);
+        st.total += tm.ms(0, 8);
+    }
+    g_stats = st;
+    return IA_OK;
+}
+
+}  // namespace ia
+
+using namespace ia;
+
+// =====================================================================
+extern "C" {
+
+const char *ia_last_error(void) { return g_err; }
+const char *ia_version(void) { return "inverseacoustics_amd 0.1 (gfx950)"; }
+
+ia_status ia_set_air(double c, double rho) {
+    if (!(c > 0) || !(rho > 0) || !std::isfinite(c) || !std::isfinite(rho))
+        return fail(IA_ERR_ARG, "ia_set_air: c and rho must be positive and finite");
+    air().c = c;
+    air().rho = rho;
+    return IA_OK;
+}
+
+ia_status ia_get_air(double *c, double *rho) {
+    if (c) *c = air().c;
+    if (rho) *rho = air().rho;
+    return IA_OK;
+}
+
+int ia_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+void ia_release(void) { (void)hipDeviceSynchronize(); }
+
+ia_status ia_last_stats(ia_stats *st) {
+    if (!st) return fail(IA_ERR_ARG, "ia_last_stats: NULL");
+    *st = g_stats;
+    return IA_OK;
+}
+
+// ------------------------------------------------------------- sweep
+static ia_status sweep_check(const ia_sweep_desc *desc, const double *k, const double *r_s,
+                             const double *src, const double *r_h, const ia_z *b) {
+    if (!desc) return fail(IA_ERR_ARG, "ia_esm_sweep: desc is NULL");
+    if (desc->m <= 0 || desc->l <= 0 || desc->n_freq <= 0)
+        return fail(IA_ERR_ARG, "ia_esm_sweep: m, l and n_freq must be positive");
+    if (!k || !r_s || !src || !r_h || !b) return fail(IA_ERR_ARG, "ia_esm_sweep: NULL input");
+    if (desc->reg.filter < IA_FILTER_TIKHONOV || desc->reg.filter > IA_FILTER_DSVD)
+        return fail(IA_ERR_ARG, "ia_esm_sweep: bad filter");
+    if (desc->reg.select < IA_SELECT_GCV || desc->reg.select > IA_SELECT_QUASIOPT)
+        return fail(IA_ERR_ARG, "ia_esm_sweep: bad select");
+    if (desc->reg.filter == IA_FILTER_TSVD && desc->reg.select == IA_SELECT_FIXED &&
+        (desc->reg.k < 1 || desc->reg.k > std::min(desc->m, desc->l)))
+        return fail(IA_ERR_ARG, "ia_esm_sweep: k out of range");
+    return IA_OK;
+}
+
+ia_status ia_esm_sweep_device(const ia_sweep_desc *desc, const double *k, const double *r_s,
+                              const double *src, const double *nrm, const double *r_h,
+                              const ia_z *b, ia_sweep_out *out, void *stream) {
+    IA_TRY(check_device());
+    IA_TRY(sweep_check(desc, k, r_s, src, r_h, b));
+    ia_sweep_out o{};
+    if (out) o = *out;
+    hipStream_t s = (hipStream_t)stream;
+    if (desc->precision == IA_PREC_MIXED)
+        return esm_sweep_impl<float>(*desc, k, r_s, src, nrm, r_h, (const cx<double> *)b, o.lambda,
+                                     o.lambda_index, o.k_tsvd, o.power, o.sigma_max,
+                                     (cx<double> *)o.velocity, (cx<double> *)o.pressure, o.status,
+                                     s);
+    return esm_sweep_impl<double>(*desc, k, r_s, src, nrm, r_h, (const cx<double> *)b, o.lambda,
+                                  o.lambda_index, o.k_tsvd, o.power, o.sigma_max,
+                                  (cx<double> *)o.velocity, (cx<double> *)o.pressure, o.status, s);
+}
+
+ia_status ia_esm_sweep(const ia_sweep_desc *desc, const double *k, const double *r_s,
+                       const double *src, const double *nrm, const double *r_h, const ia_z *b,
+                       ia_sweep_out *out) {
+    IA_TRY(check_device());
+    IA_TRY(sweep_check(desc, k, r_s, src, r_h, b));
+    const int m = desc->m, l = desc->l, F = desc->n_freq;
+    if (!is_device_ptr(k) && !finite_all(k, F))
+        return fail(IA_ERR_NONFINITE, "ia_esm_sweep: non-finite wavenumber");
+    if (!is_device_ptr(k))
+        for (int f = 0; f < F; ++f)
+            if (!(k[f] > 0)) return fail(IA_ERR_ARG, "ia_esm_sweep: k must be > 0");
+    if (!is_device_ptr(r_s) && !finite_all(r_s, 3 * (size_t)m))
+        return fail(IA_ERR_NONFINITE, "ia_esm_sweep: non-finite surface coordinates");
+    if (!is_device_ptr(r_h) && !finite_all(r_h, 3 * (size_t)l))
+        return fail(IA_ERR_NONFINITE, "ia_esm_sweep: non-finite microphone coordinates");
+    if (!is_device_ptr(b) && !finite_all((const double *)b, 2 * (size_t)l * F))
+        return fail(IA_ERR_NONFINITE, "ia_esm_sweep: non-finite pressures");
+    hipStream_t s = nullptr;
+    DevBuf dk, drs, dsrc, dnrm, drh, db;
+    IA_TRY(to_device(dk, k, sizeof(double) * F, s));
+    IA_TRY(to_device(drs, r_s, sizeof(double) * 3 * m, s));
+    IA_TRY(to_device(dsrc, src, sizeof(double) * 3 * m, s));
+    if (nrm) IA_TRY(to_device(dnrm, nrm, sizeof(double) * 3 * m, s));
+    IA_TRY(to_device(drh, r_h, sizeof(double) * 3 * l, s));
+    IA_TRY(to_device(db, b, sizeof(ia_z) * (size_t)l * F, s));
+    ia_sweep_out o{};
+    if (out) o = *out;
+    // device staging for outputs
+    DevBuf ol, oi, ok, ow, osm, ov, op, ost;
+    ia_sweep_out dout{};
+    if (o.lambda) { IA_TRY(ol.alloc(sizeof(double) * F)); dout.lambda = ol.as<double>(); }
+    if (o.lambda_index) { IA_TRY(oi.alloc(sizeof(int) * F)); dout.lambda_index = oi.as<int>(); }
+    if (o.k_tsvd) { IA_TRY(ok.alloc(sizeof(int) * F)); dout.k_tsvd = ok.as<int>(); }
+    if (o.power) { IA_TRY(ow.alloc(sizeof(double) * F)); dout.power = ow.as<double>(); }
+    if (o.sigma_max) { IA_TRY(osm.alloc(sizeof(double) * F)); dout.sigma_max = osm.as<double>(); }
+    if (o.velocity) { IA_TRY(ov.alloc(sizeof(ia_z) * (size_t)F * m)); dout.velocity = ov.as<ia_z>(); }
+    if (o.pressure) { IA_TRY(op.alloc(sizeof(ia_z) * (size_t)F * m)); dout.pressure = op.as<ia_z>(); }
+    if (o.status) { IA_TRY(ost.alloc(sizeof(int) * F)); dout.status = ost.as<int>(); }
+    IA_TRY(ia_esm_sweep_device(desc, dk.as<double>(), drs.as<double>(), dsrc.as<double>(),
+                               nrm ? dnrm.as<double>() : nullptr, drh.as<double>(), db.as<ia_z>(),
+                               &dout, s));
+    IA_HIP(hipStreamSynchronize(s));
+    if (o.lambda) IA_HIP(hipMemcpy(o.lambda, dout.lambda, sizeof(double) * F, hipMemcpyDeviceToHost));
+    if (o.lambda_index) IA_HIP(hipMemcpy(o.lambda_index, dout.lambda_index, sizeof(int) * F, hipMemcpyDeviceToHost));
+    if (o.k_tsvd) IA_HIP(hipMemcpy(o.k_tsvd, dout.k_tsvd, sizeof(int) * F, hipMemcpyDeviceToHost));
+    if (o.power) IA_HIP(hipMemcpy(o.power, dout.power, sizeof(double) * F, hipMemcpyDeviceToHost));
+    if (o.sigma_max) IA_HIP(hipMemcpy(o.sigma_max, dout.sigma_max, sizeof(double) * F, hipMemcpyDeviceToHost));
+    if (o.velocity) IA_HIP(hipMemcpy(o.velocity, dout.velocity, sizeof(ia_z) * (size_t)F * m, hipMemcpyDeviceToHost));
+    if (o.pressure) IA_HIP(hipMemcpy(o.pressure, dout.pressure, sizeof(ia_z) * (size_t)F * m, hipMemcpyDeviceToHost));
+    if (o.status) {
+        IA_HIP(hipMemcpy(o.status, dout.status, sizeof(int) * F, hipMemcpyDeviceToHost));
+        for (int f = 0; f < F; ++f) o.status[f] = o.status[f] ? IA_ERR_SINGULAR : IA_OK;
+    }
+    return IA_OK;
+}
+
+// --------------------------------------------------------------- ESM
+ia_status ia_esm(int m, int l, const double *r_ms, const double *r_lh, const double *src,
+                 const double *nrm, double k, ia_z **m_e, ia_z **h_e, ia_z **d, ia_z **m_sq,
+                 ia_z **m_hq) {
+    IA_TRY(check_device());
+    if (m <= 0 || l <= 0) return fail(IA_ERR_ARG, "ia_esm: m and l must be positive");
+    if (!r_ms || !r_lh || !src) return fail(IA_ERR_ARG, "ia_esm: NULL coordinates");
+    if (!(k > 0) || !std::isfinite(k)) return fail(IA_ERR_ARG, "ia_esm: k must be > 0");
+    hipStream_t s = nullptr;
+    DevBuf drs, drh, dsrc, dnrm, dk, bD, bMsq, bMhq, bipiv, binfo;
+    IA_TRY(to_device(drs, r_ms, sizeof(double) * 3 * m, s));
+    IA_TRY(to_device(drh, r_lh, sizeof(double) * 3 * l, s));
+    IA_TRY(to_device(dsrc, src, sizeof(double) * 3 * m, s));
+    if (nrm) IA_TRY(to_device(dnrm, nrm, sizeof(double) * 3 * m, s));
+    IA_TRY(to_device(dk, &k, sizeof(double), s));
+    IA_TRY(bD.alloc(sizeof(cx<double>) * (size_t)m * m));
+    IA_TRY(bMsq.alloc(sizeof(cx<double>) * (size_t)m * m));
+    IA_TRY(bMhq.alloc(sizeof(cx<double>) * (size_t)m * l));
+    IA_TRY(bipiv.alloc(sizeof(int) * m));
+    IA_TRY(binfo.alloc(sizeof(int)));
+    const double rho_c = air().rho * air().c;
+    launch_esm_D<double>(m, 1, dk.as<double>(), drs.as<double>(), dsrc.as<double>(),
+                         nrm ? dnrm.as<double>() : nullptr, bD.as<cx<double>>(), 0, s);
+    launch_esm_M<double>(m, m, 1, dk.as<double>(), rho_c, drs.as<double>(), dsrc.as<double>(),
+                         bMsq.as<cx<double>>(), 0, false, s);
+    launch_esm_M<double>(l, m, 1, dk.as<double>(), rho_c, drh.as<double>(), dsrc.as<double>(),
+                         bMhq.as<cx<double>>(), 0, false, s);
+    IA_HIP(hipGetLastError());
+    if (d) IA_TRY(host_out<ia_z>(d, bD.p, (size_t)m * m));
+    if (m_sq) IA_TRY(host_out<ia_z>(m_sq, bMsq.p, (size_t)m * m));
+    if (m_hq) IA_TRY(host_out<ia_z>(m_hq, bMhq.p, (size_t)m * l));
+    if (m_e || h_e) {
+        IA_HIP(hipMemsetAsync(binfo.p, 0, sizeof(int), s));
+        IA_TRY(getrf_batched<double>(m, bD.as<cx<double>>(), m, 0, bipiv.as<int>(),
+                                     binfo.as<int>(), 1, s));
+        int info = 0;
+        IA_HIP(hipMemcpy(&info, binfo.p, sizeof(int), hipMemcpyDeviceToHost));
+        if (info) return fail(IA_ERR_SINGULAR, "ia_esm: D is singular (zero pivot %d)", info);
+        // X D = M  <=>  D^H X^H = M^H
+        DevBuf bT;
+        auto solve_right = [&](DevBuf &M, int rows, ia_z **outp) -> ia_status {
+            IA_TRY(bT.alloc(sizeof(cx<double>) * (size_t)rows * m));
+            launch_conj_transpose<double>(rows, m, M.as<cx<double>>(), 0, bT.as<cx<double>>(), 0,
+                                          1, s);
+            IA_TRY(getrs_batched<double>('C', m, rows, bD.as<cx<double>>(), m, 0, bipiv.as<int>(),
+                                         bT.as<cx<double>>(), m, 0, 1, s));
+            launch_conj_transpose<double>(m, rows, bT.as<cx<double>>(), 0, M.as<cx<double>>(), 0,
+                                          1, s);
+            IA_HIP(hipGetLastError());
+            return host_out<ia_z>(outp, M.p, (size_t)rows * m);
+        };
+        if (m_e) IA_TRY(solve_right(bMsq, m, m_e));
+        if (h_e) IA_TRY(solve_right(bMhq, l, h_e));
+    }
+    IA_HIP(hipDeviceSynchronize());
+    return IA_OK;
+}
+
+static void legacy_report(const char *fn, ia_status st) {
+    if (st != IA_OK) fprintf(stderr, "%s: %s\n", fn, ia_last_error());
+}
+
+void esm(int m, int l, double *r_ms, double *r_lh, ia_z **m_e, ia_z **h, double k, double r_i) {
+    if (m_e) *m_e = nullptr;
+    if (h) *h = nullptr;
+    if (m <= 0 || !r_ms) {
+        legacy_report("esm", fail(IA_ERR_ARG, "esm: bad arguments"));
+        return;
+    }
+    std::vector<double> src(3 * (size_t)m);
+    std::vector<double> host_rs(3 * (size_t)m);
+    if (is_device_ptr(r_ms))
+        (void)hipMemcpy(host_rs.data(), r_ms, sizeof(double) * 3 * m, hipMemcpyDeviceToHost);
+    else
+        memcpy(host_rs.data(), r_ms, sizeof(double) * 3 * m);
+    for (size_t i = 0; i < 3 * (size_t)m; ++i) src[i] = r_i * host_rs[i];
+    legacy_report("esm", ia_esm(m, l, host_rs.data(), r_lh, src.data(), nullptr, k, m_e, h,
+                                nullptr, nullptr, nullptr));
+}
+
+void esmgpu(int m, int l, double *r_msg, double *r_lhg, ia_z **me, ia_z **he, double k,
+            double r_i) {
+    esm(m, l, r_msg, r_lhg, me, he, k, r_i);
+}
+
+// --------------------------------------------------------------- SVD
+ia_status ia_csvd(int m, int n, const ia_z *a, ia_z **u, double **s_out, ia_z **v) {
+    IA_TRY(check_device());
+    if (m <= 0 || n <= 0 || !a) return fail(IA_ERR_ARG, "ia_csvd: bad arguments");
+    hipStream_t s = nullptr;
+    const bool tall = (m >= n);
+    const int R = tall ? m : n, C = tall ? n : m;
+    DevBuf bA, bY, bUY, bVY;
+    IA_TRY(to_device(bA, a, sizeof(ia_z) * (size_t)m * n, s));
+    if (tall) {
+        IA_TRY(bY.alloc(sizeof(cx<double>) * (size_t)R * C));
+        IA_HIP(hipMemcpyAsync(bY.p, bA.p, sizeof(cx<double>) * (size_t)R * C,
+                              hipMemcpyDeviceToDevice, s));
+    } else {
+        IA_TRY(bY.alloc(sizeof(cx<double>) * (size_t)R * C));
+        launch_conj_transpose<double>(m, n, bA.as<cx<double>>(), 0, bY.as<cx<double>>(), 0, 1, s);
+    }
+    SvdWork<double> sw;
+    IA_TRY(sw.alloc(R, C, 1));
+    IA_TRY(svd_factor<double>(sw, bY.as<cx<double>>(), s));
+    // U_Y = U_b [P_B; 0] (R x C), V_Y = V_b Q_B (C x C)
+    IA_TRY(bUY.alloc(sizeof(cx<double>) * (size_t)R * C));
+    IA_TRY(bVY.alloc(sizeof(cx<double>) * (size_t)C * C));
+    launch_rows_to_cols(R, C, sw.PBt.as<double>(), bUY.as<cx<double>>(), 1, s);
+    launch_rows_to_cols(C, C, sw.QBt.as<double>(), bVY.as<cx<double>>(), 1, s);
+    launch_apply_left<double>(false, R, C, bY.as<cx<double>>(), R, 0, sw.tauL.as<cx<double>>(),
+                              bUY.as<cx<double>>(), R, C, 1, s);
+    launch_apply_right<double>(false, C, sw.Vr.as<cx<double>>(), sw.tauR.as<cx<double>>(),
+                               bVY.as<cx<double>>(), C, C, 1, s);
+    IA_HIP(hipGetLastError());
+    IA_HIP(hipDeviceSynchronize());
+    IA_TRY(host_out<double>(s_out, sw.sig.p, C));
+    if (tall) {
+        IA_TRY(host_out<ia_z>(u, bUY.p, (size_t)R * C));
+        IA_TRY(host_out<ia_z>(v, bVY.p, (size_t)C * C));
+    } else {
+        IA_TRY(host_out<ia_z>(u, bVY.p, (size_t)C * C));
+        IA_TRY(host_out<ia_z>(v, bUY.p, (size_t)R * C));
+    }
+    return IA_OK;
+}
+
+void csvd(int m, int n, ia_z *a, ia_z **u, double **s, ia_z **v) {
+    if (u) *u = nullptr;
+    if (s) *s = nullptr;
+    if (v) *v = nullptr;
+    legacy_report("csvd", ia_csvd(m, n, a, u, s, v));
+}
+
+// x = V[:, :p] (g .* (U[:, :p]^H b)) with g from a filter; the two products
+// run on the device (GEMM kernel), g is formed on the host from s.
+static ia_status filtered_solve(int m, int n, int p, const ia_z *u, const double *s,
+                                const ia_z *v, const ia_z *b, int kind, double lam, int kk,
+                                ia_z **x) {
+    IA_TRY(check_device());
+    if (m <= 0 || n <= 0 || p <= 0 || !u || !s || !v || !b || !x)
+        return fail(IA_ERR_ARG, "bad arguments");
+    hipStream_t st = nullptr;
+    DevBuf dU, dV, db, dbeta, dx;
+    IA_TRY(to_device(dU, u, sizeof(ia_z) * (size_t)m * p, st));
+    IA_TRY(to_device(dV, v, sizeof(ia_z) * (size_t)n * p, st));
+    IA_TRY(to_device(db, b, sizeof(ia_z) * (size_t)m, st));
+    IA_TRY(dbeta.alloc(sizeof(ia_z) * p));
+    IA_TRY(dx.alloc(sizeof(ia_z) * n));
+    const cx<double> one{1, 0}, zero{0, 0};
+    launch_gemm<double>('C', 'N', p, 1, m, one, dU.as<cx<double>>(), m, 0, db.as<cx<double>>(), m,
+                        0, zero, dbeta.as<cx<double>>(), p, 0, 1, st);
+    std::vector<ia_z> beta(p);
+    IA_HIP(hipMemcpy(beta.data(), dbeta.p, sizeof(ia_z) * p, hipMemcpyDeviceToHost));
+    for (int i = 0; i < p; ++i) {
+        double g;
+        if (kind == 0)
+            g = (i < kk) ? 1.0 / s[i] : 0.0;
+        else if (kind == 1)
+            g = s[i] / (s[i] * s[i] + lam * lam);
+        else
+            g = 1.0 / (s[i] + lam);
+        beta[i].re *= g;
+        beta[i].im *= g;
+    }
+    IA_HIP(hipMemcpy(dbeta.p, beta.data(), sizeof(ia_z) * p, hipMemcpyHostToDevice));
+    launch_gemm<double>('N', 'N', n, 1, p, one, dV.as<cx<double>>(), n, 0,
+                        dbeta.as<cx<double>>(), p, 0, zero, dx.as<cx<double>>(), n, 0, 1, st);
+    IA_HIP(hipGetLastError());
+    IA_HIP(hipDeviceSynchronize());
+    return host_out<ia_z>(x, dx.p, n);
+}
+
+void tsvd(int m, int n, int p, int k, ia_z *u, double *s, ia_z *v, ia_z *b, ia_z **x) {
+    if (x) *x = nullptr;
+    if (k < 1 || k > p) {  // tsvd.cuh:124-126
+        legacy_report("tsvd", fail(IA_ERR_ARG, "Parameter k out of range..."));
+        return;
+    }
+    legacy_report("tsvd", filtered_solve(m, n, p, u, s, v, b, 0, 0.0, k, x));
+}
+
+ia_status ia_tikhonov(int m, int n, int p, const ia_z *u, const double *s, const ia_z *v,
+                      const ia_z *b, double lambda, ia_z **x) {
+    if (!(lambda >= 0)) return fail(IA_ERR_ARG, "ia_tikhonov: lambda must be >= 0");
+    return filtered_solve(m, n, p, u, s, v, b, 1, lambda, 0, x);
+}
+
+ia_status ia_dsvd(int m, int n, int p, const ia_z *u, const double *s, const ia_z *v,
+                  const ia_z *b, double lambda, ia_z **x) {
+    if (!(lambda >= 0)) return fail(IA_ERR_ARG, "ia_dsvd: lambda must be >= 0");
+    return filtered_solve(m, n, p, u, s, v, b, 2, lambda, 0, x);
+}
+
+ia_status ia_regularize(int m, int n, int p, const ia_z *u, const double *s, const ia_z *v,
+                        const ia_z *b, const ia_reg_params *prm, ia_z **x, double *lambda,
+                        int *index) {
+    IA_TRY(check_device());
+    if (!prm || m <= 0 || n <= 0 || p <= 0 || !u || !s || !v || !b)
+        return fail(IA_ERR_ARG, "ia_regularize: bad arguments");
+    hipStream_t st = nullptr;
+    DevBuf dU, ds, db, dbeta, dphi, dbn2, dlam, didx, dk, dV, dx;
+    IA_TRY(to_device(dU, u, sizeof(ia_z) * (size_t)m * p, st));
+    IA_TRY(to_device(dV, v, sizeof(ia_z) * (size_t)n * p, st));
+    IA_TRY(to_device(ds, s, sizeof(double) * p, st));
+    IA_TRY(to_device(db, b, sizeof(ia_z) * m, st));
+    IA_TRY(dbeta.alloc(sizeof(ia_z) * p));
+    IA_TRY(dphi.alloc(sizeof(ia_z) * p));
+    IA_TRY(dbn2.alloc(sizeof(double)));
+    IA_TRY(dlam.alloc(sizeof(double)));
+    IA_TRY(didx.alloc(sizeof(int)));
+    IA_TRY(dk.alloc(sizeof(int)));
+    IA_TRY(dx.alloc(sizeof(ia_z) * n));
+    const cx<double> one{1, 0}, zero{0, 0};
+    launch_gemm<double>('C', 'N', p, 1, m, one, dU.as<cx<double>>(), m, 0, db.as<cx<double>>(), m,
+                        0, zero, dbeta.as<cx<double>>(), p, 0, 1, st);
+    launch_norm2<double>(m, db.as<cx<double>>(), m, dbn2.as<double>(), 1, st);
+    RegDev rd{(int)prm->filter, (int)prm->select, prm->n_lambda > 0 ? prm->n_lambda : 50, prm->k,
+              prm->decades > 0 ? prm->decades : 8.0, prm->lambda};
+    if (rd.n_lambda > 512) return fail(IA_ERR_ARG, "n_lambda must be <= 512");
+    if (prm->filter == IA_FILTER_TSVD && prm->select == IA_SELECT_FIXED && (prm->k < 0 || prm->k > p))
+        return fail(IA_ERR_ARG, "k out of range");
+    launch_select(p, m, ds.as<double>(), dbeta.as<cx<double>>(), dbn2.as<double>(), rd,
+                  dlam.as<double>(), didx.as<int>(), dk.as<int>(), dphi.as<cx<double>>(), 1, st);
+    launch_gemm<double>('N', 'N', n, 1, p, one, dV.as<cx<double>>(), n, 0, dphi.as<cx<double>>(),
+                        p, 0, zero, dx.as<cx<double>>(), n, 0, 1, st);
+    IA_HIP(hipGetLastError());
+    IA_HIP(hipDeviceSynchronize());
+    double lam_h = 0;
+    int idx_h = 0, k_h = 0;
+    IA_HIP(hipMemcpy(&lam_h, dlam.p, sizeof(double), hipMemcpyDeviceToHost));
+    IA_HIP(hipMemcpy(&idx_h, didx.p, sizeof(int), hipMemcpyDeviceToHost));
+    IA_HIP(hipMemcpy(&k_h, dk.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (lambda) *lambda = lam_h;
+    if (index) *index = (prm->filter == IA_FILTER_TSVD) ? (prm->select == IA_SELECT_FIXED ? prm->k : k_h) : idx_h;
+    if (x) IA_TRY(host_out<ia_z>(x, dx.p, n));
+    return IA_OK;
+}
+
+// ------------------------------------------------------------ dense
+}  // extern "C"
+
+namespace ia {
+template <typename T, typename Z>
+ia_status gemm_host(char opa, char opb, int m, int n, int k, Z alpha, const Z *a, int lda,
+                           const Z *b, int ldb, Z beta, Z *c, int ldc) {
+    IA_TRY(check_device());
+    opa = (char)toupper(opa);
+    opb = (char)toupper(opb);
+    if ((opa != 'N' && opa != 'C') || (opb != 'N' && opb != 'C'))
+        return fail(IA_ERR_ARG, "gemm: op must be 'N' or 'C'");
+    if (m < 0 || n < 0 || k < 0 || !a || !b || !c) return fail(IA_ERR_ARG, "gemm: bad arguments");
+    const int arows = (opa == 'N') ? m : k, acols = (opa == 'N') ? k : m;
+    const int brows = (opb == 'N') ? k : n, bcols = (opb == 'N') ? n : k;
+    if (lda < std::max(1, arows) || ldb < std::max(1, brows) || ldc < std::max(1, m))
+        return fail(IA_ERR_ARG, "gemm: leading dimension too small");
+    if (m == 0 || n == 0) return IA_OK;
+    DevBuf dA, dB, dC;
+    IA_TRY(to_device(dA, a, sizeof(Z) * (size_t)lda * std::max(acols, 1), nullptr));
+    IA_TRY(to_device(dB, b, sizeof(Z) * (size_t)ldb * std::max(bcols, 1), nullptr));
+    IA_TRY(to_device(dC, c, sizeof(Z) * (size_t)ldc * n, nullptr));
+    launch_gemm<T>(opa, opb, m, n, k, cx<T>{alpha.re, alpha.im}, dA.as<cx<T>>(), lda, 0,
+                   dB.as<cx<T>>(), ldb, 0, cx<T>{beta.re, beta.im}, dC.as<cx<T>>(), ldc, 0, 1,
+                   nullptr);
+    IA_HIP(hipGetLastError());
+    IA_HIP(hipMemcpy(c, dC.p, sizeof(Z) * (size_t)ldc * n, hipMemcpyDefault));
+    return IA_OK;
+}
+
+}  // namespace ia
+
+extern "C" {
+
+ia_status ia_zgemm(char opa, char opb, int m, int n, int k, ia_z alpha, const ia_z *a, int lda,
+                   const ia_z *b, int ldb, ia_z beta, ia_z *c, int ldc) {
+    return gemm_host<double, ia_z>(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+ia_status ia_cgemm(char opa, char opb, int m, int n, int k, ia_c alpha, const ia_c *a, int lda,
+                   const ia_c *b, int ldb, ia_c beta, ia_c *c, int ldc) {
+    return gemm_host<float, ia_c>(opa, opb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+ia_status ia_zgesv(char op, int n, int nrhs, const ia_z *d, ia_z *b) {
+    IA_TRY(check_device());
+    op = (char)toupper(op);
+    if ((op != 'N' && op != 'C') || n <= 0 || nrhs < 0 || !d || !b)
+        return fail(IA_ERR_ARG, "ia_zgesv: bad arguments");
+    if (nrhs == 0) return IA_OK;
+    DevBuf dD, dB, dp, di;
+    IA_TRY(to_device(dD, d, sizeof(ia_z) * (size_t)n * n, nullptr));
+    IA_TRY(to_device(dB, b, sizeof(ia_z) * (size_t)n * nrhs, nullptr));
+    IA_TRY(dp.alloc(sizeof(int) * n));
+    IA_TRY(di.alloc(sizeof(int)));
+    IA_HIP(hipMemset(di.p, 0, sizeof(int)));
+    IA_TRY(getrf_batched<double>(n, dD.as<cx<double>>(), n, 0, dp.as<int>(), di.as<int>(), 1, nullptr));
+    int info = 0;
+    IA_HIP(hipMemcpy(&info, di.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (info) return fail(IA_ERR_SINGULAR, "ia_zgesv: singular matrix (zero pivot %d)", info);
+    IA_TRY(getrs_batched<double>(op, n, nrhs, dD.as<cx<double>>(), n, 0, dp.as<int>(),
+                                 dB.as<cx<double>>(), n, 0, 1, nullptr));
+    IA_HIP(hipMemcpy(b, dB.p, sizeof(ia_z) * (size_t)n * nrhs, hipMemcpyDefault));
+    return IA_OK;
+}
+
+// ------------------------------------------------------------ utils
+static ia_status matvec_host(int m, int n, const ia_z *A, const ia_z *x, ia_z **y) {
+    IA_TRY(check_device());
+    if (m <= 0 || n <= 0 || !A || !x || !y) return fail(IA_ERR_ARG, "bad arguments");
+    DevBuf dA, dx, dy;
+    IA_TRY(to_device(dA, A, sizeof(ia_z) * (size_t)m * n, nullptr));
+    IA_TRY(to_device(dx, x, sizeof(ia_z) * n, nullptr));
+    IA_TRY(dy.alloc(sizeof(ia_z) * m));
+    const cx<double> one{1, 0}, zero{0, 0};
+    launch_gemm<double>('N', 'N', m, 1, n, one, dA.as<cx<double>>(), m, 0, dx.as<cx<double>>(), n,
+                        0, zero, dy.as<cx<double>>(), m, 0, 1, nullptr);
+    IA_HIP(hipGetLastError());
+    IA_HIP(hipDeviceSynchronize());
+    return host_out<ia_z>(y, dy.p, m);
+}
+
+void to_pressure(int m, int n, ia_z *me, ia_z *x, ia_z **p) {
+    if (p) *p = nullptr;
+    legacy_report("to_pressure", matvec_host(m, n, me, x, p));
+}
+
+void to_velocity(int m, int n, ia_z *qe, ia_z *x, ia_z **v) {
+    if (v) *v = nullptr;
+    legacy_report("to_velocity", matvec_host(m, n, qe, x, v));
+}
+
+void sound_power(int m, ia_z *p, ia_z *v, double *w, double s) {
+    ia_status st = check_device();
+    if (st == IA_OK && (m <= 0 || !p || !v || !w)) st = fail(IA_ERR_ARG, "sound_power: bad arguments");
+    if (st == IA_OK) {
+        DevBuf dp, dv, dw;
+        st = to_device(dp, p, sizeof(ia_z) * m, nullptr);
+        if (st == IA_OK) st = to_device(dv, v, sizeof(ia_z) * m, nullptr);
+        if (st == IA_OK) st = dw.alloc(sizeof(double));
+        if (st == IA_OK) {
+            launch_power<double>(m, dv.as<cx<double>>(), m, dp.as<cx<double>>(), m, s / m,
+                                 dw.as<double>(), 1, nullptr);
+            if (hipMemcpy(w, dw.p, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+                st = fail(IA_ERR_DEVICE, "sound_power: copy failed");
+        }
+    }
+    legacy_report("sound_power", st);
+}
+
+void sphere(double *r_ms, double *r_lh, ia_z **p_s, ia_z **p_h, ia_z **v_s, double k, double u,
+            double a, int m, int l) {
+    // utils.cuh:384-412: p = (a/r) rho c u (ika/(ika-1)) e^{ik(r-a)}, v_s = u
+    const double rho = air().rho, c = air().c;
+    auto field = [&](const double *pts, int n, ia_z **out) {
+        *out = (ia_z *)malloc(sizeof(ia_z) * (n ? n : 1));
+        if (!*out) return;
+        for (int i = 0; i < n; ++i) {
+            double x = pts[i], y = pts[i + n], z = pts[i + 2 * n];
+            double r = std::sqrt(x * x + y * y + z * z);
+            // ika/(ika - 1) = ika(-ika - 1)/((ka)^2 + 1) = ((ka)^2 - i ka)/(1 + (ka)^2)
+            double ka = k * a, den = 1.0 + ka * ka;
+            double fr = ka * ka / den, fi = -ka / den;
+            double ph = k * (r - a), cs = std::cos(ph), sn = std::sin(ph);
+            double amp = (a / r) * rho * c * u;
+            (*out)[i].re = amp * (fr * cs - fi * sn);
+            (*out)[i].im = amp * (fr * sn + fi * cs);
+        }
+    };
+    if (p_s) field(r_ms, m, p_s);
+    if (p_h) field(r_lh, l, p_h);
+    if (v_s) {
+        *v_s = (ia_z *)malloc(sizeof(ia_z) * (m ? m : 1));
+        if (*v_s)
+            for (int i = 0; i < m; ++i) (*v_s)[i] = ia_z{u, 0.0};
+    }
+}
+
+void cartesian_2_spherical(int m, double *r_xx, double **azelr) {
+    // utils.cuh:414-427 (azimuth wrapped to [0, 2pi); elevation fix-up for x < 0)
+    *azelr = (double *)malloc(sizeof(double) * 3 * (m ? m : 1));
+    if (!*azelr) return;
+    for (int i = 0; i < m; ++i) {
+        double x = r_xx[i], y = r_xx[i + m], z = r_xx[i + 2 * m];
+        double xy = x * x + y * y;
+        (*azelr)[i + 2 * m] = std::sqrt(xy + z * z);
+        double az = std::atan2(y, x);
+        if (y < 0) az += 2 * M_PI;
+        double el = std::atan2(std::sqrt(xy), z);
+        if (x < 0) el -= 2 * M_PI;
+        (*azelr)[i] = az;
+        (*azelr)[i + m] = el;
+    }
+}
+
+ia_status ia_geometry(const char *file, double k, double t, double **r_ms, double **r_lh,
+                      int **e_n, int *m_out, int *l_out, int *n_out) {
+    // utils.cuh:291-382: Gmsh MSH 2.2, $Nodes then $Elements, type-2 triangles,
+    // degenerate triangles dropped, field points r_ms (1 + 2 pi t / k).
+    if (!file || !r_ms || !r_lh || !e_n || !m_out || !l_out || !n_out)
+        return fail(IA_ERR_ARG, "ia_geometry: NULL argument");
+    if (!(k > 0)) return fail(IA_ERR_ARG, "ia_geometry: k must be > 0");
+    FILE *fp = fopen(file, "r");
+    if (!fp) return fail(IA_ERR_IO, "Can't open %s", file);
+    std::vector<char> line(1 << 16);
+    auto next = [&]() -> bool { return fgets(line.data(), (int)line.size(), fp) != nullptr; };
+    auto is = [&](const char *tag) { return strncmp(line.data(), tag, strlen(tag)) == 0; };
+    bool ok = false;
+    while (next())
+        if (is("$Nodes")) {
+            ok = true;
+            break;
+        }
+    if (!ok || !next()) {
+        fclose(fp);
+        return fail(IA_ERR_IO, "%s: no $Nodes section", file);
+    }
+    int m = atoi(line.data());
+    if (m <= 0) {
+        fclose(fp);
+        return fail(IA_ERR_IO, "%s: bad node count", file);
+    }
+    std::vector<double> xyz(3 * (size_t)m);
+    int i = 0;
+    while (next() && !is("$EndNodes")) {
+        if (i >= m) {
+            fclose(fp);
+            return fail(IA_ERR_IO, "%s: more nodes than declared", file);
+        }
+        char *pl;
+        strtol(line.data(), &pl, 10);
+        xyz[i] = strtod(pl, &pl);
+        xyz[i + m] = strtod(pl, &pl);
+        xyz[i + 2 * m] = strtod(pl, nullptr);
+        ++i;
+    }
+    if (i != m) {
+        fclose(fp);
+        return fail(IA_ERR_IO, "%s: %d nodes read, %d declared", file, i, m);
+    }
+    ok = false;
+    while (next())
+        if (is("$Elements")) {
+            ok = true;
+            break;
+        }
+    if (!ok || !next()) {
+        fclose(fp);
+        return fail(IA_ERR_IO, "%s: no $Elements section", file);
+    }
+    std::vector<int> tri;
+    while (next() && !is("$EndElements")) {
+        char *pl;
+        strtol(line.data(), &pl, 10);
+        long typ = strtol(pl, &pl, 10);
+        if (typ != 2) continue;
+        long ntags = strtol(pl, &pl, 10);
+        for (long q = 0; q < ntags; ++q) strtol(pl, &pl, 10);
+        int a = (int)strtol(pl, &pl, 10) - 1, b = (int)strtol(pl, &pl, 10) - 1,
+            c = (int)strtol(pl, nullptr, 10) - 1;
+        if (a < 0 || b < 0 || c < 0 || a >= m || b >= m || c >= m) {
+            fclose(fp);
+            return fail(IA_ERR_IO, "%s: element node index out of range", file);
+        }
+        if (a != b && a != c && b != c) {  // utils.cuh:337
+            tri.push_back(a);
+            tri.push_back(b);
+            tri.push_back(c);
+        }
+    }
+    fclose(fp);
+    const int n = (int)(tri.size() / 3);
+    *r_ms = (double *)malloc(sizeof(double) * 3 * m);
+    *r_lh = (double *)malloc(sizeof(double) * 3 * m);
+    *e_n = (int *)malloc(sizeof(int) * 3 * (n ? n : 1));
+    if (!*r_ms || !*r_lh || !*e_n) return fail(IA_ERR_NOMEM, "ia_geometry: malloc");
+    memcpy(*r_ms, xyz.data(), sizeof(double) * 3 * m);
+    const double dot = (2.0 * M_PI / k) * t;
+    for (int q = 0; q < 3 * m; ++q) (*r_lh)[q] = xyz[q] * (1.0 + dot);
+    for (int e = 0; e < n; ++e) {
+        (*e_n)[e] = tri[3 * e];
+        (*e_n)[e + n] = tri[3 * e + 1];
+        (*e_n)[e + 2 * n] = tri[3 * e + 2];
+    }
+    *m_out = m;
+    *l_out = m;
+    *n_out = n;
+    return IA_OK;
+}
+
+void geometry(double **r_ms, double **r_lh, int **e_n, double k, double t, int *m_out, int *l_out,
+              int *n_out, char *file) {
+    ia_status st = ia_geometry(file, k, t, r_ms, r_lh, e_n, m_out, l_out, n_out);
+    legacy_report("geometry", st);
+}
+
+}  // extern "C"

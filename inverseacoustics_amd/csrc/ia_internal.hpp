@@ -1,0 +1,205 @@
+// Internal declarations shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/ia.h"
+
+namespace ia {
+
+// ------------------------------------------------------------ complex
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+    T re, im;
+};
+using z64 = cx<double>;
+using c32 = cx<float>;
+
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> mk(T r, T i) { return cx<T>{r, i}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator-(cx<T> a) { return {-a.re, -a.im}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> operator*(T s, cx<T> a) { return {s * a.re, s * a.im}; }
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.re, -a.im}; }
+template <typename T>
+__host__ __device__ __forceinline__ T abs2(cx<T> a) { return a.re * a.re + a.im * a.im; }
+// conj(a) * b
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T> b) {
+    return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+template <typename T>
+__host__ __device__ __forceinline__ cx<T> cdiv(cx<T> a, cx<T> b) {
+    // Smith's algorithm
+    if (fabs(b.re) >= fabs(b.im)) {
+        T r = b.im / b.re, d = b.re + b.im * r;
+        return {(a.re + a.im * r) / d, (a.im - a.re * r) / d};
+    } else {
+        T r = b.re / b.im, d = b.im + b.re * r;
+        return {(a.re * r + a.im) / d, (a.im * r - a.re) / d};
+    }
+}
+template <typename To, typename Ti>
+__host__ __device__ __forceinline__ cx<To> cvt(cx<Ti> a) { return {(To)a.re, (To)a.im}; }
+
+// ------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+ia_status fail(ia_status st, const char *fmt, ...);
+
+#define IA_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return ::ia::fail(IA_ERR_DEVICE, "%s:%d %s: %s", __FILE__, __LINE__,  \
+                              #call, hipGetErrorString(e_));                      \
+    } while (0)
+
+#define IA_TRY(expr)                         \
+    do {                                     \
+        ia_status s_ = (expr);               \
+        if (s_ != IA_OK) return s_;          \
+    } while (0)
+
+// ------------------------------------------------------ device buffer
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    ia_status alloc(size_t n) {
+        if (n <= bytes && p) return IA_OK;
+        reset();
+        if (n == 0) return IA_OK;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(IA_ERR_NOMEM, "hipMalloc(%zu): %s", n, hipGetErrorString(e));
+        }
+        bytes = n;
+        return IA_OK;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+// air constants of this thread
+struct Air {
+    double c = 343.5;   // CUDA/constants.cuh:2
+    double rho = 1.19;  // CUDA/constants.cuh:5
+};
+Air &air();
+
+bool is_device_ptr(const void *p);
+// Copies n bytes of `src` (host or device) into a fresh device buffer.
+ia_status to_device(DevBuf &dst, const void *src, size_t n, hipStream_t s);
+
+// --------------------------------------------------------- launchers
+// All launchers are asynchronous on `s`; matrices are column-major,
+// batched with element strides.  T is double or float.
+
+// ESM assembly (esm.hip)
+template <typename T>
+void launch_esm_D(int m, int nf, const double *k, const double *r_s, const double *src,
+                  const double *nrm, cx<T> *D, size_t strideD, hipStream_t s);
+template <typename T>
+void launch_esm_M(int rows, int m, int nf, const double *k, double rho_c, const double *r_rows,
+                  const double *src, cx<T> *M, size_t strideM, bool conj_transpose, hipStream_t s);
+template <typename T>
+void launch_esm_msq_apply(int m, int nf, const double *k, double rho_c, const double *r_s,
+                          const double *src, const cx<T> *q, size_t strideq, cx<T> *p,
+                          size_t stridep, hipStream_t s);
+
+// complex GEMM (gemm.hip): C = alpha op(A) op(B) + beta C, op in {'N','C'}
+template <typename T>
+void launch_gemm(char opA, char opB, int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda,
+                 size_t strideA, const cx<T> *B, int ldb, size_t strideB, cx<T> beta, cx<T> *C,
+                 int ldc, size_t strideC, int batch, hipStream_t s);
+
+// LU (lu.hip)
+template <typename T>
+ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int *info, int batch,
+                        hipStream_t s);
+// Solve op(A) X = B with A = P^T L U from getrf; op 'N' or 'C'.  B n x nrhs.
+template <typename T>
+ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_t strideA,
+                        const int *ipiv, cx<T> *B, int ldb, size_t strideB, int batch,
+                        hipStream_t s);
+
+// SVD building blocks (svd.hip)
+size_t bidiag_lds_bytes(int R, int C, size_t elem);
+template <typename T>
+void launch_bidiag(int R, int C, cx<T> *Y, int ldy, size_t strideY, double *d, double *e,
+                   cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, int batch, hipStream_t s);
+void launch_bisect(int n, const double *d, const double *e, double *sigma, int batch,
+                   hipStream_t s);
+size_t inviter_work_bytes(int n, int sub);
+// QBt/PBt: n x n per batch item, vector i stored as row i (QBt[i + k n]).
+void launch_inviter(int n, const double *d, const double *e, const double *sigma, double *QBt,
+                    double *PBt, double *work, int sub, int batch, hipStream_t s);
+// v <- U_b v (herm=false) or U_b^H v, length-R vectors, nvec per batch item
+template <typename T>
+void launch_apply_left(bool herm, int R, int C, const cx<T> *Y, int ldy, size_t strideY,
+                       const cx<T> *tauL, cx<double> *v, size_t stridev, int nvec, int batch,
+                       hipStream_t s);
+// v <- V_b v or V_b^H v, length-C vectors (Vr: C x C per batch item)
+template <typename T>
+void launch_apply_right(bool herm, int C, const cx<T> *Vr, const cx<T> *tauR, cx<double> *v,
+                        size_t stridev, int nvec, int batch, hipStream_t s);
+
+// regularisation (reg.hip)
+struct RegDev {
+    int filter, select, n_lambda, k_fixed;
+    double decades, lambda_fixed;
+};
+// Given sigma (p), c = Q_B^T V_b^H b (p, complex), |b|^2, rows: selects the
+// parameter and writes phi (p, complex) such that x = V phi.
+void launch_select(int p, int rows, const double *sigma, const cx<double> *beta,
+                   const double *bnorm2, RegDev prm, double *lam_out, int *idx_out, int *k_out,
+                   cx<double> *phi, int batch, hipStream_t s);
+
+// small helpers (util.hip)
+void launch_real_matvec(int n, const double *Q, int transpose, const cx<double> *x,
+                        size_t stridex, cx<double> *y, size_t stridey, int batch, hipStream_t s);
+template <typename T>
+void launch_norm2(int n, const cx<T> *x, size_t stride, double *out, int batch, hipStream_t s);
+template <typename Td, typename Ts>
+void launch_convert(size_t n, const cx<Ts> *src, cx<Td> *dst, hipStream_t s);
+template <typename T>
+void launch_power(int m, const cx<T> *v, size_t stridev, const cx<T> *p, size_t stridep,
+                  double scale, double *W, int batch, hipStream_t s);
+
+}  // namespace ia
+
+namespace ia {
+// util.hip additions
+// dst (R x C column-major, per batch) column i = row i of Pt (C x C, Pt[i + k C]),
+// rows C..R-1 zero.
+void launch_rows_to_cols(int R, int C, const double *Pt, cx<double> *dst, int batch, hipStream_t s);
+// dst (cols x rows) = src^H (rows x cols), batched.
+template <typename T>
+void launch_conj_transpose(int rows, int cols, const cx<T> *src, size_t ss, cx<T> *dst, size_t sd,
+                           int batch, hipStream_t s);
+// y[b][0..n) = x[b][0..n), y[b][n..ny) = 0 (cx<double>)
+void launch_pad_copy(int n, int ny, const cx<double> *x, size_t sx, cx<double> *y, size_t sy,
+                     int batch, hipStream_t s);
+}  // namespace ia

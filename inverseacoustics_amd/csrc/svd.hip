@@ -1,0 +1,573 @@
+// Batched complex SVD building blocks.
+//
+// Replaces the reference's cusolverDnZgesvd ('S','A') in csvd
+// (CUDA/tsvd.cuh:39,93) and LAPACK zgesvd_ in invr (CUDA/invr.cuh:889).
+// Algorithm (one frequency per workgroup, all frequencies of a batch in one
+// launch):
+//   1. Householder bidiagonalisation Y = U_b B V_b^H (R >= C), B real upper
+//      bidiagonal (LAPACK gebrd conventions: reflectors H = I - tau u u^H
+//      with u_0 = 1, beta real).  The left update of step j, the product
+//      t = Y v_j and the next column's reflector are fused so that the
+//      trailing matrix is read and written twice per column.
+//   2. singular values of B by bisection on the Golub-Kahan tridiagonal
+//      T = tridiag(d_0, e_0, d_1, e_1, ...) whose eigenvalues are +-sigma
+//      (Sturm counts; one thread per singular value);
+//   3. singular vectors of B by inverse iteration on T - sigma I
+//      (pivoted tridiagonal LU, three solves, one thread per vector); the
+//      eigenvector interleaves (q_0, p_0, q_1, p_1, ...) with B q = sigma p;
+//   4. back-transformation: only the vectors a solve needs (U^H b, V phi)
+//      are pushed through the reflectors.
+#include "ia_internal.hpp"
+
+namespace ia {
+
+namespace {
+
+constexpr int BT = 1024;  // bidiagonalisation workgroup
+constexpr int NW = BT / 64;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// block sum over BT threads; red must hold NW doubles; all threads get result
+__device__ __forceinline__ double block_sum(double v, double *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) s += red[i];
+    return s;
+}
+
+struct Refl {
+    double beta;
+    cx<double> tau, scale;
+};
+
+// LAPACK zlarfg: H^H (alpha, x) = (beta, 0), H = I - tau u u^H, u = (1, x*scale)
+__device__ __forceinline__ Refl larfg(cx<double> alpha, double xn2) {
+    Refl r;
+    if (xn2 == 0.0 && alpha.im == 0.0) {
+        r.beta = alpha.re;
+        r.tau = {0, 0};
+        r.scale = {0, 0};
+        return r;
+    }
+    double nrm = sqrt(alpha.re * alpha.re + alpha.im * alpha.im + xn2);
+    double beta = (alpha.re >= 0) ? -nrm : nrm;
+    r.beta = beta;
+    r.tau = {(beta - alpha.re) / beta, -alpha.im / beta};
+    r.scale = cdiv(cx<double>{1, 0}, cx<double>{alpha.re - beta, alpha.im});
+    return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(BT) void k_bidiag(int R, int C, cx<T> *__restrict__ Yall, int ldy,
+                                              size_t sY, double *__restrict__ dall,
+                                              double *__restrict__ eall, cx<T> *__restrict__ tauLall,
+                                              cx<T> *__restrict__ tauRall,
+                                              cx<T> *__restrict__ Vrall) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *u = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *tt = u + R;
+    cx<T> *w = tt + R;
+    cx<T> *v = w + C;
+    double *red = reinterpret_cast<double *>(v + C);
+    __shared__ cx<double> bc_val;
+
+    const int b = blockIdx.x;
+    cx<T> *Y = Yall + (size_t)b * sY;
+    double *d = dall + (size_t)b * C;
+    double *e = eall + (size_t)b * C;
+    cx<T> *tauL = tauLall + (size_t)b * C;
+    cx<T> *tauR = tauRall + (size_t)b * C;
+    cx<T> *Vr = Vrall + (size_t)b * C * C;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+
+    // ---- prologue: reflector of column 0, then w = Y^H u over columns 1..C-1
+    {
+        double p = 0;
+        for (int r = 1 + t; r < R; r += BT) p += (double)abs2(Y[r]);
+        double xn2 = block_sum(p, red);
+        cx<double> alpha = cvt<double>(Y[0]);
+        Refl rf = larfg(alpha, xn2);
+        cx<T> sc = cvt<T>(rf.scale);
+        for (int r = 1 + t; r < R; r += BT) {
+            cx<T> uv = Y[r] * sc;
+            u[r] = uv;
+            Y[r] = uv;
+        }
+        if (t == 0) {
+            u[0] = cx<T>{1, 0};
+            Y[0] = cx<T>{(T)rf.beta, 0};
+            d[0] = rf.beta;
+            tauL[0] = cvt<T>(rf.tau);
+        }
+        __syncthreads();
+        for (int c = 1 + wv; c < C; c += NW) {
+            const cx<T> *Yc = Y + (size_t)c * ldy;
+            double ar = 0, ai = 0;
+            for (int r = lane; r < R; r += 64) {
+                cx<T> y = Yc[r], uu = u[r];
+                ar += (double)(uu.re * y.re + uu.im * y.im);
+                ai += (double)(uu.re * y.im - uu.im * y.re);
+            }
+            ar = wave_sum(ar);
+            ai = wave_sum(ai);
+            if (lane == 0) w[c] = cx<T>{(T)ar, (T)ai};
+        }
+        __syncthreads();
+    }
+    cx<T> tauj = tauL[0];
+    for (int j = 0; j < C - 1; ++j) {
+        const cx<T> ctau = conj(tauj);
+        // ---- (a) row j after the left update; right reflector of row j
+        double p = 0;
+        for (int c = j + 1 + t; c < C; c += BT) {
+            cx<T> yj = Y[(size_t)c * ldy + j] - ctau * w[c];
+            v[c] = yj;
+            if (c >= j + 2) p += (double)abs2(yj);
+        }
+        double xn2 = block_sum(p, red);  // syncs: v visible
+        cx<double> alpha = conj(cvt<double>(v[j + 1]));
+        Refl rr = larfg(alpha, xn2);
+        const cx<T> scr = cvt<T>(rr.scale);
+        const cx<T> sig = cvt<T>(rr.tau);
+        __syncthreads();
+        for (int c = j + 1 + t; c < C; c += BT) {
+            cx<T> vv = (c == j + 1) ? cx<T>{1, 0} : conj(v[c]) * scr;
+            v[c] = vv;
+            Vr[(size_t)j * C + c] = vv;
+            Y[(size_t)c * ldy + j] = (c == j + 1) ? cx<T>{(T)rr.beta, 0} : vv;
+        }
+        if (t == 0) {
+            e[j] = rr.beta;
+            tauR[j] = sig;
+        }
+        __syncthreads();
+        // ---- (b) rows j+1..R-1: left update, t = Y v, final column j+1
+        double pn = 0;
+        for (int r = j + 1 + t; r < R; r += BT) {
+            const cx<T> ur = u[r];
+            const cx<T> cu = ctau * ur;
+            cx<T> acc{0, 0}, y1{0, 0};
+            for (int c = j + 1; c < C; ++c) {
+                cx<T> *yp = Y + (size_t)c * ldy + r;
+                cx<T> y = *yp - cu * w[c];
+                *yp = y;
+                acc = acc + y * v[c];
+                if (c == j + 1) y1 = y;
+            }
+            tt[r] = acc;
+            cx<T> ncol = y1 - sig * acc;
+            Y[(size_t)(j + 1) * ldy + r] = ncol;
+            if (r >= j + 2) pn += (double)abs2(ncol);
+        }
+        double xn2b = block_sum(pn, red);  // syncs: tt and column j+1 visible
+        if (t == 0) bc_val = cvt<double>(Y[(size_t)(j + 1) * ldy + j + 1]);
+        __syncthreads();
+        Refl rl = larfg(bc_val, xn2b);
+        const cx<T> scl = cvt<T>(rl.scale);
+        for (int r = j + 2 + t; r < R; r += BT) {
+            cx<T> *yp = Y + (size_t)(j + 1) * ldy + r;
+            cx<T> uv = *yp * scl;
+            u[r] = uv;
+            *yp = uv;
+        }
+        if (t == 0) {
+            u[j + 1] = cx<T>{1, 0};
+            Y[(size_t)(j + 1) * ldy + j + 1] = cx<T>{(T)rl.beta, 0};
+            d[j + 1] = rl.beta;
+            tauL[j + 1] = cvt<T>(rl.tau);
+        }
+        __syncthreads();
+        // ---- (c) columns j+2..C-1: right update, next w = Y^H u_{j+1}
+        for (int c = j + 2 + wv; c < C; c += NW) {
+            cx<T> *Yc = Y + (size_t)c * ldy;
+            const cx<T> sv = sig * conj(v[c]);
+            double ar = 0, ai = 0;
+            for (int r = j + 1 + lane; r < R; r += 64) {
+                cx<T> y = Yc[r] - tt[r] * sv;
+                Yc[r] = y;
+                cx<T> uu = u[r];
+                ar += (double)(uu.re * y.re + uu.im * y.im);
+                ai += (double)(uu.re * y.im - uu.im * y.re);
+            }
+            ar = wave_sum(ar);
+            ai = wave_sum(ai);
+            if (lane == 0) w[c] = cx<T>{(T)ar, (T)ai};
+        }
+        __syncthreads();
+        tauj = cvt<T>(rl.tau);
+    }
+    if (t == 0 && C > 0) e[C - 1] = 0.0;
+}
+
+// ------------------------------------------------------------ bisection
+// #eigenvalues of the zero-diagonal symmetric tridiagonal with squared
+// off-diagonals t2[0..N-2] that are < x.
+__device__ __forceinline__ int sturm_count(int N, const double *t2, double x, double pivmin) {
+    int cnt = 0;
+    double q = -x;
+    if (q < 0) cnt++;
+    for (int i = 0; i < N - 1; ++i) {
+        if (fabs(q) < pivmin) q = -pivmin;
+        q = -x - t2[i] / q;
+        if (q < 0) cnt++;
+    }
+    return cnt;
+}
+
+__global__ __launch_bounds__(256) void k_bisect(int n, const double *__restrict__ dall,
+                                               const double *__restrict__ eall,
+                                               double *__restrict__ sall) {
+    extern __shared__ double t2[];
+    __shared__ double s_bound, s_pivmin;
+    const int b = blockIdx.y;
+    const double *d = dall + (size_t)b * n;
+    const double *e = eall + (size_t)b * n;
+    const int N = 2 * n;
+    const int t = threadIdx.x;
+    double lmax = 0, gmax = 0;
+    for (int i = t; i < N - 1; i += blockDim.x) {
+        double v = (i & 1) ? e[i >> 1] : d[i >> 1];
+        t2[i] = v * v;
+        lmax = fmax(lmax, v * v);
+    }
+    for (int i = t; i < n; i += blockDim.x) {
+        double g = fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0);
+        double g2 = fabs(d[i]) + (i < n - 1 ? fabs(e[i]) : 0.0);
+        gmax = fmax(gmax, fmax(g, g2));
+    }
+    __shared__ double red[2][256];
+    red[0][t] = lmax;
+    red[1][t] = gmax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            red[0][t] = fmax(red[0][t], red[0][t + s]);
+            red[1][t] = fmax(red[1][t], red[1][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        s_bound = red[1][0] * (1.0 + 8.0 * 2.220446049250313e-16) + 1e-300;
+        s_pivmin = fmax(red[0][0] * 2.2250738585072014e-308, 1e-300);
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + t;
+    if (i >= n) return;
+    const double bound = s_bound, pivmin = s_pivmin;
+    const double eps = 2.220446049250313e-16;
+    const int target = n - 1 - i;  // sigma_i (descending) has target values below it
+    double lo = 0.0, hi = bound;
+    for (int it = 0; it < 128; ++it) {
+        if (hi - lo <= fmax(2.0 * eps * hi, 1e-3 * eps * bound)) break;
+        double mid = 0.5 * (lo + hi);
+        int c = sturm_count(N, t2, mid, pivmin) - n;
+        if (c > target)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    sall[(size_t)b * n + i] = 0.5 * (lo + hi);
+}
+
+// ----------------------------------------------------- inverse iteration
+// One thread per singular value: pivoted LU of T - sigma I (N = 2n), three
+// solves from a deterministic pseudo-random start, split into q (even) and
+// p (odd).  Scratch is interleaved across threads (element k of thread g at
+// k*G + g) so that every step is a coalesced access.  Outputs QBt, PBt hold
+// the vectors as ROWS: QBt[i + k n] = q_i[k].
+__device__ __forceinline__ double hash01(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return (double)(x >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+}
+
+__global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__restrict__ dall,
+                                               const double *__restrict__ eall,
+                                               const double *__restrict__ sall,
+                                               double *__restrict__ QBt, double *__restrict__ PBt,
+                                               double *__restrict__ work, int G) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int bl = blockIdx.y;       // local batch index within this call
+    const int b = b0 + bl;
+    if (i >= n) return;
+    const int g = bl * n + i;        // scratch column
+    const int N = 2 * n;
+    const double *d = dall + (size_t)b * n;
+    const double *e = eall + (size_t)b * n;
+    const double sigma = sall[(size_t)b * n + i];
+    double *DD = work;                       // U diagonal
+    double *DU = work + (size_t)N * G;       // U first superdiagonal
+    double *DU2 = work + (size_t)2 * N * G;  // U second superdiagonal
+    double *DL = work + (size_t)3 * N * G;   // multipliers
+    double *X = work + (size_t)4 * N * G;    // iterate
+    unsigned char *PV = reinterpret_cast<unsigned char *>(work + (size_t)5 * N * G);
+#define AT(arr, k) arr[(size_t)(k) * G + g]
+    auto toff = [&](int k) { return (k & 1) ? e[k >> 1] : d[k >> 1]; };
+    double tnorm = 0;
+    for (int k = 0; k < N - 1; ++k) tnorm = fmax(tnorm, fabs(toff(k)));
+    const double eps = 2.220446049250313e-16;
+    const double tiny = fmax(eps * tnorm, 1e-300);
+    // ---- factor (LAPACK gttrf)
+    for (int k = 0; k < N; ++k) {
+        AT(DD, k) = -sigma;
+        if (k < N - 1) {
+            AT(DU, k) = toff(k);
+            AT(DL, k) = toff(k);
+        }
+        if (k < N - 2) AT(DU2, k) = 0.0;
+    }
+    for (int k = 0; k < N - 1; ++k) {
+        double dk = AT(DD, k), dlk = AT(DL, k);
+        if (fabs(dk) >= fabs(dlk)) {
+            PV[(size_t)k * G + g] = 0;
+            double fact = (dk != 0.0) ? dlk / dk : 0.0;
+            AT(DL, k) = fact;
+            AT(DD, k + 1) = AT(DD, k + 1) - fact * AT(DU, k);
+        } else {
+            PV[(size_t)k * G + g] = 1;
+            double fact = dk / dlk;
+            AT(DD, k) = dlk;
+            AT(DL, k) = fact;
+            double temp = AT(DU, k);
+            double dk1 = AT(DD, k + 1);
+            AT(DU, k) = dk1;
+            AT(DD, k + 1) = temp - fact * dk1;
+            if (k < N - 2) {
+                double du1 = AT(DU, k + 1);
+                AT(DU2, k) = du1;
+                AT(DU, k + 1) = -fact * du1;
+            }
+        }
+    }
+    for (int k = 0; k < N; ++k)
+        if (fabs(AT(DD, k)) < tiny) AT(DD, k) = (AT(DD, k) < 0) ? -tiny : tiny;
+    // ---- start vector
+    for (int k = 0; k < N; ++k) AT(X, k) = hash01(((uint64_t)b << 40) ^ ((uint64_t)i << 20) ^ k) + 0.25;
+    for (int itn = 0; itn < 3; ++itn) {
+        // forward: L
+        for (int k = 0; k < N - 1; ++k) {
+            double xk = AT(X, k), xk1 = AT(X, k + 1);
+            if (PV[(size_t)k * G + g] == 0) {
+                AT(X, k + 1) = xk1 - AT(DL, k) * xk;
+            } else {
+                AT(X, k) = xk1;
+                AT(X, k + 1) = xk - AT(DL, k) * xk1;
+            }
+        }
+        // backward: U
+        double scale = 0;
+        double x2 = 0, x1 = 0;  // x[k+2], x[k+1]
+        for (int k = N - 1; k >= 0; --k) {
+            double s = AT(X, k);
+            if (k < N - 1) s -= AT(DU, k) * x1;
+            if (k < N - 2) s -= AT(DU2, k) * x2;
+            s /= AT(DD, k);
+            AT(X, k) = s;
+            x2 = x1;
+            x1 = s;
+            scale = fmax(scale, fabs(s));
+        }
+        double inv = (scale > 0) ? 1.0 / scale : 1.0;
+        for (int k = 0; k < N; ++k) AT(X, k) *= inv;
+    }
+    // ---- split, normalise, fix the sign of p so that p^T B q >= 0
+    double nq = 0, np = 0;
+    for (int k = 0; k < n; ++k) {
+        double q = AT(X, 2 * k), p = AT(X, 2 * k + 1);
+        nq += q * q;
+        np += p * p;
+    }
+    nq = sqrt(nq);
+    np = sqrt(np);
+    double iq = nq > 0 ? 1.0 / nq : 0.0, ip = np > 0 ? 1.0 / np : 0.0;
+    double sgn = 0;
+    for (int k = 0; k < n; ++k) {
+        double bq = d[k] * AT(X, 2 * k) + (k < n - 1 ? e[k] * AT(X, 2 * k + 2) : 0.0);
+        sgn += AT(X, 2 * k + 1) * bq;
+    }
+    if (sgn < 0) ip = -ip;
+    double *Qb = QBt + (size_t)b * n * n;
+    double *Pb = PBt + (size_t)b * n * n;
+    for (int k = 0; k < n; ++k) {
+        Qb[(size_t)k * n + i] = AT(X, 2 * k) * iq;
+        Pb[(size_t)k * n + i] = AT(X, 2 * k + 1) * ip;
+    }
+#undef AT
+}
+
+// --------------------------------------------------- reflector application
+// v <- U_b v (FWD=false: H_0 H_1 ... H_{C-1} v) or U_b^H v, length-R vectors;
+// reflector j lives in column j of Y below the diagonal (unit at row j).
+template <typename T, bool HERM>
+__global__ __launch_bounds__(256) void k_apply_left(int R, int C, const cx<T> *__restrict__ Yall,
+                                                   int ldy, size_t sY,
+                                                   const cx<T> *__restrict__ tauall,
+                                                   cx<double> *__restrict__ vall, size_t sv,
+                                                   int nvec) {
+    __shared__ double red[2][4];
+    const int vec = blockIdx.x, b = blockIdx.y;
+    const cx<T> *Y = Yall + (size_t)b * sY;
+    const cx<T> *tau = tauall + (size_t)b * C;
+    cx<double> *v = vall + (size_t)b * nvec * sv + (size_t)vec * sv;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int jj = 0; jj < C; ++jj) {
+        const int j = HERM ? jj : (C - 1 - jj);
+        cx<double> tj = cvt<double>(tau[j]);
+        if (tj.re == 0.0 && tj.im == 0.0) continue;
+        const cx<T> *uc = Y + (size_t)j * ldy;
+        double ar = 0, ai = 0;
+        for (int r = j + t; r < R; r += 256) {
+            cx<double> uu = (r == j) ? cx<double>{1, 0} : cvt<double>(uc[r]);
+            cx<double> x = v[r];
+            ar += uu.re * x.re + uu.im * x.im;
+            ai += uu.re * x.im - uu.im * x.re;
+        }
+        ar = wave_sum(ar);
+        ai = wave_sum(ai);
+        __syncthreads();
+        if (lane == 0) {
+            red[0][w] = ar;
+            red[1][w] = ai;
+        }
+        __syncthreads();
+        cx<double> dot{red[0][0] + red[0][1] + red[0][2] + red[0][3],
+                       red[1][0] + red[1][1] + red[1][2] + red[1][3]};
+        cx<double> f = (HERM ? conj(tj) : tj) * dot;
+        for (int r = j + t; r < R; r += 256) {
+            cx<double> uu = (r == j) ? cx<double>{1, 0} : cvt<double>(uc[r]);
+            v[r] = v[r] - uu * f;
+        }
+    }
+}
+
+// v <- V_b v or V_b^H v, length-C vectors; reflector j lives in Vr column j,
+// rows j+1..C-1 (unit at row j+1).
+template <typename T, bool HERM>
+__global__ __launch_bounds__(256) void k_apply_right(int C, const cx<T> *__restrict__ Vrall,
+                                                    const cx<T> *__restrict__ tauall,
+                                                    cx<double> *__restrict__ vall, size_t sv,
+                                                    int nvec) {
+    __shared__ double red[2][4];
+    const int vec = blockIdx.x, b = blockIdx.y;
+    const cx<T> *Vr = Vrall + (size_t)b * C * C;
+    const cx<T> *tau = tauall + (size_t)b * C;
+    cx<double> *v = vall + (size_t)b * nvec * sv + (size_t)vec * sv;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    for (int jj = 0; jj < C - 1; ++jj) {
+        const int j = HERM ? jj : (C - 2 - jj);
+        cx<double> tj = cvt<double>(tau[j]);
+        if (tj.re == 0.0 && tj.im == 0.0) continue;
+        const cx<T> *vc = Vr + (size_t)j * C;
+        double ar = 0, ai = 0;
+        for (int r = j + 1 + t; r < C; r += 256) {
+            cx<double> uu = cvt<double>(vc[r]);
+            cx<double> x = v[r];
+            ar += uu.re * x.re + uu.im * x.im;
+            ai += uu.re * x.im - uu.im * x.re;
+        }
+        ar = wave_sum(ar);
+        ai = wave_sum(ai);
+        __syncthreads();
+        if (lane == 0) {
+            red[0][w] = ar;
+            red[1][w] = ai;
+        }
+        __syncthreads();
+        cx<double> dot{red[0][0] + red[0][1] + red[0][2] + red[0][3],
+                       red[1][0] + red[1][1] + red[1][2] + red[1][3]};
+        cx<double> f = (HERM ? conj(tj) : tj) * dot;
+        for (int r = j + 1 + t; r < C; r += 256) {
+            cx<double> uu = cvt<double>(vc[r]);
+            v[r] = v[r] - uu * f;
+        }
+    }
+}
+
+}  // namespace
+
+size_t bidiag_lds_bytes(int R, int C, size_t elem) { return (2 * (size_t)R + 2 * (size_t)C) * elem + NW * sizeof(double) + 64; }
+
+template <typename T>
+void launch_bidiag(int R, int C, cx<T> *Y, int ldy, size_t strideY, double *d, double *e,
+                   cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, int batch, hipStream_t s) {
+    size_t lds = bidiag_lds_bytes(R, C, sizeof(cx<T>));
+    (void)hipFuncSetAttribute((const void *)k_bidiag<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(k_bidiag<T>, dim3(batch), dim3(BT), lds, s, R, C, Y, ldy, strideY, d, e,
+                       tauL, tauR, Vr);
+}
+
+void launch_bisect(int n, const double *d, const double *e, double *sigma, int batch,
+                   hipStream_t s) {
+    size_t lds = sizeof(double) * (2 * (size_t)n);
+    (void)hipFuncSetAttribute((const void *)k_bisect, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    dim3 grid((n + 255) / 256, batch);
+    hipLaunchKernelGGL(k_bisect, grid, dim3(256), lds, s, n, d, e, sigma);
+}
+
+size_t inviter_work_bytes(int n, int sub) {
+    size_t G = (size_t)n * sub, N = 2 * (size_t)n;
+    return 5 * N * G * sizeof(double) + N * G + 256;
+}
+
+void launch_inviter(int n, const double *d, const double *e, const double *sigma, double *QBt,
+                    double *PBt, double *work, int sub, int batch, hipStream_t s) {
+    for (int b0 = 0; b0 < batch; b0 += sub) {
+        int nb = min(sub, batch - b0);
+        dim3 grid((n + 63) / 64, nb);
+        hipLaunchKernelGGL(k_inviter, grid, dim3(64), 0, s, n, b0, d, e, sigma, QBt, PBt, work,
+                           n * sub);
+    }
+}
+
+template <typename T>
+void launch_apply_left(bool herm, int R, int C, const cx<T> *Y, int ldy, size_t strideY,
+                       const cx<T> *tauL, cx<double> *v, size_t stridev, int nvec, int batch,
+                       hipStream_t s) {
+    dim3 grid(nvec, batch);
+    if (herm)
+        hipLaunchKernelGGL((k_apply_left<T, true>), grid, dim3(256), 0, s, R, C, Y, ldy, strideY,
+                           tauL, v, stridev, nvec);
+    else
+        hipLaunchKernelGGL((k_apply_left<T, false>), grid, dim3(256), 0, s, R, C, Y, ldy, strideY,
+                           tauL, v, stridev, nvec);
+}
+
+template <typename T>
+void launch_apply_right(bool herm, int C, const cx<T> *Vr, const cx<T> *tauR, cx<double> *v,
+                        size_t stridev, int nvec, int batch, hipStream_t s) {
+    dim3 grid(nvec, batch);
+    if (herm)
+        hipLaunchKernelGGL((k_apply_right<T, true>), grid, dim3(256), 0, s, C, Vr, tauR, v, stridev,
+                           nvec);
+    else
+        hipLaunchKernelGGL((k_apply_right<T, false>), grid, dim3(256), 0, s, C, Vr, tauR, v,
+                           stridev, nvec);
+}
+
+#define INST(T)                                                                                  \
+    template void launch_bidiag<T>(int, int, cx<T> *, int, size_t, double *, double *, cx<T> *, \
+                                   cx<T> *, cx<T> *, int, hipStream_t);                          \
+    template void launch_apply_left<T>(bool, int, int, const cx<T> *, int, size_t,              \
+                                       const cx<T> *, cx<double> *, size_t, int, int,           \
+                                       hipStream_t);                                            \
+    template void launch_apply_right<T>(bool, int, const cx<T> *, const cx<T> *, cx<double> *,  \
+                                        size_t, int, int, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace ia

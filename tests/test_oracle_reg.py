@@ -1,0 +1,159 @@
+"""Pins of the regularisation oracle (oracle/regtools.py) against library
+routines (LAPACK via numpy/scipy, scipy's bounded Brent) and brute force."""
+import numpy as np
+import pytest
+import scipy.linalg as sl
+from scipy.optimize import fminbound
+
+from oracle import regtools as rt
+
+
+def _rand(m, n, seed=0, decay=None):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+    if decay is not None:
+        U, s, Vh = np.linalg.svd(A, full_matrices=False)
+        s = 10.0 ** (-decay * np.arange(len(s)) / len(s))
+        A = (U * s) @ Vh
+    b = rng.standard_normal(m) + 1j * rng.standard_normal(m)
+    return A, b
+
+
+@pytest.mark.parametrize("shape", [(30, 20), (20, 30), (25, 25)])
+def test_csvd_against_gesvd(shape):
+    A, _ = _rand(*shape, seed=3)
+    U, s, V = rt.csvd(A)
+    U2, s2, Vh2 = sl.svd(A, full_matrices=False, lapack_driver="gesvd")
+    p = min(shape)
+    assert U.shape == (shape[0], p) and V.shape == (shape[1], p)
+    np.testing.assert_allclose(s, s2, rtol=1e-13, atol=1e-14 * s2[0])
+    np.testing.assert_allclose((U * s) @ V.conj().T, A, atol=1e-13 * s2[0])
+    # phase-invariant: projectors agree
+    np.testing.assert_allclose(U @ U.conj().T, U2 @ U2.conj().T, atol=1e-12)
+
+
+def test_tsvd_full_rank_is_least_squares():
+    A, b = _rand(30, 20, seed=4)
+    U, s, V = rt.csvd(A)
+    x = rt.tsvd(U, s, V, b, 20)
+    np.testing.assert_allclose(x, np.linalg.lstsq(A, b, rcond=None)[0], rtol=1e-11)
+    assert np.all(rt.tsvd(U, s, V, b, 0) == 0)
+    with pytest.raises(ValueError):
+        rt.tsvd(U, s, V, b, 21)
+
+
+@pytest.mark.parametrize("shape", [(30, 20), (20, 30)])
+def test_tikhonov_normal_equations(shape):
+    A, b = _rand(*shape, seed=5, decay=6)
+    U, s, V = rt.csvd(A)
+    lam = 1e-2 * s[0]
+    x = rt.tikhonov(U, s, V, b, lam)
+    n = A.shape[1]
+    xr = np.linalg.solve(A.conj().T @ A + lam ** 2 * np.eye(n), A.conj().T @ b)
+    np.testing.assert_allclose(x, xr, rtol=1e-9)
+
+
+def test_dsvd_closed_forms():
+    a = np.array([3.0, 1.5, 0.2])
+    A = np.diag(a).astype(complex)
+    b = np.array([1 + 1j, 2.0, -1j])
+    U, s, V = rt.csvd(A)
+    np.testing.assert_allclose(rt.dsvd(U, s, V, b, 0.7), b / (a + 0.7), rtol=1e-13)
+    A2, b2 = _rand(20, 12, seed=6)
+    U, s, V = rt.csvd(A2)
+    np.testing.assert_allclose(rt.dsvd(U, s, V, b2, 0.0), np.linalg.lstsq(A2, b2, rcond=None)[0], rtol=1e-11)
+
+
+def _brute_gcv_tik(A, b, lam):
+    m, n = A.shape
+    M = A.conj().T @ A + lam ** 2 * np.eye(n)
+    x = np.linalg.solve(M, A.conj().T @ b)
+    infl = A @ np.linalg.solve(M, A.conj().T)
+    return np.linalg.norm(A @ x - b) ** 2 / (m - np.trace(infl).real) ** 2
+
+
+@pytest.mark.parametrize("shape", [(24, 16), (16, 24)])
+def test_gcv_tikhonov_grid_brute_force(shape):
+    A, b = _rand(*shape, seed=7, decay=8)
+    U, s, V = rt.csvd(A)
+    beta = U.conj().T @ b
+    rows = A.shape[0]
+    delta0 = max(np.linalg.norm(b) ** 2 - np.linalg.norm(beta) ** 2, 0) if rows > len(s) else 0.0
+    idx, lam, G = rt.gcv_tikhonov_grid(s, beta, rows, delta0, npoints=12, decades=6)
+    Gb = np.array([_brute_gcv_tik(A, b, l) for l in lam])
+    np.testing.assert_allclose(G, Gb, rtol=1e-6)
+    assert idx == int(np.argmin(Gb))
+    assert lam[0] == pytest.approx(s[0] * 1e-6) and lam[-1] == pytest.approx(s[0])
+
+
+def test_gcv_tsvd_brute_force():
+    A, b = _rand(30, 18, seed=8, decay=6)
+    U, s, V = rt.csvd(A)
+    beta = U.conj().T @ b
+    delta0 = np.linalg.norm(b) ** 2 - np.linalg.norm(beta) ** 2
+    k, G = rt.gcv_tsvd(beta, 30, 18, delta0)
+    for kk in range(1, 18):
+        x = rt.tsvd(U, s, V, b, kk)
+        Gb = np.linalg.norm(A @ x - b) ** 2 / (30 - kk) ** 2
+        assert G[kk - 1] == pytest.approx(Gb, rel=1e-9)
+    assert k == int(np.argmin(G)) + 1
+
+
+@pytest.mark.parametrize("fun,a,b", [
+    (lambda x: (x - 0.3) ** 2, -1.0, 2.0),
+    (lambda x: np.cos(x), 2.0, 5.0),
+    (lambda x: x ** 4 - 3 * x ** 2 + x, -2.5, 0.0),
+    (lambda x: abs(x - 1e-3) + 1e-9 * x, 1e-6, 1.0),
+])
+def test_fminbnd_matches_scipy_fminbound(fun, a, b):
+    x = rt.fminbnd(fun, a, b, tol_x=1e-4)
+    xs = fminbound(fun, a, b, xtol=1e-4)
+    assert x == pytest.approx(xs, abs=2e-4)
+
+
+def test_lcfun_is_negative_curvature():
+    A, b = _rand(30, 20, seed=9, decay=5)
+    U, s, V = rt.csvd(A)
+    beta = U.conj().T @ b
+    xi = beta / s
+    beta_ls = np.concatenate([beta, [np.linalg.norm(b - U @ beta)]])
+    lam0 = 1e-2 * s[0]
+
+    def logs(l):
+        x = rt.tikhonov(U, s, V, b, l)
+        return np.log(np.linalg.norm(A @ x - b)), np.log(np.linalg.norm(x))
+
+    h = 1e-4 * lam0
+    (x0, y0), (xp, yp), (xm, ym) = logs(lam0), logs(lam0 + h), logs(lam0 - h)
+    dx, dy = (xp - xm) / (2 * h), (yp - ym) / (2 * h)
+    ddx, ddy = (xp - 2 * x0 + xm) / h ** 2, (yp - 2 * y0 + ym) / h ** 2
+    kappa = (dx * ddy - ddx * dy) / (dx ** 2 + dy ** 2) ** 1.5
+    g = rt.lcfun(lam0, s, beta_ls, xi, "t")[0]
+    assert g == pytest.approx(-kappa, rel=1e-3)
+
+
+def test_quasiopt_is_half_lambda_dx_dlambda():
+    A, b = _rand(20, 20, seed=10, decay=4)
+    U, s, V = rt.csvd(A)
+    reg_min, grid, Q2 = rt.quasiopt(U, s, b, "t")
+    for j in (10, 100, 180):
+        l = grid[j]
+        h = 1e-6 * l
+        dx = (rt.tikhonov(U, s, V, b, l + h) - rt.tikhonov(U, s, V, b, l - h)) / (2 * h)
+        assert Q2[j] == pytest.approx(0.5 * np.linalg.norm(l * dx), rel=1e-5)
+    assert grid[-1] <= reg_min <= grid[0]
+
+
+def test_gcv_regtools_literal_grid_and_refinement():
+    A, b = _rand(24, 16, seed=11, decay=6)
+    U, s, V = rt.csvd(A)
+    reg_min, grid, G, gi = rt.gcv_regtools(U, s, b, "d")
+    assert len(grid) == 200 and grid[0] == pytest.approx(s[0]) and np.all(np.diff(grid) < 0)
+    # the literal gcvG grid uses the damped filter lam/(s+lam) for both methods
+    beta = U.conj().T @ b
+    d0 = np.linalg.norm(b) ** 2 - np.linalg.norm(beta) ** 2
+    j = 57
+    f = grid[j] / (s + grid[j])
+    assert G[j] == pytest.approx((np.linalg.norm(f * beta) ** 2 + d0) / ((24 - 16) + f.sum()) ** 2, rel=1e-12)
+    lo, hi = grid[min(gi + 1, 199)], grid[max(gi - 1, 0)]
+    assert lo <= reg_min <= hi
