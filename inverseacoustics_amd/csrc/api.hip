@@ -165,7 +165,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
                                 int *status, hipStream_t s) {
     const int m = D.m, l = D.l, F = D.n_freq;
     if (l > m) return fail(IA_ERR_ARG, "ia_esm_sweep: l (%d) > m (%d) is not supported", l, m);
-    int chunk = D.chunk > 0 ? D.chunk : std::min(F, 64);
+    int chunk = D.chunk > 0 ? D.chunk : std::min(F, 256);
     chunk = std::min(chunk, F);
     const int R = m, C = l;  // Y = H_e^H (m x l), A = Y^H
     const double rho_c = air().rho * air().c;

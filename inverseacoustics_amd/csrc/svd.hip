@@ -449,6 +449,7 @@ __global__ __launch_bounds__(256) void k_apply_left(int R, int C, const cx<T> *_
             cx<double> uu = (r == j) ? cx<double>{1, 0} : cvt<double>(uc[r]);
             v[r] = v[r] - uu * f;
         }
+        __syncthreads();  // the next reflector's dot product reads rows other waves wrote
     }
 }
 
@@ -492,6 +493,7 @@ __global__ __launch_bounds__(256) void k_apply_right(int C, const cx<T> *__restr
             cx<double> uu = cvt<double>(vc[r]);
             v[r] = v[r] - uu * f;
         }
+        __syncthreads();  // the next reflector's dot product reads rows other waves wrote
     }
 }
 
