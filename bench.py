@@ -29,6 +29,10 @@ def _parse():
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=3, help="frequencies in the CPU sample")
+    ap.add_argument("--only", choices=("both", "fp64", "mixed"), default="both",
+                    help="profile runs: time one form only")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -97,6 +101,18 @@ def main():
         ia.check(L.ia_last_stats(st))
         return dt, o, {n: getattr(st, n) for n, _ in _lib.ia_stats._fields_}
 
+    if a.copy_probe_mib:
+        n = a.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    if a.only != "both":
+        dt, o, st = timed(1 if a.only == "mixed" else 0)
+        print(json.dumps({"profile_run": a.only, "ms_per_step": round(dt * 1e3, 2),
+                          "phases_ms": {k: round(v, 2) for k, v in st.items()}}))
+        return
     t_mixed, o_mixed, st_mixed = timed(1)
     t_fp64, o_fp64, st_fp64 = timed(0)
 
@@ -124,8 +140,11 @@ def main():
     bytes_per_launch = bidiag_bytes(16) * chunk
     launch_s = st_fp64["bidiag"] / launches / 1e3
     achieved = bytes_per_launch / launch_s / 1e9
+    # HBM bytes per launch measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (copy-kernel
+    # scale factor 1.0) for this kernel at this shape: profiles/esm_sweep_256f_1024x2048_fp64_hbm_counters.txt
+    traffic = (3627259531.1 + 7212270390.7) * 1024 if (F, m, l, chunk) == (256, 2048, 1024, 256) else None
     roofline = {"bound": "hbm", "kernel": "k_bidiag<double> (FP64 form)", "achieved": round(achieved, 1),
-                "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
+                "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": traffic,
                 "launch_ms": round(launch_s * 1e3, 3), "frequencies_per_launch": chunk,
                 "dominant_phase_fp64": dom64}
 
