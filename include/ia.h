@@ -79,6 +79,21 @@ ia_status ia_esm(int m, int l, const double *r_ms, const double *r_lh,
                  const double *src, const double *nrm, double k,
                  ia_z **m_e, ia_z **h_e, ia_z **d, ia_z **m_sq, ia_z **m_hq);
 
+/* ------------------------------------------------- HELS (hels.cuh) */
+/* replaces hels(), CUDA/hels.cuh:222 and helsgpu(), hels.cuh:185.
+ * r_ms_s (3m), r_lh_s (3l): spherical (azimuth, elevation, radius) SoA as
+ * cartesian_2_spherical returns them.  m_e (m x n_t) = h_n(k r) Y_n^m,
+ * h (l x n_t), q_e (m x n_t) = k h_n'(k r) Y_n^m / (i rho c k);
+ * n_t = (n_sh+1)^2, column i -> n = floor(sqrt(i)), m = i - n - n^2
+ * (CUDA order, hels.cuh:92).  FP64 recurrences (the CPU/GSL semantics),
+ * 0 <= n_sh <= 80. */
+void hels(int m, int l, double *r_ms_s, double *r_lh_s, ia_z **m_e, ia_z **h,
+          ia_z **q_e, double k, int n_sh);
+void helsgpu(int m, int l, double *r_ms_sg, double *r_lh_sg, ia_z **me,
+             ia_z **he, ia_z **qe, double k, int n_sh);
+ia_status ia_hels(int m, int l, const double *r_ms_s, const double *r_lh_s,
+                  double k, int n_sh, ia_z **m_e, ia_z **h, ia_z **q_e);
+
 /* --------------------------------------------- SVD + filters (tsvd.cuh) */
 /* replaces csvd(), CUDA/tsvd.cuh:10: compact SVD a = u diag(s) v^H.
  * m >= n: u m x n, s n, v n x n;  m < n: u m x m, s m, v n x m. */
@@ -199,6 +214,93 @@ ia_status ia_esm_sweep_device(const ia_sweep_desc *desc, const double *k,
                               const double *r_s, const double *src,
                               const double *nrm, const double *r_h,
                               const ia_z *b, ia_sweep_out *out, void *stream);
+
+/* Batched HELS sweep with a per-frequency expansion order (uneven system
+ * sizes are grouped by order): H (l x n_t) regularised solve for the
+ * coefficients c, then v = Q_e c (to_velocity) and p = M_e c (to_pressure)
+ * on the m surface points, W = 1/2 Re(v^H p) S/m.  Host pointers:
+ * order[n_freq], k[n_freq], azelr_s (3m), azelr_h (3l), b (n_freq*l). */
+typedef struct {
+    int m, l, n_freq;
+    ia_precision precision;
+    ia_reg_params reg;
+    double area;
+} ia_hels_desc;
+ia_status ia_hels_sweep(const ia_hels_desc *desc, const int *order,
+                        const double *k, const double *azelr_s,
+                        const double *azelr_h, const ia_z *b,
+                        ia_sweep_out *out);
+
+/* ------------------------------------------------- IBEM (ibem.cuh) */
+/* Quadrature tables: regular rule (n_q points: xi1, xi2, w SoA) and three
+ * singular pages of n_qs points each (page p: xi1, xi2, w SoA at
+ * 3*n_qs*p).  ia_quadrature_load reads the reference's text format
+ * (count line, then "xi1 xi2 w" rows; IPFs with $PAGE 1..3 sections,
+ * CUDA/ibem.cuh:431-477); the returned buffers are malloc'd.  Without a
+ * call to ia_set_quadrature the library uses Radon's 7-point degree-5
+ * rule and Duffy-collapsed 40x40 Gauss-Legendre singular pages. */
+ia_status ia_quadrature_load(const char *ipf_path, const char *ipfs_path,
+                             int *n_q, double **ipf, int *n_qs, double **ipfs);
+/* Set the tables used by the IBEM / radiation-mode entry points of this
+ * thread (copied); n_q = n_qs = 0 restores the built-in rules. */
+ia_status ia_set_quadrature(int n_q, const double *ipf, int n_qs,
+                            const double *ipfs);
+/* replaces ibem(), CUDA/ibem.cuh:424 and ibemgpu(), ibem.cuh:263.
+ * m_e = Z = A_ss \ B_ss (m x m; least squares when n_ch CHIEF points are
+ * given), h = H_b = A_hs Z - B_hs (l x m); r_ms (3m) and r_lh (3l) SoA,
+ * e_n (3n) 0-based triangles SoA, cp (3 n_ch) CHIEF points SoA. */
+void ibem(int m, int l, int n, double *r_ms, int *e_n, double *r_lh,
+          ia_z **m_e, ia_z **h, double k, int n_ch, double *cp);
+void ibemgpu(int m, int l, int n, double *r_msg, int *e_ng, double *r_lhg,
+             ia_z **me, ia_z **he, double k, int n_ch, double *cp);
+ia_status ia_ibem(int m, int l, int n, const double *r_ms, const int *e_n,
+                  const double *r_lh, double k, int n_ch, const double *cp,
+                  ia_z **z, ia_z **h_b);
+
+/* Batched IBEM sweep (ibemS.m pipeline per frequency): assemble Z and H_b
+ * for every k[f], regularised solve H_b v = b_f, p_s = Z v and
+ * W = 1/2 Re(v^H p_s) S/m.  Host pointers; b holds n_freq columns of l. */
+typedef struct {
+    int m, l, n_elem, n_ch, n_freq;
+    ia_precision precision;   /* MIXED: complex64 assembly and SVD        */
+    ia_reg_params reg;
+    double area;
+    int chunk;                /* frequencies per device batch (0 = auto)   */
+} ia_ibem_desc;
+ia_status ia_ibem_sweep(const ia_ibem_desc *desc, const double *k,
+                        const double *r_ms, const int *e_n, const double *r_lh,
+                        const double *cp, const ia_z *b, ia_sweep_out *out);
+
+/* ------------------------------------ radiation modes (invr.cuh) */
+/* replaces invr(), CUDA/invr.cuh:457 and invrgpu(), invr.cuh:263, with the
+ * argument order of the reference: m_e = Z Q (m x n_am), h = H_b Q
+ * (l x n_am), q_e = Q (m x n_am), the leading left singular vectors of
+ * R = real(G Z) (MATLAB/invrS.m:160-161), G the Gram matrix of the linear
+ * shape functions. */
+void invr(int m, int l, int n, double *r_ms, int *e_n, double *r_lh,
+          ia_z **m_e, ia_z **h, ia_z **q_e, double k, int n_ch, double *cp,
+          int n_am);
+void invrgpu(int m, int l, int n, double *r_msg, int *e_ng, double *r_lhg,
+             ia_z **me, ia_z **he, ia_z **qe, double k, int n_ch, double *cp,
+             int n_rm);
+ia_status ia_invr(int m, int l, int n, const double *r_ms, const int *e_n,
+                  const double *r_lh, double k, int n_ch, const double *cp,
+                  int n_am, ia_z **q, ia_z **m_r, ia_z **h_r, ia_z **g_mass);
+/* Elemental-radiator radiation resistance of np patches of area a_e
+ * (r_p: 3 np SoA): R_ij = (w^2 rho a_e^2/(4 pi c)) sin(k r_ij)/(k r_ij),
+ * R_ii with sin(x)/x = 1; returns the n_modes leading eigenvalues
+ * (descending) and real modes (np x n_modes, largest entry positive). */
+ia_status ia_radiation_modes(int np, const double *r_p, double a_e, double k,
+                             int n_modes, double **eigval, double **modes);
+
+/* Batched regularised solve of A_f x_f = b_f for any transfer matrices
+ * (rows x cols each, column-major, batch stride rows*cols; host or device
+ * pointers): SVD, parameter choice, filtered solution.  lambda, index
+ * (grid index), k_tsvd and sigma_max are [batch] outputs (any may be NULL). */
+ia_status ia_regsolve(int rows, int cols, int batch, const ia_z *a,
+                      const ia_z *b, const ia_reg_params *prm,
+                      ia_precision precision, ia_z *x, double *lambda,
+                      int *index, int *k_tsvd, double *sigma_max);
 
 /* Phase timings (ms) of the last sweep on this thread. */
 typedef struct {

@@ -20,7 +20,8 @@ from ._lib import IAError, check, lib, soa, zf
 __all__ = [
     "IAError", "set_air", "get_air", "device_count", "esm", "esmgpu", "esm_matrices", "csvd",
     "tsvd", "tikhonov", "dsvd", "regularize", "to_pressure", "to_velocity", "sound_power",
-    "sphere", "cartesian_2_spherical", "geometry", "esm_sweep", "SweepResult", "zgemm", "cgemm", "zgesv",
+    "sphere", "cartesian_2_spherical", "geometry", "esm_sweep", "SweepResult", "zgemm", "cgemm", "zgesv", "hels", "hels_sweep", "regsolve",
+    "quadrature_load", "set_quadrature", "ibem", "ibem_sweep", "invr", "radiation_modes",
     "FILTERS", "SELECTS",
 ]
 
@@ -250,6 +251,155 @@ def geometry(file, k, t):
     return r_ms, r_lh, e_n
 
 
+# ----------------------------------------------------------------- HELS
+def hels(m, l, r_ms_s, r_lh_s, k, n_sh):
+    """CUDA/hels.cuh:222 ``hels``: (M_e (m x n_t), H (l x n_t), Q_e (m x n_t)),
+    inputs in spherical (azimuth, elevation, radius) coordinates."""
+    rs, rh = soa(r_ms_s), soa(r_lh_s)
+    n_t = (n_sh + 1) ** 2
+    me, h, qe = _lib._zp(), _lib._zp(), _lib._zp()
+    lib().hels(int(m), int(l), _lib.dptr(rs), _lib.dptr(rh), C.byref(me), C.byref(h), C.byref(qe),
+               float(k), int(n_sh))
+    return (_lib.take_z(me, m * n_t, (m, n_t)), _lib.take_z(h, l * n_t, (l, n_t)),
+            _lib.take_z(qe, m * n_t, (m, n_t)))
+
+
+def hels_sweep(k, order, azelr_s, azelr_h, b, area, precision="fp64", filter="tikhonov",
+               select="gcv", n_lambda=50, decades=8.0, lam=0.0, kfix=0):
+    """Batched HELS sweep (uneven orders): returns a ``SweepResult``."""
+    k = np.ascontiguousarray(k, dtype=np.float64).ravel()
+    F = len(k)
+    order = np.ascontiguousarray(order, dtype=np.int32).ravel()
+    rs, rh = soa(azelr_s), soa(azelr_h)
+    m, l = len(rs) // 3, len(rh) // 3
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).reshape(F, l))
+    desc = _lib.ia_hels_desc(m, l, F, 1 if precision == "mixed" else 0,
+                             _reg(filter, select, n_lambda, decades, lam, kfix), float(area))
+    out = _sweep_outputs(F, m)
+    o = _sweep_struct(out)
+    check(lib().ia_hels_sweep(C.byref(desc), order.ctypes.data_as(_lib._ip), _lib.dptr(k),
+                              _lib.dptr(rs), _lib.dptr(rh), _lib.zptr(b), C.byref(o)))
+    return SweepResult(out["lam"], out["lam_index"], out["k_tsvd"], out["power"], out["sigma_max"],
+                       out["velocity"], out["pressure"], out["status"], {})
+
+
+def regsolve(A, b, filter="tikhonov", select="gcv", n_lambda=50, decades=8.0, lam=0.0, k=0,
+             precision="fp64"):
+    """Batched regularised solve: A (batch, rows, cols), b (batch, rows) ->
+    (x (batch, cols), lambda, index, k_tsvd, sigma_max)."""
+    A = np.asarray(A, dtype=np.complex128)
+    if A.ndim == 2:
+        A = A[None]
+    batch, rows, cols = A.shape
+    Af = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))  # column-major per item
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).reshape(batch, rows))
+    x = np.zeros((batch, cols), np.complex128)
+    la, ks, sm = np.zeros(batch), np.zeros(batch, np.int32), np.zeros(batch)
+    ix = np.zeros(batch, np.int32)
+    prm = _reg(filter, select, n_lambda, decades, lam, k)
+    check(lib().ia_regsolve(rows, cols, batch, _lib.zptr(Af), _lib.zptr(b), C.byref(prm),
+                            1 if precision == "mixed" else 0, _lib.zptr(x), _lib.dptr(la),
+                            ix.ctypes.data_as(_lib._ip), ks.ctypes.data_as(_lib._ip), _lib.dptr(sm)))
+    return x, la, ix, ks, sm
+
+
+# ----------------------------------------------------------------- IBEM
+def quadrature_load(ipf_path, ipfs_path):
+    """Read the reference's IPF.txt / IPFs.txt tables (CUDA/ibem.cuh:431-477):
+    returns (rule (n_q, 3), pages (3, n_qs, 3)) with rows (xi1, xi2, w)."""
+    nq, nqs = C.c_int(), C.c_int()
+    ipf, ipfs = _lib._dp(), _lib._dp()
+    check(lib().ia_quadrature_load(str(ipf_path).encode(), str(ipfs_path).encode(), C.byref(nq),
+                                   C.byref(ipf), C.byref(nqs), C.byref(ipfs)))
+    rule = _lib.take_d(ipf, 3 * nq.value).reshape(3, nq.value).T.copy()
+    pages = _lib.take_d(ipfs, 9 * nqs.value).reshape(3, 3, nqs.value).transpose(0, 2, 1).copy()
+    return rule, pages
+
+
+def set_quadrature(rule=None, pages=None):
+    """Use `rule` (n_q, 3) and singular `pages` (3, n_qs, 3) for the IBEM and
+    radiation-mode entry points of this thread; no arguments restores the
+    built-in rules (Radon 7-point, Duffy 40x40 Gauss-Legendre)."""
+    if rule is None and pages is None:
+        check(lib().ia_set_quadrature(0, None, 0, None))
+        return
+    rule = np.asarray(rule, dtype=np.float64)
+    pages = np.asarray(pages, dtype=np.float64)
+    r = np.ascontiguousarray(rule.T).ravel()
+    p = np.ascontiguousarray(pages.transpose(0, 2, 1)).ravel()
+    check(lib().ia_set_quadrature(rule.shape[0], _lib.dptr(r), pages.shape[1], _lib.dptr(p)))
+
+
+def _mesh_args(r_ms, e_n, r_lh, cp):
+    rs = soa(r_ms)
+    E = np.asarray(e_n, dtype=np.int32)
+    en = np.ascontiguousarray((E.T if E.ndim == 2 and E.shape[1] == 3 else E).ravel())
+    rh = soa(r_lh) if r_lh is not None and len(r_lh) else np.zeros(0)
+    c = soa(cp) if cp is not None and len(cp) else np.zeros(0)
+    return rs, en, rh, c, len(rs) // 3, len(en) // 3, len(rh) // 3, len(c) // 3
+
+
+def ibem(r_ms, e_n, r_lh, k, cp=None):
+    """CUDA/ibem.cuh:424 ``ibem``: (Z (m x m), H_b (l x m)).  r_ms (m, 3),
+    e_n (n, 3) 0-based triangles, r_lh (l, 3), cp (n_ch, 3) CHIEF points."""
+    rs, en, rh, c, m, n, l, nch = _mesh_args(r_ms, e_n, r_lh, cp)
+    z, h = _lib._zp(), _lib._zp()
+    check(lib().ia_ibem(m, l, n, _lib.dptr(rs), en.ctypes.data_as(_lib._ip),
+                        _lib.dptr(rh) if l else None, float(k), nch, _lib.dptr(c) if nch else None,
+                        C.byref(z), C.byref(h) if l else None))
+    Z = _lib.take_z(z, m * m, (m, m))
+    return Z, (_lib.take_z(h, l * m, (l, m)) if l else np.zeros((0, m), np.complex128))
+
+
+def ibem_sweep(k, r_ms, e_n, r_lh, b, area, cp=None, precision="fp64", filter="tikhonov",
+               select="gcv", n_lambda=50, decades=8.0, lam=0.0, kfix=0, chunk=0):
+    """Batched IBEM sweep: assemble Z, H_b per wavenumber, regularised solve
+    of H_b v = b[f], p_s = Z v and W.  Returns a ``SweepResult``."""
+    k = np.ascontiguousarray(k, dtype=np.float64).ravel()
+    F = len(k)
+    rs, en, rh, c, m, n, l, nch = _mesh_args(r_ms, e_n, r_lh, cp)
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).reshape(F, l))
+    desc = _lib.ia_ibem_desc(m, l, n, nch, F, 1 if precision == "mixed" else 0,
+                             _reg(filter, select, n_lambda, decades, lam, kfix), float(area),
+                             int(chunk))
+    out = _sweep_outputs(F, m)
+    o = _sweep_struct(out)
+    check(lib().ia_ibem_sweep(C.byref(desc), _lib.dptr(k), _lib.dptr(rs),
+                              en.ctypes.data_as(_lib._ip), _lib.dptr(rh),
+                              _lib.dptr(c) if nch else None, _lib.zptr(b), C.byref(o)))
+    return SweepResult(out["lam"], out["lam_index"], out["k_tsvd"], out["power"], out["sigma_max"],
+                       out["velocity"], out["pressure"], out["status"], {})
+
+
+def invr(r_ms, e_n, r_lh, k, n_am, cp=None):
+    """Radiation modes (CUDA/invr.cuh:457, MATLAB/invrS.m:157-170):
+    returns (Q (m x n_am), M_r = Z Q, H_r = H_b Q, G (m x m))."""
+    rs, en, rh, c, m, n, l, nch = _mesh_args(r_ms, e_n, r_lh, cp)
+    q, mr, hr, g = _lib._zp(), _lib._zp(), _lib._zp(), _lib._zp()
+    check(lib().ia_invr(m, l, n, _lib.dptr(rs), en.ctypes.data_as(_lib._ip),
+                        _lib.dptr(rh) if l else None, float(k), nch, _lib.dptr(c) if nch else None,
+                        int(n_am), C.byref(q), C.byref(mr), C.byref(hr) if l else None,
+                        C.byref(g)))
+    na = min(int(n_am), m)
+    Q = _lib.take_z(q, m * na, (m, na))
+    Mr = _lib.take_z(mr, m * na, (m, na))
+    Hr = _lib.take_z(hr, l * na, (l, na)) if l else np.zeros((0, na), np.complex128)
+    G = _lib.take_z(g, m * m, (m, m)).real.copy()
+    return Q, Mr, Hr, G
+
+
+def radiation_modes(pts, area, k, n_modes):
+    """Elemental-radiator radiation modes: (eigenvalues (n_modes,), modes
+    (np, n_modes)) of R_ij = (w^2 rho A^2/(4 pi c)) sin(k r_ij)/(k r_ij)."""
+    p = soa(pts)
+    npt = len(p) // 3
+    ev, md = _lib._dp(), _lib._dp()
+    check(lib().ia_radiation_modes(npt, _lib.dptr(p), float(area), float(k), int(n_modes),
+                                   C.byref(ev), C.byref(md)))
+    nm = min(int(n_modes), npt)
+    return _lib.take_d(ev, nm), _lib.take_d(md, npt * nm).reshape(nm, npt).T.copy()
+
+
 # ------------------------------------------------------------ ESM sweep
 @dataclass
 class SweepResult:
@@ -262,6 +412,18 @@ class SweepResult:
     pressure: np.ndarray   # (F, m)
     status: np.ndarray
     stats: dict
+
+
+def _sweep_outputs(F, m):
+    return dict(lam=np.zeros(F), lam_index=np.zeros(F, np.int32), k_tsvd=np.zeros(F, np.int32),
+                power=np.zeros(F), sigma_max=np.zeros(F), velocity=np.zeros((F, m), np.complex128),
+                pressure=np.zeros((F, m), np.complex128), status=np.zeros(F, np.int32))
+
+
+def _sweep_struct(out):
+    return _lib.ia_sweep_out(*(out[n].ctypes.data for n in
+                               ("lam", "lam_index", "k_tsvd", "power", "sigma_max", "velocity",
+                                "pressure", "status")))
 
 
 def esm_sweep(k, r_s, src, r_h, b, area, normals=None, precision="fp64", filter="tikhonov",
@@ -278,12 +440,8 @@ def esm_sweep(k, r_s, src, r_h, b, area, normals=None, precision="fp64", filter=
     desc = _lib.ia_sweep_desc(m, l, F, 1 if precision == "mixed" else 0,
                               _reg(filter, select, n_lambda, decades, lam, kfix), float(area),
                               int(chunk))
-    out = dict(lam=np.zeros(F), lam_index=np.zeros(F, np.int32), k_tsvd=np.zeros(F, np.int32),
-               power=np.zeros(F), sigma_max=np.zeros(F), velocity=np.zeros((F, m), np.complex128),
-               pressure=np.zeros((F, m), np.complex128), status=np.zeros(F, np.int32))
-    o = _lib.ia_sweep_out(*(out[n].ctypes.data for n in
-                            ("lam", "lam_index", "k_tsvd", "power", "sigma_max", "velocity",
-                             "pressure", "status")))
+    out = _sweep_outputs(F, m)
+    o = _sweep_struct(out)
     check(lib().ia_esm_sweep(C.byref(desc), _lib.dptr(k), _lib.dptr(rs), _lib.dptr(sq),
                              _lib.dptr(nr) if nr is not None else None, _lib.dptr(rh),
                              _lib.zptr(b), C.byref(o)))

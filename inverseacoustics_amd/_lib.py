@@ -39,6 +39,17 @@ class ia_sweep_out(C.Structure):
                 ("pressure", C.c_void_p), ("status", C.c_void_p)]
 
 
+class ia_hels_desc(C.Structure):
+    _fields_ = [("m", C.c_int), ("l", C.c_int), ("n_freq", C.c_int), ("precision", C.c_int),
+                ("reg", ia_reg_params), ("area", C.c_double)]
+
+
+class ia_ibem_desc(C.Structure):
+    _fields_ = [("m", C.c_int), ("l", C.c_int), ("n_elem", C.c_int), ("n_ch", C.c_int),
+                ("n_freq", C.c_int), ("precision", C.c_int), ("reg", ia_reg_params),
+                ("area", C.c_double), ("chunk", C.c_int)]
+
+
 class ia_stats(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("assemble", "factor", "solve_he", "bidiag", "bisect", "vectors", "select",
@@ -88,6 +99,31 @@ SIGNATURES = {
     "ia_zgemm": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, ia_z, _zp, C.c_int, _zp,
                            C.c_int, ia_z, _zp, C.c_int]),
     "ia_zgesv": (C.c_int, [C.c_char, C.c_int, C.c_int, _zp, _zp]),
+    "hels": (None, [C.c_int, C.c_int, _dp, _dp, _zpp, _zpp, _zpp, C.c_double, C.c_int]),
+    "helsgpu": (None, [C.c_int, C.c_int, _dp, _dp, _zpp, _zpp, _zpp, C.c_double, C.c_int]),
+    "ia_hels": (C.c_int, [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_int, _zpp, _zpp, _zpp]),
+    "ia_hels_sweep": (C.c_int, [C.POINTER(ia_hels_desc), _ip, _dp, _dp, _dp, _zp,
+                                C.POINTER(ia_sweep_out)]),
+    "ia_regsolve": (C.c_int, [C.c_int, C.c_int, C.c_int, _zp, _zp, C.POINTER(ia_reg_params), C.c_int,
+                              _zp, _dp, _ip, _ip, _dp]),
+    "ia_quadrature_load": (C.c_int, [C.c_char_p, C.c_char_p, _ip, _dpp, _ip, _dpp]),
+    "ia_set_quadrature": (C.c_int, [C.c_int, _dp, C.c_int, _dp]),
+    "ibem": (None, [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, _zpp, _zpp, C.c_double, C.c_int,
+                    _dp]),
+    "ibemgpu": (None, [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, _zpp, _zpp, C.c_double, C.c_int,
+                       _dp]),
+    "ia_ibem": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, C.c_double, C.c_int, _dp,
+                          _zpp, _zpp]),
+    "ia_ibem_sweep": (C.c_int, [C.POINTER(ia_ibem_desc), _dp, _dp, _ip, _dp, _dp, _zp,
+                                C.POINTER(ia_sweep_out)]),
+    "invr": (None, [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, _zpp, _zpp, _zpp, C.c_double,
+                    C.c_int, _dp, C.c_int]),
+    "invrgpu": (None, [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, _zpp, _zpp, _zpp, C.c_double,
+                       C.c_int, _dp, C.c_int]),
+    "ia_invr": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, C.c_double, C.c_int, _dp,
+                          C.c_int, _zpp, _zpp, _zpp, _zpp]),
+    "ia_radiation_modes": (C.c_int, [C.c_int, _dp, C.c_double, C.c_double, C.c_int, _dpp,
+                                     _dpp]),
     "ia_cgemm": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, ia_c, C.c_void_p, C.c_int,
                            C.c_void_p, C.c_int, ia_c, C.c_void_p, C.c_int]),
 }

@@ -62,80 +62,6 @@ ia_status to_device(DevBuf &dst, const void *src, size_t n, hipStream_t s) {
 
 static thread_local ia_stats g_stats;
 
-static bool finite_all(const double *p, size_t n) {
-    for (size_t i = 0; i < n; ++i)
-        if (!std::isfinite(p[i])) return false;
-    return true;
-}
-
-static ia_status check_device() {
-    int n = 0;
-    hipError_t e = hipGetDeviceCount(&n);
-    if (e != hipSuccess || n == 0) {
-        (void)hipGetLastError();
-        return fail(IA_ERR_DEVICE, "no HIP device available (this library has no CPU fallback)");
-    }
-    return IA_OK;
-}
-
-template <typename T>
-static ia_status host_out(T **dst, const void *dsrc, size_t count) {
-    if (!dst) return IA_OK;
-    *dst = (T *)malloc(sizeof(T) * (count ? count : 1));
-    if (!*dst) return fail(IA_ERR_NOMEM, "malloc(%zu)", sizeof(T) * count);
-    if (count) IA_HIP(hipMemcpy(*dst, dsrc, sizeof(T) * count, hipMemcpyDeviceToHost));
-    return IA_OK;
-}
-
-// ------------------------------------------------------ SVD driver
-// Factor Y (R x C, R >= C, ld R, batch stride R*C, complex T) in place:
-// bidiagonal reflectors in Y/Vr/tau, singular values sig (descending),
-// bidiagonal singular vectors QBt/PBt (vector i in row i).
-template <typename T>
-struct SvdWork {
-    DevBuf d, e, sig, tauL, tauR, Vr, QBt, PBt, work;
-    int R = 0, C = 0, batch = 0, sub = 1;
-    ia_status alloc(int R_, int C_, int batch_) {
-        R = R_;
-        C = C_;
-        batch = batch_;
-        size_t bc = (size_t)batch * C;
-        IA_TRY(d.alloc(bc * sizeof(double)));
-        IA_TRY(e.alloc(bc * sizeof(double)));
-        IA_TRY(sig.alloc(bc * sizeof(double)));
-        IA_TRY(tauL.alloc(bc * sizeof(cx<T>)));
-        IA_TRY(tauR.alloc(bc * sizeof(cx<T>)));
-        IA_TRY(Vr.alloc(bc * C * sizeof(cx<T>)));
-        IA_TRY(QBt.alloc(bc * C * sizeof(double)));
-        IA_TRY(PBt.alloc(bc * C * sizeof(double)));
-        // inverse-iteration scratch bounded to ~1 GiB
-        size_t per = inviter_work_bytes(C, 1);
-        sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)batch, ((size_t)1 << 30) / per));
-        IA_TRY(work.alloc(inviter_work_bytes(C, sub)));
-        return IA_OK;
-    }
-};
-
-template <typename T>
-static ia_status svd_factor(SvdWork<T> &w, cx<T> *Y, hipStream_t s) {
-    const int R = w.R, C = w.C, batch = w.batch;
-    size_t lds = bidiag_lds_bytes(R, C, sizeof(cx<T>));
-    if (lds > 160 * 1024)
-        return fail(IA_ERR_ARG, "SVD of %d x %d exceeds the on-chip bidiagonalisation limit", R, C);
-    launch_bidiag<T>(R, C, Y, R, (size_t)R * C, w.d.template as<double>(), w.e.template as<double>(),
-                     w.tauL.template as<cx<T>>(), w.tauR.template as<cx<T>>(),
-                     w.Vr.template as<cx<T>>(), batch, s);
-    IA_HIP(hipGetLastError());
-    launch_bisect(C, w.d.template as<double>(), w.e.template as<double>(), w.sig.template as<double>(),
-                  batch, s);
-    IA_HIP(hipGetLastError());
-    launch_inviter(C, w.d.template as<double>(), w.e.template as<double>(),
-                   w.sig.template as<double>(), w.QBt.template as<double>(),
-                   w.PBt.template as<double>(), w.work.template as<double>(), w.sub, batch, s);
-    IA_HIP(hipGetLastError());
-    return IA_OK;
-}
-
 // ------------------------------------------------------ sweep engine
 struct Timer {
     hipEvent_t ev[10];

@@ -145,6 +145,15 @@ ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_
                         const int *ipiv, cx<T> *B, int ldb, size_t strideB, int batch,
                         hipStream_t s);
 
+// upper-triangular solve A X = B with the upper triangle of A (non-unit)
+template <typename T>
+ia_status trsm_upper_batched(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
+                             int ldb, size_t strideB, int batch, hipStream_t s);
+// Householder QR in place (reflectors below the diagonal, R above)
+template <typename T>
+ia_status launch_geqr2(int R, int C, cx<T> *A, int lda, size_t strideA, cx<T> *tau, int batch,
+                       hipStream_t s);
+
 // SVD building blocks (svd.hip)
 size_t bidiag_lds_bytes(int R, int C, size_t elem);
 template <typename T>
@@ -202,4 +211,87 @@ void launch_conj_transpose(int rows, int cols, const cx<T> *src, size_t ss, cx<T
 // y[b][0..n) = x[b][0..n), y[b][n..ny) = 0 (cx<double>)
 void launch_pad_copy(int n, int ny, const cx<double> *x, size_t sx, cx<double> *y, size_t sy,
                      int batch, hipStream_t s);
+}  // namespace ia
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
+namespace ia {
+// ------------------------------------------------ shared host helpers
+
+inline bool finite_all(const double *p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+inline ia_status check_device() {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return fail(IA_ERR_DEVICE, "no HIP device available (this library has no CPU fallback)");
+    }
+    return IA_OK;
+}
+
+template <typename T>
+inline ia_status host_out(T **dst, const void *dsrc, size_t count) {
+    if (!dst) return IA_OK;
+    *dst = (T *)malloc(sizeof(T) * (count ? count : 1));
+    if (!*dst) return fail(IA_ERR_NOMEM, "malloc(%zu)", sizeof(T) * count);
+    if (count) IA_HIP(hipMemcpy(*dst, dsrc, sizeof(T) * count, hipMemcpyDeviceToHost));
+    return IA_OK;
+}
+
+// ------------------------------------------------------ SVD driver
+// Factor Y (R x C, R >= C, ld R, batch stride R*C, complex T) in place:
+// bidiagonal reflectors in Y/Vr/tau, singular values sig (descending),
+// bidiagonal singular vectors QBt/PBt (vector i in row i).
+template <typename T>
+struct SvdWork {
+    DevBuf d, e, sig, tauL, tauR, Vr, QBt, PBt, work;
+    int R = 0, C = 0, batch = 0, sub = 1;
+    ia_status alloc(int R_, int C_, int batch_) {
+        R = R_;
+        C = C_;
+        batch = batch_;
+        size_t bc = (size_t)batch * C;
+        IA_TRY(d.alloc(bc * sizeof(double)));
+        IA_TRY(e.alloc(bc * sizeof(double)));
+        IA_TRY(sig.alloc(bc * sizeof(double)));
+        IA_TRY(tauL.alloc(bc * sizeof(cx<T>)));
+        IA_TRY(tauR.alloc(bc * sizeof(cx<T>)));
+        IA_TRY(Vr.alloc(bc * C * sizeof(cx<T>)));
+        IA_TRY(QBt.alloc(bc * C * sizeof(double)));
+        IA_TRY(PBt.alloc(bc * C * sizeof(double)));
+        // inverse-iteration scratch bounded to ~1 GiB
+        size_t per = inviter_work_bytes(C, 1);
+        sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)batch, ((size_t)1 << 30) / per));
+        IA_TRY(work.alloc(inviter_work_bytes(C, sub)));
+        return IA_OK;
+    }
+};
+
+template <typename T>
+inline ia_status svd_factor(SvdWork<T> &w, cx<T> *Y, hipStream_t s) {
+    const int R = w.R, C = w.C, batch = w.batch;
+    size_t lds = bidiag_lds_bytes(R, C, sizeof(cx<T>));
+    if (lds > 160 * 1024)
+        return fail(IA_ERR_ARG, "SVD of %d x %d exceeds the on-chip bidiagonalisation limit", R, C);
+    launch_bidiag<T>(R, C, Y, R, (size_t)R * C, w.d.template as<double>(), w.e.template as<double>(),
+                     w.tauL.template as<cx<T>>(), w.tauR.template as<cx<T>>(),
+                     w.Vr.template as<cx<T>>(), batch, s);
+    IA_HIP(hipGetLastError());
+    launch_bisect(C, w.d.template as<double>(), w.e.template as<double>(), w.sig.template as<double>(),
+                  batch, s);
+    IA_HIP(hipGetLastError());
+    launch_inviter(C, w.d.template as<double>(), w.e.template as<double>(),
+                   w.sig.template as<double>(), w.QBt.template as<double>(),
+                   w.PBt.template as<double>(), w.work.template as<double>(), w.sub, batch, s);
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
 }  // namespace ia

@@ -321,6 +321,17 @@ ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_
     return IA_OK;
 }
 
+template <typename T>
+ia_status trsm_upper_batched(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
+                             int ldb, size_t strideB, int batch, hipStream_t s) {
+    if (n <= 0 || nrhs <= 0 || batch <= 0) return IA_OK;
+    trsm_blocked<T, 2>(n, nrhs, A, lda, strideA, B, ldb, strideB, batch, s);
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
+template ia_status trsm_upper_batched<double>(int, int, const cx<double> *, int, size_t,
+                                              cx<double> *, int, size_t, int, hipStream_t);
 template ia_status getrf_batched<double>(int, cx<double> *, int, size_t, int *, int *, int,
                                          hipStream_t);
 template ia_status getrf_batched<float>(int, cx<float> *, int, size_t, int *, int *, int,

@@ -497,6 +497,68 @@ __global__ __launch_bounds__(256) void k_apply_right(int C, const cx<T> *__restr
     }
 }
 
+// Unblocked Householder QR A = Q R (R >= C), one workgroup per matrix, the
+// LAPACK geqr2 conventions of the bidiagonalisation's left reflectors:
+// reflector j in column j below the diagonal (unit at row j), R on and above
+// the diagonal.  Used for the CHIEF least-squares solves (the reference's
+// cusolverDnZZgels, CUDA/ibem.cuh:389-391).
+template <typename T>
+__global__ __launch_bounds__(BT) void k_geqr2(int R, int C, cx<T> *__restrict__ Aall, int lda,
+                                             size_t sA, cx<T> *__restrict__ tauall) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *u = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *w = u + R;
+    double *red = reinterpret_cast<double *>(w + C);
+    __shared__ cx<double> bc_val;
+    const int b = blockIdx.x;
+    cx<T> *A = Aall + (size_t)b * sA;
+    cx<T> *tau = tauall + (size_t)b * C;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    for (int j = 0; j < C; ++j) {
+        cx<T> *Aj = A + (size_t)j * lda;
+        double p = 0;
+        for (int r = j + 1 + t; r < R; r += BT) p += (double)abs2(Aj[r]);
+        double xn2 = block_sum(p, red);
+        if (t == 0) bc_val = cvt<double>(Aj[j]);
+        __syncthreads();
+        Refl rf = larfg(bc_val, xn2);
+        const cx<T> sc = cvt<T>(rf.scale);
+        for (int r = j + 1 + t; r < R; r += BT) {
+            cx<T> uv = Aj[r] * sc;
+            u[r] = uv;
+            Aj[r] = uv;
+        }
+        if (t == 0) {
+            u[j] = cx<T>{1, 0};
+            Aj[j] = cx<T>{(T)rf.beta, 0};
+            tau[j] = cvt<T>(rf.tau);
+        }
+        __syncthreads();
+        for (int c = j + 1 + wv; c < C; c += NW) {
+            const cx<T> *Ac = A + (size_t)c * lda;
+            double ar = 0, ai = 0;
+            for (int r = j + lane; r < R; r += 64) {
+                cx<T> y = Ac[r], uu = u[r];
+                ar += (double)(uu.re * y.re + uu.im * y.im);
+                ai += (double)(uu.re * y.im - uu.im * y.re);
+            }
+            ar = wave_sum(ar);
+            ai = wave_sum(ai);
+            if (lane == 0) w[c] = cx<T>{(T)ar, (T)ai};
+        }
+        __syncthreads();
+        const cx<T> ctau = conj(cvt<T>(rf.tau));
+        for (int r = j + t; r < R; r += BT) {
+            const cx<T> cu = ctau * u[r];
+            for (int c = j + 1; c < C; ++c) {
+                cx<T> *yp = A + (size_t)c * lda + r;
+                *yp = *yp - cu * w[c];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 size_t bidiag_lds_bytes(int R, int C, size_t elem) { return (2 * (size_t)R + 2 * (size_t)C) * elem + NW * sizeof(double) + 64; }
@@ -536,6 +598,17 @@ void launch_inviter(int n, const double *d, const double *e, const double *sigma
 }
 
 template <typename T>
+ia_status launch_geqr2(int R, int C, cx<T> *A, int lda, size_t strideA, cx<T> *tau, int batch,
+                       hipStream_t s) {
+    size_t lds = ((size_t)R + C) * sizeof(cx<T>) + NW * sizeof(double) + 64;
+    if (lds > 160 * 1024) return fail(IA_ERR_ARG, "QR of %d x %d exceeds the on-chip limit", R, C);
+    (void)hipFuncSetAttribute((const void *)k_geqr2<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds);
+    hipLaunchKernelGGL(k_geqr2<T>, dim3(batch), dim3(BT), lds, s, R, C, A, lda, strideA, tau);
+    return IA_OK;
+}
+
+template <typename T>
 void launch_apply_left(bool herm, int R, int C, const cx<T> *Y, int ldy, size_t strideY,
                        const cx<T> *tauL, cx<double> *v, size_t stridev, int nvec, int batch,
                        hipStream_t s) {
@@ -559,6 +632,11 @@ void launch_apply_right(bool herm, int C, const cx<T> *Vr, const cx<T> *tauR, cx
         hipLaunchKernelGGL((k_apply_right<T, false>), grid, dim3(256), 0, s, C, Vr, tauR, v,
                            stridev, nvec);
 }
+
+template ia_status launch_geqr2<double>(int, int, cx<double> *, int, size_t, cx<double> *, int,
+                                         hipStream_t);
+template ia_status launch_geqr2<float>(int, int, cx<float> *, int, size_t, cx<float> *, int,
+                                       hipStream_t);
 
 #define INST(T)                                                                                  \
     template void launch_bidiag<T>(int, int, cx<T> *, int, size_t, double *, double *, cx<T> *, \

@@ -243,3 +243,142 @@ def plate_scenario(freqs=None, n_freq=256, fmin=200.0, fmax=1500.0, d=0.02,
 def soa(pts: np.ndarray) -> np.ndarray:
     """(n,3) -> contiguous x-block, y-block, z-block (the reference's layout)."""
     return np.ascontiguousarray(np.asarray(pts, dtype=np.float64).T).ravel()
+
+
+# ---------------------------------------------------------------- HELS
+@dataclass
+class HelsScenario:
+    """Inputs of a HELS sweep: spherical coordinates (az, el, r) as
+    ``cartesian_2_spherical`` returns them, per-frequency expansion order."""
+    freqs: np.ndarray
+    k: np.ndarray
+    order: np.ndarray
+    azelr_s: np.ndarray   # (m, 3) reconstruction surface
+    azelr_h: np.ndarray   # (l, 3) microphones
+    b: np.ndarray         # (F, l)
+    area: float
+    rho: float = RHO_DEFAULT
+    c: float = C_DEFAULT
+    truth: dict = field(default_factory=dict)
+
+
+def to_spherical(pts):
+    """(az in [0, 2 pi), el = polar angle (minus 2 pi where x < 0), r) -- the
+    reference's cartesian_2_spherical (CUDA/utils.cuh:414-427)."""
+    p = np.asarray(pts, np.float64)
+    xy = p[:, 0] ** 2 + p[:, 1] ** 2
+    r = np.sqrt(xy + p[:, 2] ** 2)
+    az = np.arctan2(p[:, 1], p[:, 0])
+    az = np.where(p[:, 1] < 0, az + 2 * math.pi, az)
+    el = np.arctan2(np.sqrt(xy), p[:, 2])
+    el = np.where(p[:, 0] < 0, el - 2 * math.pi, el)
+    return np.stack([az, el, r], axis=1)
+
+
+def hels_scenario(freqs=None, n_freq=128, fmin=50.0, fmax=3000.0, a=0.12, n_src=5, l=256,
+                  r_mic=0.3, m=256, nmax=16, snr_db=30.0, seed=1003, rho=RHO_DEFAULT, c=C_DEFAULT):
+    """n_src seeded monopoles inside r < 0.6 a with CN(0,1) amplitudes,
+    l Fibonacci microphones at r_mic, reconstruction on m Fibonacci points at
+    r = a, expansion order N_f = min(nmax, ceil(k a) + 4)."""
+    if freqs is None:
+        freqs = np.linspace(fmin, fmax, n_freq)
+    freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+    k = 2 * math.pi * freqs / c
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal((n_src, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    src = d * (0.6 * a * rng.uniform(0, 1, (n_src, 1)) ** (1 / 3))
+    amp = (rng.standard_normal(n_src) + 1j * rng.standard_normal(n_src)) / math.sqrt(2)
+    r_h = fibonacci_sphere(l, r_mic)
+    r_s = fibonacci_sphere(m, a)
+    order = np.minimum(nmax, np.ceil(k * a).astype(int) + 4)
+    b = np.empty((len(freqs), l), np.complex128)
+    for f, kf in enumerate(k):
+        R = np.linalg.norm(r_h[:, None, :] - src[None, :, :], axis=2)
+        p = (1j * rho * c * kf * np.exp(1j * kf * R) / (4 * math.pi * R)) @ amp
+        b[f] = add_noise(p, snr_db, rng)
+    return HelsScenario(freqs, k, order, to_spherical(r_s), to_spherical(r_h), b,
+                        4 * math.pi * a * a, rho, c, truth=dict(src=src, amp=amp))
+
+
+# ----------------------------------------------------------------- IBEM
+def icosphere(subdiv: int, radius: float = 1.0):
+    """Icosahedron refined `subdiv` times by edge midpoints projected onto the
+    sphere: (nodes (m, 3), triangles (n, 3) 0-based), each triangle ordered
+    so that (P0 - P2) x (P1 - P2) points outward (the reference's normal,
+    ``MATLAB/ibemS.m:39``).  subdiv 3: 642 nodes, 1280 triangles."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [np.array(p, float) / math.sqrt(1 + t * t) for p in
+             ((-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t),
+              (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1))]
+    tris = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+            (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+            (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        mids: dict = {}
+
+        def mid(i, j):
+            key = (i, j) if i < j else (j, i)
+            if key not in mids:
+                p = verts[i] + verts[j]
+                verts.append(p / np.linalg.norm(p))
+                mids[key] = len(verts) - 1
+            return mids[key]
+
+        nxt = []
+        for a, b, c in tris:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nxt += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        tris = nxt
+    X = radius * np.array(verts)
+    E = np.array(tris, dtype=np.int64)
+    nrm = np.cross(X[E[:, 0]] - X[E[:, 2]], X[E[:, 1]] - X[E[:, 2]])
+    inward = np.einsum("ij,ij->i", nrm, X[E].mean(axis=1)) < 0
+    E[inward] = E[inward][:, [1, 0, 2]]
+    return X, E
+
+
+@dataclass
+class IbemScenario:
+    """Inputs of one IBEM sweep on a closed triangulated surface."""
+    freqs: np.ndarray
+    k: np.ndarray
+    X: np.ndarray              # (m, 3) nodes
+    E: np.ndarray              # (n, 3) 0-based triangles, outward (P0-P2)x(P1-P2)
+    r_h: np.ndarray            # (l, 3) microphones
+    chief: np.ndarray          # (n_ch, 3) interior CHIEF points (may be empty)
+    b: np.ndarray              # (F, l)
+    area: float
+    rho: float
+    c: float
+    truth: dict = field(default_factory=dict)
+
+
+def ibem_scenario(freqs=None, n_freq=64, fmin=100.0, fmax=2000.0, subdiv=3, a=0.1, l=128,
+                  r_mic=0.25, V0=1.0, V1=0.5, snr_db=30.0, n_chief=0, seed=1002,
+                  rho=RHO_DEFAULT, c=C_DEFAULT):
+    """Pulsating + oscillating sphere (normal velocity V0 + V1 cos(theta)) on an
+    icosphere of radius a, l Fibonacci microphones at r_mic with seeded
+    complex Gaussian noise; n_chief seeded interior CHIEF points at r < 0.6 a."""
+    if freqs is None:
+        freqs = np.linspace(fmin, fmax, n_freq)
+    freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+    k = 2 * math.pi * freqs / c
+    X, E = icosphere(subdiv, a)
+    r_h = fibonacci_sphere(l, r_mic)
+    rng = np.random.default_rng(seed)
+    chief = np.zeros((0, 3))
+    if n_chief:
+        d = rng.standard_normal((n_chief, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        chief = d * (0.6 * a * rng.uniform(0.2, 1.0, (n_chief, 1)))
+    b = np.empty((len(freqs), l), np.complex128)
+    W = np.empty(len(freqs))
+    for f, kf in enumerate(k):
+        b[f] = add_noise(sphere_modal_pressure(kf, a, r_h, V0, V1, rho, c), snr_db, rng)
+        W[f] = sphere_modal_power(kf, a, V0, V1, rho, c)
+    # polyhedron area: the sum of the flat triangles
+    S = 0.5 * float(np.sum(np.linalg.norm(np.cross(X[E[:, 0]] - X[E[:, 2]],
+                                                    X[E[:, 1]] - X[E[:, 2]]), axis=1)))
+    return IbemScenario(freqs, k, X, E, r_h, chief, b, S, rho, c,
+                        truth=dict(v_s=sphere_modal_velocity(X, V0, V1), W=W))

@@ -28,12 +28,20 @@ EPS = np.finfo(np.float64).eps
 
 
 # ------------------------------------------------------------------ SVD
-def csvd(A):
+def csvd(A, driver="gesdd"):
     """Compact SVD A = U diag(s) V^H (csvdG.m:17-24, tsvd.cuh:10-120).
 
     m >= n: U m x n, s n, V n x n.  m < n: U m x m, s m, V n x m (the reference
-    works on A^H for this case, tsvd.cuh:93-103)."""
-    U, s, Vh = np.linalg.svd(np.asarray(A), full_matrices=False)
+    works on A^H for this case, tsvd.cuh:93-103).  ``driver`` selects the
+    LAPACK routine (gesdd, numpy's default, or gesvd, the routine cuSOLVER and
+    the reference's CPU invr use); comparing the two measures how far two
+    backward-stable FP64 SVDs of the same matrix can disagree."""
+    if driver == "gesdd":
+        U, s, Vh = np.linalg.svd(np.asarray(A), full_matrices=False)
+    else:
+        import scipy.linalg as sl
+
+        U, s, Vh = sl.svd(np.asarray(A), full_matrices=False, lapack_driver=driver)
     return U, s, Vh.conj().T
 
 

@@ -85,3 +85,19 @@ def test_compute_without_gpu_fails_loudly(ia):
         pytest.skip("GPU present")
     with pytest.raises(ia.IAError):
         ia.csvd(np.eye(3))
+
+
+def test_quadrature_loader_reads_reference_tables():
+    """ia_quadrature_load is host-only parsing: it reads the reference's text
+    tables exactly as the oracle's loaders do."""
+    from pathlib import Path
+
+    import inverseacoustics_amd as ia
+    from oracle import ibem as ib
+
+    gold = Path(__file__).parent / "golden"
+    rule, pages = ia.quadrature_load(gold / "IPF.txt", gold / "IPFs.txt")
+    assert np.array_equal(rule, ib.load_ipf(gold / "IPF.txt"))
+    assert np.array_equal(pages, ib.load_ipfs(gold / "IPFs.txt"))
+    with pytest.raises(ia.IAError):
+        ia.quadrature_load(gold / "missing.txt", gold / "IPFs.txt")
