@@ -135,14 +135,10 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
                                 (size_t)m * l, nf, s));
         tm.mark();  // 3
         sw.batch = nf;
-        {
-            size_t lds = bidiag_lds_bytes(R, C, sizeof(cx<T>));
-            if (lds > 160 * 1024)
-                return fail(IA_ERR_ARG, "ESM sweep %d x %d exceeds the on-chip limit", R, C);
-            launch_bidiag<T>(R, C, Y, R, (size_t)R * C, sw.d.template as<double>(),
-                             sw.e.template as<double>(), sw.tauL.template as<cx<T>>(),
-                             sw.tauR.template as<cx<T>>(), sw.Vr.template as<cx<T>>(), nf, s);
-        }
+        IA_TRY(launch_gebrd<T>(R, C, Y, R, (size_t)R * C, sw.d.template as<double>(),
+                               sw.e.template as<double>(), sw.tauL.template as<cx<T>>(),
+                               sw.tauR.template as<cx<T>>(), sw.Vr.template as<cx<T>>(),
+                               sw.gw.template as<cx<T>>(), nf, s));
         tm.mark();  // 4
         launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(), nf, s);
         tm.mark();  // 5

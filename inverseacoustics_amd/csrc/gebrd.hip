@@ -1,0 +1,311 @@
+// Panel-blocked Householder bidiagonalisation Y = U_b B V_b^H (R >= C).
+//
+// Same reflectors and storage as the fused kernel of svd.hip (left
+// reflector j in column j of Y below the diagonal with tauL[j], right
+// reflector j in row j of Vr with tauR[j], B = bidiag(d, e)), computed with
+// the deferred-update scheme of LAPACK's gebrd/labrd:  inside a panel of NB
+// columns the trailing matrix is not touched; the matrix the unblocked
+// algorithm would see at step i is
+//     A_i = A - U Yw^H - Xw Vw^H        (U: left vectors, Vw: right vectors,
+//                                        Yw = tau A^H u, Xw = sigma A' v)
+// and the two matrix-vector products of a step read the trailing block
+// once each.  After the panel, the trailing block is updated by two complex
+// GEMMs of rank NB on the matrix cores.  Per column the HBM traffic is two
+// reads of the trailing block, against two reads and two writes for the
+// fused unblocked kernel.
+//
+// One 1024-thread workgroup per frequency for the panel kernel.  LDS holds
+// the current left vector u (R), the right vector v (C), the panel's small
+// dot products and the partial sums of the row-split matrix-vector product.
+#include "householder.hpp"
+
+namespace ia {
+
+namespace {
+
+using hh::BT;
+using hh::NW;
+using hh::Refl;
+using hh::block_sum;
+using hh::larfg;
+using hh::wave_sum;
+
+constexpr int NB = 32;
+
+template <typename T>
+__global__ __launch_bounds__(BT) void k_labrd(int R, int C, int j0, int nbp,
+                                             cx<T> *__restrict__ Yall, int ldy, size_t sY,
+                                             double *__restrict__ dall, double *__restrict__ eall,
+                                             cx<T> *__restrict__ tauLall,
+                                             cx<T> *__restrict__ tauRall,
+                                             cx<T> *__restrict__ Vrall, cx<T> *__restrict__ Xwall,
+                                             cx<T> *__restrict__ Ywall,
+                                             cx<T> *__restrict__ Vwall) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<double> *part = reinterpret_cast<cx<double> *>(smem_raw);  // NW * 64
+    cx<double> *pu = part + NW * 64;                                // NB: u^H U_k
+    cx<double> *pq = pu + NB;                                       // NB: u^H Xw_k
+    cx<double> *py = pq + NB;                                       // NB+1: Yw_k^H v
+    cx<double> *pv = py + NB + 1;                                   // NB: Vw_k^H v
+    cx<double> *cy = pv + NB;                                       // NB: conj(Yw[j,k])
+    cx<double> *cv = cy + NB;                                       // NB: conj(Vw[j,k])
+    cx<double> *uj = cv + NB;                                       // NB: U[j,k]
+    cx<double> *xj = uj + NB;                                       // NB: Xw[j,k]
+    double *red = reinterpret_cast<double *>(xj + NB);              // NW
+    cx<T> *u = reinterpret_cast<cx<T> *>(red + NW);                 // R
+    cx<T> *v = u + R;                                               // C
+
+    const int b = blockIdx.x;
+    cx<T> *Y = Yall + (size_t)b * sY;
+    double *d = dall + (size_t)b * C;
+    double *e = eall + (size_t)b * C;
+    cx<T> *tauL = tauLall + (size_t)b * C;
+    cx<T> *tauR = tauRall + (size_t)b * C;
+    cx<T> *Vr = Vrall + (size_t)b * C * C;
+    cx<T> *Xw = Xwall + (size_t)b * R * NB;
+    cx<T> *Yw = Ywall + (size_t)b * C * NB;
+    cx<T> *Vw = Vwall + (size_t)b * C * NB;
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+
+    for (int i = 0; i < nbp; ++i) {
+        const int j = j0 + i;
+        // ---- row-j coefficients of the deferred updates
+        if (t < i) {
+            cy[t] = conj(cvt<double>(Yw[j + (size_t)t * C]));
+            cv[t] = conj(cvt<double>(Vw[j + (size_t)t * C]));
+            uj[t] = cvt<double>(Y[j + (size_t)(j0 + t) * ldy]);
+            xj[t] = cvt<double>(Xw[j + (size_t)t * R]);
+        }
+        __syncthreads();
+        // ---- column j of A_i (rows >= j) and its left reflector
+        double p = 0;
+        for (int r = j + t; r < R; r += BT) {
+            cx<double> a = cvt<double>(Y[r + (size_t)j * ldy]);
+            for (int k = 0; k < i; ++k)
+                a = a - cvt<double>(Y[r + (size_t)(j0 + k) * ldy]) * cy[k] -
+                    cvt<double>(Xw[r + (size_t)k * R]) * cv[k];
+            u[r] = cvt<T>(a);
+            if (r > j) p += abs2(a);
+        }
+        const double xn2 = block_sum(p, red);  // syncs: u visible
+        const Refl rf = larfg(cvt<double>(u[j]), xn2);
+        const cx<T> sc = cvt<T>(rf.scale);
+        __syncthreads();  // u[j] read by all before it is overwritten
+        for (int r = j + 1 + t; r < R; r += BT) {
+            const cx<T> uv = u[r] * sc;
+            u[r] = uv;
+            Y[r + (size_t)j * ldy] = uv;
+        }
+        if (t == 0) {
+            u[j] = cx<T>{1, 0};
+            Y[j + (size_t)j * ldy] = cx<T>{(T)rf.beta, 0};
+            d[j] = rf.beta;
+            tauL[j] = cvt<T>(rf.tau);
+        }
+        __syncthreads();
+        if (j == C - 1) {
+            if (t == 0) e[j] = 0.0;
+            break;
+        }
+        // ---- w = u^H [A trailing | U | Xw] (one wave per column)
+        const int nt = C - 1 - j;
+        const int nvc = nt + 2 * i;
+        for (int vc = wv; vc < nvc; vc += NW) {
+            const cx<T> *col;
+            if (vc < nt)
+                col = Y + (size_t)(j + 1 + vc) * ldy;
+            else if (vc < nt + i)
+                col = Y + (size_t)(j0 + vc - nt) * ldy;
+            else
+                col = Xw + (size_t)(vc - nt - i) * R;
+            double ar0 = 0, ai0 = 0, ar1 = 0, ai1 = 0;
+            int r = j + lane;
+            for (; r + 64 < R; r += 128) {
+                const cx<T> y0 = col[r], y1 = col[r + 64];
+                const cx<T> u0 = u[r], u1 = u[r + 64];
+                ar0 += (double)u0.re * y0.re + (double)u0.im * y0.im;
+                ai0 += (double)u0.re * y0.im - (double)u0.im * y0.re;
+                ar1 += (double)u1.re * y1.re + (double)u1.im * y1.im;
+                ai1 += (double)u1.re * y1.im - (double)u1.im * y1.re;
+            }
+            if (r < R) {
+                const cx<T> y0 = col[r], u0 = u[r];
+                ar0 += (double)u0.re * y0.re + (double)u0.im * y0.im;
+                ai0 += (double)u0.re * y0.im - (double)u0.im * y0.re;
+            }
+            const double ar = wave_sum(ar0 + ar1), ai = wave_sum(ai0 + ai1);
+            if (lane == 0) {
+                const cx<double> w{ar, ai};
+                if (vc < nt)
+                    v[j + 1 + vc] = cvt<T>(w);  // raw u^H A, corrected below
+                else if (vc < nt + i)
+                    pu[vc - nt] = w;
+                else
+                    pq[vc - nt - i] = w;
+            }
+        }
+        __syncthreads();
+        // ---- y = tau conj(u^H A_i) and row j of (I - conj(tau) u u^H) A_i
+        const cx<double> tau = rf.tau;
+        double pr = 0;
+        for (int c = j + 1 + t; c < C; c += BT) {
+            cx<double> w = cvt<double>(v[c]);
+            cx<double> rowc = cvt<double>(Y[j + (size_t)c * ldy]);
+            for (int k = 0; k < i; ++k) {
+                const cx<double> cyk = conj(cvt<double>(Yw[c + (size_t)k * C]));
+                const cx<double> cvk = conj(cvt<double>(Vw[c + (size_t)k * C]));
+                w = w - pu[k] * cyk - pq[k] * cvk;
+                rowc = rowc - uj[k] * cyk - xj[k] * cvk;
+            }
+            const cx<double> y = tau * conj(w);
+            Yw[c + (size_t)i * C] = cvt<T>(y);
+            rowc = rowc - conj(y);
+            v[c] = cvt<T>(rowc);
+            if (c >= j + 2) pr += abs2(rowc);
+        }
+        const double xr2 = block_sum(pr, red);  // syncs: v, Yw[:, i] visible
+        const Refl rr = larfg(conj(cvt<double>(v[j + 1])), xr2);
+        const cx<T> scr = cvt<T>(rr.scale);
+        __syncthreads();  // v[j+1] read by all before it is overwritten
+        for (int c = j + 1 + t; c < C; c += BT) {
+            const cx<T> vv = (c == j + 1) ? cx<T>{1, 0} : conj(v[c]) * scr;
+            v[c] = vv;
+            Vr[(size_t)j * C + c] = vv;
+            Vw[c + (size_t)i * C] = vv;
+            Y[j + (size_t)c * ldy] = (c == j + 1) ? cx<T>{(T)rr.beta, 0} : vv;
+        }
+        if (t == 0) {
+            e[j] = rr.beta;
+            tauR[j] = cvt<T>(rr.tau);
+        }
+        __syncthreads();
+        // ---- py[k] = Yw_k^H v (k <= i), pv[k] = Vw_k^H v (k < i), over c > j
+        for (int q = wv; q < 2 * i + 1; q += NW) {
+            const cx<T> *col = (q <= i) ? Yw + (size_t)q * C : Vw + (size_t)(q - i - 1) * C;
+            double ar = 0, ai = 0;
+            for (int c = j + 1 + lane; c < C; c += 64) {
+                const cx<T> a = col[c], vv = v[c];
+                ar += (double)a.re * vv.re + (double)a.im * vv.im;
+                ai += (double)a.re * vv.im - (double)a.im * vv.re;
+            }
+            ar = wave_sum(ar);
+            ai = wave_sum(ai);
+            if (lane == 0) {
+                if (q <= i)
+                    py[q] = cx<double>{ar, ai};
+                else
+                    pv[q - i - 1] = cx<double>{ar, ai};
+            }
+        }
+        __syncthreads();
+        // ---- x = sigma (A v - U py - Xw pv - u py_i), rows r > j
+        const cx<double> sig = rr.tau;
+        const int r0 = j + 1;
+        const int nrb = (R - r0 + 63) / 64;
+        if (nrb >= NW) {
+            for (int rb = wv; rb < nrb; rb += NW) {
+                const int r = r0 + rb * 64 + lane;
+                if (r >= R) continue;
+                cx<double> acc0{0, 0}, acc1{0, 0};
+                int c = j + 1;
+                for (; c + 1 < C; c += 2) {
+                    const cx<T> a0 = Y[r + (size_t)c * ldy], a1 = Y[r + (size_t)(c + 1) * ldy];
+                    acc0 = acc0 + cvt<double>(a0) * cvt<double>(v[c]);
+                    acc1 = acc1 + cvt<double>(a1) * cvt<double>(v[c + 1]);
+                }
+                if (c < C) acc0 = acc0 + cvt<double>(Y[r + (size_t)c * ldy]) * cvt<double>(v[c]);
+                cx<double> acc = acc0 + acc1;
+                for (int k = 0; k < i; ++k)
+                    acc = acc - cvt<double>(Y[r + (size_t)(j0 + k) * ldy]) * py[k] -
+                          cvt<double>(Xw[r + (size_t)k * R]) * pv[k];
+                acc = acc - cvt<double>(u[r]) * py[i];
+                Xw[r + (size_t)i * R] = cvt<T>(sig * acc);
+            }
+        } else {
+            // fewer row blocks than waves: split the trailing columns into
+            // ns slices and reduce the partial sums through LDS
+            const int ns = NW / nrb;
+            const int rb = wv % nrb, sl = wv / nrb;
+            const int r = r0 + rb * 64 + lane;
+            cx<double> acc{0, 0};
+            if (sl < ns && r < R) {
+                const int chunk = (nt + ns - 1) / ns;
+                const int c0 = j + 1 + sl * chunk, c1 = min(C, c0 + chunk);
+                cx<double> acc1{0, 0};
+                int c = c0;
+                for (; c + 1 < c1; c += 2) {
+                    const cx<T> a0 = Y[r + (size_t)c * ldy], a1 = Y[r + (size_t)(c + 1) * ldy];
+                    acc = acc + cvt<double>(a0) * cvt<double>(v[c]);
+                    acc1 = acc1 + cvt<double>(a1) * cvt<double>(v[c + 1]);
+                }
+                if (c < c1) acc = acc + cvt<double>(Y[r + (size_t)c * ldy]) * cvt<double>(v[c]);
+                acc = acc + acc1;
+                if (sl == 0) {
+                    for (int k = 0; k < i; ++k)
+                        acc = acc - cvt<double>(Y[r + (size_t)(j0 + k) * ldy]) * py[k] -
+                              cvt<double>(Xw[r + (size_t)k * R]) * pv[k];
+                    acc = acc - cvt<double>(u[r]) * py[i];
+                }
+            }
+            part[wv * 64 + lane] = acc;
+            __syncthreads();
+            if (wv < nrb && r < R) {
+                cx<double> s{0, 0};
+                for (int q = 0; q < ns; ++q) s = s + part[(q * nrb + wv) * 64 + lane];
+                Xw[r + (size_t)i * R] = cvt<T>(sig * s);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+size_t gebrd_lds_bytes(int R, int C, size_t elem) {
+    return ((size_t)NW * 64 + 8 * NB + 1) * sizeof(cx<double>) + NW * sizeof(double) +
+           ((size_t)R + C) * elem;
+}
+
+size_t gebrd_work_elems(int R, int C) { return ((size_t)R + 2 * (size_t)C) * NB; }
+
+template <typename T>
+ia_status launch_gebrd(int R, int C, cx<T> *Y, int ldy, size_t sY, double *d, double *e,
+                       cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, cx<T> *work, int batch,
+                       hipStream_t s) {
+    if (R < C) return fail(IA_ERR_ARG, "bidiagonalisation needs R >= C (%d x %d)", R, C);
+    const size_t lds = gebrd_lds_bytes(R, C, sizeof(cx<T>));
+    if (lds > 160 * 1024)
+        return fail(IA_ERR_ARG, "bidiagonalisation of %d x %d exceeds the on-chip limit", R, C);
+    IA_HIP(hipFuncSetAttribute((const void *)k_labrd<T>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    cx<T> *Xw = work;
+    cx<T> *Yw = Xw + (size_t)batch * R * NB;
+    cx<T> *Vw = Yw + (size_t)batch * C * NB;
+    const cx<T> mone{-1, 0}, one{1, 0};
+    for (int j0 = 0; j0 < C; j0 += NB) {
+        const int nbp = std::min(NB, C - j0);
+        hipLaunchKernelGGL(k_labrd<T>, dim3(batch), dim3(BT), lds, s, R, C, j0, nbp, Y, ldy, sY,
+                           d, e, tauL, tauR, Vr, Xw, Yw, Vw);
+        const int s0 = j0 + nbp;
+        if (s0 < C) {
+            const int M = R - s0, N = C - s0;
+            cx<T> *A = Y + s0 + (size_t)s0 * ldy;
+            // A -= U Yw^H + Xw Vw^H on rows/columns s0..
+            launch_gemm<T>('N', 'C', M, N, nbp, mone, Y + s0 + (size_t)j0 * ldy, ldy, sY, Yw + s0,
+                           C, (size_t)C * NB, one, A, ldy, sY, batch, s);
+            launch_gemm<T>('N', 'C', M, N, nbp, mone, Xw + s0, R, (size_t)R * NB, Vw + s0, C,
+                           (size_t)C * NB, one, A, ldy, sY, batch, s);
+        }
+    }
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
+template ia_status launch_gebrd<double>(int, int, cx<double> *, int, size_t, double *, double *,
+                                        cx<double> *, cx<double> *, cx<double> *, cx<double> *,
+                                        int, hipStream_t);
+template ia_status launch_gebrd<float>(int, int, cx<float> *, int, size_t, double *, double *,
+                                       cx<float> *, cx<float> *, cx<float> *, cx<float> *, int,
+                                       hipStream_t);
+
+}  // namespace ia

@@ -154,8 +154,16 @@ template <typename T>
 ia_status launch_geqr2(int R, int C, cx<T> *A, int lda, size_t strideA, cx<T> *tau, int batch,
                        hipStream_t s);
 
-// SVD building blocks (svd.hip)
+// SVD building blocks (svd.hip, gebrd.hip)
 size_t bidiag_lds_bytes(int R, int C, size_t elem);
+// panel-blocked bidiagonalisation (same outputs as launch_bidiag); work holds
+// gebrd_work_elems(R, C) complex T per batch item
+size_t gebrd_lds_bytes(int R, int C, size_t elem);
+size_t gebrd_work_elems(int R, int C);
+template <typename T>
+ia_status launch_gebrd(int R, int C, cx<T> *Y, int ldy, size_t sY, double *d, double *e,
+                       cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, cx<T> *work, int batch,
+                       hipStream_t s);
 template <typename T>
 void launch_bidiag(int R, int C, cx<T> *Y, int ldy, size_t strideY, double *d, double *e,
                    cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, int batch, hipStream_t s);
@@ -251,7 +259,7 @@ inline ia_status host_out(T **dst, const void *dsrc, size_t count) {
 // bidiagonal singular vectors QBt/PBt (vector i in row i).
 template <typename T>
 struct SvdWork {
-    DevBuf d, e, sig, tauL, tauR, Vr, QBt, PBt, work;
+    DevBuf d, e, sig, tauL, tauR, Vr, QBt, PBt, work, gw;
     int R = 0, C = 0, batch = 0, sub = 1;
     ia_status alloc(int R_, int C_, int batch_) {
         R = R_;
@@ -266,9 +274,19 @@ struct SvdWork {
         IA_TRY(Vr.alloc(bc * C * sizeof(cx<T>)));
         IA_TRY(QBt.alloc(bc * C * sizeof(double)));
         IA_TRY(PBt.alloc(bc * C * sizeof(double)));
-        // inverse-iteration scratch bounded to ~1 GiB
+        IA_TRY(gw.alloc(gebrd_work_elems(R, C) * batch * sizeof(cx<T>)));
+        // inverse-iteration scratch: as many frequencies per launch as a
+        // quarter of the free device memory holds (1..32 GiB), so one launch
+        // covers a whole 256-frequency batch (one thread per singular value
+        // needs >= 1024 * 256 threads to fill the device)
         size_t per = inviter_work_bytes(C, 1);
-        sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)batch, ((size_t)1 << 30) / per));
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            (void)hipGetLastError();
+            fr = (size_t)4 << 30;
+        }
+        const size_t cap = std::min<size_t>((size_t)32 << 30, std::max<size_t>((size_t)1 << 30, fr / 4));
+        sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)batch, cap / per));
         IA_TRY(work.alloc(inviter_work_bytes(C, sub)));
         return IA_OK;
     }
@@ -277,13 +295,10 @@ struct SvdWork {
 template <typename T>
 inline ia_status svd_factor(SvdWork<T> &w, cx<T> *Y, hipStream_t s) {
     const int R = w.R, C = w.C, batch = w.batch;
-    size_t lds = bidiag_lds_bytes(R, C, sizeof(cx<T>));
-    if (lds > 160 * 1024)
-        return fail(IA_ERR_ARG, "SVD of %d x %d exceeds the on-chip bidiagonalisation limit", R, C);
-    launch_bidiag<T>(R, C, Y, R, (size_t)R * C, w.d.template as<double>(), w.e.template as<double>(),
-                     w.tauL.template as<cx<T>>(), w.tauR.template as<cx<T>>(),
-                     w.Vr.template as<cx<T>>(), batch, s);
-    IA_HIP(hipGetLastError());
+    IA_TRY(launch_gebrd<T>(R, C, Y, R, (size_t)R * C, w.d.template as<double>(),
+                           w.e.template as<double>(), w.tauL.template as<cx<T>>(),
+                           w.tauR.template as<cx<T>>(), w.Vr.template as<cx<T>>(),
+                           w.gw.template as<cx<T>>(), batch, s));
     launch_bisect(C, w.d.template as<double>(), w.e.template as<double>(), w.sig.template as<double>(),
                   batch, s);
     IA_HIP(hipGetLastError());
