@@ -60,7 +60,8 @@ ia_status ia_set_air(double c, double rho);
 ia_status ia_get_air(double *c, double *rho);
 /* Number of visible HIP devices (0 when no GPU). */
 int ia_device_count(void);
-/* Release cached device state of this library on the current device. */
+/* Wait for outstanding device work of this library on the current device
+ * (no device state is cached between calls, so there is nothing to free). */
 void ia_release(void);
 
 /* -------------------------------------------------- ESM (esm.cuh) */
@@ -185,6 +186,11 @@ typedef struct {
     ia_reg_params reg;
     double area;              /* S in W = 1/2 Re(v^H p) S/m                */
     int chunk;                /* frequencies per device batch (0 = auto)   */
+    /* IA_PREC_MIXED only: a frequency whose selection is nearly tied (the
+     * GCV / L-curve value of the runner-up parameter within this relative
+     * gap of the chosen one) is solved again in FP64 and its outputs
+     * replaced.  0 = 2e-5, < 0 = never. */
+    double recheck_gap;
 } ia_sweep_desc;
 
 typedef struct {
@@ -196,6 +202,9 @@ typedef struct {
     ia_z *velocity;           /* [n_freq * m] surface normal velocity      */
     ia_z *pressure;           /* [n_freq * m] surface pressure             */
     int *status;              /* [n_freq] ia_status per frequency          */
+    double *selection_gap;    /* [n_freq] relative GCV / L-curve gap between
+                                 the chosen and the runner-up parameter (k
+                                 for TSVD); small = nearly tied choice     */
 } ia_sweep_out;
 
 /* Solve an ESM sweep: for each frequency f, assemble D, M_hq at k[f],
@@ -302,10 +311,15 @@ ia_status ia_regsolve(int rows, int cols, int batch, const ia_z *a,
                       ia_precision precision, ia_z *x, double *lambda,
                       int *index, int *k_tsvd, double *sigma_max);
 
-/* Phase timings (ms) of the last sweep on this thread. */
+/* Phase timings (ms) of the last sweep on this thread, plus the summed
+ * HIP-event duration, algorithmic bytes and launch count of the panel
+ * bidiagonalisation kernel (the sweep's dominant kernel). */
 typedef struct {
     double assemble, factor, solve_he, bidiag, bisect, vectors, select,
            post, total;
+    double bidiag_kernel_ms, bidiag_kernel_bytes, bidiag_kernel_launches;
+    double rechecked;         /* frequencies solved again in FP64 (mixed)  */
+    double recheck_ms;        /* time of those FP64 solves                 */
 } ia_stats;
 ia_status ia_last_stats(ia_stats *st);
 

@@ -412,25 +412,32 @@ class SweepResult:
     pressure: np.ndarray   # (F, m)
     status: np.ndarray
     stats: dict
+    selection_gap: np.ndarray | None = None
 
 
 def _sweep_outputs(F, m):
     return dict(lam=np.zeros(F), lam_index=np.zeros(F, np.int32), k_tsvd=np.zeros(F, np.int32),
                 power=np.zeros(F), sigma_max=np.zeros(F), velocity=np.zeros((F, m), np.complex128),
-                pressure=np.zeros((F, m), np.complex128), status=np.zeros(F, np.int32))
+                pressure=np.zeros((F, m), np.complex128), status=np.zeros(F, np.int32),
+                gap=np.zeros(F))
 
 
-def _sweep_struct(out):
-    return _lib.ia_sweep_out(*(out[n].ctypes.data for n in
-                               ("lam", "lam_index", "k_tsvd", "power", "sigma_max", "velocity",
-                                "pressure", "status")))
+def _sweep_struct(out, with_gap=False):
+    o = _lib.ia_sweep_out(*(out[n].ctypes.data for n in
+                            ("lam", "lam_index", "k_tsvd", "power", "sigma_max", "velocity",
+                             "pressure", "status")))
+    if with_gap:
+        o.selection_gap = out["gap"].ctypes.data
+    return o
 
 
 def esm_sweep(k, r_s, src, r_h, b, area, normals=None, precision="fp64", filter="tikhonov",
-              select="gcv", n_lambda=50, decades=8.0, lam=0.0, kfix=0, chunk=0):
+              select="gcv", n_lambda=50, decades=8.0, lam=0.0, kfix=0, chunk=0, recheck_gap=0.0):
     """Batched ESM sweep on the GPU: for every wavenumber k[f], assemble, solve
     H_e v = b[f] with the selected regularisation, reconstruct the surface
-    pressure and sound power.  Returns a ``SweepResult``."""
+    pressure and sound power.  Returns a ``SweepResult``.  In the mixed form a
+    frequency whose parameter choice is within `recheck_gap` (relative; 0 =
+    2e-5, negative = never) of a tie is solved again in FP64."""
     k = np.ascontiguousarray(k, dtype=np.float64).ravel()
     F = len(k)
     rs, sq, rh = soa(r_s), soa(src), soa(r_h)
@@ -439,9 +446,9 @@ def esm_sweep(k, r_s, src, r_h, b, area, normals=None, precision="fp64", filter=
     nr = soa(normals) if normals is not None else None
     desc = _lib.ia_sweep_desc(m, l, F, 1 if precision == "mixed" else 0,
                               _reg(filter, select, n_lambda, decades, lam, kfix), float(area),
-                              int(chunk))
+                              int(chunk), float(recheck_gap))
     out = _sweep_outputs(F, m)
-    o = _sweep_struct(out)
+    o = _sweep_struct(out, with_gap=True)
     check(lib().ia_esm_sweep(C.byref(desc), _lib.dptr(k), _lib.dptr(rs), _lib.dptr(sq),
                              _lib.dptr(nr) if nr is not None else None, _lib.dptr(rh),
                              _lib.zptr(b), C.byref(o)))
@@ -449,4 +456,5 @@ def esm_sweep(k, r_s, src, r_h, b, area, normals=None, precision="fp64", filter=
     check(lib().ia_last_stats(C.byref(st)))
     stats = {n: getattr(st, n) for n, _ in _lib.ia_stats._fields_}
     return SweepResult(out["lam"], out["lam_index"], out["k_tsvd"], out["power"],
-                       out["sigma_max"], out["velocity"], out["pressure"], out["status"], stats)
+                       out["sigma_max"], out["velocity"], out["pressure"], out["status"], stats,
+                       out["gap"])

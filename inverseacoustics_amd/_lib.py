@@ -30,13 +30,14 @@ class ia_reg_params(C.Structure):
 
 class ia_sweep_desc(C.Structure):
     _fields_ = [("m", C.c_int), ("l", C.c_int), ("n_freq", C.c_int), ("precision", C.c_int),
-                ("reg", ia_reg_params), ("area", C.c_double), ("chunk", C.c_int)]
+                ("reg", ia_reg_params), ("area", C.c_double), ("chunk", C.c_int),
+                ("recheck_gap", C.c_double)]
 
 
 class ia_sweep_out(C.Structure):
     _fields_ = [("lambda_", C.c_void_p), ("lambda_index", C.c_void_p), ("k_tsvd", C.c_void_p),
                 ("power", C.c_void_p), ("sigma_max", C.c_void_p), ("velocity", C.c_void_p),
-                ("pressure", C.c_void_p), ("status", C.c_void_p)]
+                ("pressure", C.c_void_p), ("status", C.c_void_p), ("selection_gap", C.c_void_p)]
 
 
 class ia_hels_desc(C.Structure):
@@ -53,7 +54,8 @@ class ia_ibem_desc(C.Structure):
 class ia_stats(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("assemble", "factor", "solve_he", "bidiag", "bisect", "vectors", "select",
-                 "post", "total")]
+                 "post", "total", "bidiag_kernel_ms", "bidiag_kernel_bytes",
+                 "bidiag_kernel_launches", "rechecked", "recheck_ms")]
 
 
 _zp = C.POINTER(ia_z)

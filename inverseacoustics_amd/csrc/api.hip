@@ -88,7 +88,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
                                 const double *src, const double *nrm, const double *r_h,
                                 const cx<double> *b, double *lam, int *lidx, int *ktsvd,
                                 double *power, double *smax, cx<double> *vel, cx<double> *pres,
-                                int *status, hipStream_t s) {
+                                int *status, double *gap, hipStream_t s) {
     const int m = D.m, l = D.l, F = D.n_freq;
     if (l > m) return fail(IA_ERR_ARG, "ia_esm_sweep: l (%d) > m (%d) is not supported", l, m);
     int chunk = D.chunk > 0 ? D.chunk : std::min(F, 256);
@@ -99,7 +99,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
                D.reg.k, D.reg.decades > 0 ? D.reg.decades : 8.0, D.reg.lambda};
     if (prm.n_lambda > 512) return fail(IA_ERR_ARG, "n_lambda must be <= 512");
 
-    DevBuf bD, bY, bipiv, binfo, bc, bbeta, bphi, bx, bq, bps, bpsd, bn2, blam, bidx, bk, bW;
+    DevBuf bD, bY, bipiv, binfo, bc, bbeta, bphi, bx, bq, bps, bpsd, bn2, bcn2, blam, bidx, bk, bW;
     IA_TRY(bD.alloc(sizeof(cx<T>) * (size_t)chunk * m * m));
     IA_TRY(bY.alloc(sizeof(cx<T>) * (size_t)chunk * m * l));
     IA_TRY(bipiv.alloc(sizeof(int) * (size_t)chunk * m));
@@ -112,10 +112,12 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
     IA_TRY(bps.alloc(sizeof(cx<T>) * (size_t)chunk * m));
     IA_TRY(bpsd.alloc(sizeof(cx<double>) * (size_t)chunk * m));
     IA_TRY(bn2.alloc(sizeof(double) * (size_t)chunk));
+    IA_TRY(bcn2.alloc(sizeof(double) * (size_t)chunk));
     SvdWork<T> sw;
     IA_TRY(sw.alloc(R, C, chunk));
 
     Timer tm(s);
+    KernelClock clk;
     ia_stats st{};
     for (int f0 = 0; f0 < F; f0 += chunk) {
         const int nf = std::min(chunk, F - f0);
@@ -138,7 +140,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
         IA_TRY(launch_gebrd<T>(R, C, Y, R, (size_t)R * C, sw.d.template as<double>(),
                                sw.e.template as<double>(), sw.tauL.template as<cx<T>>(),
                                sw.tauR.template as<cx<T>>(), sw.Vr.template as<cx<T>>(),
-                               sw.gw.template as<cx<T>>(), nf, s));
+                               sw.gw.template as<cx<T>>(), nf, s, &clk));
         tm.mark();  // 4
         launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(), nf, s);
         tm.mark();  // 5
@@ -151,11 +153,13 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
         launch_norm2<double>(l, bc.as<cx<double>>(), l, bn2.as<double>(), nf, s);
         launch_apply_right<T>(true, C, sw.Vr.template as<cx<T>>(), sw.tauR.template as<cx<T>>(), bc.as<cx<double>>(),
                               C, 1, nf, s);
+        launch_norm2<double>(C, bc.as<cx<double>>(), C, bcn2.as<double>(), nf, s);
         launch_real_matvec(C, sw.QBt.template as<double>(), 0, bc.as<cx<double>>(), C,
                            bbeta.as<cx<double>>(), C, nf, s);
         launch_select(C, l, sw.sig.template as<double>(), bbeta.as<cx<double>>(), bn2.as<double>(), prm,
                       lam ? lam + f0 : nullptr, lidx ? lidx + f0 : nullptr,
-                      ktsvd ? ktsvd + f0 : nullptr, bphi.as<cx<double>>(), nf, s);
+                      ktsvd ? ktsvd + f0 : nullptr, bphi.as<cx<double>>(), nf, s,
+                      gap ? gap + f0 : nullptr, bcn2.as<double>(), svd_floor_rel<T>());
         // x = U_b [P_B phi; 0]
         launch_real_matvec(C, sw.PBt.template as<double>(), 1, bphi.as<cx<double>>(), C,
                            bc.as<cx<double>>(), C, nf, s);
@@ -208,6 +212,9 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
         st.post += tm.ms(7, 8);
         st.total += tm.ms(0, 8);
     }
+    st.bidiag_kernel_ms = clk.ms();
+    st.bidiag_kernel_bytes = clk.bytes;
+    st.bidiag_kernel_launches = clk.launches;
     g_stats = st;
     return IA_OK;
 }
@@ -278,14 +285,92 @@ ia_status ia_esm_sweep_device(const ia_sweep_desc *desc, const double *k, const 
     ia_sweep_out o{};
     if (out) o = *out;
     hipStream_t s = (hipStream_t)stream;
-    if (desc->precision == IA_PREC_MIXED)
-        return esm_sweep_impl<float>(*desc, k, r_s, src, nrm, r_h, (const cx<double> *)b, o.lambda,
-                                     o.lambda_index, o.k_tsvd, o.power, o.sigma_max,
-                                     (cx<double> *)o.velocity, (cx<double> *)o.pressure, o.status,
-                                     s);
-    return esm_sweep_impl<double>(*desc, k, r_s, src, nrm, r_h, (const cx<double> *)b, o.lambda,
-                                  o.lambda_index, o.k_tsvd, o.power, o.sigma_max,
-                                  (cx<double> *)o.velocity, (cx<double> *)o.pressure, o.status, s);
+    if (desc->precision != IA_PREC_MIXED)
+        return esm_sweep_impl<double>(*desc, k, r_s, src, nrm, r_h, (const cx<double> *)b,
+                                      o.lambda, o.lambda_index, o.k_tsvd, o.power, o.sigma_max,
+                                      (cx<double> *)o.velocity, (cx<double> *)o.pressure,
+                                      o.status, o.selection_gap, s);
+    const int F = desc->n_freq, m = desc->m, l = desc->l;
+    const double tau = desc->recheck_gap == 0.0 ? 2e-5 : desc->recheck_gap;
+    DevBuf dgap;
+    double *gap = o.selection_gap;
+    if (!gap && tau > 0) {
+        IA_TRY(dgap.alloc(sizeof(double) * F));
+        gap = dgap.as<double>();
+    }
+    IA_TRY(esm_sweep_impl<float>(*desc, k, r_s, src, nrm, r_h, (const cx<double> *)b, o.lambda,
+                                 o.lambda_index, o.k_tsvd, o.power, o.sigma_max,
+                                 (cx<double> *)o.velocity, (cx<double> *)o.pressure, o.status,
+                                 gap, s));
+    if (!(tau > 0)) return IA_OK;
+    // ---- certified selection: nearly tied frequencies again in FP64
+    ia_stats st = g_stats;
+    std::vector<double> hgap(F);
+    IA_HIP(hipMemcpyAsync(hgap.data(), gap, sizeof(double) * F, hipMemcpyDeviceToHost, s));
+    IA_HIP(hipStreamSynchronize(s));
+    std::vector<int> L;
+    for (int f = 0; f < F; ++f)
+        if (!(hgap[f] >= tau)) L.push_back(f);  // NaN gaps are rechecked too
+    if (L.empty()) return IA_OK;
+    const int nL = (int)L.size();
+    hipEvent_t e0, e1;
+    IA_HIP(hipEventCreate(&e0));
+    IA_HIP(hipEventCreate(&e1));
+    IA_HIP(hipEventRecord(e0, s));
+    DevBuf kk, bb, tl, ti, tk, tw, tsm, tv, tp, tst, tg;
+    IA_TRY(kk.alloc(sizeof(double) * nL));
+    IA_TRY(bb.alloc(sizeof(ia_z) * (size_t)nL * l));
+    for (int q = 0; q < nL; ++q) {
+        IA_HIP(hipMemcpyAsync(kk.as<double>() + q, k + L[q], sizeof(double), hipMemcpyDefault, s));
+        IA_HIP(hipMemcpyAsync(bb.as<ia_z>() + (size_t)q * l, b + (size_t)L[q] * l,
+                              sizeof(ia_z) * l, hipMemcpyDefault, s));
+    }
+    IA_TRY(tl.alloc(sizeof(double) * nL));
+    IA_TRY(ti.alloc(sizeof(int) * nL));
+    IA_TRY(tk.alloc(sizeof(int) * nL));
+    IA_TRY(tw.alloc(sizeof(double) * nL));
+    IA_TRY(tsm.alloc(sizeof(double) * nL));
+    IA_TRY(tv.alloc(sizeof(ia_z) * (size_t)nL * m));
+    IA_TRY(tp.alloc(sizeof(ia_z) * (size_t)nL * m));
+    IA_TRY(tst.alloc(sizeof(int) * nL));
+    IA_TRY(tg.alloc(sizeof(double) * nL));
+    ia_sweep_desc d2 = *desc;
+    d2.n_freq = nL;
+    d2.precision = IA_PREC_FP64;
+    d2.chunk = 0;
+    IA_TRY(esm_sweep_impl<double>(d2, kk.as<double>(), r_s, src, nrm, r_h, bb.as<cx<double>>(),
+                                  tl.as<double>(), ti.as<int>(), tk.as<int>(), tw.as<double>(),
+                                  tsm.as<double>(), tv.as<cx<double>>(), tp.as<cx<double>>(),
+                                  tst.as<int>(), tg.as<double>(), s));
+    for (int q = 0; q < nL; ++q) {
+        const int f = L[q];
+        if (o.lambda) IA_HIP(hipMemcpyAsync(o.lambda + f, tl.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
+        if (o.lambda_index) IA_HIP(hipMemcpyAsync(o.lambda_index + f, ti.as<int>() + q, sizeof(int), hipMemcpyDefault, s));
+        if (o.k_tsvd) IA_HIP(hipMemcpyAsync(o.k_tsvd + f, tk.as<int>() + q, sizeof(int), hipMemcpyDefault, s));
+        if (o.power) IA_HIP(hipMemcpyAsync(o.power + f, tw.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
+        if (o.sigma_max) IA_HIP(hipMemcpyAsync(o.sigma_max + f, tsm.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
+        if (o.status) IA_HIP(hipMemcpyAsync(o.status + f, tst.as<int>() + q, sizeof(int), hipMemcpyDefault, s));
+        if (o.selection_gap)
+            IA_HIP(hipMemcpyAsync(o.selection_gap + f, tg.as<double>() + q, sizeof(double),
+                                  hipMemcpyDefault, s));
+        if (o.velocity)
+            IA_HIP(hipMemcpyAsync(o.velocity + (size_t)f * m, tv.as<ia_z>() + (size_t)q * m,
+                                  sizeof(ia_z) * m, hipMemcpyDefault, s));
+        if (o.pressure)
+            IA_HIP(hipMemcpyAsync(o.pressure + (size_t)f * m, tp.as<ia_z>() + (size_t)q * m,
+                                  sizeof(ia_z) * m, hipMemcpyDefault, s));
+    }
+    IA_HIP(hipEventRecord(e1, s));
+    IA_HIP(hipStreamSynchronize(s));
+    float rms = 0;
+    (void)hipEventElapsedTime(&rms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    st.rechecked = nL;
+    st.recheck_ms = rms;
+    st.total += rms;
+    g_stats = st;
+    return IA_OK;
 }
 
 ia_status ia_esm_sweep(const ia_sweep_desc *desc, const double *k, const double *r_s,
@@ -316,7 +401,7 @@ ia_status ia_esm_sweep(const ia_sweep_desc *desc, const double *k, const double 
     ia_sweep_out o{};
     if (out) o = *out;
     // device staging for outputs
-    DevBuf ol, oi, ok, ow, osm, ov, op, ost;
+    DevBuf ol, oi, ok, ow, osm, ov, op, ost, og;
     ia_sweep_out dout{};
     if (o.lambda) { IA_TRY(ol.alloc(sizeof(double) * F)); dout.lambda = ol.as<double>(); }
     if (o.lambda_index) { IA_TRY(oi.alloc(sizeof(int) * F)); dout.lambda_index = oi.as<int>(); }
@@ -326,6 +411,7 @@ ia_status ia_esm_sweep(const ia_sweep_desc *desc, const double *k, const double 
     if (o.velocity) { IA_TRY(ov.alloc(sizeof(ia_z) * (size_t)F * m)); dout.velocity = ov.as<ia_z>(); }
     if (o.pressure) { IA_TRY(op.alloc(sizeof(ia_z) * (size_t)F * m)); dout.pressure = op.as<ia_z>(); }
     if (o.status) { IA_TRY(ost.alloc(sizeof(int) * F)); dout.status = ost.as<int>(); }
+    if (o.selection_gap) { IA_TRY(og.alloc(sizeof(double) * F)); dout.selection_gap = og.as<double>(); }
     IA_TRY(ia_esm_sweep_device(desc, dk.as<double>(), drs.as<double>(), dsrc.as<double>(),
                                nrm ? dnrm.as<double>() : nullptr, drh.as<double>(), db.as<ia_z>(),
                                &dout, s));
@@ -337,6 +423,8 @@ ia_status ia_esm_sweep(const ia_sweep_desc *desc, const double *k, const double 
     if (o.sigma_max) IA_HIP(hipMemcpy(o.sigma_max, dout.sigma_max, sizeof(double) * F, hipMemcpyDeviceToHost));
     if (o.velocity) IA_HIP(hipMemcpy(o.velocity, dout.velocity, sizeof(ia_z) * (size_t)F * m, hipMemcpyDeviceToHost));
     if (o.pressure) IA_HIP(hipMemcpy(o.pressure, dout.pressure, sizeof(ia_z) * (size_t)F * m, hipMemcpyDeviceToHost));
+    if (o.selection_gap)
+        IA_HIP(hipMemcpy(o.selection_gap, dout.selection_gap, sizeof(double) * F, hipMemcpyDeviceToHost));
     if (o.status) {
         IA_HIP(hipMemcpy(o.status, dout.status, sizeof(int) * F, hipMemcpyDeviceToHost));
         for (int f = 0; f < F; ++f) o.status[f] = o.status[f] ? IA_ERR_SINGULAR : IA_OK;

@@ -5,6 +5,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/ia.h"
 
@@ -160,10 +161,40 @@ size_t bidiag_lds_bytes(int R, int C, size_t elem);
 // gebrd_work_elems(R, C) complex T per batch item
 size_t gebrd_lds_bytes(int R, int C, size_t elem);
 size_t gebrd_work_elems(int R, int C);
+// HIP events around every launch of one kernel on its stream, with the
+// algorithmic bytes those launches move (read after the stream is synced)
+struct KernelClock {
+    std::vector<hipEvent_t> ev;
+    double bytes = 0;
+    int launches = 0;
+    KernelClock() = default;
+    KernelClock(const KernelClock &) = delete;
+    KernelClock &operator=(const KernelClock &) = delete;
+    ~KernelClock() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    void mark(hipStream_t s) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) == hipSuccess) {
+            (void)hipEventRecord(e, s);
+            ev.push_back(e);
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    double ms() const {
+        double t = 0;
+        for (size_t i = 0; i + 1 < ev.size(); i += 2) {
+            float x = 0;
+            if (hipEventElapsedTime(&x, ev[i], ev[i + 1]) == hipSuccess) t += x;
+        }
+        return t;
+    }
+};
 template <typename T>
 ia_status launch_gebrd(int R, int C, cx<T> *Y, int ldy, size_t sY, double *d, double *e,
                        cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, cx<T> *work, int batch,
-                       hipStream_t s);
+                       hipStream_t s, KernelClock *clk = nullptr);
 template <typename T>
 void launch_bidiag(int R, int C, cx<T> *Y, int ldy, size_t strideY, double *d, double *e,
                    cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, int batch, hipStream_t s);
@@ -190,9 +221,18 @@ struct RegDev {
 };
 // Given sigma (p), c = Q_B^T V_b^H b (p, complex), |b|^2, rows: selects the
 // parameter and writes phi (p, complex) such that x = V phi.
+// gap_out (optional): relative GCV / L-curve gap between the chosen and the
+// runner-up parameter (discrete-GCV k for TSVD), INFINITY when none.
+// Singular values below floor_rel * s_1 are pooled: their coordinates are
+// replaced by one component of energy |c|^2 - sum(resolved |beta|^2) at s = 0
+// (cnorm2 = |c|^2 of the vector with beta = Q^T c; NULL: sum |beta|^2).
+// floor_rel: 1e-12 for an FP64 SVD, 1e-6 for a complex64 one.
 void launch_select(int p, int rows, const double *sigma, const cx<double> *beta,
                    const double *bnorm2, RegDev prm, double *lam_out, int *idx_out, int *k_out,
-                   cx<double> *phi, int batch, hipStream_t s);
+                   cx<double> *phi, int batch, hipStream_t s, double *gap_out = nullptr,
+                   const double *cnorm2 = nullptr, double floor_rel = 1e-12);
+template <typename T>
+constexpr double svd_floor_rel() { return sizeof(T) == sizeof(double) ? 1e-12 : 1e-6; }
 
 // small helpers (util.hip)
 void launch_real_matvec(int n, const double *Q, int transpose, const cx<double> *x,

@@ -29,7 +29,8 @@ void launch_gram(int m, const double *X, const int *E, int ne, const int *adj_pt
 template <typename T>
 ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, size_t sA,
                            const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
-                           size_t sx, double *lam, int *idx, int *kts, double *smax, hipStream_t s);
+                           size_t sx, double *lam, int *idx, int *kts, double *smax, hipStream_t s,
+                           double *gap = nullptr);
 RegDev to_regdev(const ia_reg_params &p);
 ia_status check_reg(const ia_reg_params &p, int pmax);
 

@@ -98,10 +98,11 @@ __device__ double fminbnd(F fun, double ax, double bx, double tol_x) {
     return xf;
 }
 
-// lcfunG.m: negative L-curve curvature at one lambda (sequential over p)
+// lcfunG.m: negative L-curve curvature at one lambda (sequential over p);
+// pool = energy of the unresolved components (filter limit s -> 0: f = 0)
 __device__ double lcfun1(double la, int p, const double *s, const double *ab2, const double *axi2,
-                         bool tik, bool LS, double rhoLS2) {
-    double eta2 = 0, rho2 = 0, phi = 0, psi = 0, dphi = 0, dpsi = 0;
+                         bool tik, bool LS, double rhoLS2, double pool) {
+    double eta2 = 0, rho2 = pool, phi = 0, psi = 0, dphi = 0, dpsi = 0;
     for (int i = 0; i < p; ++i) {
         double f = tik ? (s[i] * s[i]) / (s[i] * s[i] + la * la) : s[i] / (s[i] + la);
         double cf = 1.0 - f;
@@ -126,8 +127,10 @@ __device__ double lcfun1(double la, int p, const double *s, const double *ab2, c
 
 __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *__restrict__ sall,
                                                const cx<double> *__restrict__ ball,
-                                               const double *__restrict__ bn2all, RegDev prm,
-                                               double *lam_out, int *idx_out, int *k_out,
+                                               const double *__restrict__ bn2all,
+                                               const double *__restrict__ cn2all, double floor_rel,
+                                               RegDev prm, double *lam_out, int *idx_out,
+                                               int *k_out, double *gap_out,
                                                cx<double> *__restrict__ phiall) {
     extern __shared__ double sh[];
     double *s = sh;            // p
@@ -136,30 +139,54 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
     double *axi2 = sh + 3 * p + 1;  // |beta/s|^2 (0 where s = 0), p
     __shared__ double Gv[512];
     __shared__ int Gi[512];
-    __shared__ double s_total, s_lam;
+    __shared__ double s_total, s_pool, s_lam, s_gap, s_gapk;
     __shared__ int s_k, s_idx;
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     const double *sg = sall + (size_t)b * p;
     const cx<double> *beta = ball + (size_t)b * p;
     cx<double> *phi = phiall + (size_t)b * p;
+    // Singular values below floor_rel * s_1 sit at the round-off floor of the
+    // SVD: their vectors are not resolved (a cluster's computed vectors need
+    // not be orthogonal), so their coordinates are pooled.  The pool's energy
+    // is |c|^2 minus the resolved part (c = the vector whose coordinates in
+    // the singular basis are beta, |c| exact under the Householder
+    // transforms) and it enters every sum with the filter's s -> 0 limit,
+    // which differs from the per-component weight by O((floor/lambda)^2).
+    // When that energy is within the orthogonality error of the computed
+    // vectors (below 1e-9 |c|^2 for an FP64 SVD, 1e-5 for complex64) the
+    // difference carries no information and the pooled coordinates' own sum
+    // is kept; with nothing below the floor there is no pool.
+    const double sfloor = floor_rel * sg[0];
     for (int i = t; i < p; i += blockDim.x) {
         s[i] = sg[i];
-        ab2[i] = abs2(beta[i]);
-        axi2[i] = (sg[i] > 0) ? ab2[i] / (sg[i] * sg[i]) : 0.0;
+        const bool res = !(sg[i] < sfloor);
+        ab2[i] = res ? abs2(beta[i]) : 0.0;
+        axi2[i] = (res && sg[i] > 0) ? ab2[i] / (sg[i] * sg[i]) : 0.0;
     }
     __syncthreads();
     if (t == 0) {
-        double acc = 0;
+        double acc = 0, allb = 0;
         tail[p] = 0;
         for (int i = p - 1; i >= 0; --i) {
             acc += ab2[i];
             tail[i] = acc;
         }
-        s_total = acc;
+        for (int i = 0; i < p; ++i) allb += abs2(beta[i]);
+        const double tot = cn2all ? cn2all[b] : allb;
+        int npool = 0;
+        for (int i = 0; i < p; ++i) npool += (sg[i] < sfloor) ? 1 : 0;
+        const double direct = allb - acc;  // sum of the pooled |beta|^2
+        const double comp = tot - acc;
+        const double ortho = (floor_rel < 1e-9) ? 1e-9 : 1e-5;
+        const double pool = (npool == 0) ? 0.0 : (comp > ortho * tot) ? comp : fmax(direct, 0.0);
+        for (int i = 0; i <= p; ++i) tail[i] += pool;
+        s_total = tot;
+        s_pool = pool;
     }
     __syncthreads();
     const double bn2 = bn2all[b];
     const double total = s_total;
+    const double pool = s_pool;
     const double delta0 = (rows > p && bn2 - total > 0) ? bn2 - total : 0.0;
     const double smax = s[0];
 
@@ -189,6 +216,25 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
         }
         if (t == 0) s_k = (p > 1) ? Gi[0] : p;
         __syncthreads();
+        // relative gap to the runner-up k (how far a perturbation of the
+        // spectrum must move G before the chosen index changes)
+        const double gbest = Gv[0];
+        __syncthreads();
+        double second = INFINITY;
+        for (int k = 1 + t; k < p; k += blockDim.x) {
+            if (k == s_k) continue;
+            double den = (double)(rows - k);
+            second = fmin(second, (tail[k] + delta0) / (den * den));
+        }
+        Gv[t] = second;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (t < st) Gv[t] = fmin(Gv[t], Gv[t + st]);
+            __syncthreads();
+        }
+        if (t == 0) s_gapk = (gbest > 0 && gbest < INFINITY) ? (Gv[0] - gbest) / gbest : INFINITY;
+        if (t == 0) s_gap = INFINITY;
+        __syncthreads();
     }
     const bool tik = (prm.filter == IA_FILTER_TIKHONOV);
     double lam = 0.0;
@@ -207,7 +253,7 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                         num += fc * fc * ab2[i];
                         sf += fc;
                     }
-                    num = wsum(num);
+                    num = wsum(num) + pool;
                     sf = wsum(sf);
                     double den = (double)(rows - p) + sf;
                     val = (num + delta0) / (den * den);
@@ -226,6 +272,7 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                     }
 #pragma unroll
                     for (int q = 0; q < 6; ++q) sums[q] = wsum(sums[q]);
+                    sums[1] += pool;
                     if (rows > p) sums[1] += delta0;
                     double eta = sqrt(sums[0]), rho = sqrt(sums[1]);
                     double deta = sums[2] / eta, drho = -sums[3] / rho;
@@ -250,6 +297,10 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                         best = Gv[j];
                         bj = j;
                     }
+                double second = INFINITY;
+                for (int j = 0; j < nl; ++j)
+                    if (j != bj) second = fmin(second, Gv[j]);
+                s_gap = (best < INFINITY && best != 0) ? (second - best) / fabs(best) : INFINITY;
                 s_idx = bj;
                 s_lam = smax * pow(10.0, -prm.decades + prm.decades * bj / (double)(nl - 1));
             }
@@ -271,7 +322,7 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                     rpm = fmax(s[p - 1], s[0] * 16.0 * eps);
                     ratio = pow(s[0] / rpm, 1.0 / (np - 1));
                 }
-                double best = INFINITY;
+                double best = INFINITY, second = INFINITY;
                 int bj = 0;
                 const bool LS = (rows > p);
                 const double m_n = (double)(rows - p);
@@ -280,7 +331,7 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                     double val = 0;
                     if (prm.select == IA_SELECT_GCV_REGTOOLS) {
                         // gcvG.m:38-40: f = lam/(s+lam) on the grid for both methods
-                        double num = 0, sf = 0;
+                        double num = pool, sf = 0;
                         for (int i = 0; i < p; ++i) {
                             double f = la / (s[i] + la);
                             num += f * f * ab2[i];
@@ -288,7 +339,7 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                         }
                         val = (num + delta0) / ((m_n + sf) * (m_n + sf));
                     } else if (prm.select == IA_SELECT_LCURVE_REGTOOLS) {
-                        val = lcfun1(la, p, s, ab2, axi2, tik, LS, delta0);
+                        val = lcfun1(la, p, s, ab2, axi2, tik, LS, delta0, pool);
                     } else {
                         double q2 = 0;
                         for (int i = 0; i < p; ++i) {
@@ -299,17 +350,21 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                         val = sqrt(q2);
                     }
                     if (val < best) {
+                        second = best;
                         best = val;
                         bj = j;
+                    } else if (val < second) {
+                        second = val;
                     }
                 }
+                s_gap = (best < INFINITY && best != 0) ? (second - best) / fabs(best) : INFINITY;
                 double lo = pow(ratio, (double)(np - 1 - min(bj + 1, np - 1))) * rpm;
                 double hi = pow(ratio, (double)(np - 1 - max(bj - 1, 0))) * rpm;
                 double res;
                 if (prm.select == IA_SELECT_GCV_REGTOOLS) {
                     res = fminbnd(
                         [&](double la) {
-                            double num = 0, sf = 0;
+                            double num = pool, sf = 0;
                             for (int i = 0; i < p; ++i) {
                                 double f = tik ? (la * la) / (s[i] + la * la) : la / (s[i] + la);
                                 num += f * f * ab2[i];
@@ -319,7 +374,7 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                         },
                         lo, hi, 1e-4);
                 } else if (prm.select == IA_SELECT_LCURVE_REGTOOLS) {
-                    res = fminbnd([&](double la) { return lcfun1(la, p, s, ab2, axi2, tik, LS, delta0); },
+                    res = fminbnd([&](double la) { return lcfun1(la, p, s, ab2, axi2, tik, LS, delta0, pool); },
                                   lo, hi, 1e-4);
                 } else {
                     res = fminbnd(
@@ -359,6 +414,9 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
         if (lam_out) lam_out[b] = lam;
         if (idx_out) idx_out[b] = idx;
         if (k_out) k_out[b] = s_k;
+        // gap of the selection that decides x: the lambda search, or the
+        // discrete-GCV k when the filter is TSVD
+        if (gap_out) gap_out[b] = (prm.filter == IA_FILTER_TSVD) ? s_gapk : s_gap;
     }
 }
 
@@ -366,12 +424,13 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
 
 void launch_select(int p, int rows, const double *sigma, const cx<double> *beta,
                    const double *bnorm2, RegDev prm, double *lam_out, int *idx_out, int *k_out,
-                   cx<double> *phi, int batch, hipStream_t s) {
+                   cx<double> *phi, int batch, hipStream_t s, double *gap_out,
+                   const double *cnorm2, double floor_rel) {
     size_t lds = sizeof(double) * (4 * (size_t)p + 1);
     (void)hipFuncSetAttribute((const void *)k_select, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
-    hipLaunchKernelGGL(k_select, dim3(batch), dim3(256), lds, s, p, rows, sigma, beta, bnorm2, prm,
-                       lam_out, idx_out, k_out, phi);
+    hipLaunchKernelGGL(k_select, dim3(batch), dim3(256), lds, s, p, rows, sigma, beta, bnorm2,
+                       cnorm2, floor_rel, prm, lam_out, idx_out, k_out, gap_out, phi);
 }
 
 }  // namespace ia

@@ -26,10 +26,10 @@ template <typename T>
 ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, size_t sA,
                            const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
                            size_t sx, double *lam, int *idx, int *kts, double *smax,
-                           hipStream_t s) {
+                           hipStream_t s, double *gap = nullptr) {
     const bool tall = rows >= cols;
     const int R = tall ? rows : cols, C = tall ? cols : rows;
-    DevBuf bY, bYd, bc, bbeta, bphi, bn2, bv;
+    DevBuf bY, bYd, bc, bbeta, bphi, bn2, bcn2, bv;
     IA_TRY(bY.alloc(sizeof(cx<T>) * (size_t)R * C * batch));
     cx<T> *Y = bY.as<cx<T>>();
     cx<double> *Yd = nullptr;
@@ -56,6 +56,7 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
     IA_TRY(bphi.alloc(sizeof(cx<double>) * (size_t)C * batch));
     IA_TRY(bv.alloc(sizeof(cx<double>) * (size_t)R * batch));
     IA_TRY(bn2.alloc(sizeof(double) * batch));
+    IA_TRY(bcn2.alloc(sizeof(double) * batch));
     cx<double> *c = bc.as<cx<double>>();
     launch_norm2<double>(rows, b, sb, bn2.as<double>(), batch, s);
     if (tall) {
@@ -64,6 +65,7 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
                                 sizeof(cx<double>) * rows, batch, hipMemcpyDeviceToDevice, s));
         launch_apply_left<T>(true, R, C, Y, R, (size_t)R * C, sw.tauL.template as<cx<T>>(), c, R, 1,
                              batch, s);
+        launch_norm2<double>(C, c, R, bcn2.as<double>(), batch, s);
         launch_real_matvec(C, sw.PBt.template as<double>(), 0, c, R, bbeta.as<cx<double>>(), C,
                            batch, s);
     } else {
@@ -72,11 +74,13 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
                                 sizeof(cx<double>) * rows, batch, hipMemcpyDeviceToDevice, s));
         launch_apply_right<T>(true, C, sw.Vr.template as<cx<T>>(), sw.tauR.template as<cx<T>>(), c,
                               C, 1, batch, s);
+        launch_norm2<double>(C, c, C, bcn2.as<double>(), batch, s);
         launch_real_matvec(C, sw.QBt.template as<double>(), 0, c, C, bbeta.as<cx<double>>(), C,
                            batch, s);
     }
     launch_select(C, rows, sw.sig.template as<double>(), bbeta.as<cx<double>>(), bn2.as<double>(),
-                  prm, lam, idx, kts, bphi.as<cx<double>>(), batch, s);
+                  prm, lam, idx, kts, bphi.as<cx<double>>(), batch, s, gap, bcn2.as<double>(),
+                  svd_floor_rel<T>());
     cx<double> *v = bv.as<cx<double>>();
     if (tall) {
         // x = V_b (Q_B phi)
@@ -107,10 +111,11 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
 template ia_status regsolve_batched<double>(int, int, int, const cx<double> *, size_t,
                                             const cx<double> *, size_t, const RegDev &,
                                             cx<double> *, size_t, double *, int *, int *, double *,
-                                            hipStream_t);
+                                            hipStream_t, double *);
 template ia_status regsolve_batched<float>(int, int, int, const cx<double> *, size_t,
                                            const cx<double> *, size_t, const RegDev &, cx<double> *,
-                                           size_t, double *, int *, int *, double *, hipStream_t);
+                                           size_t, double *, int *, int *, double *, hipStream_t,
+                                           double *);
 
 RegDev to_regdev(const ia_reg_params &p) {
     return RegDev{(int)p.filter, (int)p.select, p.n_lambda > 0 ? p.n_lambda : 50, p.k,

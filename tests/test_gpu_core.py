@@ -183,3 +183,23 @@ def test_utils_products(ia):
     np.testing.assert_allclose(ia.to_velocity(M, x), M @ x, rtol=1e-13)
     p, v = _cr(30, 20), _cr(30, 21)
     assert ia.sound_power(p, v, 3.0) == pytest.approx(0.5 * np.real(np.vdot(v, p)) * 3.0 / 30, rel=1e-13)
+
+
+@pytest.mark.parametrize("shape", [(96, 40), (300, 257), (64, 64)])
+def test_bidiag_split_and_fused_forms_agree(ia, shape, monkeypatch):
+    """Small batches run the bidiagonalisation as four launches per column
+    (products spread over many workgroups), large ones as one fused
+    workgroup per matrix; both must give the oracle's solution."""
+    rng = np.random.default_rng(11)
+    B = 3
+    A = rng.standard_normal((B,) + shape) + 1j * rng.standard_normal((B,) + shape)
+    b = rng.standard_normal((B, shape[0])) + 1j * rng.standard_normal((B, shape[0]))
+    monkeypatch.setenv("IA_GEBRD_SPLIT_BELOW", "1000")
+    xs, *_ = ia.regsolve(A, b, filter="tikhonov", select="fixed", lam=0.05)
+    monkeypatch.setenv("IA_GEBRD_SPLIT_BELOW", "0")
+    xf, *_ = ia.regsolve(A, b, filter="tikhonov", select="fixed", lam=0.05)
+    for i in range(B):
+        U, s, V = rt.csvd(A[i])
+        x0 = rt.tikhonov(U, s, V, b[i], 0.05)
+        assert np.linalg.norm(xs[i] - x0) <= 1e-12 * np.linalg.norm(x0)
+        assert np.linalg.norm(xf[i] - x0) <= 1e-12 * np.linalg.norm(x0)

@@ -89,3 +89,44 @@ def test_sweep_rejects_bad_inputs(ia):
     assert e.value.status == 5
     with pytest.raises(ia.IAError):
         ia.esm_sweep(S.k, S.r_s, S.src, S.r_h, S.b, S.area, filter="tsvd", select="fixed", kfix=10_000)
+
+
+def test_mixed_recheck_replaces_near_ties_with_fp64(ia):
+    """Every frequency whose selection is within the recheck gap of a tie is
+    solved again in FP64: with a gap large enough to catch all of them the
+    mixed sweep returns the FP64 sweep bit for bit; with rechecking off it
+    stays within 0.1 dB."""
+    P = sc.plate_scenario(freqs=np.linspace(300.0, 1400.0, 6), mic_n=(16, 16), mic_pitch=0.06,
+                          src_n=(32, 16), ref_n=(64, 50))
+    r = _run(ia, P)
+    every = _run(ia, P, precision="mixed", recheck_gap=1e30)
+    np.testing.assert_array_equal(every.lam_index, r.lam_index)
+    np.testing.assert_array_equal(every.velocity, r.velocity)
+    raw = _run(ia, P, precision="mixed", recheck_gap=-1.0)
+    assert np.max(np.abs(post.power_db(raw.power) - post.power_db(r.power))) < 0.1
+    cert = _run(ia, P, precision="mixed")
+    np.testing.assert_array_equal(cert.lam_index, r.lam_index)
+
+
+def test_full_size_plate_sweep_matches_oracle(ia):
+    """1024 microphones x 2048 sources (H_e condition ~1e15): the FP64 form
+    against the LAPACK oracle at six frequencies of the 200-1500 Hz grid, run
+    once tiled to a 256-frequency batch (the fused bidiagonalisation, one
+    workgroup per frequency) and once as a 6-frequency batch (the split
+    form); the mixed form picks the same lambda index within 0.1 dB."""
+    grid = np.linspace(200.0, 1500.0, 256)
+    pick = grid[[0, 66, 140, 222, 240, 255]]
+    P = sc.plate_scenario(freqs=pick)
+    ia.set_air(P.c, P.rho)
+    tile = np.arange(256) % len(pick)
+    big = ia.esm_sweep(P.k[tile], P.r_s, P.src, P.r_h, P.b[tile], P.area, normals=P.normals)
+    sub = _run(ia, P)
+    for f in range(len(pick)):
+        o = _oracle(P, f)
+        for res, i in ((big, f), (big, f + len(pick) * 10), (sub, f)):
+            assert res.lam_index[i] == o["idx"] and res.k_tsvd[i] == o["k_tsvd"], (f, i)
+            assert np.linalg.norm(res.velocity[i] - o["x"]) <= 1e-10 * np.linalg.norm(o["x"]), f
+            assert res.power[i] == pytest.approx(o["W"], rel=1e-10)
+    rm = _run(ia, P, precision="mixed")
+    assert np.max(np.abs(post.power_db(rm.power) - post.power_db(sub.power))) < 0.1
+    np.testing.assert_array_equal(rm.lam_index, sub.lam_index)
