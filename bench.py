@@ -182,10 +182,13 @@ def main():
     print(json.dumps(res))
 
 
-# HBM bytes per k_labrd launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, scaled by
-# the counter factor of a plain device copy) keyed by (form, frequencies, m, l, chunk);
-# the profiles/ summaries hold the runs these figures come from.
-HBM_TRAFFIC_PER_LAUNCH = {}
+# HBM bytes per k_labrd launch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; FETCH scaled by
+# 1.99, the WRITE/FETCH ratio of a plain 1 GiB device copy in the same runs) keyed by
+# (form, frequencies, m, l, chunk): profiles/labrd_esm_sweep_256f_1024x2048_hbm_counters.txt
+HBM_TRAFFIC_PER_LAUNCH = {
+    ("fp64", 256, 2048, 1024, 0): 253578247811,
+    ("mixed", 256, 2048, 1024, 0): 128157784819,
+}
 
 
 def cpu_baseline(P, nf):
