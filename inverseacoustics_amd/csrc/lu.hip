@@ -20,7 +20,8 @@ namespace ia {
 
 namespace {
 
-constexpr int NB = 32;
+constexpr int NB = 32;    // diagonal solves and panel columns
+constexpr int NBO = 128;  // rank of the trailing GEMM updates
 
 template <typename T>
 __device__ __forceinline__ T cabs1(cx<T> a) { return fabs(a.re) + fabs(a.im); }
@@ -216,38 +217,67 @@ __global__ void k_copy_cols(int n, int nrhs, const cx<T> *src, int lds, size_t s
     dst[(size_t)b * strideD + (size_t)c * ldd + r] = src[(size_t)b * strideS + (size_t)c * lds + r];
 }
 
+// Two-level blocking: diagonal blocks of NBO columns are solved with NB-wide
+// diagonal solves and rank-NB updates confined to the block, then the rest of
+// B gets one rank-NBO GEMM update per block (four times fewer passes over B
+// than rank-NB updates).
 template <typename T, int MODE>
 void trsm_blocked(int n, int nrhs, const cx<T> *A, int lda, size_t sA, cx<T> *B, int ldb,
                   size_t sB, int batch, hipStream_t s) {
     const cx<T> mone{-1, 0}, one{1, 0};
     dim3 dgrid((nrhs + 255) / 256, batch);
     const bool forward = (MODE == 0 || MODE == 1);
-    const int nblk = (n + NB - 1) / NB;
-    for (int bi = 0; bi < nblk; ++bi) {
-        const int blk = forward ? bi : (nblk - 1 - bi);
-        const int i0 = blk * NB, jb = min(NB, n - i0);
-        hipLaunchKernelGGL((k_trsm_diag<T, MODE>), dgrid, dim3(256), 0, s, n, i0, jb, A, lda, sA,
-                           B, ldb, sB, nrhs);
+    const int nob = (n + NBO - 1) / NBO;
+    for (int ob = 0; ob < nob; ++ob) {
+        const int blkO = forward ? ob : (nob - 1 - ob);
+        const int o0 = blkO * NBO, ojb = min(NBO, n - o0), o1 = o0 + ojb;
+        const int nblk = (ojb + NB - 1) / NB;
+        for (int bi = 0; bi < nblk; ++bi) {
+            const int blk = forward ? bi : (nblk - 1 - bi);
+            const int i0 = o0 + blk * NB, jb = min(NB, o1 - i0);
+            hipLaunchKernelGGL((k_trsm_diag<T, MODE>), dgrid, dim3(256), 0, s, n, i0, jb, A, lda,
+                               sA, B, ldb, sB, nrhs);
+            if (forward) {
+                const int rest = o1 - i0 - jb;
+                if (rest > 0) {
+                    if (MODE == 0)  // B[i0+jb:o1] -= L[i0+jb:o1, I] X_I
+                        launch_gemm<T>('N', 'N', rest, nrhs, jb, mone,
+                                       A + (size_t)i0 * lda + i0 + jb, lda, sA, B + i0, ldb, sB,
+                                       one, B + i0 + jb, ldb, sB, batch, s);
+                    else  // B[i0+jb:o1] -= (U[I, i0+jb:o1])^H X_I
+                        launch_gemm<T>('C', 'N', rest, nrhs, jb, mone,
+                                       A + (size_t)(i0 + jb) * lda + i0, lda, sA, B + i0, ldb, sB,
+                                       one, B + i0 + jb, ldb, sB, batch, s);
+                }
+            } else {
+                const int rest = i0 - o0;
+                if (rest > 0) {
+                    if (MODE == 2)  // B[o0:i0] -= U[o0:i0, I] X_I
+                        launch_gemm<T>('N', 'N', rest, nrhs, jb, mone, A + (size_t)i0 * lda + o0,
+                                       lda, sA, B + i0, ldb, sB, one, B + o0, ldb, sB, batch, s);
+                    else  // B[o0:i0] -= (L[I, o0:i0])^H X_I
+                        launch_gemm<T>('C', 'N', rest, nrhs, jb, mone, A + (size_t)o0 * lda + i0,
+                                       lda, sA, B + i0, ldb, sB, one, B + o0, ldb, sB, batch, s);
+                }
+            }
+        }
         if (forward) {
-            int rest = n - i0 - jb;
+            const int rest = n - o1;
             if (rest > 0) {
-                if (MODE == 0)  // B[i0+jb:, :] -= L[i0+jb:, I] X_I
-                    launch_gemm<T>('N', 'N', rest, nrhs, jb, mone, A + (size_t)i0 * lda + i0 + jb,
-                                   lda, sA, B + i0, ldb, sB, one, B + i0 + jb, ldb, sB, batch, s);
-                else  // B[i0+jb:, :] -= (U[I, i0+jb:])^H X_I
-                    launch_gemm<T>('C', 'N', rest, nrhs, jb, mone,
-                                   A + (size_t)(i0 + jb) * lda + i0, lda, sA, B + i0, ldb, sB,
-                                   one, B + i0 + jb, ldb, sB, batch, s);
+                if (MODE == 0)  // B[o1:] -= L[o1:, O] X_O
+                    launch_gemm<T>('N', 'N', rest, nrhs, ojb, mone, A + (size_t)o0 * lda + o1, lda,
+                                   sA, B + o0, ldb, sB, one, B + o1, ldb, sB, batch, s);
+                else  // B[o1:] -= (U[O, o1:])^H X_O
+                    launch_gemm<T>('C', 'N', rest, nrhs, ojb, mone, A + (size_t)o1 * lda + o0, lda,
+                                   sA, B + o0, ldb, sB, one, B + o1, ldb, sB, batch, s);
             }
-        } else {
-            if (i0 > 0) {
-                if (MODE == 2)  // B[:i0] -= U[:i0, I] X_I
-                    launch_gemm<T>('N', 'N', i0, nrhs, jb, mone, A + (size_t)i0 * lda, lda, sA,
-                                   B + i0, ldb, sB, one, B, ldb, sB, batch, s);
-                else  // B[:i0] -= (L[I, :i0])^H X_I
-                    launch_gemm<T>('C', 'N', i0, nrhs, jb, mone, A + i0, lda, sA, B + i0, ldb, sB,
-                                   one, B, ldb, sB, batch, s);
-            }
+        } else if (o0 > 0) {
+            if (MODE == 2)  // B[:o0] -= U[:o0, O] X_O
+                launch_gemm<T>('N', 'N', o0, nrhs, ojb, mone, A + (size_t)o0 * lda, lda, sA,
+                               B + o0, ldb, sB, one, B, ldb, sB, batch, s);
+            else  // B[:o0] -= (L[O, :o0])^H X_O
+                launch_gemm<T>('C', 'N', o0, nrhs, ojb, mone, A + o0, lda, sA, B + o0, ldb, sB,
+                               one, B, ldb, sB, batch, s);
         }
     }
 }
@@ -259,24 +289,37 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
                         hipStream_t s) {
     if (n <= 0 || batch <= 0) return IA_OK;
     const cx<T> mone{-1, 0}, one{1, 0};
-    for (int k = 0; k < n; k += NB) {
-        const int jb = min(NB, n - k);
-        hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(512), 0, s, n, k, jb, A, lda, strideA,
-                           ipiv, info);
-        if (n - jb > 0) {
-            dim3 g((n - jb + 255) / 256, batch);
-            hipLaunchKernelGGL(k_laswp<T>, g, dim3(256), 0, s, n, k, jb, A, lda, strideA, ipiv);
+    // panels of NBO columns, each factored as NB-wide sub-panels whose
+    // updates stay inside the panel; then U12 = L11^-1 A12 and one rank-NBO
+    // update of the trailing matrix per panel
+    for (int p0 = 0; p0 < n; p0 += NBO) {
+        const int pend = min(n, p0 + NBO);
+        for (int k = p0; k < pend; k += NB) {
+            const int jb = min(NB, pend - k);
+            hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(512), 0, s, n, k, jb, A, lda,
+                               strideA, ipiv, info);
+            if (n - jb > 0) {
+                dim3 g((n - jb + 255) / 256, batch);
+                hipLaunchKernelGGL(k_laswp<T>, g, dim3(256), 0, s, n, k, jb, A, lda, strideA, ipiv);
+            }
+            const int rin = pend - k - jb;
+            if (rin > 0) {
+                dim3 dgrid((rin + 255) / 256, batch);
+                hipLaunchKernelGGL((k_trsm_diag<T, 0>), dgrid, dim3(256), 0, s, n, k, jb,
+                                   (const cx<T> *)A + 0, lda, strideA,
+                                   A + (size_t)(k + jb) * lda, lda, strideA, rin);
+                launch_gemm<T>('N', 'N', n - k - jb, rin, jb, mone, A + (size_t)k * lda + k + jb,
+                               lda, strideA, A + (size_t)(k + jb) * lda + k, lda, strideA, one,
+                               A + (size_t)(k + jb) * lda + k + jb, lda, strideA, batch, s);
+            }
         }
-        const int rest = n - k - jb;
+        const int rest = n - pend, pjb = pend - p0;
         if (rest > 0) {
-            // U12 = L11^-1 A12
-            dim3 dgrid((rest + 255) / 256, batch);
-            hipLaunchKernelGGL((k_trsm_diag<T, 0>), dgrid, dim3(256), 0, s, n, k, jb,
-                               (const cx<T> *)A + 0, lda, strideA,
-                               A + (size_t)(k + jb) * lda, lda, strideA, rest);
-            launch_gemm<T>('N', 'N', rest, rest, jb, mone, A + (size_t)k * lda + k + jb, lda,
-                           strideA, A + (size_t)(k + jb) * lda + k, lda, strideA, one,
-                           A + (size_t)(k + jb) * lda + k + jb, lda, strideA, batch, s);
+            trsm_blocked<T, 0>(pjb, rest, A + (size_t)p0 * lda + p0, lda, strideA,
+                               A + (size_t)pend * lda + p0, lda, strideA, batch, s);
+            launch_gemm<T>('N', 'N', rest, rest, pjb, mone, A + (size_t)p0 * lda + pend, lda,
+                           strideA, A + (size_t)pend * lda + p0, lda, strideA, one,
+                           A + (size_t)pend * lda + pend, lda, strideA, batch, s);
         }
     }
     IA_HIP(hipGetLastError());
