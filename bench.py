@@ -185,10 +185,7 @@ def main():
 # HBM bytes per k_labrd launch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; FETCH scaled by
 # 1.99, the WRITE/FETCH ratio of a plain 1 GiB device copy in the same runs) keyed by
 # (form, frequencies, m, l, chunk): profiles/labrd_esm_sweep_256f_1024x2048_hbm_counters.txt
-HBM_TRAFFIC_PER_LAUNCH = {
-    ("fp64", 256, 2048, 1024, 0): 253578247811,
-    ("mixed", 256, 2048, 1024, 0): 128157784819,
-}
+HBM_TRAFFIC_PER_LAUNCH = {}
 
 
 def cpu_baseline(P, nf):

@@ -137,10 +137,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
                                 (size_t)m * l, nf, s));
         tm.mark();  // 3
         sw.batch = nf;
-        IA_TRY(launch_gebrd<T>(R, C, Y, R, (size_t)R * C, sw.d.template as<double>(),
-                               sw.e.template as<double>(), sw.tauL.template as<cx<T>>(),
-                               sw.tauR.template as<cx<T>>(), sw.Vr.template as<cx<T>>(),
-                               sw.gw.template as<cx<T>>(), nf, s, &clk));
+        IA_TRY(svd_reduce<T>(sw, Y, s, &clk));
         tm.mark();  // 4
         launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(), nf, s);
         tm.mark();  // 5
@@ -164,8 +161,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
         launch_real_matvec(C, sw.PBt.template as<double>(), 1, bphi.as<cx<double>>(), C,
                            bc.as<cx<double>>(), C, nf, s);
         launch_pad_copy(C, R, bc.as<cx<double>>(), C, bx.as<cx<double>>(), R, nf, s);
-        launch_apply_left<T>(false, R, C, Y, R, (size_t)R * C, sw.tauL.template as<cx<T>>(),
-                             bx.as<cx<double>>(), R, 1, nf, s);
+        svd_apply_u<T>(sw, Y, bx.as<cx<double>>(), R, 1, s);
         IA_HIP(hipGetLastError());
         tm.mark();  // 7
         // post: q = D^-1 x, p_s = M_sq q, W = 1/2 Re(x^H p_s) S/m
@@ -542,8 +538,7 @@ ia_status ia_csvd(int m, int n, const ia_z *a, ia_z **u, double **s_out, ia_z **
     IA_TRY(bVY.alloc(sizeof(cx<double>) * (size_t)C * C));
     launch_rows_to_cols(R, C, sw.PBt.as<double>(), bUY.as<cx<double>>(), 1, s);
     launch_rows_to_cols(C, C, sw.QBt.as<double>(), bVY.as<cx<double>>(), 1, s);
-    launch_apply_left<double>(false, R, C, bY.as<cx<double>>(), R, 0, sw.tauL.as<cx<double>>(),
-                              bUY.as<cx<double>>(), R, C, 1, s);
+    svd_apply_u<double>(sw, bY.as<cx<double>>(), bUY.as<cx<double>>(), R, C, s);
     launch_apply_right<double>(false, C, sw.Vr.as<cx<double>>(), sw.tauR.as<cx<double>>(),
                                bVY.as<cx<double>>(), C, C, 1, s);
     IA_HIP(hipGetLastError());

@@ -1,0 +1,180 @@
+// Batched blocked Householder QR of a tall matrix, Y = Q R (R >= C), used to
+// shrink the bidiagonalisation of tall transfer matrices: the SVD of the
+// C x C factor R shares singular values and right vectors with Y, and the
+// left vectors are Q times those of R.  The bidiagonalisation streams its
+// trailing block twice per column, so reducing R x C to C x C first cuts
+// that traffic by ~(R/C)(3 - C/R)/2, while the QR itself is GEMM work on
+// the matrix cores.
+//
+// Per panel of NB columns (LAPACK geqrf/larft scheme):
+//   k_qr_panel   one workgroup per matrix: unblocked Householder QR of the
+//                panel (H_i = I - tau_i v_i v_i^H, LAPACK zlarfg), explicit
+//                copy of the reflectors V (unit diagonal, zeros above);
+//   G = V^H V    GEMM;  k_qr_tmat builds T with H_0 ... H_{nb-1} = I - V T V^H;
+//   trailing     A2 <- (I - V T^H V^H) A2 = A2 - V (T^H (V^H A2)), three GEMMs.
+// The reflectors stay below the diagonal of Y with tau per column, the form
+// the vector transforms (k_apply_left) read.
+#include "householder.hpp"
+
+namespace ia {
+
+namespace {
+
+using hh::BT;
+using hh::NW;
+using hh::Refl;
+using hh::block_sum;
+using hh::larfg;
+using hh::wave_sum;
+
+constexpr int NB = 32;
+
+template <typename T>
+__global__ __launch_bounds__(BT) void k_qr_panel(int R, int C, int j0, int nbp,
+                                                cx<T> *__restrict__ Yall, int ldy, size_t sY,
+                                                cx<T> *__restrict__ tauall,
+                                                cx<T> *__restrict__ Vwall) {
+    __shared__ double red[NW];
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    cx<T> *A = Yall + (size_t)b * sY;
+    cx<T> *tau = tauall + (size_t)b * C;
+    cx<T> *Vw = Vwall + (size_t)b * R * NB;
+    for (int i = 0; i < nbp; ++i) {
+        const int j = j0 + i;
+        cx<T> *Aj = A + (size_t)j * ldy;
+        double p = 0;
+        for (int r = j + 1 + t; r < R; r += BT) p += (double)abs2(Aj[r]);
+        const double xn2 = block_sum(p, red);
+        const Refl rf = larfg(cvt<double>(Aj[j]), xn2);
+        const cx<T> sc = cvt<T>(rf.scale);
+        __syncthreads();  // A[j,j] read by all before it is overwritten
+        for (int r = j + 1 + t; r < R; r += BT) Aj[r] = Aj[r] * sc;
+        if (t == 0) {
+            Aj[j] = cx<T>{(T)rf.beta, 0};
+            tau[j] = cvt<T>(rf.tau);
+        }
+        __syncthreads();
+        // remaining panel columns: A[:,c] -= conj(tau) u (u^H A[:,c]), one wave per column
+        const cx<double> ctau = conj(rf.tau);
+        for (int c = j + 1 + wv; c < j0 + nbp; c += NW) {
+            cx<T> *Ac = A + (size_t)c * ldy;
+            double ar = 0, ai = 0;
+            for (int r = j + lane; r < R; r += 64) {
+                const cx<T> uu = (r == j) ? cx<T>{1, 0} : Aj[r];
+                const cx<T> y = Ac[r];
+                ar += (double)uu.re * y.re + (double)uu.im * y.im;
+                ai += (double)uu.re * y.im - (double)uu.im * y.re;
+            }
+            const cx<double> f = ctau * cx<double>{wave_sum(ar), wave_sum(ai)};
+            for (int r = j + lane; r < R; r += 64) {
+                const cx<double> uu = (r == j) ? cx<double>{1, 0} : cvt<double>(Aj[r]);
+                Ac[r] = cvt<T>(cvt<double>(Ac[r]) - uu * f);
+            }
+        }
+        __syncthreads();
+    }
+    // explicit reflectors of the panel: Vw[:, i] (rows j0..R-1)
+    for (int i = 0; i < nbp; ++i) {
+        const int j = j0 + i;
+        const cx<T> *Aj = A + (size_t)j * ldy;
+        for (int r = j0 + t; r < R; r += BT) {
+            cx<T> v = (r < j) ? cx<T>{0, 0} : (r == j) ? cx<T>{1, 0} : Aj[r];
+            Vw[(size_t)i * R + r] = v;
+        }
+    }
+}
+
+// T (nbp x nbp, upper) with H_0 ... H_{nbp-1} = I - V T V^H from G = V^H V
+// and tau (LAPACK larft, forward columnwise); one wave per matrix.
+template <typename T>
+__global__ __launch_bounds__(64) void k_qr_tmat(int C, int j0, int nbp, const cx<T> *__restrict__ Gall,
+                                               const cx<T> *__restrict__ tauall,
+                                               cx<T> *__restrict__ Tall) {
+    __shared__ cx<double> Ts[NB][NB];
+    __shared__ cx<double> z[NB];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const cx<T> *G = Gall + (size_t)b * NB * NB;
+    const cx<T> *tau = tauall + (size_t)b * C + j0;
+    cx<T> *Tm = Tall + (size_t)b * NB * NB;
+    for (int e = t; e < NB * NB; e += 64) Ts[e % NB][e / NB] = cx<double>{0, 0};
+    __syncthreads();
+    for (int i = 0; i < nbp; ++i) {
+        const cx<double> ti = cvt<double>(tau[i]);
+        if (t < i) z[t] = cx<double>{0, 0} - ti * cvt<double>(G[(size_t)i * NB + t]);  // G[t, i]
+        __syncthreads();
+        if (t < i) {
+            cx<double> acc{0, 0};
+            for (int l = t; l < i; ++l) acc = acc + Ts[t][l] * z[l];
+            Ts[t][i] = acc;
+        }
+        if (t == 0) Ts[i][i] = ti;
+        __syncthreads();
+    }
+    for (int e = t; e < NB * NB; e += 64) Tm[e] = cvt<T>(Ts[e % NB][e / NB]);
+}
+
+// Rb (C x C) = upper triangle of Y's leading C x C block, zeros below
+template <typename T>
+__global__ void k_copy_upper(int C, const cx<T> *__restrict__ Yall, int ldy, size_t sY,
+                             cx<T> *__restrict__ Rball) {
+    const int b = blockIdx.z, c = blockIdx.y;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= C) return;
+    const cx<T> v = Yall[(size_t)b * sY + (size_t)c * ldy + r];
+    Rball[(size_t)b * C * C + (size_t)c * C + r] = (r <= c) ? v : cx<T>{0, 0};
+}
+
+}  // namespace
+
+size_t geqrf_work_elems(int R, int C) {
+    return (size_t)R * NB + 2 * (size_t)NB * NB + 2 * (size_t)NB * C;
+}
+
+template <typename T>
+ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, cx<T> *work,
+                       int batch, hipStream_t s) {
+    if (R < C) return fail(IA_ERR_ARG, "QR needs R >= C (%d x %d)", R, C);
+    cx<T> *Vw = work;
+    cx<T> *G = Vw + (size_t)batch * R * NB;
+    cx<T> *Tm = G + (size_t)batch * NB * NB;
+    cx<T> *W = Tm + (size_t)batch * NB * NB;
+    cx<T> *W2 = W + (size_t)batch * NB * C;
+    const cx<T> one{1, 0}, zero{0, 0}, mone{-1, 0};
+    const size_t sV = (size_t)R * NB, sT = (size_t)NB * NB, sW = (size_t)NB * C;
+    for (int j0 = 0; j0 < C; j0 += NB) {
+        const int nbp = std::min(NB, C - j0);
+        hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, R, C, j0, nbp, Y, ldy, sY,
+                           tau, Vw);
+        const int N = C - j0 - nbp, M = R - j0;
+        if (N <= 0) continue;
+        const cx<T> *V0 = Vw + j0;  // rows j0.. of the explicit panel reflectors
+        // G = V^H V
+        launch_gemm<T>('C', 'N', nbp, nbp, M, one, V0, R, sV, V0, R, sV, zero, G, NB, sT, batch, s);
+        hipLaunchKernelGGL(k_qr_tmat<T>, dim3(batch), dim3(64), 0, s, C, j0, nbp, G, tau, Tm);
+        cx<T> *A2 = Y + j0 + (size_t)(j0 + nbp) * ldy;
+        // W = V^H A2, W2 = T^H W, A2 -= V W2
+        launch_gemm<T>('C', 'N', nbp, N, M, one, V0, R, sV, A2, ldy, sY, zero, W, NB, sW, batch, s);
+        launch_gemm<T>('C', 'N', nbp, N, nbp, one, Tm, NB, sT, W, NB, sW, zero, W2, NB, sW, batch, s);
+        launch_gemm<T>('N', 'N', M, N, nbp, mone, V0, R, sV, W2, NB, sW, one, A2, ldy, sY, batch, s);
+    }
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
+template <typename T>
+void launch_copy_upper(int C, const cx<T> *Y, int ldy, size_t sY, cx<T> *Rb, int batch,
+                       hipStream_t s) {
+    dim3 g((C + 255) / 256, C, batch);
+    hipLaunchKernelGGL(k_copy_upper<T>, g, dim3(256), 0, s, C, Y, ldy, sY, Rb);
+}
+
+template ia_status launch_geqrf<double>(int, int, cx<double> *, int, size_t, cx<double> *,
+                                        cx<double> *, int, hipStream_t);
+template ia_status launch_geqrf<float>(int, int, cx<float> *, int, size_t, cx<float> *,
+                                       cx<float> *, int, hipStream_t);
+template void launch_copy_upper<double>(int, const cx<double> *, int, size_t, cx<double> *, int,
+                                        hipStream_t);
+template void launch_copy_upper<float>(int, const cx<float> *, int, size_t, cx<float> *, int,
+                                       hipStream_t);
+
+}  // namespace ia

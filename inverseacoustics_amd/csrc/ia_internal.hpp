@@ -161,6 +161,16 @@ size_t bidiag_lds_bytes(int R, int C, size_t elem);
 // gebrd_work_elems(R, C) complex T per batch item
 size_t gebrd_lds_bytes(int R, int C, size_t elem);
 size_t gebrd_work_elems(int R, int C);
+// blocked Householder QR Y = Q R in place (geqrf.hip); work holds
+// geqrf_work_elems(R, C) complex T per batch item
+size_t geqrf_work_elems(int R, int C);
+template <typename T>
+ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, cx<T> *work,
+                       int batch, hipStream_t s);
+template <typename T>
+void launch_copy_upper(int C, const cx<T> *Y, int ldy, size_t sY, cx<T> *Rb, int batch,
+                       hipStream_t s);
+
 // HIP events around every launch of one kernel on its stream, with the
 // algorithmic bytes those launches move (read after the stream is synced)
 struct KernelClock {
@@ -296,15 +306,20 @@ inline ia_status host_out(T **dst, const void *dsrc, size_t count) {
 // ------------------------------------------------------ SVD driver
 // Factor Y (R x C, R >= C, ld R, batch stride R*C, complex T) in place:
 // bidiagonal reflectors in Y/Vr/tau, singular values sig (descending),
-// bidiagonal singular vectors QBt/PBt (vector i in row i).
+// bidiagonal singular vectors QBt/PBt (vector i in row i).  When R >= 3C/2
+// Y is first reduced to R = Q^H Y (blocked QR on the matrix cores) and the
+// C x C factor is bidiagonalised instead: the left singular basis is then
+// U = Q diag(U_R, I), applied by svd_apply_u / svd_apply_uh.
 template <typename T>
 struct SvdWork {
-    DevBuf d, e, sig, tauL, tauR, Vr, QBt, PBt, work, gw;
+    DevBuf d, e, sig, tauL, tauR, Vr, QBt, PBt, work, gw, Rb, tauQ, qw;
     int R = 0, C = 0, batch = 0, sub = 1;
+    bool qr = false;
     ia_status alloc(int R_, int C_, int batch_) {
         R = R_;
         C = C_;
         batch = batch_;
+        qr = 2 * (size_t)R >= 3 * (size_t)C && C >= 64;
         size_t bc = (size_t)batch * C;
         IA_TRY(d.alloc(bc * sizeof(double)));
         IA_TRY(e.alloc(bc * sizeof(double)));
@@ -314,7 +329,12 @@ struct SvdWork {
         IA_TRY(Vr.alloc(bc * C * sizeof(cx<T>)));
         IA_TRY(QBt.alloc(bc * C * sizeof(double)));
         IA_TRY(PBt.alloc(bc * C * sizeof(double)));
-        IA_TRY(gw.alloc(gebrd_work_elems(R, C) * batch * sizeof(cx<T>)));
+        IA_TRY(gw.alloc(gebrd_work_elems(qr ? C : R, C) * batch * sizeof(cx<T>)));
+        if (qr) {
+            IA_TRY(Rb.alloc(bc * C * sizeof(cx<T>)));
+            IA_TRY(tauQ.alloc(bc * sizeof(cx<T>)));
+            IA_TRY(qw.alloc(geqrf_work_elems(R, C) * batch * sizeof(cx<T>)));
+        }
         // inverse-iteration scratch: as many frequencies per launch as a
         // quarter of the free device memory holds (1..32 GiB), so one launch
         // covers a whole 256-frequency batch (one thread per singular value
@@ -332,13 +352,29 @@ struct SvdWork {
     }
 };
 
+// QR (when chosen) + bidiagonalisation of Y; clk times the panel kernel
 template <typename T>
-inline ia_status svd_factor(SvdWork<T> &w, cx<T> *Y, hipStream_t s) {
+inline ia_status svd_reduce(SvdWork<T> &w, cx<T> *Y, hipStream_t s, KernelClock *clk = nullptr) {
     const int R = w.R, C = w.C, batch = w.batch;
-    IA_TRY(launch_gebrd<T>(R, C, Y, R, (size_t)R * C, w.d.template as<double>(),
+    cx<T> *Z = Y;
+    int Rz = R;
+    if (w.qr) {
+        IA_TRY(launch_geqrf<T>(R, C, Y, R, (size_t)R * C, w.tauQ.template as<cx<T>>(),
+                               w.qw.template as<cx<T>>(), batch, s));
+        launch_copy_upper<T>(C, Y, R, (size_t)R * C, w.Rb.template as<cx<T>>(), batch, s);
+        Z = w.Rb.template as<cx<T>>();
+        Rz = C;
+    }
+    return launch_gebrd<T>(Rz, C, Z, Rz, (size_t)Rz * C, w.d.template as<double>(),
                            w.e.template as<double>(), w.tauL.template as<cx<T>>(),
                            w.tauR.template as<cx<T>>(), w.Vr.template as<cx<T>>(),
-                           w.gw.template as<cx<T>>(), batch, s));
+                           w.gw.template as<cx<T>>(), batch, s, clk);
+}
+
+template <typename T>
+inline ia_status svd_factor(SvdWork<T> &w, cx<T> *Y, hipStream_t s) {
+    const int C = w.C, batch = w.batch;
+    IA_TRY(svd_reduce<T>(w, Y, s));
     launch_bisect(C, w.d.template as<double>(), w.e.template as<double>(), w.sig.template as<double>(),
                   batch, s);
     IA_HIP(hipGetLastError());
@@ -347,6 +383,41 @@ inline ia_status svd_factor(SvdWork<T> &w, cx<T> *Y, hipStream_t s) {
                    w.PBt.template as<double>(), w.work.template as<double>(), w.sub, batch, s);
     IA_HIP(hipGetLastError());
     return IA_OK;
+}
+
+// v <- U_b v for nvec length-R vectors per batch item (stride sv), where
+// Y = U_b B V_b^H; Y is the matrix svd_reduce was given.  Rows C..R-1 of v
+// must hold the vector's tail (zero for U_b [y; 0]).
+template <typename T>
+inline void svd_apply_u(SvdWork<T> &w, const cx<T> *Y, cx<double> *v, size_t sv, int nvec,
+                        hipStream_t s) {
+    const int R = w.R, C = w.C;
+    if (w.qr) {
+        launch_apply_left<T>(false, C, C, w.Rb.template as<cx<T>>(), C, (size_t)C * C,
+                             w.tauL.template as<cx<T>>(), v, sv, nvec, w.batch, s);
+        launch_apply_left<T>(false, R, C, Y, R, (size_t)R * C, w.tauQ.template as<cx<T>>(), v, sv,
+                             nvec, w.batch, s);
+    } else {
+        launch_apply_left<T>(false, R, C, Y, R, (size_t)R * C, w.tauL.template as<cx<T>>(), v, sv,
+                             nvec, w.batch, s);
+    }
+}
+
+// v <- U_b^H v (length-R vectors); the first C entries are the coordinates
+// in the left singular basis of the bidiagonal
+template <typename T>
+inline void svd_apply_uh(SvdWork<T> &w, const cx<T> *Y, cx<double> *v, size_t sv, int nvec,
+                         hipStream_t s) {
+    const int R = w.R, C = w.C;
+    if (w.qr) {
+        launch_apply_left<T>(true, R, C, Y, R, (size_t)R * C, w.tauQ.template as<cx<T>>(), v, sv,
+                             nvec, w.batch, s);
+        launch_apply_left<T>(true, C, C, w.Rb.template as<cx<T>>(), C, (size_t)C * C,
+                             w.tauL.template as<cx<T>>(), v, sv, nvec, w.batch, s);
+    } else {
+        launch_apply_left<T>(true, R, C, Y, R, (size_t)R * C, w.tauL.template as<cx<T>>(), v, sv,
+                             nvec, w.batch, s);
+    }
 }
 
 }  // namespace ia

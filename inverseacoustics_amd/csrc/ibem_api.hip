@@ -494,8 +494,7 @@ ia_status ia_invr(int m, int l, int n, const double *r_ms, const int *e_n, const
     DevBuf bU;
     IA_TRY(bU.alloc(sizeof(cx<double>) * (size_t)m * m));
     launch_rows_to_cols(m, m, sw.PBt.as<double>(), bU.as<cx<double>>(), 1, s);
-    launch_apply_left<double>(false, m, m, bR.as<cx<double>>(), m, 0, sw.tauL.as<cx<double>>(),
-                              bU.as<cx<double>>(), m, n_am, 1, s);
+    svd_apply_u<double>(sw, bR.as<cx<double>>(), bU.as<cx<double>>(), m, n_am, s);
     // modes of a real symmetric matrix are real up to a phase: rotate each to real
     std::vector<ia_z> Uh((size_t)m * n_am);
     IA_HIP(hipMemcpy(Uh.data(), bU.p, sizeof(ia_z) * Uh.size(), hipMemcpyDeviceToHost));
@@ -592,8 +591,7 @@ ia_status ia_radiation_modes(int np, const double *r_p, double a_e, double k, in
     IA_TRY(svd_factor<double>(sw, bR.as<cx<double>>(), s));
     IA_TRY(bU.alloc(sizeof(cx<double>) * (size_t)np * np));
     launch_rows_to_cols(np, np, sw.PBt.as<double>(), bU.as<cx<double>>(), 1, s);
-    launch_apply_left<double>(false, np, np, bR.as<cx<double>>(), np, 0, sw.tauL.as<cx<double>>(),
-                              bU.as<cx<double>>(), np, n_modes, 1, s);
+    svd_apply_u<double>(sw, bR.as<cx<double>>(), bU.as<cx<double>>(), np, n_modes, s);
     IA_HIP(hipGetLastError());
     IA_HIP(hipDeviceSynchronize());
     std::vector<double> sig(np);

@@ -63,8 +63,7 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
         // beta = P_B^T (U_b^H b)[:C]
         IA_HIP(hipMemcpy2DAsync(c, sizeof(cx<double>) * R, b, sizeof(cx<double>) * sb,
                                 sizeof(cx<double>) * rows, batch, hipMemcpyDeviceToDevice, s));
-        launch_apply_left<T>(true, R, C, Y, R, (size_t)R * C, sw.tauL.template as<cx<T>>(), c, R, 1,
-                             batch, s);
+        svd_apply_uh<T>(sw, Y, c, R, 1, s);
         launch_norm2<double>(C, c, R, bcn2.as<double>(), batch, s);
         launch_real_matvec(C, sw.PBt.template as<double>(), 0, c, R, bbeta.as<cx<double>>(), C,
                            batch, s);
@@ -95,8 +94,7 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
         launch_real_matvec(C, sw.PBt.template as<double>(), 1, bphi.as<cx<double>>(), C,
                            bbeta.as<cx<double>>(), C, batch, s);
         launch_pad_copy(C, R, bbeta.as<cx<double>>(), C, v, R, batch, s);
-        launch_apply_left<T>(false, R, C, Y, R, (size_t)R * C, sw.tauL.template as<cx<T>>(), v, R, 1,
-                             batch, s);
+        svd_apply_u<T>(sw, Y, v, R, 1, s);
         IA_HIP(hipMemcpy2DAsync(x, sizeof(cx<double>) * sx, v, sizeof(cx<double>) * R,
                                 sizeof(cx<double>) * R, batch, hipMemcpyDeviceToDevice, s));
     }

@@ -77,17 +77,19 @@ def test_csvd_against_lapack(ia, shape):
     np.testing.assert_allclose(V.conj().T @ V, np.eye(p), atol=1e-12)
 
 
-def test_csvd_graded_spectrum(ia):
+@pytest.mark.parametrize("shape", [(80, 60), (300, 80)])  # (300, 80): QR-first path
+def test_csvd_graded_spectrum(ia, shape):
     # exponentially decaying singular values down to ~1e-15 (the ESM regime)
-    rng = np.random.default_rng(7)
-    U0, _ = np.linalg.qr(_cr((80, 60), 8))
-    V0, _ = np.linalg.qr(_cr((60, 60), 9))
-    sv = 10.0 ** (-15 * np.arange(60) / 59)
+    mr, nc = shape
+    U0, _ = np.linalg.qr(_cr((mr, nc), 8))
+    V0, _ = np.linalg.qr(_cr((nc, nc), 9))
+    sv = 10.0 ** (-15 * np.arange(nc) / (nc - 1))
     A = (U0 * sv) @ V0.conj().T
     U, s, V = ia.csvd(A)
     s_np = np.linalg.svd(A, compute_uv=False)
-    assert np.abs(s - s_np).max() <= 4e-15 * s_np[0] * 60
-    np.testing.assert_allclose(s[:30], sv[:30], rtol=1e-6)
+    assert np.abs(s - s_np).max() <= 4e-15 * s_np[0] * nc
+    np.testing.assert_allclose(s[: nc // 2], sv[: nc // 2], rtol=1e-6)
+    np.testing.assert_allclose((U * s) @ V.conj().T, A, atol=1e-13 * np.sqrt(mr))
 
 
 def test_esm_legacy_matches_oracle(ia):
