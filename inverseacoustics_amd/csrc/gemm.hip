@@ -16,6 +16,10 @@
 // Workgroup tile 64x64, K step 16, 4 waves (one per SIMD), operands staged
 // in LDS as split real/imaginary planes in k-major order so that the MFMA
 // operand reads of 16 consecutive lanes hit 16 consecutive words.
+#include <rocblas/rocblas.h>
+
+#include <cstdlib>
+
 #include "ia_internal.hpp"
 
 namespace ia {
@@ -185,11 +189,65 @@ void dispatch_beta(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, si
 
 }  // namespace
 
+namespace {
+
+// rocBLAS handle of the current device for this host thread (created once)
+rocblas_handle blas_handle() {
+    static thread_local rocblas_handle h[16] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (!h[dev] && rocblas_create_handle(&h[dev]) != rocblas_status_success) h[dev] = nullptr;
+    return h[dev];
+}
+
+rocblas_operation blas_op(char op) {
+    return op == 'C' ? rocblas_operation_conjugate_transpose : rocblas_operation_none;
+}
+
+// Plain batched products go to rocBLAS (Tensile kernels tuned for gfx950
+// reach 70-95 % of the matrix-core peak on these shapes, against ~35 % for
+// the single-buffered 64 x 64 tile above, which remains for tiny products
+// and when IA_GEMM_OWN is set).  Returns false when not handled.
+template <typename T>
+bool blas_gemm(char opA, char opB, int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda,
+               size_t sA, const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc,
+               size_t sC, int batch, hipStream_t s) {
+    if ((double)M * N * K < 32768.0 || getenv("IA_GEMM_OWN")) return false;
+    rocblas_handle h = blas_handle();
+    if (!h || rocblas_set_stream(h, s) != rocblas_status_success) return false;
+    rocblas_status st;
+    if (sizeof(T) == sizeof(double))
+        st = rocblas_zgemm_strided_batched(
+            h, blas_op(opA), blas_op(opB), M, N, K,
+            reinterpret_cast<const rocblas_double_complex *>(&alpha),
+            reinterpret_cast<const rocblas_double_complex *>(A), lda, (rocblas_stride)sA,
+            reinterpret_cast<const rocblas_double_complex *>(B), ldb, (rocblas_stride)sB,
+            reinterpret_cast<const rocblas_double_complex *>(&beta),
+            reinterpret_cast<rocblas_double_complex *>(C), ldc, (rocblas_stride)sC, batch);
+    else
+        st = rocblas_cgemm_strided_batched(
+            h, blas_op(opA), blas_op(opB), M, N, K,
+            reinterpret_cast<const rocblas_float_complex *>(&alpha),
+            reinterpret_cast<const rocblas_float_complex *>(A), lda, (rocblas_stride)sA,
+            reinterpret_cast<const rocblas_float_complex *>(B), ldb, (rocblas_stride)sB,
+            reinterpret_cast<const rocblas_float_complex *>(&beta),
+            reinterpret_cast<rocblas_float_complex *>(C), ldc, (rocblas_stride)sC, batch);
+    return st == rocblas_status_success;
+}
+
+}  // namespace
+
 template <typename T>
 void launch_gemm(char opA, char opB, int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda,
                  size_t strideA, const cx<T> *B, int ldb, size_t strideB, cx<T> beta, cx<T> *C,
                  int ldc, size_t strideC, int batch, hipStream_t s) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
+    if (blas_gemm<T>(opA, opB, M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
+                     strideC, batch, s))
+        return;
     if (opA == 'N' && opB == 'N')
         dispatch_beta<T, 'N', 'N'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
                                    strideC, batch, s);

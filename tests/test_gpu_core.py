@@ -16,9 +16,21 @@ def _cr(shape, seed):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
+@pytest.fixture(params=["library", "own"])
+def gemm_path(request, monkeypatch):
+    """Large products run on rocBLAS, tiny ones (and all with IA_GEMM_OWN set)
+    on the hand-written MFMA tile kernel; both are checked."""
+    if request.param == "own":
+        monkeypatch.setenv("IA_GEMM_OWN", "1")
+    else:
+        monkeypatch.delenv("IA_GEMM_OWN", raising=False)
+    return request.param
+
+
 @pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("C", "C")])
-@pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 5, 3), (64, 64, 16), (130, 77, 45), (5, 200, 1)])
-def test_zgemm(ia, opa, opb, mnk):
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 5, 3), (64, 64, 16), (130, 77, 45), (5, 200, 1),
+                                 (300, 257, 96)])
+def test_zgemm(ia, opa, opb, mnk, gemm_path):
     m, n, k = mnk
     A = _cr((m, k) if opa == "N" else (k, m), 1)
     B = _cr((k, n) if opb == "N" else (n, k), 2)
@@ -31,8 +43,8 @@ def test_zgemm(ia, opa, opb, mnk):
 
 
 @pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("C", "C")])
-@pytest.mark.parametrize("mnk", [(17, 5, 3), (130, 77, 45)])
-def test_cgemm_fp32(ia, opa, opb, mnk):
+@pytest.mark.parametrize("mnk", [(17, 5, 3), (130, 77, 45), (300, 257, 96)])
+def test_cgemm_fp32(ia, opa, opb, mnk, gemm_path):
     m, n, k = mnk
     A = _cr((m, k) if opa == "N" else (k, m), 1).astype(np.complex64)
     B = _cr((k, n) if opb == "N" else (n, k), 2).astype(np.complex64)
