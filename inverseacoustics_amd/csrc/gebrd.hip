@@ -35,6 +35,168 @@ using hh::wave_sum;
 
 constexpr int NB = 32;
 
+// Rows are read in groups of G consecutive entries per lane: G = 2 for
+// complex64 (one 16-byte load per lane, as for complex128), else 1.
+template <typename T, int G>
+struct RowGroup;
+template <typename T>
+struct RowGroup<T, 1> {
+    static __device__ __forceinline__ void load(const cx<T> *col, int g, cx<double> (&o)[1]) {
+        o[0] = cvt<double>(col[g]);
+    }
+};
+template <>
+struct RowGroup<float, 2> {
+    static __device__ __forceinline__ void load(const cx<float> *col, int g, cx<double> (&o)[2]) {
+        const float4 a = reinterpret_cast<const float4 *>(col)[g];
+        o[0] = cx<double>{a.x, a.y};
+        o[1] = cx<double>{a.z, a.w};
+    }
+};
+
+// Unroll depth of the streaming loops: UL independent 16-byte loads per lane
+// are in flight before their products are accumulated (a CU needs ~64 KB in
+// flight to cover the HBM latency at its share of the bandwidth).
+#ifndef IA_UL32
+#define IA_UL32 4
+#endif
+#ifndef IA_UL64
+#define IA_UL64 2
+#endif
+template <typename T>
+constexpr int unroll_of() { return sizeof(T) == sizeof(double) ? IA_UL64 : IA_UL32; }
+
+// sum_{r >= j} conj(u_r) col_r by one wave (rows below j of u are zero)
+template <typename T, int G>
+__device__ __forceinline__ cx<double> wave_dot_u(const cx<T> *col, const cx<T> *u, int j, int R,
+                                                  int lane) {
+    constexpr int UL = unroll_of<T>();
+    double ar = 0, ai = 0;
+    const int ng = R / G;
+    int g = j / G + lane;
+    for (; g + 64 * (UL - 1) < ng; g += 64 * UL) {
+        cx<double> y[UL][G], uu[UL][G];
+#pragma unroll
+        for (int q = 0; q < UL; ++q) RowGroup<T, G>::load(col, g + 64 * q, y[q]);
+#pragma unroll
+        for (int q = 0; q < UL; ++q) RowGroup<T, G>::load(u, g + 64 * q, uu[q]);
+#pragma unroll
+        for (int q = 0; q < UL; ++q)
+#pragma unroll
+            for (int e = 0; e < G; ++e) {
+                ar += uu[q][e].re * y[q][e].re + uu[q][e].im * y[q][e].im;
+                ai += uu[q][e].re * y[q][e].im - uu[q][e].im * y[q][e].re;
+            }
+    }
+    for (; g < ng; g += 64) {
+        cx<double> y0[G], u0[G];
+        RowGroup<T, G>::load(col, g, y0);
+        RowGroup<T, G>::load(u, g, u0);
+#pragma unroll
+        for (int e = 0; e < G; ++e) {
+            ar += u0[e].re * y0[e].re + u0[e].im * y0[e].im;
+            ai += u0[e].re * y0[e].im - u0[e].im * y0[e].re;
+        }
+    }
+    return cx<double>{wave_sum(ar), wave_sum(ai)};
+}
+
+// acc[e] += sum_{c in [c0, c1)} A[G g + e, c] v[c]
+template <typename T, int G>
+__device__ __forceinline__ void rows_times_v(const cx<T> *Y, size_t ldy, const cx<T> *v, int g,
+                                             int c0, int c1, cx<double> (&acc)[G]) {
+    constexpr int UL = unroll_of<T>();
+    int c = c0;
+    for (; c + UL - 1 < c1; c += UL) {
+        cx<double> a[UL][G];
+#pragma unroll
+        for (int q = 0; q < UL; ++q) RowGroup<T, G>::load(Y + (size_t)(c + q) * ldy, g, a[q]);
+#pragma unroll
+        for (int q = 0; q < UL; ++q) {
+            const cx<double> vq = cvt<double>(v[c + q]);
+#pragma unroll
+            for (int e = 0; e < G; ++e) acc[e] = acc[e] + a[q][e] * vq;
+        }
+    }
+    for (; c < c1; ++c) {
+        cx<double> a0[G];
+        RowGroup<T, G>::load(Y + (size_t)c * ldy, g, a0);
+        const cx<double> v0 = cvt<double>(v[c]);
+#pragma unroll
+        for (int e = 0; e < G; ++e) acc[e] = acc[e] + a0[e] * v0;
+    }
+}
+
+// acc[q] -= U[r, k] py[k] + Xw[r, k] pv[k] (k < i) + u[r] py[i], r = G g + q
+template <typename T, int G>
+__device__ __forceinline__ void rows_corrections(const cx<T> *Y, size_t ldy, const cx<T> *Xw,
+                                                 int R, const cx<T> *u, int j0, int i, int g,
+                                                 const cx<double> *py, const cx<double> *pv,
+                                                 cx<double> (&acc)[G]) {
+    for (int k = 0; k < i; ++k) {
+        cx<double> uk[G], xk[G];
+        RowGroup<T, G>::load(Y + (size_t)(j0 + k) * ldy, g, uk);
+        RowGroup<T, G>::load(Xw + (size_t)k * R, g, xk);
+#pragma unroll
+        for (int q = 0; q < G; ++q) acc[q] = acc[q] - uk[q] * py[k] - xk[q] * pv[k];
+    }
+    cx<double> ur[G];
+    RowGroup<T, G>::load(u, g, ur);
+#pragma unroll
+    for (int q = 0; q < G; ++q) acc[q] = acc[q] - ur[q] * py[i];
+}
+
+// x = sigma (A v - U py - Xw pv - u py_i) for rows r > j of the step
+template <typename T, int G>
+__device__ __forceinline__ void panel_gemv2(const cx<T> *Y, int ldy, cx<T> *Xw, int R, int C,
+                                            const cx<T> *u, const cx<T> *v, int j0, int i, int j,
+                                            const cx<double> *py, const cx<double> *pv,
+                                            cx<double> sig, cx<double> *part, int lane, int wv) {
+    const int ng = R / G, g0 = (j + 1) / G;
+    const int nrb = (ng - g0 + 63) / 64;
+    const int nt = C - 1 - j;
+    if (nrb >= NW) {
+        for (int rb = wv; rb < nrb; rb += NW) {
+            const int g = g0 + rb * 64 + lane;
+            if (g >= ng) continue;
+            cx<double> acc[G];
+#pragma unroll
+            for (int q = 0; q < G; ++q) acc[q] = cx<double>{0, 0};
+            rows_times_v<T, G>(Y, ldy, v, g, j + 1, C, acc);
+            rows_corrections<T, G>(Y, ldy, Xw, R, u, j0, i, g, py, pv, acc);
+#pragma unroll
+            for (int q = 0; q < G; ++q)
+                if (G * g + q > j) Xw[G * g + q + (size_t)i * R] = cvt<T>(sig * acc[q]);
+        }
+    } else {
+        // fewer row blocks than waves: split the trailing columns into ns
+        // slices and reduce the partial sums through LDS
+        const int ns = NW / nrb;
+        const int rb = wv % nrb, sl = wv / nrb;
+        const int g = g0 + rb * 64 + lane;
+        cx<double> acc[G];
+#pragma unroll
+        for (int q = 0; q < G; ++q) acc[q] = cx<double>{0, 0};
+        if (sl < ns && g < ng) {
+            const int chunk = (nt + ns - 1) / ns;
+            const int c0 = j + 1 + sl * chunk, c1 = min(C, c0 + chunk);
+            rows_times_v<T, G>(Y, ldy, v, g, c0, c1, acc);
+            if (sl == 0) rows_corrections<T, G>(Y, ldy, Xw, R, u, j0, i, g, py, pv, acc);
+        }
+#pragma unroll
+        for (int q = 0; q < G; ++q) part[(wv * 64 + lane) * 2 + q] = acc[q];
+        __syncthreads();
+        if (wv < nrb && g < ng) {
+#pragma unroll
+            for (int q = 0; q < G; ++q) {
+                cx<double> sum{0, 0};
+                for (int z = 0; z < ns; ++z) sum = sum + part[((z * nrb + wv) * 64 + lane) * 2 + q];
+                if (G * g + q > j) Xw[G * g + q + (size_t)i * R] = cvt<T>(sig * sum);
+            }
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(BT) void k_labrd(int R, int C, int j0, int nbp,
                                              cx<T> *__restrict__ Yall, int ldy, size_t sY,
@@ -45,8 +207,8 @@ __global__ __launch_bounds__(BT) void k_labrd(int R, int C, int j0, int nbp,
                                              cx<T> *__restrict__ Ywall,
                                              cx<T> *__restrict__ Vwall) {
     extern __shared__ unsigned char smem_raw[];
-    cx<double> *part = reinterpret_cast<cx<double> *>(smem_raw);  // NW * 64
-    cx<double> *pu = part + NW * 64;                                // NB: u^H U_k
+    cx<double> *part = reinterpret_cast<cx<double> *>(smem_raw);  // NW * 64 * 2
+    cx<double> *pu = part + NW * 64 * 2;                            // NB: u^H U_k
     cx<double> *pq = pu + NB;                                       // NB: u^H Xw_k
     cx<double> *py = pq + NB;                                       // NB+1: Yw_k^H v
     cx<double> *pv = py + NB + 1;                                   // NB: Vw_k^H v
@@ -69,6 +231,7 @@ __global__ __launch_bounds__(BT) void k_labrd(int R, int C, int j0, int nbp,
     cx<T> *Yw = Ywall + (size_t)b * C * NB;
     cx<T> *Vw = Vwall + (size_t)b * C * NB;
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const bool pairs = ((R | ldy) & 1) == 0;  // complex64 rows read two at a time
 
     for (int i = 0; i < nbp; ++i) {
         const int j = j0 + i;
@@ -101,6 +264,7 @@ __global__ __launch_bounds__(BT) void k_labrd(int R, int C, int j0, int nbp,
         }
         if (t == 0) {
             u[j] = cx<T>{1, 0};
+            if (j & 1) u[j - 1] = cx<T>{0, 0};  // row pairs start at even rows
             Y[j + (size_t)j * ldy] = cx<T>{(T)rf.beta, 0};
             d[j] = rf.beta;
             tauL[j] = cvt<T>(rf.tau);
@@ -121,22 +285,16 @@ __global__ __launch_bounds__(BT) void k_labrd(int R, int C, int j0, int nbp,
                 col = Y + (size_t)(j0 + vc - nt) * ldy;
             else
                 col = Xw + (size_t)(vc - nt - i) * R;
-            double ar0 = 0, ai0 = 0, ar1 = 0, ai1 = 0;
-            int r = j + lane;
-            for (; r + 64 < R; r += 128) {
-                const cx<T> y0 = col[r], y1 = col[r + 64];
-                const cx<T> u0 = u[r], u1 = u[r + 64];
-                ar0 += (double)u0.re * y0.re + (double)u0.im * y0.im;
-                ai0 += (double)u0.re * y0.im - (double)u0.im * y0.re;
-                ar1 += (double)u1.re * y1.re + (double)u1.im * y1.im;
-                ai1 += (double)u1.re * y1.im - (double)u1.im * y1.re;
+            cx<double> wd;
+            if constexpr (sizeof(T) == 4) {
+                if (pairs)
+                    wd = wave_dot_u<T, 2>(col, u, j, R, lane);
+                else
+                    wd = wave_dot_u<T, 1>(col, u, j, R, lane);
+            } else {
+                wd = wave_dot_u<T, 1>(col, u, j, R, lane);
             }
-            if (r < R) {
-                const cx<T> y0 = col[r], u0 = u[r];
-                ar0 += (double)u0.re * y0.re + (double)u0.im * y0.im;
-                ai0 += (double)u0.re * y0.im - (double)u0.im * y0.re;
-            }
-            const double ar = wave_sum(ar0 + ar1), ai = wave_sum(ai0 + ai1);
+            const double ar = wd.re, ai = wd.im;
             if (lane == 0) {
                 const cx<double> w{ar, ai};
                 if (vc < nt)
@@ -203,60 +361,13 @@ __global__ __launch_bounds__(BT) void k_labrd(int R, int C, int j0, int nbp,
         __syncthreads();
         // ---- x = sigma (A v - U py - Xw pv - u py_i), rows r > j
         const cx<double> sig = rr.tau;
-        const int r0 = j + 1;
-        const int nrb = (R - r0 + 63) / 64;
-        if (nrb >= NW) {
-            for (int rb = wv; rb < nrb; rb += NW) {
-                const int r = r0 + rb * 64 + lane;
-                if (r >= R) continue;
-                cx<double> acc0{0, 0}, acc1{0, 0};
-                int c = j + 1;
-                for (; c + 1 < C; c += 2) {
-                    const cx<T> a0 = Y[r + (size_t)c * ldy], a1 = Y[r + (size_t)(c + 1) * ldy];
-                    acc0 = acc0 + cvt<double>(a0) * cvt<double>(v[c]);
-                    acc1 = acc1 + cvt<double>(a1) * cvt<double>(v[c + 1]);
-                }
-                if (c < C) acc0 = acc0 + cvt<double>(Y[r + (size_t)c * ldy]) * cvt<double>(v[c]);
-                cx<double> acc = acc0 + acc1;
-                for (int k = 0; k < i; ++k)
-                    acc = acc - cvt<double>(Y[r + (size_t)(j0 + k) * ldy]) * py[k] -
-                          cvt<double>(Xw[r + (size_t)k * R]) * pv[k];
-                acc = acc - cvt<double>(u[r]) * py[i];
-                Xw[r + (size_t)i * R] = cvt<T>(sig * acc);
-            }
+        if constexpr (sizeof(T) == 4) {
+            if (pairs)
+                panel_gemv2<T, 2>(Y, ldy, Xw, R, C, u, v, j0, i, j, py, pv, sig, part, lane, wv);
+            else
+                panel_gemv2<T, 1>(Y, ldy, Xw, R, C, u, v, j0, i, j, py, pv, sig, part, lane, wv);
         } else {
-            // fewer row blocks than waves: split the trailing columns into
-            // ns slices and reduce the partial sums through LDS
-            const int ns = NW / nrb;
-            const int rb = wv % nrb, sl = wv / nrb;
-            const int r = r0 + rb * 64 + lane;
-            cx<double> acc{0, 0};
-            if (sl < ns && r < R) {
-                const int chunk = (nt + ns - 1) / ns;
-                const int c0 = j + 1 + sl * chunk, c1 = min(C, c0 + chunk);
-                cx<double> acc1{0, 0};
-                int c = c0;
-                for (; c + 1 < c1; c += 2) {
-                    const cx<T> a0 = Y[r + (size_t)c * ldy], a1 = Y[r + (size_t)(c + 1) * ldy];
-                    acc = acc + cvt<double>(a0) * cvt<double>(v[c]);
-                    acc1 = acc1 + cvt<double>(a1) * cvt<double>(v[c + 1]);
-                }
-                if (c < c1) acc = acc + cvt<double>(Y[r + (size_t)c * ldy]) * cvt<double>(v[c]);
-                acc = acc + acc1;
-                if (sl == 0) {
-                    for (int k = 0; k < i; ++k)
-                        acc = acc - cvt<double>(Y[r + (size_t)(j0 + k) * ldy]) * py[k] -
-                              cvt<double>(Xw[r + (size_t)k * R]) * pv[k];
-                    acc = acc - cvt<double>(u[r]) * py[i];
-                }
-            }
-            part[wv * 64 + lane] = acc;
-            __syncthreads();
-            if (wv < nrb && r < R) {
-                cx<double> s{0, 0};
-                for (int q = 0; q < ns; ++q) s = s + part[(q * nrb + wv) * 64 + lane];
-                Xw[r + (size_t)i * R] = cvt<T>(sig * s);
-            }
+            panel_gemv2<T, 1>(Y, ldy, Xw, R, C, u, v, j0, i, j, py, pv, sig, part, lane, wv);
         }
         __syncthreads();
     }
@@ -484,7 +595,7 @@ __global__ __launch_bounds__(256) void k_lb_gemv2(int R, int C, int j0, int i,
 }  // namespace
 
 size_t gebrd_lds_bytes(int R, int C, size_t elem) {
-    return ((size_t)NW * 64 + 8 * NB + 1) * sizeof(cx<double>) + NW * sizeof(double) +
+    return ((size_t)NW * 64 * 2 + 8 * NB + 1) * sizeof(cx<double>) + NW * sizeof(double) +
            ((size_t)R + C) * elem;
 }
 
