@@ -133,10 +133,11 @@ def main():
     db_c, idx_c, vel_c = errors(o_cert)
 
     # ---- roofline of the dominant kernel: the panel bidiagonalisation
-    # k_labrd (HBM-bound).  Per step j it reads the trailing block of the
-    # m x l matrix H_e^H twice (rows >= j and > j, columns > j), so a launch
-    # over panel columns j0..j0+31 moves sum_j (2(m-j)-1)(l-1-j) * sizeof(c)
-    # bytes per frequency, times the frequencies of the launch.  The library
+    # k_labrd (HBM-bound) of the l x l triangular factor of H_e^H = Q R.  Per
+    # step j it reads the trailing block twice (rows >= j and > j, columns
+    # > j), so a launch over panel columns j0..j0+31 moves
+    # sum_j (2(l-j)-1)(l-1-j) * sizeof(c) bytes per frequency, times the
+    # frequencies of the launch.  The library
     # brackets every launch with HIP events on the sweep's stream and
     # reports the summed duration, bytes and launch count.
     def roof(st, form, esz):
@@ -199,7 +200,10 @@ def main():
 # HBM bytes per k_labrd launch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; FETCH scaled by
 # 1.99, the WRITE/FETCH ratio of a plain 1 GiB device copy in the same runs) keyed by
 # (form, frequencies, m, l, chunk): profiles/labrd_esm_sweep_256f_1024x2048_hbm_counters.txt
-HBM_TRAFFIC_PER_LAUNCH = {}
+HBM_TRAFFIC_PER_LAUNCH = {
+    ("fp64", 256, 2048, 1024, 0): 103207968929,
+    ("mixed", 256, 2048, 1024, 0): 52029598145,
+}
 
 
 def cpu_baseline(P, nf):

@@ -60,8 +60,9 @@ ia_status ia_set_air(double c, double rho);
 ia_status ia_get_air(double *c, double *rho);
 /* Number of visible HIP devices (0 when no GPU). */
 int ia_device_count(void);
-/* Wait for outstanding device work of this library on the current device
- * (no device state is cached between calls, so there is nothing to free). */
+/* Wait for outstanding device work of this thread and free the device
+ * workspace the sweeps keep between calls (it only grows, per host thread
+ * and precision, so repeated sweeps allocate nothing). */
 void ia_release(void);
 
 /* -------------------------------------------------- ESM (esm.cuh) */

@@ -83,6 +83,27 @@ struct Timer {
     }
 };
 
+// Device workspace of the ESM sweep, kept per host thread and precision
+// between calls (buffers only grow); ia_release() frees it.  Allocated with
+// new and never destroyed at thread exit, so no hipFree runs after the HIP
+// runtime has shut down.
+template <typename T>
+struct SweepCache {
+    DevBuf bD, bY, bipiv, binfo, bc, bbeta, bphi, bx, bq, bps, bpsd, bn2, bcn2;
+    SvdWork<T> sw;
+    void release() {
+        for (DevBuf *b : {&bD, &bY, &bipiv, &binfo, &bc, &bbeta, &bphi, &bx, &bq, &bps, &bpsd,
+                          &bn2, &bcn2, &sw.d, &sw.e, &sw.sig, &sw.tauL, &sw.tauR, &sw.Vr, &sw.QBt,
+                          &sw.PBt, &sw.work, &sw.gw, &sw.Rb, &sw.tauQ, &sw.qw})
+            b->reset();
+    }
+};
+template <typename T>
+static SweepCache<T> &sweep_cache() {
+    static thread_local SweepCache<T> *c = new SweepCache<T>();
+    return *c;
+}
+
 template <typename T>
 static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const double *r_s,
                                 const double *src, const double *nrm, const double *r_h,
@@ -99,7 +120,10 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
                D.reg.k, D.reg.decades > 0 ? D.reg.decades : 8.0, D.reg.lambda};
     if (prm.n_lambda > 512) return fail(IA_ERR_ARG, "n_lambda must be <= 512");
 
-    DevBuf bD, bY, bipiv, binfo, bc, bbeta, bphi, bx, bq, bps, bpsd, bn2, bcn2, blam, bidx, bk, bW;
+    SweepCache<T> &W = sweep_cache<T>();
+    DevBuf &bD = W.bD, &bY = W.bY, &bipiv = W.bipiv, &binfo = W.binfo, &bc = W.bc,
+           &bbeta = W.bbeta, &bphi = W.bphi, &bx = W.bx, &bq = W.bq, &bps = W.bps, &bpsd = W.bpsd,
+           &bn2 = W.bn2, &bcn2 = W.bcn2;
     IA_TRY(bD.alloc(sizeof(cx<T>) * (size_t)chunk * m * m));
     IA_TRY(bY.alloc(sizeof(cx<T>) * (size_t)chunk * m * l));
     IA_TRY(bipiv.alloc(sizeof(int) * (size_t)chunk * m));
@@ -113,7 +137,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
     IA_TRY(bpsd.alloc(sizeof(cx<double>) * (size_t)chunk * m));
     IA_TRY(bn2.alloc(sizeof(double) * (size_t)chunk));
     IA_TRY(bcn2.alloc(sizeof(double) * (size_t)chunk));
-    SvdWork<T> sw;
+    SvdWork<T> &sw = W.sw;
     IA_TRY(sw.alloc(R, C, chunk));
 
     Timer tm(s);
@@ -248,7 +272,11 @@ int ia_device_count(void) {
     return n;
 }
 
-void ia_release(void) { (void)hipDeviceSynchronize(); }
+void ia_release(void) {
+    (void)hipDeviceSynchronize();
+    sweep_cache<double>().release();
+    sweep_cache<float>().release();
+}
 
 ia_status ia_last_stats(ia_stats *st) {
     if (!st) return fail(IA_ERR_ARG, "ia_last_stats: NULL");

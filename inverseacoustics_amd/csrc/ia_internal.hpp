@@ -340,13 +340,18 @@ struct SvdWork {
         // covers a whole 256-frequency batch (one thread per singular value
         // needs >= 1024 * 256 threads to fill the device)
         size_t per = inviter_work_bytes(C, 1);
-        size_t fr = 0, tot = 0;
-        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
-            (void)hipGetLastError();
-            fr = (size_t)4 << 30;
+        if (work.bytes >= inviter_work_bytes(C, batch)) {
+            sub = batch;  // a kept buffer already holds the whole batch
+        } else {
+            size_t fr = 0, tot = 0;
+            if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+                (void)hipGetLastError();
+                fr = (size_t)4 << 30;
+            }
+            const size_t cap = std::min<size_t>((size_t)32 << 30,
+                                                std::max<size_t>((size_t)1 << 30, fr / 4));
+            sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)batch, cap / per));
         }
-        const size_t cap = std::min<size_t>((size_t)32 << 30, std::max<size_t>((size_t)1 << 30, fr / 4));
-        sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)batch, cap / per));
         IA_TRY(work.alloc(inviter_work_bytes(C, sub)));
         return IA_OK;
     }
