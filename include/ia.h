@@ -159,7 +159,10 @@ typedef enum {
     IA_SELECT_FIXED = 2,      /* use lambda / k given by the caller         */
     IA_SELECT_GCV_REGTOOLS = 3,   /* literal gcvG.m: 200-point grid + fminbnd */
     IA_SELECT_LCURVE_REGTOOLS = 4,/* literal l_curveG.m: 200 points + fminbnd */
-    IA_SELECT_QUASIOPT = 5        /* quasioptG.m: 200 points + fminbnd        */
+    IA_SELECT_QUASIOPT = 5,       /* quasioptG.m: 200 points + fminbnd        */
+    IA_SELECT_RANK = 6            /* TSVD: k = numerical rank (singular values
+                                     > eps max(rows, cols) s_1): the minimum-
+                                     norm least-squares solution            */
 } ia_select;
 
 typedef struct {
@@ -311,6 +314,24 @@ ia_status ia_regsolve(int rows, int cols, int batch, const ia_z *a,
                       const ia_z *b, const ia_reg_params *prm,
                       ia_precision precision, ia_z *x, double *lambda,
                       int *index, int *k_tsvd, double *sigma_max);
+
+/* ADMM regularised solutions (MATLAB/ADMMG.m:1-89) of a batch of systems
+ * A_f c_f = b_f (rows x cols each, column-major, batch stride rows*cols;
+ * host or device pointers), FP64.  norm 1: L1 (complex soft threshold,
+ * continuation lambda = 5 lambda0 * 0.75^k while > lambda0 / 10); norm 2:
+ * L2; norm 3: Lp with 0 < p < 1 (generalised soft thresholding, n_gst
+ * fixed-point steps).  Iterations stop when |c - c_old|, |c - x| and
+ * |u - u_old| are all <= epsilon, at most max_iter per continuation stage.
+ * The start is the minimum-norm least-squares solution (ADMMG.m uses
+ * MATLAB's A\b).  c: [batch * cols]; iterations: [batch] (may be NULL). */
+typedef struct {
+    double lambda0, rho, gamma, epsilon;
+    int max_iter, norm;
+    double p;
+    int n_gst;
+} ia_admm_params;
+ia_status ia_admm(int rows, int cols, int batch, const ia_z *a, const ia_z *b,
+                  const ia_admm_params *prm, ia_z *c, int *iterations);
 
 /* Phase timings (ms) of the last sweep on this thread, plus the summed
  * HIP-event duration, algorithmic bytes and launch count of the panel

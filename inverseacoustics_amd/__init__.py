@@ -21,13 +21,13 @@ __all__ = [
     "IAError", "set_air", "get_air", "device_count", "esm", "esmgpu", "esm_matrices", "csvd",
     "tsvd", "tikhonov", "dsvd", "regularize", "to_pressure", "to_velocity", "sound_power",
     "sphere", "cartesian_2_spherical", "geometry", "esm_sweep", "SweepResult", "zgemm", "cgemm", "zgesv", "hels", "hels_sweep", "regsolve",
-    "quadrature_load", "set_quadrature", "ibem", "ibem_sweep", "invr", "radiation_modes",
+    "quadrature_load", "set_quadrature", "ibem", "ibem_sweep", "invr", "radiation_modes", "admm",
     "FILTERS", "SELECTS",
 ]
 
 FILTERS = {"tikhonov": 0, "tsvd": 1, "dsvd": 2}
 SELECTS = {"gcv": 0, "lcurve": 1, "fixed": 2, "gcv_regtools": 3, "lcurve_regtools": 4,
-           "quasiopt": 5}
+           "quasiopt": 5, "rank": 6}
 
 
 def set_air(c: float, rho: float):
@@ -301,6 +301,25 @@ def regsolve(A, b, filter="tikhonov", select="gcv", n_lambda=50, decades=8.0, la
                             1 if precision == "mixed" else 0, _lib.zptr(x), _lib.dptr(la),
                             ix.ctypes.data_as(_lib._ip), ks.ctypes.data_as(_lib._ip), _lib.dptr(sm)))
     return x, la, ix, ks, sm
+
+
+def admm(A, b, lambda0, rho=1.0, gamma=1.0, epsilon=1e-6, max_iter=100, norm=1, p=0.5, n_gst=5):
+    """ADMM regularised solution (MATLAB/ADMMG.m): A (batch, rows, cols) or
+    (rows, cols), b (batch, rows) -> (c (batch, cols), iterations (batch,))."""
+    A = np.asarray(A, dtype=np.complex128)
+    single = A.ndim == 2
+    if single:
+        A = A[None]
+    batch, rows, cols = A.shape
+    Af = np.ascontiguousarray(np.transpose(A, (0, 2, 1)))
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).reshape(batch, rows))
+    c = np.zeros((batch, cols), np.complex128)
+    it = np.zeros(batch, np.int32)
+    prm = _lib.ia_admm_params(float(lambda0), float(rho), float(gamma), float(epsilon),
+                              int(max_iter), int(norm), float(p), int(n_gst))
+    check(lib().ia_admm(rows, cols, batch, _lib.zptr(Af), _lib.zptr(b), C.byref(prm),
+                        _lib.zptr(c), it.ctypes.data_as(_lib._ip)))
+    return (c[0], it[0]) if single else (c, it)
 
 
 # ----------------------------------------------------------------- IBEM

@@ -51,6 +51,12 @@ class ia_ibem_desc(C.Structure):
                 ("area", C.c_double), ("chunk", C.c_int)]
 
 
+class ia_admm_params(C.Structure):
+    _fields_ = [("lambda0", C.c_double), ("rho", C.c_double), ("gamma", C.c_double),
+                ("epsilon", C.c_double), ("max_iter", C.c_int), ("norm", C.c_int),
+                ("p", C.c_double), ("n_gst", C.c_int)]
+
+
 class ia_stats(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("assemble", "factor", "solve_he", "bidiag", "bisect", "vectors", "select",
@@ -126,6 +132,8 @@ SIGNATURES = {
                           C.c_int, _zpp, _zpp, _zpp, _zpp]),
     "ia_radiation_modes": (C.c_int, [C.c_int, _dp, C.c_double, C.c_double, C.c_int, _dpp,
                                      _dpp]),
+    "ia_admm": (C.c_int, [C.c_int, C.c_int, C.c_int, _zp, _zp, C.POINTER(ia_admm_params), _zp,
+                          _ip]),
     "ia_cgemm": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, ia_c, C.c_void_p, C.c_int,
                            C.c_void_p, C.c_int, ia_c, C.c_void_p, C.c_int]),
 }

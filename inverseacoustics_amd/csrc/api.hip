@@ -293,7 +293,7 @@ static ia_status sweep_check(const ia_sweep_desc *desc, const double *k, const d
     if (!k || !r_s || !src || !r_h || !b) return fail(IA_ERR_ARG, "ia_esm_sweep: NULL input");
     if (desc->reg.filter < IA_FILTER_TIKHONOV || desc->reg.filter > IA_FILTER_DSVD)
         return fail(IA_ERR_ARG, "ia_esm_sweep: bad filter");
-    if (desc->reg.select < IA_SELECT_GCV || desc->reg.select > IA_SELECT_QUASIOPT)
+    if (desc->reg.select < IA_SELECT_GCV || desc->reg.select > IA_SELECT_RANK)
         return fail(IA_ERR_ARG, "ia_esm_sweep: bad select");
     if (desc->reg.filter == IA_FILTER_TSVD && desc->reg.select == IA_SELECT_FIXED &&
         (desc->reg.k < 1 || desc->reg.k > std::min(desc->m, desc->l)))

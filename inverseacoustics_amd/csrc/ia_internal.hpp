@@ -156,8 +156,7 @@ ia_status launch_geqr2(int R, int C, cx<T> *A, int lda, size_t strideA, cx<T> *t
                        hipStream_t s);
 
 // SVD building blocks (svd.hip, gebrd.hip)
-size_t bidiag_lds_bytes(int R, int C, size_t elem);
-// panel-blocked bidiagonalisation (same outputs as launch_bidiag); work holds
+// panel-blocked bidiagonalisation Y = U_b B V_b^H (R >= C); work holds
 // gebrd_work_elems(R, C) complex T per batch item
 size_t gebrd_lds_bytes(int R, int C, size_t elem);
 size_t gebrd_work_elems(int R, int C);
@@ -170,6 +169,10 @@ ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, c
 template <typename T>
 void launch_copy_upper(int C, const cx<T> *Y, int ldy, size_t sY, cx<T> *Rb, int batch,
                        hipStream_t s);
+
+// ADMM (admm.hip): A (rows x cols, batch stride rows*cols), b, c on the device
+ia_status admm_batched(int rows, int cols, int batch, const cx<double> *A, const cx<double> *b,
+                       const ia_admm_params &P, cx<double> *c, int *iters, hipStream_t s);
 
 // HIP events around every launch of one kernel on its stream, with the
 // algorithmic bytes those launches move (read after the stream is synced)
@@ -205,9 +208,6 @@ template <typename T>
 ia_status launch_gebrd(int R, int C, cx<T> *Y, int ldy, size_t sY, double *d, double *e,
                        cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, cx<T> *work, int batch,
                        hipStream_t s, KernelClock *clk = nullptr);
-template <typename T>
-void launch_bidiag(int R, int C, cx<T> *Y, int ldy, size_t strideY, double *d, double *e,
-                   cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, int batch, hipStream_t s);
 void launch_bisect(int n, const double *d, const double *e, double *sigma, int batch,
                    hipStream_t s);
 size_t inviter_work_bytes(int n, int sub);

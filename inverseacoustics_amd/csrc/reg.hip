@@ -397,6 +397,14 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
         }
     } else if (prm.select == IA_SELECT_FIXED) {
         kk = prm.k_fixed;
+    } else if (prm.select == IA_SELECT_RANK) {
+        // numerical rank with the cutoff of LAPACK-based least squares
+        // (numpy lstsq rcond = eps max(rows, cols))
+        const int dim = prm.k_fixed > 0 ? prm.k_fixed : max(rows, p);  // max(rows, cols)
+        const double cut = 2.220446049250313e-16 * (double)dim * smax;
+        int r = 0;
+        for (int i = 0; i < p; ++i) r += (s[i] > cut) ? 1 : 0;
+        kk = r;
     }
     // ---- filtered coefficients
     for (int i = t; i < p; i += blockDim.x) {

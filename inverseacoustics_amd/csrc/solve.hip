@@ -77,8 +77,10 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
         launch_real_matvec(C, sw.QBt.template as<double>(), 0, c, C, bbeta.as<cx<double>>(), C,
                            batch, s);
     }
+    RegDev prm2 = prm;
+    if (prm2.select == IA_SELECT_RANK) prm2.k_fixed = std::max(rows, cols);  // rank cutoff dimension
     launch_select(C, rows, sw.sig.template as<double>(), bbeta.as<cx<double>>(), bn2.as<double>(),
-                  prm, lam, idx, kts, bphi.as<cx<double>>(), batch, s, gap, bcn2.as<double>(),
+                  prm2, lam, idx, kts, bphi.as<cx<double>>(), batch, s, gap, bcn2.as<double>(),
                   svd_floor_rel<T>());
     cx<double> *v = bv.as<cx<double>>();
     if (tall) {
@@ -123,7 +125,7 @@ RegDev to_regdev(const ia_reg_params &p) {
 ia_status check_reg(const ia_reg_params &p, int pmax) {
     if (p.filter < IA_FILTER_TIKHONOV || p.filter > IA_FILTER_DSVD)
         return fail(IA_ERR_ARG, "bad filter");
-    if (p.select < IA_SELECT_GCV || p.select > IA_SELECT_QUASIOPT)
+    if (p.select < IA_SELECT_GCV || p.select > IA_SELECT_RANK)
         return fail(IA_ERR_ARG, "bad select");
     if (p.n_lambda > 512) return fail(IA_ERR_ARG, "n_lambda must be <= 512");
     if (p.filter == IA_FILTER_TSVD && p.select == IA_SELECT_FIXED && (p.k < 1 || p.k > pmax))

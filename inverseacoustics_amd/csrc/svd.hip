@@ -2,13 +2,11 @@
 //
 // Replaces the reference's cusolverDnZgesvd ('S','A') in csvd
 // (CUDA/tsvd.cuh:39,93) and LAPACK zgesvd_ in invr (CUDA/invr.cuh:889).
-// Algorithm (one frequency per workgroup, all frequencies of a batch in one
-// launch):
+// Algorithm (all frequencies of a batch in one launch per stage):
 //   1. Householder bidiagonalisation Y = U_b B V_b^H (R >= C), B real upper
 //      bidiagonal (LAPACK gebrd conventions: reflectors H = I - tau u u^H
-//      with u_0 = 1, beta real).  The left update of step j, the product
-//      t = Y v_j and the next column's reflector are fused so that the
-//      trailing matrix is read and written twice per column.
+//      with u_0 = 1, beta real), after a QR pre-reduction of tall Y
+//      (geqrf.hip, gebrd.hip);
 //   2. singular values of B by bisection on the Golub-Kahan tridiagonal
 //      T = tridiag(d_0, e_0, d_1, e_1, ...) whose eigenvalues are +-sigma
 //      (Sturm counts; one thread per singular value);
@@ -29,148 +27,6 @@ using hh::Refl;
 using hh::block_sum;
 using hh::larfg;
 using hh::wave_sum;
-
-template <typename T>
-__global__ __launch_bounds__(BT) void k_bidiag(int R, int C, cx<T> *__restrict__ Yall, int ldy,
-                                              size_t sY, double *__restrict__ dall,
-                                              double *__restrict__ eall, cx<T> *__restrict__ tauLall,
-                                              cx<T> *__restrict__ tauRall,
-                                              cx<T> *__restrict__ Vrall) {
-    extern __shared__ unsigned char smem_raw[];
-    cx<T> *u = reinterpret_cast<cx<T> *>(smem_raw);
-    cx<T> *tt = u + R;
-    cx<T> *w = tt + R;
-    cx<T> *v = w + C;
-    double *red = reinterpret_cast<double *>(v + C);
-    __shared__ cx<double> bc_val;
-
-    const int b = blockIdx.x;
-    cx<T> *Y = Yall + (size_t)b * sY;
-    double *d = dall + (size_t)b * C;
-    double *e = eall + (size_t)b * C;
-    cx<T> *tauL = tauLall + (size_t)b * C;
-    cx<T> *tauR = tauRall + (size_t)b * C;
-    cx<T> *Vr = Vrall + (size_t)b * C * C;
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-
-    // ---- prologue: reflector of column 0, then w = Y^H u over columns 1..C-1
-    {
-        double p = 0;
-        for (int r = 1 + t; r < R; r += BT) p += (double)abs2(Y[r]);
-        double xn2 = block_sum(p, red);
-        cx<double> alpha = cvt<double>(Y[0]);
-        Refl rf = larfg(alpha, xn2);
-        cx<T> sc = cvt<T>(rf.scale);
-        for (int r = 1 + t; r < R; r += BT) {
-            cx<T> uv = Y[r] * sc;
-            u[r] = uv;
-            Y[r] = uv;
-        }
-        if (t == 0) {
-            u[0] = cx<T>{1, 0};
-            Y[0] = cx<T>{(T)rf.beta, 0};
-            d[0] = rf.beta;
-            tauL[0] = cvt<T>(rf.tau);
-        }
-        __syncthreads();
-        for (int c = 1 + wv; c < C; c += NW) {
-            const cx<T> *Yc = Y + (size_t)c * ldy;
-            double ar = 0, ai = 0;
-            for (int r = lane; r < R; r += 64) {
-                cx<T> y = Yc[r], uu = u[r];
-                ar += (double)(uu.re * y.re + uu.im * y.im);
-                ai += (double)(uu.re * y.im - uu.im * y.re);
-            }
-            ar = wave_sum(ar);
-            ai = wave_sum(ai);
-            if (lane == 0) w[c] = cx<T>{(T)ar, (T)ai};
-        }
-        __syncthreads();
-    }
-    cx<T> tauj = tauL[0];
-    for (int j = 0; j < C - 1; ++j) {
-        const cx<T> ctau = conj(tauj);
-        // ---- (a) row j after the left update; right reflector of row j
-        double p = 0;
-        for (int c = j + 1 + t; c < C; c += BT) {
-            cx<T> yj = Y[(size_t)c * ldy + j] - ctau * w[c];
-            v[c] = yj;
-            if (c >= j + 2) p += (double)abs2(yj);
-        }
-        double xn2 = block_sum(p, red);  // syncs: v visible
-        cx<double> alpha = conj(cvt<double>(v[j + 1]));
-        Refl rr = larfg(alpha, xn2);
-        const cx<T> scr = cvt<T>(rr.scale);
-        const cx<T> sig = cvt<T>(rr.tau);
-        __syncthreads();
-        for (int c = j + 1 + t; c < C; c += BT) {
-            cx<T> vv = (c == j + 1) ? cx<T>{1, 0} : conj(v[c]) * scr;
-            v[c] = vv;
-            Vr[(size_t)j * C + c] = vv;
-            Y[(size_t)c * ldy + j] = (c == j + 1) ? cx<T>{(T)rr.beta, 0} : vv;
-        }
-        if (t == 0) {
-            e[j] = rr.beta;
-            tauR[j] = sig;
-        }
-        __syncthreads();
-        // ---- (b) rows j+1..R-1: left update, t = Y v, final column j+1
-        double pn = 0;
-        for (int r = j + 1 + t; r < R; r += BT) {
-            const cx<T> ur = u[r];
-            const cx<T> cu = ctau * ur;
-            cx<T> acc{0, 0}, y1{0, 0};
-            for (int c = j + 1; c < C; ++c) {
-                cx<T> *yp = Y + (size_t)c * ldy + r;
-                cx<T> y = *yp - cu * w[c];
-                *yp = y;
-                acc = acc + y * v[c];
-                if (c == j + 1) y1 = y;
-            }
-            tt[r] = acc;
-            cx<T> ncol = y1 - sig * acc;
-            Y[(size_t)(j + 1) * ldy + r] = ncol;
-            if (r >= j + 2) pn += (double)abs2(ncol);
-        }
-        double xn2b = block_sum(pn, red);  // syncs: tt and column j+1 visible
-        if (t == 0) bc_val = cvt<double>(Y[(size_t)(j + 1) * ldy + j + 1]);
-        __syncthreads();
-        Refl rl = larfg(bc_val, xn2b);
-        const cx<T> scl = cvt<T>(rl.scale);
-        for (int r = j + 2 + t; r < R; r += BT) {
-            cx<T> *yp = Y + (size_t)(j + 1) * ldy + r;
-            cx<T> uv = *yp * scl;
-            u[r] = uv;
-            *yp = uv;
-        }
-        if (t == 0) {
-            u[j + 1] = cx<T>{1, 0};
-            Y[(size_t)(j + 1) * ldy + j + 1] = cx<T>{(T)rl.beta, 0};
-            d[j + 1] = rl.beta;
-            tauL[j + 1] = cvt<T>(rl.tau);
-        }
-        __syncthreads();
-        // ---- (c) columns j+2..C-1: right update, next w = Y^H u_{j+1}
-        for (int c = j + 2 + wv; c < C; c += NW) {
-            cx<T> *Yc = Y + (size_t)c * ldy;
-            const cx<T> sv = sig * conj(v[c]);
-            double ar = 0, ai = 0;
-            for (int r = j + 1 + lane; r < R; r += 64) {
-                cx<T> y = Yc[r] - tt[r] * sv;
-                Yc[r] = y;
-                cx<T> uu = u[r];
-                ar += (double)(uu.re * y.re + uu.im * y.im);
-                ai += (double)(uu.re * y.im - uu.im * y.re);
-            }
-            ar = wave_sum(ar);
-            ai = wave_sum(ai);
-            if (lane == 0) w[c] = cx<T>{(T)ar, (T)ai};
-        }
-        __syncthreads();
-        tauj = cvt<T>(rl.tau);
-    }
-    if (t == 0 && C > 0) e[C - 1] = 0.0;
-}
 
 // ------------------------------------------------------------ bisection
 // #eigenvalues of the zero-diagonal symmetric tridiagonal with squared
@@ -524,17 +380,6 @@ __global__ __launch_bounds__(BT) void k_geqr2(int R, int C, cx<T> *__restrict__ 
 
 }  // namespace
 
-size_t bidiag_lds_bytes(int R, int C, size_t elem) { return (2 * (size_t)R + 2 * (size_t)C) * elem + NW * sizeof(double) + 64; }
-
-template <typename T>
-void launch_bidiag(int R, int C, cx<T> *Y, int ldy, size_t strideY, double *d, double *e,
-                   cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, int batch, hipStream_t s) {
-    size_t lds = bidiag_lds_bytes(R, C, sizeof(cx<T>));
-    (void)hipFuncSetAttribute((const void *)k_bidiag<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
-    hipLaunchKernelGGL(k_bidiag<T>, dim3(batch), dim3(BT), lds, s, R, C, Y, ldy, strideY, d, e,
-                       tauL, tauR, Vr);
-}
 
 void launch_bisect(int n, const double *d, const double *e, double *sigma, int batch,
                    hipStream_t s) {
@@ -602,8 +447,6 @@ template ia_status launch_geqr2<float>(int, int, cx<float> *, int, size_t, cx<fl
                                        hipStream_t);
 
 #define INST(T)                                                                                  \
-    template void launch_bidiag<T>(int, int, cx<T> *, int, size_t, double *, double *, cx<T> *, \
-                                   cx<T> *, cx<T> *, int, hipStream_t);                          \
     template void launch_apply_left<T>(bool, int, int, const cx<T> *, int, size_t,              \
                                        const cx<T> *, cx<double> *, size_t, int, int,           \
                                        hipStream_t);                                            \
