@@ -333,6 +333,47 @@ typedef struct {
 ia_status ia_admm(int rows, int cols, int batch, const ia_z *a, const ia_z *b,
                   const ia_admm_params *prm, ia_z *c, int *iterations);
 
+/* ------------------------------------- scenarios (MATLAB/linesource.m) */
+/* Analytic line source (MATLAB/linesource.m:1-65), batched over n_freq
+ * wavenumbers k: a source of length len and width a on the y axis, centred
+ * at the origin, vibrating with velocity u0, sampled at n_ls points.
+ * xp = i (rho c u0 k a / 2) trapz(e^{ikr}/r) at the l field points r_lh
+ * (l x n_freq); xsf the same on the m surface points r_ms, vsf the normal
+ * velocity (u0 a / 2) trapz((ikr - 1) e^{ikr}/r^2 n.(x - y)/r) there
+ * (m x n_freq each), n the normalised sum of (x1 - x3) x (x2 - x3) over a
+ * node's triangles of e_n (3 n_elem, 0-based SoA).  rho and c are
+ * arguments, as in linesource.m.  Geometry: host pointers; outputs: host or
+ * device pointers.  m = 0 or l = 0 skips that output. */
+ia_status ia_linesource(int n_ls, int m, int n_elem, const double *r_ms,
+                        const int *e_n, int l, const double *r_lh, double rho,
+                        double c, double u0, const double *k, int n_freq,
+                        double a, double len, ia_z *xp, ia_z *xsf, ia_z *vsf);
+
+/* ------------------------------------ text output (CUDA/utils.cuh) */
+/* replace write_file_cuDoubleComplex(), write_file_string(),
+ * write_file_double() (utils.cuh:32-74): mode is the fopen mode ("w" or
+ * "a"); complex entries "%f,%f" and real entries "%f", separated by ';',
+ * one '\n' at the end.  Status variants return IA_ERR_IO when the file
+ * cannot be opened (the reference writes through a NULL stream). */
+ia_status ia_write_complex(const char *mode, const char *file, const ia_z *v,
+                           int n);
+ia_status ia_write_double(const char *mode, const char *file, const double *v,
+                          int n);
+ia_status ia_write_string(const char *mode, const char *file, const char *text);
+void write_file_cuDoubleComplex(const char *wa, char *file, ia_z *v, int n);
+void write_file_string(const char *wa, char *file, char *text);
+void write_file_double(const char *wa, char *file, double *v, int n);
+/* replaces write_matrix_cuDoubleComplex() (utils.cuh:116): r x c column-major
+ * matrix, row by row, "%f,%f," (fmt "f") or "%e,%e," (fmt "e") per entry,
+ * '\n' per row and a final '\n'.  The legacy name writes "matrix.txt". */
+ia_status ia_write_matrix(const char *fmt, const char *file, const ia_z *a,
+                          int r, int c);
+void write_matrix_cuDoubleComplex(const char *d, ia_z *a, int r, int c);
+/* replace print_matrix_cuDoubleComplex(), print_matrix_double()
+ * (utils.cuh:76,159): the same matrices on stdout. */
+void print_matrix_cuDoubleComplex(const char *d, ia_z *a, int r, int c);
+void print_matrix_double(const char *d, double *a, int r, int c);
+
 /* Phase timings (ms) of the last sweep on this thread, plus the summed
  * HIP-event duration, algorithmic bytes and launch count of the panel
  * bidiagonalisation kernel (the sweep's dominant kernel). */

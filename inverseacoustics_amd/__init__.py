@@ -22,6 +22,7 @@ __all__ = [
     "tsvd", "tikhonov", "dsvd", "regularize", "to_pressure", "to_velocity", "sound_power",
     "sphere", "cartesian_2_spherical", "geometry", "esm_sweep", "SweepResult", "zgemm", "cgemm", "zgesv", "hels", "hels_sweep", "regsolve",
     "quadrature_load", "set_quadrature", "ibem", "ibem_sweep", "invr", "radiation_modes", "admm",
+    "linesource", "write_file_complex", "write_file_double", "write_file_string", "write_matrix",
     "FILTERS", "SELECTS",
 ]
 
@@ -477,3 +478,55 @@ def esm_sweep(k, r_s, src, r_h, b, area, normals=None, precision="fp64", filter=
     return SweepResult(out["lam"], out["lam_index"], out["k_tsvd"], out["power"],
                        out["sigma_max"], out["velocity"], out["pressure"], out["status"], stats,
                        out["gap"])
+
+
+# ------------------------------------------------------------ scenarios
+def linesource(n_ls, r_ms, e_n, r_lh, rho, c, U0, k, a, L):
+    """MATLAB/linesource.m ``[xp, xsf, vsf] = linesource(n_ls, r_ms, e_n,
+    r_lh, rho, c, U0, k, a, L)``: pressure at the field points r_lh (l, 3),
+    pressure and normal velocity at the surface nodes r_ms (m, 3) of the
+    triangles e_n (n, 3, 0-based) for a line source of length L and width a
+    on the y axis.  k may be a scalar (outputs (l,), (m,), (m,)) or an array
+    of F wavenumbers (outputs (F, l), (F, m), (F, m))."""
+    kk = np.ascontiguousarray(np.atleast_1d(np.asarray(k, dtype=np.float64)).ravel())
+    F = len(kk)
+    rs, en, rh, _, m, n, l, _ = _mesh_args(r_ms, e_n, r_lh, None)
+    xp = np.zeros((F, l), np.complex128)
+    xsf = np.zeros((F, m), np.complex128)
+    vsf = np.zeros((F, m), np.complex128)
+    check(lib().ia_linesource(int(n_ls), m, n, _lib.dptr(rs), en.ctypes.data_as(_lib._ip), l,
+                              _lib.dptr(rh) if l else None, float(rho), float(c), float(U0),
+                              _lib.dptr(kk), F, float(a), float(L),
+                              _lib.zptr(xp) if l else None, _lib.zptr(xsf) if m else None,
+                              _lib.zptr(vsf) if m else None))
+    if np.ndim(k) == 0:
+        return xp[0], xsf[0], vsf[0]
+    return xp, xsf, vsf
+
+
+# ---------------------------------------------------------- text output
+def write_file_complex(mode, file, v):
+    """CUDA/utils.cuh:32 ``write_file_cuDoubleComplex``: one line of
+    "%f,%f" entries separated by ';' (mode "w" or "a")."""
+    v = np.ascontiguousarray(np.asarray(v, dtype=np.complex128).ravel())
+    check(lib().ia_write_complex(str(mode).encode(), str(file).encode(), _lib.zptr(v), len(v)))
+
+
+def write_file_double(mode, file, v):
+    """CUDA/utils.cuh:58 ``write_file_double``: one line of "%f" entries
+    separated by ';'."""
+    v = np.ascontiguousarray(np.asarray(v, dtype=np.float64).ravel())
+    check(lib().ia_write_double(str(mode).encode(), str(file).encode(), _lib.dptr(v), len(v)))
+
+
+def write_file_string(mode, file, text):
+    """CUDA/utils.cuh:50 ``write_file_string``."""
+    check(lib().ia_write_string(str(mode).encode(), str(file).encode(), str(text).encode()))
+
+
+def write_matrix(fmt, file, a):
+    """CUDA/utils.cuh:116 ``write_matrix_cuDoubleComplex`` into `file`: rows
+    of "%f,%f," (fmt "f") or "%e,%e," (fmt "e") entries, then a blank line."""
+    A = zf(np.atleast_2d(a))
+    r, cc = A.shape
+    check(lib().ia_write_matrix(str(fmt).encode(), str(file).encode(), _lib.zptr(A), r, cc))

@@ -136,6 +136,19 @@ SIGNATURES = {
                           _ip]),
     "ia_cgemm": (C.c_int, [C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, ia_c, C.c_void_p, C.c_int,
                            C.c_void_p, C.c_int, ia_c, C.c_void_p, C.c_int]),
+    "ia_linesource": (C.c_int, [C.c_int, C.c_int, C.c_int, _dp, _ip, C.c_int, _dp, C.c_double,
+                                C.c_double, C.c_double, _dp, C.c_int, C.c_double, C.c_double,
+                                _zp, _zp, _zp]),
+    "ia_write_complex": (C.c_int, [C.c_char_p, C.c_char_p, _zp, C.c_int]),
+    "ia_write_double": (C.c_int, [C.c_char_p, C.c_char_p, _dp, C.c_int]),
+    "ia_write_string": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p]),
+    "ia_write_matrix": (C.c_int, [C.c_char_p, C.c_char_p, _zp, C.c_int, C.c_int]),
+    "write_file_cuDoubleComplex": (None, [C.c_char_p, C.c_char_p, _zp, C.c_int]),
+    "write_file_string": (None, [C.c_char_p, C.c_char_p, C.c_char_p]),
+    "write_file_double": (None, [C.c_char_p, C.c_char_p, _dp, C.c_int]),
+    "write_matrix_cuDoubleComplex": (None, [C.c_char_p, _zp, C.c_int, C.c_int]),
+    "print_matrix_cuDoubleComplex": (None, [C.c_char_p, _zp, C.c_int, C.c_int]),
+    "print_matrix_double": (None, [C.c_char_p, _dp, C.c_int, C.c_int]),
 }
 
 _lib = None
