@@ -94,7 +94,8 @@ struct SweepCache {
     void release() {
         for (DevBuf *b : {&bD, &bY, &bipiv, &binfo, &bc, &bbeta, &bphi, &bx, &bq, &bps, &bpsd,
                           &bn2, &bcn2, &sw.d, &sw.e, &sw.sig, &sw.tauL, &sw.tauR, &sw.Vr, &sw.QBt,
-                          &sw.PBt, &sw.work, &sw.gw, &sw.Rb, &sw.tauQ, &sw.qw})
+                          &sw.PBt, &sw.work, &sw.gw, &sw.Rb, &sw.tauQ, &sw.qw, &sw.Bs, &sw.uR2,
+                          &sw.uL2, &sw.tR2, &sw.tL2, &sw.Pm, &sw.tauP})
             b->reset();
     }
 };
@@ -172,8 +173,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
         const cx<double> *bf = b + (size_t)f0 * l;
         IA_HIP(hipMemcpyAsync(bc.p, bf, sizeof(cx<double>) * (size_t)nf * l, hipMemcpyDefault, s));
         launch_norm2<double>(l, bc.as<cx<double>>(), l, bn2.as<double>(), nf, s);
-        launch_apply_right<T>(true, C, sw.Vr.template as<cx<T>>(), sw.tauR.template as<cx<T>>(), bc.as<cx<double>>(),
-                              C, 1, nf, s);
+        svd_apply_v<T>(sw, true, bc.as<cx<double>>(), C, 1, s);
         launch_norm2<double>(C, bc.as<cx<double>>(), C, bcn2.as<double>(), nf, s);
         launch_real_matvec(C, sw.QBt.template as<double>(), 0, bc.as<cx<double>>(), C,
                            bbeta.as<cx<double>>(), C, nf, s);
@@ -567,8 +567,7 @@ ia_status ia_csvd(int m, int n, const ia_z *a, ia_z **u, double **s_out, ia_z **
     launch_rows_to_cols(R, C, sw.PBt.as<double>(), bUY.as<cx<double>>(), 1, s);
     launch_rows_to_cols(C, C, sw.QBt.as<double>(), bVY.as<cx<double>>(), 1, s);
     svd_apply_u<double>(sw, bY.as<cx<double>>(), bUY.as<cx<double>>(), R, C, s);
-    launch_apply_right<double>(false, C, sw.Vr.as<cx<double>>(), sw.tauR.as<cx<double>>(),
-                               bVY.as<cx<double>>(), C, C, 1, s);
+    svd_apply_v<double>(sw, false, bVY.as<cx<double>>(), C, C, s);
     IA_HIP(hipGetLastError());
     IA_HIP(hipDeviceSynchronize());
     IA_TRY(host_out<double>(s_out, sw.sig.p, C));

@@ -124,6 +124,32 @@ __global__ void k_copy_upper(int C, const cx<T> *__restrict__ Yall, int ldy, siz
     Rball[(size_t)b * C * C + (size_t)c * C + r] = (r <= c) ? v : cx<T>{0, 0};
 }
 
+// LQ panel of the band reduction, as a QR of the conjugate transpose:
+// P[(k + q) + (k + i) ldp] = conj(Z[k + i, k + NB + q]), q < N2, i < nbp
+template <typename T>
+__global__ void k_lq_gather(int k, int N2, int nbp, const cx<T> *__restrict__ Zall, int ldz,
+                            size_t sZ, cx<T> *__restrict__ Pall, int ldp, size_t sP) {
+    const int b = blockIdx.z, i = blockIdx.y;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N2 || i >= nbp) return;
+    const cx<T> z = Zall[(size_t)b * sZ + (size_t)(k + NB + q) * ldz + (k + i)];
+    Pall[(size_t)b * sP + (size_t)(k + i) * ldp + (k + q)] = conj(z);
+}
+
+// rows k..k+nbp-1 of Z after the LQ panel: Z[k + i, k + NB + q] =
+// conj(R_w[q, i]) for q <= i (lower triangle), zero beyond
+template <typename T>
+__global__ void k_lq_scatter(int k, int nbp, cx<T> *__restrict__ Zall, int ldz, size_t sZ,
+                             const cx<T> *__restrict__ Pall, int ldp, size_t sP) {
+    const int b = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nbp * nbp) return;
+    const int i = e % nbp, q = e / nbp;
+    const cx<T> r = Pall[(size_t)b * sP + (size_t)(k + i) * ldp + (k + q)];
+    Zall[(size_t)b * sZ + (size_t)(k + NB + q) * ldz + (k + i)] =
+        (q <= i) ? conj(r) : cx<T>{0, 0};
+}
+
 }  // namespace
 
 size_t geqrf_work_elems(int R, int C) {
@@ -161,6 +187,80 @@ ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, c
     return IA_OK;
 }
 
+// Stage one of the two-stage bidiagonalisation: Z (R x C, R >= C) ->
+// Q1^H Z P1, upper band of width NB in the leading C x C block.  Per panel
+// k: QR of the NB columns k.. (reflectors below the diagonal of Z, tau in
+// tauQ) and its trailing update from the left; then, while columns remain
+// beyond k + NB, the LQ of rows k..k+NB-1 over columns k+NB.. computed as
+// a QR of their conjugate transpose in Pm (C-NB x C-NB per matrix; the
+// reflectors stay there, tau in tauP) and applied from the right to the
+// rows below, Z2 <- Z2 (I - V T V^H).  All trailing updates are GEMMs.
+// tauP has C - NB entries per matrix (the reflectors of Pm's columns).
+size_t band_work_elems(int R, int C) {
+    return (size_t)R * NB + 2 * (size_t)NB * NB + 2 * (size_t)NB * std::max(R, C);
+}
+
+template <typename T>
+ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *tauQ, cx<T> *Pm,
+                            cx<T> *tauP, cx<T> *work, int batch, hipStream_t s) {
+    if (R < C) return fail(IA_ERR_ARG, "band reduction needs R >= C (%d x %d)", R, C);
+    cx<T> *Vw = work;
+    cx<T> *G = Vw + (size_t)batch * R * NB;
+    cx<T> *Tm = G + (size_t)batch * NB * NB;
+    const int Wl = std::max(R, C);
+    cx<T> *W = Tm + (size_t)batch * NB * NB;
+    cx<T> *W2 = W + (size_t)batch * NB * Wl;
+    const cx<T> one{1, 0}, zero{0, 0}, mone{-1, 0};
+    const size_t sT = (size_t)NB * NB, sW = (size_t)NB * Wl;
+    const int Cp = C - NB;
+    const size_t sP = (size_t)std::max(Cp, 1) * std::max(Cp, 1);
+    for (int k = 0; k < C; k += NB) {
+        const int w = std::min(NB, C - k);
+        // ---- QR of columns k..k+w-1, left update of columns k+w..
+        hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, R, C, k, w, Z, ldz, sZ,
+                           tauQ, Vw);
+        const size_t sV = (size_t)R * NB;
+        const int N = C - k - w, M = R - k;
+        if (N > 0) {
+            const cx<T> *V0 = Vw + k;
+            launch_gemm<T>('C', 'N', w, w, M, one, V0, R, sV, V0, R, sV, zero, G, NB, sT, batch, s);
+            hipLaunchKernelGGL(k_qr_tmat<T>, dim3(batch), dim3(64), 0, s, C, k, w, G, tauQ, Tm);
+            cx<T> *A2 = Z + k + (size_t)(k + w) * ldz;
+            launch_gemm<T>('C', 'N', w, N, M, one, V0, R, sV, A2, ldz, sZ, zero, W, NB, sW, batch, s);
+            launch_gemm<T>('C', 'N', w, N, w, one, Tm, NB, sT, W, NB, sW, zero, W2, NB, sW, batch, s);
+            launch_gemm<T>('N', 'N', M, N, w, mone, V0, R, sV, W2, NB, sW, one, A2, ldz, sZ, batch,
+                           s);
+        }
+        if (k + NB >= C) continue;
+        // ---- LQ of rows k..k+NB-1 over columns k+NB..C-1
+        const int N2 = C - k - NB;           // columns of the row panel
+        const int nbp = std::min(NB, N2);    // reflectors (rows with columns left)
+        hipLaunchKernelGGL(k_lq_gather<T>, dim3((N2 + 255) / 256, nbp, batch), dim3(256), 0, s, k,
+                           N2, nbp, Z, ldz, sZ, Pm, Cp, sP);
+        hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, Cp, Cp, k, nbp, Pm, Cp, sP,
+                           tauP, Vw);
+        hipLaunchKernelGGL(k_lq_scatter<T>, dim3((nbp * nbp + 255) / 256, batch), dim3(256), 0, s,
+                           k, nbp, Z, ldz, sZ, Pm, Cp, sP);
+        // rows below the reflected ones: Z2 <- Z2 - ((Z2 V) T) V^H
+        const int M2 = R - k - nbp;
+        if (M2 <= 0) continue;
+        const size_t sV2 = (size_t)Cp * NB;
+        const cx<T> *V0 = Vw + k;
+        cx<T> *Z2 = Z + (k + nbp) + (size_t)(k + NB) * ldz;
+        launch_gemm<T>('C', 'N', nbp, nbp, N2, one, V0, Cp, sV2, V0, Cp, sV2, zero, G, NB, sT,
+                       batch, s);
+        hipLaunchKernelGGL(k_qr_tmat<T>, dim3(batch), dim3(64), 0, s, Cp, k, nbp, G, tauP, Tm);
+        launch_gemm<T>('N', 'N', M2, nbp, N2, one, Z2, ldz, sZ, V0, Cp, sV2, zero, W, Wl, sW, batch,
+                       s);
+        launch_gemm<T>('N', 'N', M2, nbp, nbp, one, W, Wl, sW, Tm, NB, sT, zero, W2, Wl, sW, batch,
+                       s);
+        launch_gemm<T>('N', 'C', M2, N2, nbp, mone, W2, Wl, sW, V0, Cp, sV2, one, Z2, ldz, sZ,
+                       batch, s);
+    }
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
 template <typename T>
 void launch_copy_upper(int C, const cx<T> *Y, int ldy, size_t sY, cx<T> *Rb, int batch,
                        hipStream_t s) {
@@ -172,6 +272,12 @@ template ia_status launch_geqrf<double>(int, int, cx<double> *, int, size_t, cx<
                                         cx<double> *, int, hipStream_t);
 template ia_status launch_geqrf<float>(int, int, cx<float> *, int, size_t, cx<float> *,
                                        cx<float> *, int, hipStream_t);
+template ia_status launch_gebrd_band<double>(int, int, cx<double> *, int, size_t, cx<double> *,
+                                             cx<double> *, cx<double> *, cx<double> *, int,
+                                             hipStream_t);
+template ia_status launch_gebrd_band<float>(int, int, cx<float> *, int, size_t, cx<float> *,
+                                            cx<float> *, cx<float> *, cx<float> *, int,
+                                            hipStream_t);
 template void launch_copy_upper<double>(int, const cx<double> *, int, size_t, cx<double> *, int,
                                         hipStream_t);
 template void launch_copy_upper<float>(int, const cx<float> *, int, size_t, cx<float> *, int,

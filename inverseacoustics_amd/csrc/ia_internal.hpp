@@ -169,6 +169,31 @@ ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, c
 template <typename T>
 void launch_copy_upper(int C, const cx<T> *Y, int ldy, size_t sY, cx<T> *Rb, int batch,
                        hipStream_t s);
+// two-stage bidiagonalisation: stage one (geqrf.hip) reduces Z (R x C) to an
+// upper band of width 32 (left reflectors below Z's diagonal, tau in tauQ;
+// right reflectors as the columns of Pm, (C-32) x (C-32) per matrix, tau in
+// tauP); stage two (bulge.hip) chases the band to the real bidiagonal d, e,
+// keeping its reflectors (uR/tR, uL/tL: bulge_refl_elems(C) slots of 32).
+constexpr int BAND_NB = 32;
+size_t band_work_elems(int R, int C);
+template <typename T>
+ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *tauQ, cx<T> *Pm,
+                            cx<T> *tauP, cx<T> *work, int batch, hipStream_t s);
+int bulge_tmax(int C);
+size_t bulge_band_elems(int C);
+size_t bulge_refl_elems(int C);
+size_t bulge_apply_lds(int C);
+// bytes the chase's tasks read and write (each task's two blocks once)
+double bulge_alg_bytes(int C, size_t elem);
+template <typename T>
+ia_status launch_bulge(int C, const cx<T> *Z, int ldz, size_t sZ, cx<T> *Bs, cx<T> *uR,
+                       cx<T> *uL, cx<T> *tR, cx<T> *tL, double *d, double *e, int batch,
+                       hipStream_t s);
+// v <- H v over one side's stage-two reflectors: fwd = true applies
+// P2^H (uR) or Q2^H (uL), fwd = false applies P2 or Q2
+template <typename T>
+void launch_bulge_apply(bool fwd, int C, const cx<T> *u, const cx<T> *tau, cx<double> *v,
+                        size_t sv, int nvec, int batch, hipStream_t s);
 
 // ADMM (admm.hip): A (rows x cols, batch stride rows*cols), b, c on the device
 ia_status admm_batched(int rows, int cols, int batch, const cx<double> *A, const cx<double> *b,
@@ -310,26 +335,48 @@ inline ia_status host_out(T **dst, const void *dsrc, size_t count) {
 // Y is first reduced to R = Q^H Y (blocked QR on the matrix cores) and the
 // C x C factor is bidiagonalised instead: the left singular basis is then
 // U = Q diag(U_R, I), applied by svd_apply_u / svd_apply_uh.
+// Two-stage bidiagonalisation for C in [128, 4096] unless IA_ONE_STAGE is
+// set (the one-stage panel kernel then handles every size).
+inline bool two_stage_for(int C) {
+    const char *e = std::getenv("IA_ONE_STAGE");
+    if (e && *e && *e != '0') return false;
+    return C >= 128 && C <= 4096;
+}
+
 template <typename T>
 struct SvdWork {
     DevBuf d, e, sig, tauL, tauR, Vr, QBt, PBt, work, gw, Rb, tauQ, qw;
+    DevBuf Bs, uR2, uL2, tR2, tL2, Pm, tauP;   // two-stage form
     int R = 0, C = 0, batch = 0, sub = 1;
-    bool qr = false;
+    bool qr = false, ts = false;
     ia_status alloc(int R_, int C_, int batch_) {
         R = R_;
         C = C_;
         batch = batch_;
         qr = 2 * (size_t)R >= 3 * (size_t)C && C >= 64;
+        ts = two_stage_for(C);
         size_t bc = (size_t)batch * C;
         IA_TRY(d.alloc(bc * sizeof(double)));
         IA_TRY(e.alloc(bc * sizeof(double)));
         IA_TRY(sig.alloc(bc * sizeof(double)));
         IA_TRY(tauL.alloc(bc * sizeof(cx<T>)));
-        IA_TRY(tauR.alloc(bc * sizeof(cx<T>)));
-        IA_TRY(Vr.alloc(bc * C * sizeof(cx<T>)));
         IA_TRY(QBt.alloc(bc * C * sizeof(double)));
         IA_TRY(PBt.alloc(bc * C * sizeof(double)));
-        IA_TRY(gw.alloc(gebrd_work_elems(qr ? C : R, C) * batch * sizeof(cx<T>)));
+        if (ts) {
+            const size_t ns = bulge_refl_elems(C), cp = (size_t)(C - BAND_NB);
+            IA_TRY(Bs.alloc(bulge_band_elems(C) * batch * sizeof(cx<T>)));
+            IA_TRY(uR2.alloc(ns * BAND_NB * batch * sizeof(cx<T>)));
+            IA_TRY(uL2.alloc(ns * BAND_NB * batch * sizeof(cx<T>)));
+            IA_TRY(tR2.alloc(ns * batch * sizeof(cx<T>)));
+            IA_TRY(tL2.alloc(ns * batch * sizeof(cx<T>)));
+            IA_TRY(Pm.alloc(cp * cp * batch * sizeof(cx<T>)));
+            IA_TRY(tauP.alloc(cp * batch * sizeof(cx<T>)));
+            IA_TRY(gw.alloc(band_work_elems(qr ? C : R, C) * batch * sizeof(cx<T>)));
+        } else {
+            IA_TRY(tauR.alloc(bc * sizeof(cx<T>)));
+            IA_TRY(Vr.alloc(bc * C * sizeof(cx<T>)));
+            IA_TRY(gw.alloc(gebrd_work_elems(qr ? C : R, C) * batch * sizeof(cx<T>)));
+        }
         if (qr) {
             IA_TRY(Rb.alloc(bc * C * sizeof(cx<T>)));
             IA_TRY(tauQ.alloc(bc * sizeof(cx<T>)));
@@ -370,6 +417,22 @@ inline ia_status svd_reduce(SvdWork<T> &w, cx<T> *Y, hipStream_t s, KernelClock 
         Z = w.Rb.template as<cx<T>>();
         Rz = C;
     }
+    if (w.ts) {
+        IA_TRY(launch_gebrd_band<T>(Rz, C, Z, Rz, (size_t)Rz * C, w.tauL.template as<cx<T>>(),
+                                    w.Pm.template as<cx<T>>(), w.tauP.template as<cx<T>>(),
+                                    w.gw.template as<cx<T>>(), batch, s));
+        if (clk) clk->mark(s);
+        IA_TRY(launch_bulge<T>(C, Z, Rz, (size_t)Rz * C, w.Bs.template as<cx<T>>(),
+                               w.uR2.template as<cx<T>>(), w.uL2.template as<cx<T>>(),
+                               w.tR2.template as<cx<T>>(), w.tL2.template as<cx<T>>(),
+                               w.d.template as<double>(), w.e.template as<double>(), batch, s));
+        if (clk) {
+            clk->mark(s);
+            clk->launches += 1;
+            clk->bytes += bulge_alg_bytes(C, sizeof(cx<T>)) * batch;
+        }
+        return IA_OK;
+    }
     return launch_gebrd<T>(Rz, C, Z, Rz, (size_t)Rz * C, w.d.template as<double>(),
                            w.e.template as<double>(), w.tauL.template as<cx<T>>(),
                            w.tauR.template as<cx<T>>(), w.Vr.template as<cx<T>>(),
@@ -397,6 +460,9 @@ template <typename T>
 inline void svd_apply_u(SvdWork<T> &w, const cx<T> *Y, cx<double> *v, size_t sv, int nvec,
                         hipStream_t s) {
     const int R = w.R, C = w.C;
+    if (w.ts)
+        launch_bulge_apply<T>(false, C, w.uL2.template as<cx<T>>(), w.tL2.template as<cx<T>>(), v,
+                              sv, nvec, w.batch, s);
     if (w.qr) {
         launch_apply_left<T>(false, C, C, w.Rb.template as<cx<T>>(), C, (size_t)C * C,
                              w.tauL.template as<cx<T>>(), v, sv, nvec, w.batch, s);
@@ -422,6 +488,35 @@ inline void svd_apply_uh(SvdWork<T> &w, const cx<T> *Y, cx<double> *v, size_t sv
     } else {
         launch_apply_left<T>(true, R, C, Y, R, (size_t)R * C, w.tauL.template as<cx<T>>(), v, sv,
                              nvec, w.batch, s);
+    }
+    if (w.ts)
+        launch_bulge_apply<T>(true, C, w.uL2.template as<cx<T>>(), w.tL2.template as<cx<T>>(), v,
+                              sv, nvec, w.batch, s);
+}
+
+// v <- V_b v (herm = false) or V_b^H v for nvec length-C vectors per batch
+// item, Y = U_b B V_b^H
+template <typename T>
+inline void svd_apply_v(SvdWork<T> &w, bool herm, cx<double> *v, size_t sv, int nvec,
+                        hipStream_t s) {
+    const int C = w.C;
+    if (!w.ts) {
+        launch_apply_right<T>(herm, C, w.Vr.template as<cx<T>>(), w.tauR.template as<cx<T>>(), v,
+                              sv, nvec, w.batch, s);
+        return;
+    }
+    const int cp = C - BAND_NB;
+    // V_b = P1 P2: P1 from the columns of Pm acting on entries BAND_NB..C-1
+    if (herm) {
+        launch_apply_left<T>(true, cp, cp, w.Pm.template as<cx<T>>(), cp, (size_t)cp * cp,
+                             w.tauP.template as<cx<T>>(), v + BAND_NB, sv, nvec, w.batch, s);
+        launch_bulge_apply<T>(true, C, w.uR2.template as<cx<T>>(), w.tR2.template as<cx<T>>(), v,
+                              sv, nvec, w.batch, s);
+    } else {
+        launch_bulge_apply<T>(false, C, w.uR2.template as<cx<T>>(), w.tR2.template as<cx<T>>(), v,
+                              sv, nvec, w.batch, s);
+        launch_apply_left<T>(false, cp, cp, w.Pm.template as<cx<T>>(), cp, (size_t)cp * cp,
+                             w.tauP.template as<cx<T>>(), v + BAND_NB, sv, nvec, w.batch, s);
     }
 }
 

@@ -71,8 +71,7 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
         // beta = Q_B^T (V_b^H b)
         IA_HIP(hipMemcpy2DAsync(c, sizeof(cx<double>) * C, b, sizeof(cx<double>) * sb,
                                 sizeof(cx<double>) * rows, batch, hipMemcpyDeviceToDevice, s));
-        launch_apply_right<T>(true, C, sw.Vr.template as<cx<T>>(), sw.tauR.template as<cx<T>>(), c,
-                              C, 1, batch, s);
+        svd_apply_v<T>(sw, true, c, C, 1, s);
         launch_norm2<double>(C, c, C, bcn2.as<double>(), batch, s);
         launch_real_matvec(C, sw.QBt.template as<double>(), 0, c, C, bbeta.as<cx<double>>(), C,
                            batch, s);
@@ -87,8 +86,7 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
         // x = V_b (Q_B phi)
         launch_real_matvec(C, sw.QBt.template as<double>(), 1, bphi.as<cx<double>>(), C, v, C,
                            batch, s);
-        launch_apply_right<T>(false, C, sw.Vr.template as<cx<T>>(), sw.tauR.template as<cx<T>>(), v,
-                              C, 1, batch, s);
+        svd_apply_v<T>(sw, false, v, C, 1, s);
         IA_HIP(hipMemcpy2DAsync(x, sizeof(cx<double>) * sx, v, sizeof(cx<double>) * C,
                                 sizeof(cx<double>) * C, batch, hipMemcpyDeviceToDevice, s));
     } else {

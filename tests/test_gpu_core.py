@@ -208,6 +208,7 @@ def test_bidiag_split_and_fused_forms_agree(ia, shape, monkeypatch):
     B = 3
     A = rng.standard_normal((B,) + shape) + 1j * rng.standard_normal((B,) + shape)
     b = rng.standard_normal((B, shape[0])) + 1j * rng.standard_normal((B, shape[0]))
+    monkeypatch.setenv("IA_ONE_STAGE", "1")
     monkeypatch.setenv("IA_GEBRD_SPLIT_BELOW", "1000")
     xs, *_ = ia.regsolve(A, b, filter="tikhonov", select="fixed", lam=0.05)
     monkeypatch.setenv("IA_GEBRD_SPLIT_BELOW", "0")
@@ -217,3 +218,52 @@ def test_bidiag_split_and_fused_forms_agree(ia, shape, monkeypatch):
         x0 = rt.tikhonov(U, s, V, b[i], 0.05)
         assert np.linalg.norm(xs[i] - x0) <= 1e-12 * np.linalg.norm(x0)
         assert np.linalg.norm(xf[i] - x0) <= 1e-12 * np.linalg.norm(x0)
+
+
+@pytest.mark.parametrize("shape", [(128, 128), (200, 129), (129, 200), (300, 257), (257, 700),
+                                   (600, 600)])
+def test_two_stage_csvd_against_lapack(ia, shape, monkeypatch):
+    """Columns >= 128: band reduction (QR / LQ panels) + bulge chasing."""
+    monkeypatch.delenv("IA_ONE_STAGE", raising=False)
+    A = _cr(shape, 12)
+    U, s, V = ia.csvd(A)
+    s0 = np.linalg.svd(A, compute_uv=False)
+    np.testing.assert_allclose(s, s0, rtol=1e-12, atol=1e-13 * s0[0])
+    np.testing.assert_allclose((U * s) @ V.conj().T, A, atol=1e-12 * s0[0] * np.sqrt(max(shape)))
+    p = min(shape)
+    np.testing.assert_allclose(U.conj().T @ U, np.eye(p), atol=1e-12)
+    np.testing.assert_allclose(V.conj().T @ V, np.eye(p), atol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(200, 200), (500, 160)])
+def test_two_stage_graded_spectrum(ia, shape, monkeypatch):
+    monkeypatch.delenv("IA_ONE_STAGE", raising=False)
+    mr, nc = shape
+    U0, _ = np.linalg.qr(_cr((mr, nc), 13))
+    V0, _ = np.linalg.qr(_cr((nc, nc), 14))
+    sv = 10.0 ** (-15 * np.arange(nc) / (nc - 1))
+    A = (U0 * sv) @ V0.conj().T
+    U, s, V = ia.csvd(A)
+    s_np = np.linalg.svd(A, compute_uv=False)
+    assert np.abs(s - s_np).max() <= 4e-15 * s_np[0] * nc
+    np.testing.assert_allclose(s[: nc // 2], sv[: nc // 2], rtol=1e-6)
+    np.testing.assert_allclose((U * s) @ V.conj().T, A, atol=1e-13 * np.sqrt(mr))
+
+
+@pytest.mark.parametrize("shape", [(300, 257), (160, 400), (700, 200)])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_one_and_two_stage_solutions_agree(ia, shape, precision, monkeypatch):
+    rng = np.random.default_rng(15)
+    B = 3
+    A = rng.standard_normal((B,) + shape) + 1j * rng.standard_normal((B,) + shape)
+    b = rng.standard_normal((B, shape[0])) + 1j * rng.standard_normal((B, shape[0]))
+    monkeypatch.setenv("IA_ONE_STAGE", "1")
+    x1, *_ = ia.regsolve(A, b, filter="tikhonov", select="fixed", lam=0.05, precision=precision)
+    monkeypatch.delenv("IA_ONE_STAGE")
+    x2, *_ = ia.regsolve(A, b, filter="tikhonov", select="fixed", lam=0.05, precision=precision)
+    tol = 1e-12 if precision == "fp64" else 2e-5
+    for i in range(B):
+        U, s, V = rt.csvd(A[i])
+        x0 = rt.tikhonov(U, s, V, b[i], 0.05)
+        assert np.linalg.norm(x1[i] - x0) <= tol * np.linalg.norm(x0)
+        assert np.linalg.norm(x2[i] - x0) <= tol * np.linalg.norm(x0)
