@@ -11,11 +11,11 @@
 //   left step:  H_L from B[c:c+L, c], B[c:c+L, c:min(C, c+L+NB)] <- H_L^H B,
 //               so B[c, c] = beta (real), B[c+1:c+L, c] = 0.
 // Band B = Q2 Bd P2^H with Q2 = H_L(1) H_L(2) ..., P2 = H_R(1) H_R(2) ... in
-// sequential (sweep, task) order.  Task (s, t) runs at time step 3 s + t:
-// tasks of one time step touch disjoint blocks, and any two tasks whose
-// blocks meet run in sequential order, so the pipelined result is the
-// sequential one bit for bit.  One workgroup per matrix; wave w runs the
-// sweeps s = w mod NWB; a barrier closes every time step.
+// sequential (sweep, task) order.  One workgroup per matrix; wave w runs the
+// sweeps s = w mod NWB in order, and task (s, t) waits (LDS progress keys)
+// until tasks 0..t+2 of sweep s-1 are done: that covers every earlier task
+// whose block meets its own, and tasks that are ready together touch
+// disjoint blocks, so the result is the sequential one bit for bit.
 //
 // Band storage: B(i, j) at j * LD + (i - j + KU), KU = 2 NB - 1, LD = 3 NB,
 // which holds the fill of the chase (up to 2 NB - 1 above the diagonal and
@@ -56,126 +56,273 @@ __global__ void k_band_extract(int C, const cx<T> *__restrict__ Zall, int ldz, s
     Ball[(size_t)b * C * BLD + e] = v;
 }
 
-// wave-level LDS handoff: lane 0's writes visible to every lane of the wave
+// Wave-level LDS handoff: one wave's LDS operations execute in order, so
+// this only has to stop the compiler from moving memory operations across
+// it (a wavefront-scope fence: no wait for outstanding global stores).
 __device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // keep each phase's loads and registers inside the phase
+    __builtin_amdgcn_sched_barrier(0);
 }
 
+// Per-wave LDS: the overlap block B[c:c+L, c:c+L] (row-major, padded), the
+// current reflector u (in the data precision) and tau, beta.
 template <typename T>
-__device__ __forceinline__ void bulge_task(int C, cx<T> *__restrict__ B, int r, int c, int L,
-                           cx<double> *__restrict__ us, cx<T> *__restrict__ uR,
-                           cx<T> *__restrict__ tR, cx<T> *__restrict__ uL, cx<T> *__restrict__ tL,
-                           int lane) {
-    cx<T> x[BNB];
-    // ---------------- right step: lane = row r + lane, columns c..c+L-1
-    const int nr = c + L - r;
-    if (lane < nr) {
+struct WaveLds {
+    cx<T> tile[BNB][BNB + 1];
+    cx<T> u[BNB];
+    cx<double> pv[2];
+};
+
+// Reflector of the L values v_q held by lanes q < L (LAPACK zlarfg on
+// (v_0, v_1..)), generated in double; u lands in us.u, tau/beta in us.pv.
+template <typename T>
+__device__ __forceinline__ void wave_larfg(cx<double> vq, int L, int lane, WaveLds<T> &us) {
+    const double a2 = (lane >= 1 && lane < L) ? vq.re * vq.re + vq.im * vq.im : 0.0;
+    const double xn2 = wave_sum(a2);
+    const cx<double> alpha{__shfl(vq.re, 0, 64), __shfl(vq.im, 0, 64)};
+    const Refl rf = larfg(alpha, xn2);
+    // entries past L are zero so that the updates need no predicates
+    if (lane < BNB) us.u[lane] = lane == 0 ? cx<T>{1, 0} : lane < L ? cvt<T>(vq * rf.scale) : cx<T>{0, 0};
+    if (lane == 0) {
+        us.pv[0] = rf.tau;
+        us.pv[1] = cx<double>{rf.beta, 0};
+    }
+    wave_sync();
+}
+
+__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// w += a * b and w += conj(a) * b with four fused multiply-adds each
+template <typename T>
+__device__ __forceinline__ void cmac(cx<T> &w, cx<T> a, cx<T> b) {
+    w.re = fmat(a.re, b.re, fmat(-a.im, b.im, w.re));
+    w.im = fmat(a.re, b.im, fmat(a.im, b.re, w.im));
+}
+template <typename T>
+__device__ __forceinline__ void cmac_conj(cx<T> &w, cx<T> a, cx<T> b) {
+    w.re = fmat(a.re, b.re, fmat(a.im, b.im, w.re));
+    w.im = fmat(a.re, b.im, fmat(-a.im, b.re, w.im));
+}
+// x -= f * conj(u)
+template <typename T>
+__device__ __forceinline__ void csub_fconj(cx<T> &x, cx<T> f, cx<T> u) {
+    x.re = fmat(-f.re, u.re, fmat(-f.im, u.im, x.re));
+    x.im = fmat(-f.im, u.re, fmat(f.re, u.im, x.im));
+}
+// y -= u * f
+template <typename T>
+__device__ __forceinline__ void csub_uf(cx<T> &y, cx<T> u, cx<T> f) {
+    y.re = fmat(-u.re, f.re, fmat(u.im, f.im, y.re));
+    y.im = fmat(-u.re, f.im, fmat(-u.im, f.re, y.im));
+}
+
+// Band offset of element (i, j): j * BLD + i - j + BKU.
+__device__ __forceinline__ int boff(int i, int j) { return j * (BLD - 1) + i + BKU; }
+
+// Opaque copy of a lane offset: addresses derived from it are recomputed at
+// their use instead of being kept live across the update (register budget).
+__device__ __forceinline__ int fresh(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// One task (s, t) on one wave.  Right step: lanes 0..nt-1 hold the rows
+// r..c-1 above the overlap block, lanes 32..32+L-1 the overlap rows
+// c..c+L-1, each over columns c..c+L-1.  Left step: lanes 0..L-1 hold the
+// overlap columns (through the LDS tile, already right-updated), lanes
+// 32..32+nR-1 the columns c+L.. right of it, each over rows c..c+L-1.  With
+// PRE the right-hand columns are loaded together with the right step's
+// rows.  FULL: L == BNB (all but the last task of a sweep).  The data is
+// updated in its own precision; reflectors are generated in double.
+//
+// Carry: the right-hand block of task t (rows c..c+BNB-1, columns
+// c+BNB..) is exactly the block above the overlap of task t+1 of the same
+// sweep, which this wave runs next, and no other task touches it in
+// between; so it stays in the LDS tile (to_tile) instead of going to memory
+// and back (from_tile: task t+1 reads its top rows there).
+template <typename T, bool PRE, bool FULL>
+__device__ __forceinline__ void bulge_task(int C, cx<T> *__restrict__ B, int r, int c, int Ld,
+                                           bool from_tile, bool to_tile, WaveLds<T> &us,
+                                           cx<T> *__restrict__ uR, cx<T> *__restrict__ tR,
+                                           cx<T> *__restrict__ uL, cx<T> *__restrict__ tL,
+                                           int lane) {
+    const int L = FULL ? BNB : Ld;
+    const int nt = c - r;                                 // 1 (first task) or BNB
+    const int nR = min(C, c + L + BNB) - (c + L);         // columns right of the overlap
+    const bool top = lane < nt, ovl = lane >= 32 && lane - 32 < L;
+    const int row = top ? r + lane : c + (lane - 32);
+    const bool rcol = lane >= 32 && lane - 32 < nR;
+    const int col = c + L + (lane - 32);
+    const cx<T> *__restrict__ pr = B + boff(row, c);     // + q (BLD - 1): column c + q
+    cx<T> x[BNB], y[BNB];
+    // Loads are unconditional, with fixed offsets, so that all of them are
+    // in flight at once: entries past L lie inside the padded band buffer
+    // and are zeroed afterwards.
+    const cx<T> zero{0, 0};
+    if (top && from_tile) {
+#pragma unroll
+        for (int q = 0; q < BNB; ++q) x[q] = us.tile[lane][q];
+    } else if (top || ovl) {
+#pragma unroll
+        for (int q = 0; q < BNB; ++q) x[q] = pr[q * (BLD - 1)];
+    }
+    if (PRE && rcol) {
+        const cx<T> *__restrict__ pc = B + boff(c, col);  // + p: row c + p
+#pragma unroll
+        for (int p = 0; p < BNB; ++p) y[p] = pc[p];
+    }
+    if (!FULL) {
 #pragma unroll
         for (int q = 0; q < BNB; ++q)
-            if (q < L) x[q] = B[bidx(r + lane, c + q)];
+            if (q >= L) {
+                x[q] = zero;
+                if (PRE) y[q] = zero;
+            }
     }
-    if (lane == 0) {
-        double xn2 = 0;
-#pragma unroll
-        for (int q = 1; q < BNB; ++q)
-            if (q < L) xn2 += (double)x[q].re * x[q].re + (double)x[q].im * x[q].im;
-        const Refl rf = larfg(conj(cvt<double>(x[0])), xn2);
-        us[0] = cx<double>{1, 0};
-#pragma unroll
-        for (int q = 1; q < BNB; ++q)
-            if (q < L) us[q] = conj(cvt<double>(x[q])) * rf.scale;
-        us[BNB] = rf.tau;
-        us[BNB + 1] = cx<double>{rf.beta, 0};
-    }
-    wave_sync();
+    cx<double> rq{0, 0};
+    if (lane < L)
+        rq = conj(cvt<double>(from_tile ? us.tile[0][lane] : B[boff(r, c + lane)]));
+    wave_sync();  // tile reads done before the tile is rewritten
+    // ---------------- right step: B[r:c+L, c:c+L] <- B H_R
+    wave_larfg<T>(rq, L, lane, us);
     {
-        const cx<double> tau = us[BNB];
-        if (lane < nr) {
-            cx<double> w{0, 0};
+        const cx<T> tau = cvt<T>(us.pv[0]);
+        if (top || ovl) {
+            // u is read from LDS in chunks of 8 so that only the data block
+            // and one chunk are live in registers
+            cx<T> w0{0, 0}, w1{0, 0};   // two chains
 #pragma unroll
-            for (int q = 0; q < BNB; ++q)
-                if (q < L) w = w + cvt<double>(x[q]) * us[q];
-            const cx<double> f = tau * w;
+            for (int q0 = 0; q0 < BNB; q0 += 8) {
+#pragma unroll
+                for (int q = q0; q < q0 + 8; q += 2) {
+                    cmac(w0, x[q], us.u[q]);
+                    cmac(w1, x[q + 1], us.u[q + 1]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            const cx<T> f = tau * (w0 + w1);
+#pragma unroll
+            for (int q0 = 0; q0 < BNB; q0 += 8) {
+#pragma unroll
+                for (int q = q0; q < q0 + 8; ++q) csub_fconj(x[q], f, us.u[q]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
             if (lane == 0) {
-                x[0] = cx<T>{(T)us[BNB + 1].re, 0};
+                x[0] = cx<T>{(T)us.pv[1].re, 0};
 #pragma unroll
                 for (int q = 1; q < BNB; ++q) x[q] = cx<T>{0, 0};
-            } else {
-#pragma unroll
-                for (int q = 0; q < BNB; ++q)
-                    if (q < L) x[q] = cvt<T>(cvt<double>(x[q]) - f * conj(us[q]));
             }
+        }
+        if (top) {
+            cx<T> *__restrict__ pw = B + fresh(boff(row, c));
 #pragma unroll
             for (int q = 0; q < BNB; ++q)
-                if (q < L) B[bidx(r + lane, c + q)] = x[q];
+                if (FULL || q < L) pw[q * (BLD - 1)] = x[q];
         }
-        if (lane < L) uR[lane] = cvt<T>(us[lane]);
-        if (lane == 0) *tR = cvt<T>(tau);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // ---------------- left step: lane = column c + lane, rows c..c+L-1
-    const int nc = min(C, c + L + BNB) - c;
-    if (lane < nc) {
+        if (ovl) {
 #pragma unroll
-        for (int p = 0; p < BNB; ++p)
-            if (p < L) x[p] = B[bidx(c + p, c + lane)];
-    }
-    if (lane == 0) {
-        double xn2 = 0;
-#pragma unroll
-        for (int p = 1; p < BNB; ++p)
-            if (p < L) xn2 += (double)x[p].re * x[p].re + (double)x[p].im * x[p].im;
-        const Refl rf = larfg(cvt<double>(x[0]), xn2);
-        us[0] = cx<double>{1, 0};
-#pragma unroll
-        for (int p = 1; p < BNB; ++p)
-            if (p < L) us[p] = cvt<double>(x[p]) * rf.scale;
-        us[BNB] = rf.tau;
-        us[BNB + 1] = cx<double>{rf.beta, 0};
+            for (int q = 0; q < BNB; ++q) us.tile[lane - 32][q] = x[q];
+        }
+        if (lane < L) uR[lane] = us.u[lane];
+        if (lane == 0) *tR = cvt<T>(us.pv[0]);
     }
     wave_sync();
-    {
-        const cx<double> tau = us[BNB];
-        if (lane < nc) {
-            cx<double> w{0, 0};
+    // ---------------- left step: B[c:c+L, c:c+L+nR] <- H_L^H B
+    if (lane < L) {
+#pragma unroll
+        for (int p = 0; p < BNB; ++p) {
+            const cx<T> v = us.tile[p][lane];
+            y[p] = (FULL || p < L) ? v : zero;
+        }
+    }
+    if (!PRE && rcol) {
+        const cx<T> *__restrict__ pc = B + boff(c, col);
+#pragma unroll
+        for (int p = 0; p < BNB; ++p) y[p] = pc[p];
+        if (!FULL) {
 #pragma unroll
             for (int p = 0; p < BNB; ++p)
-                if (p < L) w = w + conj(us[p]) * cvt<double>(x[p]);
-            const cx<double> f = conj(tau) * w;
-            if (lane == 0) {
-                x[0] = cx<T>{(T)us[BNB + 1].re, 0};
+                if (p >= L) y[p] = zero;
+        }
+    }
+    const cx<double> cq = lane < L ? cvt<double>(us.tile[lane][0]) : cx<double>{0, 0};
+    wave_larfg<T>(cq, L, lane, us);
+    {
+        const cx<T> ctau = conj(cvt<T>(us.pv[0]));
+        if (lane < L || rcol) {
+            cx<T> w0{0, 0}, w1{0, 0};
 #pragma unroll
-                for (int p = 1; p < BNB; ++p) x[p] = cx<T>{0, 0};
+            for (int p0 = 0; p0 < BNB; p0 += 8) {
+#pragma unroll
+                for (int p = p0; p < p0 + 8; p += 2) {
+                    cmac_conj(w0, us.u[p], y[p]);
+                    cmac_conj(w1, us.u[p + 1], y[p + 1]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            const cx<T> f = ctau * (w0 + w1);
+#pragma unroll
+            for (int p0 = 0; p0 < BNB; p0 += 8) {
+#pragma unroll
+                for (int p = p0; p < p0 + 8; ++p) csub_uf(y[p], us.u[p], f);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            if (lane == 0) {
+                y[0] = cx<T>{(T)us.pv[1].re, 0};
+#pragma unroll
+                for (int p = 1; p < BNB; ++p) y[p] = cx<T>{0, 0};
+            }
+            if (lane >= 32 && to_tile) {
+#pragma unroll
+                for (int p = 0; p < BNB; ++p) us.tile[p][lane - 32] = y[p];
             } else {
+                cx<T> *__restrict__ pw = B + fresh(boff(c, lane < L ? c + lane : col));
 #pragma unroll
                 for (int p = 0; p < BNB; ++p)
-                    if (p < L) x[p] = cvt<T>(cvt<double>(x[p]) - us[p] * f);
+                    if (FULL || p < L) pw[p] = y[p];
             }
-#pragma unroll
-            for (int p = 0; p < BNB; ++p)
-                if (p < L) B[bidx(c + p, c + lane)] = x[p];
         }
-        if (lane < L) uL[lane] = cvt<T>(us[lane]);
-        if (lane == 0) *tL = cvt<T>(tau);
+        if (lane < L) uL[lane] = us.u[lane];
+        if (lane == 0) *tL = cvt<T>(us.pv[0]);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    wave_sync();  // u / pv reads done before the next task rewrites them
 }
 
-template <typename T, int NWB>
+// Progress keys: slot w holds s * KEYM + (tasks of sweep s done) for the
+// sweep s = w mod NWB its wave is on; keys only grow.
+constexpr int KEYM = 1 << 16;
+// A wait that outlasts this many polls (seconds) marks the result NaN
+// instead of hanging; it cannot happen when the dependencies are met.
+constexpr long SPIN_MAX = 1L << 26;
+
+__device__ __forceinline__ bool wait_key(const int *p, int need) {
+    long n = 0;
+    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+        if (++n > SPIN_MAX) return false;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return true;
+}
+
+// Dataflow chase: wave w runs sweeps s = w, w + NWB, ... in order; task
+// (s, t) starts once tasks 0..t+2 of sweep s-1 are done (the tasks of
+// sweep s-1 it overlaps); tasks that are ready together touch disjoint
+// blocks, so the result is the sequential one.
+template <typename T, int NWB, bool PRE>
 __global__ __launch_bounds__(64 * NWB) void k_bulge(int C, cx<T> *__restrict__ Ball,
                                                     cx<T> *__restrict__ uRall,
                                                     cx<T> *__restrict__ uLall,
                                                     cx<T> *__restrict__ tRall,
                                                     cx<T> *__restrict__ tLall, int tmax,
-                                                    int nsteps, double *__restrict__ dall,
+                                                    double *__restrict__ dall,
                                                     double *__restrict__ eall) {
-    __shared__ cx<double> ush[NWB][BNB + 2];
+    __shared__ WaveLds<T> ush[NWB];
+    __shared__ int prog[NWB];
+    __shared__ int bad;
     const int b = blockIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     cx<T> *B = Ball + (size_t)b * C * BLD;
     const size_t nslot = (size_t)(C - 1) * tmax;
@@ -183,27 +330,47 @@ __global__ __launch_bounds__(64 * NWB) void k_bulge(int C, cx<T> *__restrict__ B
     cx<T> *uL = uLall + (size_t)b * nslot * BNB;
     cx<T> *tR = tRall + (size_t)b * nslot;
     cx<T> *tL = tLall + (size_t)b * nslot;
-    for (int ts = 0; ts < nsteps; ++ts) {
-        const int shi = min(C - 2, ts / 3);
-        const int lo = ts - tmax + 1;
-        const int slo = lo <= 0 ? 0 : (lo + 2) / 3;
-        const int s0 = slo + ((wv - slo % NWB) + NWB) % NWB;
-        for (int s = s0; s <= shi; s += NWB) {
-            const int t = ts - 3 * s;
-            if (t < 0 || t >= ntask_of(C, s)) continue;
+    if (lane == 0) prog[wv] = -1;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    bool ok = true;
+    for (int s = wv; s <= C - 2 && ok; s += NWB) {
+        const int nts = ntask_of(C, s);
+        const int ntp = s > 0 ? ntask_of(C, s - 1) : 0;
+        for (int t = 0; t < nts; ++t) {
+            if (s > 0 && !wait_key(&prog[(s - 1) % NWB], (s - 1) * KEYM + min(t + 3, ntp))) {
+                ok = false;
+                break;
+            }
             const int r = t == 0 ? s : s + 1 + (t - 1) * BNB;
             const int c = t == 0 ? s + 1 : r + BNB;
             const int L = min(BNB, C - c);
             const size_t slot = (size_t)s * tmax + t;
-            bulge_task<T>(C, B, r, c, L, ush[wv], uR + slot * BNB, tR + slot, uL + slot * BNB,
-                          tL + slot, lane);
+            const bool from_tile = t > 0, to_tile = t + 1 < nts;
+            if (L == BNB)
+                bulge_task<T, PRE, true>(C, B, r, c, L, from_tile, to_tile, ush[wv],
+                                         uR + slot * BNB, tR + slot, uL + slot * BNB, tL + slot,
+                                         lane);
+            else
+                bulge_task<T, PRE, false>(C, B, r, c, L, from_tile, to_tile, ush[wv],
+                                          uR + slot * BNB, tR + slot, uL + slot * BNB, tL + slot,
+                                          lane);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (lane == 0)
+                __hip_atomic_store(&prog[wv], s * KEYM + t + 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        __syncthreads();
     }
+    if (!ok && lane == 0) bad = 1;
+    // a wave that gave up publishes completion so no other wave waits on it
+    if (lane == 0)
+        __hip_atomic_store(&prog[wv], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
     double *d = dall + (size_t)b * C, *e = eall + (size_t)b * C;
+    const double nan = __builtin_nan("");
     for (int i = threadIdx.x; i < C; i += blockDim.x) {
-        d[i] = (double)B[bidx(i, i)].re;
-        e[i] = i + 1 < C ? (double)B[bidx(i, i + 1)].re : 0.0;
+        d[i] = bad ? nan : (double)B[boff(i, i)].re;
+        e[i] = bad ? nan : (i + 1 < C ? (double)B[boff(i, i + 1)].re : 0.0);
     }
 }
 
@@ -282,6 +449,8 @@ __global__ __launch_bounds__(64) void k_bulge_apply(int C, const cx<T> *__restri
 int bulge_tmax(int C) { return ntask_of(C, 0); }
 
 size_t bulge_band_elems(int C) { return (size_t)C * BLD; }
+// the chase reads (and discards) up to BNB columns past a matrix's band
+size_t bulge_band_pad_elems() { return (size_t)(BNB + 1) * BLD; }
 size_t bulge_refl_elems(int C) { return C > 1 ? (size_t)(C - 1) * ntask_of(C, 0) : 0; }
 
 template <typename T>
@@ -293,12 +462,11 @@ ia_status launch_bulge(int C, const cx<T> *Z, int ldz, size_t sZ, cx<T> *Bs, cx<
     hipLaunchKernelGGL(k_band_extract<T>, dim3((unsigned)((nb + 255) / 256), batch), dim3(256), 0,
                        s, C, Z, ldz, sZ, Bs);
     const int tmax = ntask_of(C, 0);
-    int nsteps = 0;
-    for (int sw = 0; sw <= C - 2; ++sw) nsteps = std::max(nsteps, 3 * sw + ntask_of(C, sw));
-    // eight waves: a task keeps one block row (32 complex values) per lane
-    // plus its double-precision work in registers (~230-250 VGPRs)
-    hipLaunchKernelGGL((k_bulge<T, 8>), dim3(batch), dim3(512), 0, s, C, Bs, uR, uL, tR, tL, tmax,
-                       nsteps, d, e);
+    // eight waves; complex64 also holds the right-hand block in registers
+    // from the start of a task (two rows of 32 values per lane)
+    constexpr bool pre = sizeof(T) == sizeof(float);
+    hipLaunchKernelGGL((k_bulge<T, 8, pre>), dim3(batch), dim3(512), 0, s, C, Bs, uR, uL, tR, tL,
+                       tmax, d, e);
     IA_HIP(hipGetLastError());
     return IA_OK;
 }

@@ -181,6 +181,7 @@ ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *t
                             cx<T> *tauP, cx<T> *work, int batch, hipStream_t s);
 int bulge_tmax(int C);
 size_t bulge_band_elems(int C);
+size_t bulge_band_pad_elems();
 size_t bulge_refl_elems(int C);
 size_t bulge_apply_lds(int C);
 // bytes the chase's tasks read and write (each task's two blocks once)
@@ -364,7 +365,8 @@ struct SvdWork {
         IA_TRY(PBt.alloc(bc * C * sizeof(double)));
         if (ts) {
             const size_t ns = bulge_refl_elems(C), cp = (size_t)(C - BAND_NB);
-            IA_TRY(Bs.alloc(bulge_band_elems(C) * batch * sizeof(cx<T>)));
+            IA_TRY(Bs.alloc((bulge_band_elems(C) * batch + bulge_band_pad_elems()) *
+                            sizeof(cx<T>)));
             IA_TRY(uR2.alloc(ns * BAND_NB * batch * sizeof(cx<T>)));
             IA_TRY(uL2.alloc(ns * BAND_NB * batch * sizeof(cx<T>)));
             IA_TRY(tR2.alloc(ns * batch * sizeof(cx<T>)));
