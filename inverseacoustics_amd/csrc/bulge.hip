@@ -375,73 +375,80 @@ __global__ __launch_bounds__(64 * NWB) void k_bulge(int C, cx<T> *__restrict__ B
 }
 
 // v <- H v for the stage-two reflectors of one side, in sequential order
-// (FWD, with H^H: P2^H v, Q2^H v) or reverse order (P2 v, Q2 v).  One wave
-// per (vector, matrix); the vector lives in LDS; the reflectors of CH task
-// slots are staged through LDS with the next chunk's loads in flight.
-constexpr int CH = 32;
+// (FWD, with H^H: P2^H v, Q2^H v) or reverse order (P2 v, Q2 v).  Reflector
+// (s, t) acts on entries c..c+L-1 of the vector; at time step 2 s + t the
+// reflectors in use act on disjoint ranges and any two whose ranges meet
+// run in sequential order, so the result is the sequential product.  One
+// workgroup (NWA waves) per (vector, matrix), the vector in LDS; wave w
+// takes the sweeps s = w mod NWA and loads the reflector of its next step
+// while the current one is applied.
+constexpr int NWA = 16;
 
 template <typename T, bool FWD>
-__global__ __launch_bounds__(64) void k_bulge_apply(int C, const cx<T> *__restrict__ uall,
-                                                    const cx<T> *__restrict__ tall, int tmax,
-                                                    cx<double> *__restrict__ vall, size_t sv,
-                                                    int nvec) {
+__global__ __launch_bounds__(64 * NWA) void k_bulge_apply(int C, const cx<T> *__restrict__ uall,
+                                                          const cx<T> *__restrict__ tall,
+                                                          int tmax, int nsteps,
+                                                          cx<double> *__restrict__ vall,
+                                                          size_t sv, int nvec) {
     extern __shared__ cx<double> vs[];          // C entries
-    __shared__ cx<double> ub[CH][BNB];
-    __shared__ cx<double> tb[CH];
-    const int vec = blockIdx.x, b = blockIdx.y, lane = threadIdx.x;
+    const int vec = blockIdx.x, b = blockIdx.y;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const size_t nslot = (size_t)(C - 1) * tmax;
     const cx<T> *u = uall + (size_t)b * nslot * BNB;
     const cx<T> *tau = tall + (size_t)b * nslot;
     cx<double> *v = vall + (size_t)b * nvec * sv + (size_t)vec * sv;
-    for (int i = lane; i < C; i += 64) vs[i] = v[i];
-    const long nchunk = (long)((nslot + CH - 1) / CH);
-    // chunk loads: CH * BNB reflector entries, 64 lanes -> CH*BNB/64 each
-    constexpr int PER = CH * BNB / 64;
-    cx<T> pre[PER];
-    cx<T> pret{0, 0};
-    auto load = [&](long ch) {
-        const size_t base = (size_t)ch * CH;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const size_t e = base * BNB + (size_t)k * 64 + lane;
-            pre[k] = e < nslot * BNB ? u[e] : cx<T>{0, 0};
+    for (int i = threadIdx.x; i < C; i += blockDim.x) vs[i] = v[i];
+    // first task of this wave at a step (or -1): s = slo.. with s = wv mod NWA
+    auto first = [&](int ts) -> int {
+        if (ts < 0 || ts >= nsteps) return -1;
+        const int shi = min(C - 2, ts / 2);
+        const int lo = ts - tmax + 1;
+        const int slo = lo <= 0 ? 0 : (lo + 1) / 2;
+        for (int sw = slo + ((wv - slo % NWA) + NWA) % NWA; sw <= shi; sw += NWA) {
+            const int t = ts - 2 * sw;
+            if (t >= 0 && t < ntask_of(C, sw)) return sw;
         }
-        if (lane < CH) pret = base + lane < nslot ? tau[base + lane] : cx<T>{0, 0};
+        return -1;
     };
-    load(FWD ? 0 : nchunk - 1);
-    for (long it = 0; it < nchunk; ++it) {
-        const long ch = FWD ? it : nchunk - 1 - it;
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const int e = k * 64 + lane;
-            ub[e / BNB][e % BNB] = cvt<double>(pre[k]);
-        }
-        if (lane < CH) tb[lane] = cvt<double>(pret);
-        __syncthreads();
-        if (it + 1 < nchunk) load(FWD ? ch + 1 : ch - 1);
-        for (int kk = 0; kk < CH; ++kk) {
-            const int k = FWD ? kk : CH - 1 - kk;
-            const size_t slot = (size_t)ch * CH + k;
-            if (slot >= nslot) continue;
-            const int s = (int)(slot / tmax), t = (int)(slot % tmax);
-            if (t >= ntask_of(C, s)) continue;
-            const int c = t == 0 ? s + 1 : s + 1 + t * BNB;
-            const int L = min(BNB, C - c);
-            const cx<double> tk = FWD ? conj(tb[k]) : tb[k];
-            cx<double> x{0, 0}, uu{0, 0};
-            if (lane < L) {
-                x = vs[c + lane];
-                uu = ub[k][lane];
-            }
-            const cx<double> pr = conj(uu) * x;
-            const cx<double> dot{wave_sum(pr.re), wave_sum(pr.im)};
-            if (lane < L) vs[c + lane] = x - uu * (tk * dot);
-            __syncthreads();
-        }
-    }
+    auto fetch = [&](int sw, int t, cx<T> &uu, cx<T> &tt) {
+        const size_t slot = (size_t)sw * tmax + t;
+        uu = lane < BNB ? u[slot * BNB + lane] : cx<T>{0, 0};
+        tt = tau[slot];
+    };
+    const int step0 = FWD ? 0 : nsteps - 1, dstep = FWD ? 1 : -1;
+    cx<T> nu{0, 0}, nt{0, 0};
+    int ns = first(step0);
+    if (ns >= 0) fetch(ns, step0 - 2 * ns, nu, nt);
     __syncthreads();
-    for (int i = lane; i < C; i += 64) v[i] = vs[i];
+    for (int k = 0, ts = step0; k < nsteps; ++k, ts += dstep) {
+        const int s0 = ns;
+        const cx<T> cu = nu, ct = nt;
+        // prefetch the next step's first task of this wave
+        ns = first(ts + dstep);
+        if (ns >= 0) fetch(ns, ts + dstep - 2 * ns, nu, nt);
+        if (s0 >= 0) {
+            const int shi = min(C - 2, ts / 2);
+            for (int sw = s0; sw <= shi; sw += NWA) {
+                const int t = ts - 2 * sw;
+                if (t < 0 || t >= ntask_of(C, sw)) continue;
+                cx<T> uT = cu, tT = ct;
+                if (sw != s0) fetch(sw, t, uT, tT);
+                const int c = t == 0 ? sw + 1 : sw + 1 + t * BNB;
+                const int L = min(BNB, C - c);
+                const cx<double> tk = FWD ? conj(cvt<double>(tT)) : cvt<double>(tT);
+                cx<double> x{0, 0}, uu{0, 0};
+                if (lane < L) {
+                    x = vs[c + lane];
+                    uu = cvt<double>(uT);
+                }
+                const cx<double> pr = conj(uu) * x;
+                const cx<double> dot{wave_sum(pr.re), wave_sum(pr.im)};
+                if (lane < L) vs[c + lane] = x - uu * (tk * dot);
+            }
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < C; i += blockDim.x) v[i] = vs[i];
 }
 
 }  // namespace
@@ -491,14 +498,16 @@ void launch_bulge_apply(bool fwd, int C, const cx<T> *u, const cx<T> *tau, cx<do
                         size_t sv, int nvec, int batch, hipStream_t s) {
     if (C < 2) return;
     const int tmax = ntask_of(C, 0);
+    int nsteps = 0;
+    for (int sw = 0; sw <= C - 2; ++sw) nsteps = std::max(nsteps, 2 * sw + ntask_of(C, sw));
     dim3 grid(nvec, batch);
     const size_t lds = bulge_apply_lds(C);
     if (fwd)
-        hipLaunchKernelGGL((k_bulge_apply<T, true>), grid, dim3(64), lds, s, C, u, tau, tmax, v,
-                           sv, nvec);
+        hipLaunchKernelGGL((k_bulge_apply<T, true>), grid, dim3(64 * NWA), lds, s, C, u, tau, tmax,
+                           nsteps, v, sv, nvec);
     else
-        hipLaunchKernelGGL((k_bulge_apply<T, false>), grid, dim3(64), lds, s, C, u, tau, tmax, v,
-                           sv, nvec);
+        hipLaunchKernelGGL((k_bulge_apply<T, false>), grid, dim3(64 * NWA), lds, s, C, u, tau,
+                           tmax, nsteps, v, sv, nvec);
 }
 
 #define INST(T)                                                                               \
