@@ -84,6 +84,184 @@ __global__ __launch_bounds__(BT) void k_qr_panel(int R, int C, int j0, int nbp,
     }
 }
 
+// Blocked form of k_qr_panel: the panel is factored SB columns at a time
+// with those columns held in LDS (rows j0..R-1), and each sub-panel's
+// reflectors are applied to the panel's remaining columns as one block,
+// A <- (I - V T V^H)^H A: two passes over each column (V^H a, then
+// a -= V T^H (V^H a)) instead of two passes per reflector.  Same outputs
+// (reflectors below the diagonal, tau, explicit V in Vw) as k_qr_panel.
+template <typename T, int SB>
+__global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int nbp,
+                                                    cx<T> *__restrict__ Yall, int ldy, size_t sY,
+                                                    cx<T> *__restrict__ tauall,
+                                                    cx<T> *__restrict__ Vwall) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *Vs = reinterpret_cast<cx<T> *>(smem_raw);  // SB columns of M rows
+    __shared__ double red[NW];
+    __shared__ cx<double> Gs[SB][SB];
+    __shared__ cx<double> Ts[SB][SB];
+    __shared__ cx<double> Zs[NW][SB];
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    cx<T> *A = Yall + (size_t)b * sY;
+    cx<T> *tau = tauall + (size_t)b * C;
+    cx<T> *Vw = Vwall + (size_t)b * R * NB;
+    const int M = R - j0;
+    for (int cs = j0; cs < j0 + nbp; cs += SB) {
+        const int w = min(SB, j0 + nbp - cs);
+        for (int i = 0; i < w; ++i)
+            for (int r = t; r < M; r += BT) Vs[(size_t)i * M + r] = A[(size_t)(cs + i) * ldy + j0 + r];
+        __syncthreads();
+        // ---- unblocked QR of the sub-panel in LDS
+        for (int i = 0; i < w; ++i) {
+            const int rc = cs + i - j0;  // local row of the diagonal
+            cx<T> *col = Vs + (size_t)i * M;
+            double p = 0;
+            for (int r = rc + 1 + t; r < M; r += BT) p += (double)abs2(col[r]);
+            const double xn2 = block_sum(p, red);
+            const Refl rf = larfg(cvt<double>(col[rc]), xn2);
+            __syncthreads();  // col[rc] read by all before it is overwritten
+            const cx<T> sc = cvt<T>(rf.scale);
+            for (int r = rc + 1 + t; r < M; r += BT) col[r] = col[r] * sc;
+            if (t == 0) {
+                col[rc] = cx<T>{(T)rf.beta, 0};
+                tau[cs + i] = cvt<T>(rf.tau);
+            }
+            __syncthreads();
+            const cx<double> ctau = conj(rf.tau);
+            for (int ip = i + 1 + wv; ip < w; ip += NW) {
+                cx<T> *cp = Vs + (size_t)ip * M;
+                double ar = 0, ai = 0;
+                for (int r = rc + lane; r < M; r += 64) {
+                    const cx<T> uu = (r == rc) ? cx<T>{1, 0} : col[r];
+                    const cx<T> y = cp[r];
+                    ar += (double)uu.re * y.re + (double)uu.im * y.im;
+                    ai += (double)uu.re * y.im - (double)uu.im * y.re;
+                }
+                const cx<double> f = ctau * cx<double>{wave_sum(ar), wave_sum(ai)};
+                for (int r = rc + lane; r < M; r += 64) {
+                    const cx<double> uu = (r == rc) ? cx<double>{1, 0} : cvt<double>(col[r]);
+                    cp[r] = cvt<T>(cvt<double>(cp[r]) - uu * f);
+                }
+            }
+            __syncthreads();
+        }
+        // ---- write back: factored columns and explicit reflectors
+        for (int i = 0; i < w; ++i) {
+            const int rc = cs + i - j0;
+            for (int r = t; r < M; r += BT) {
+                const cx<T> v = Vs[(size_t)i * M + r];
+                A[(size_t)(cs + i) * ldy + j0 + r] = v;
+                Vw[(size_t)(cs + i - j0) * R + j0 + r] =
+                    (r < rc) ? cx<T>{0, 0} : (r == rc) ? cx<T>{1, 0} : v;
+            }
+        }
+        const int k0 = cs + w, k1 = j0 + nbp;
+        if (k0 < k1) {
+            // ---- T of the sub-panel (LAPACK larft, forward): G(j, i) = v_j^H v_i
+            const int r0 = cs - j0;
+            for (int pr = wv; pr < w * w; pr += NW) {
+                const int jj = pr / w, ii = pr % w;
+                if (jj >= ii) continue;
+                double gr = 0, gi = 0;
+                for (int r = r0 + ii + lane; r < M; r += 64) {
+                    const cx<T> vi = (r == r0 + ii) ? cx<T>{1, 0} : Vs[(size_t)ii * M + r];
+                    const cx<T> vj = (r < r0 + jj) ? cx<T>{0, 0}
+                                     : (r == r0 + jj) ? cx<T>{1, 0} : Vs[(size_t)jj * M + r];
+                    gr += (double)vj.re * vi.re + (double)vj.im * vi.im;
+                    gi += (double)vj.re * vi.im - (double)vj.im * vi.re;
+                }
+                gr = wave_sum(gr);
+                gi = wave_sum(gi);
+                if (lane == 0) Gs[jj][ii] = cx<double>{gr, gi};
+            }
+            __syncthreads();
+            if (t == 0) {
+                for (int ii = 0; ii < w; ++ii) {
+                    const cx<double> ti = cvt<double>(tau[cs + ii]);
+                    for (int jj = 0; jj < ii; ++jj) {
+                        cx<double> acc{0, 0};
+                        for (int l = jj; l < ii; ++l) acc = acc + Ts[jj][l] * Gs[l][ii];
+                        Ts[jj][ii] = cx<double>{0, 0} - ti * acc;
+                    }
+                    Ts[ii][ii] = ti;
+                }
+            }
+            __syncthreads();
+            // ---- remaining panel columns: a <- a - V T^H (V^H a), one wave per column
+            for (int k = k0 + wv; k < k1; k += NW) {
+                cx<T> *Ak = A + (size_t)k * ldy + j0;
+                cx<double> acc[SB];
+#pragma unroll
+                for (int i = 0; i < SB; ++i) acc[i] = cx<double>{0, 0};
+                for (int r = r0 + lane; r < M; r += 64) {
+                    const cx<double> a = cvt<double>(Ak[r]);
+#pragma unroll
+                    for (int i = 0; i < SB; ++i) {
+                        if (i >= w || r < r0 + i) continue;
+                        const cx<double> u =
+                            (r == r0 + i) ? cx<double>{1, 0} : cvt<double>(Vs[(size_t)i * M + r]);
+                        acc[i] = acc[i] + conj(u) * a;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < SB; ++i) {
+                    const cx<double> v{wave_sum(acc[i].re), wave_sum(acc[i].im)};
+                    if (lane == 0) Zs[wv][i] = v;
+                }
+                // z = T^H acc (lane 0), then a <- a - V z with z read from LDS
+                if (lane == 0) {
+                    cx<double> z[SB];
+                    for (int jj = 0; jj < w; ++jj) {
+                        cx<double> sacc{0, 0};
+                        for (int ii = 0; ii <= jj; ++ii) sacc = sacc + conj(Ts[ii][jj]) * Zs[wv][ii];
+                        z[jj] = sacc;
+                    }
+                    for (int jj = 0; jj < w; ++jj) Zs[wv][jj] = z[jj];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                for (int r = r0 + lane; r < M; r += 64) {
+                    cx<double> a = cvt<double>(Ak[r]);
+                    for (int i = 0; i < w; ++i) {
+                        if (r < r0 + i) break;
+                        const cx<double> u =
+                            (r == r0 + i) ? cx<double>{1, 0} : cvt<double>(Vs[(size_t)i * M + r]);
+                        a = a - u * Zs[wv][i];
+                    }
+                    Ak[r] = cvt<T>(a);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Panel QR: the LDS-blocked kernel when SB columns of the panel fit in
+// 128 KB of LDS (SB = 8, 4 or 2), else the column-by-column kernel.
+template <typename T>
+void launch_qr_panel(int R, int C, int j0, int nbp, cx<T> *Y, int ldy, size_t sY, cx<T> *tau,
+                     cx<T> *Vw, int batch, hipStream_t s) {
+    const size_t colb = sizeof(cx<T>) * (size_t)(R - j0);
+    const size_t cap = 128 * 1024;
+    auto go = [&](auto kern, int sb) {
+        const size_t lds = colb * sb;
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL(kern, dim3(batch), dim3(BT), lds, s, R, C, j0, nbp, Y, ldy, sY, tau, Vw);
+    };
+    if (colb * 8 <= cap)
+        go(k_qr_panel_lds<T, 8>, 8);
+    else if (colb * 4 <= cap)
+        go(k_qr_panel_lds<T, 4>, 4);
+    else if (colb * 2 <= cap)
+        go(k_qr_panel_lds<T, 2>, 2);
+    else
+        hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, R, C, j0, nbp, Y, ldy, sY,
+                           tau, Vw);
+}
+
 // T (nbp x nbp, upper) with H_0 ... H_{nbp-1} = I - V T V^H from G = V^H V
 // and tau (LAPACK larft, forward columnwise); one wave per matrix.
 template <typename T>
@@ -169,8 +347,7 @@ ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, c
     const size_t sV = (size_t)R * NB, sT = (size_t)NB * NB, sW = (size_t)NB * C;
     for (int j0 = 0; j0 < C; j0 += NB) {
         const int nbp = std::min(NB, C - j0);
-        hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, R, C, j0, nbp, Y, ldy, sY,
-                           tau, Vw);
+        launch_qr_panel<T>(R, C, j0, nbp, Y, ldy, sY, tau, Vw, batch, s);
         const int N = C - j0 - nbp, M = R - j0;
         if (N <= 0) continue;
         const cx<T> *V0 = Vw + j0;  // rows j0.. of the explicit panel reflectors
@@ -217,8 +394,7 @@ ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *t
     for (int k = 0; k < C; k += NB) {
         const int w = std::min(NB, C - k);
         // ---- QR of columns k..k+w-1, left update of columns k+w..
-        hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, R, C, k, w, Z, ldz, sZ,
-                           tauQ, Vw);
+        launch_qr_panel<T>(R, C, k, w, Z, ldz, sZ, tauQ, Vw, batch, s);
         const size_t sV = (size_t)R * NB;
         const int N = C - k - w, M = R - k;
         if (N > 0) {
@@ -237,8 +413,7 @@ ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *t
         const int nbp = std::min(NB, N2);    // reflectors (rows with columns left)
         hipLaunchKernelGGL(k_lq_gather<T>, dim3((N2 + 255) / 256, nbp, batch), dim3(256), 0, s, k,
                            N2, nbp, Z, ldz, sZ, Pm, Cp, sP);
-        hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, Cp, Cp, k, nbp, Pm, Cp, sP,
-                           tauP, Vw);
+        launch_qr_panel<T>(Cp, Cp, k, nbp, Pm, Cp, sP, tauP, Vw, batch, s);
         hipLaunchKernelGGL(k_lq_scatter<T>, dim3((nbp * nbp + 255) / 256, batch), dim3(256), 0, s,
                            k, nbp, Z, ldz, sZ, Pm, Cp, sP);
         // rows below the reflected ones: Z2 <- Z2 - ((Z2 V) T) V^H
