@@ -97,6 +97,195 @@ __global__ __launch_bounds__(512) void k_panel(int n, int k, int jb, cx<T> *A, i
     }
 }
 
+// Blocked form of k_panel: SB columns at a time are held in LDS (rows
+// k..n-1) and factored there (same pivot rule); each sub-panel's row
+// interchanges are then applied to the panel's other columns, and its L
+// to the panel's later columns as one rank-SB update (one pass per column
+// instead of one per pivot).  Thread 0 finally lists the panel's
+// interchanges as a row permutation, perm = (count, dst[], src[]), for
+// k_laswp_perm.
+constexpr int LBT = 1024;
+constexpr size_t LDS_CAP = 128 * 1024;
+constexpr int LNW = LBT / 64;
+constexpr int PERM_MAX = 2 * NB;
+
+template <typename T, int SB>
+__global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *A, int lda,
+                                                  size_t strideA, int *ipiv, int *info,
+                                                  int *permall) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *Vs = reinterpret_cast<cx<T> *>(smem_raw);  // SB columns of M rows
+    __shared__ T red_v[LNW];
+    __shared__ int red_i[LNW];
+    __shared__ cx<T> pinv_s;
+    __shared__ int piv_s;
+    __shared__ int prow[PERM_MAX], psrc[PERM_MAX];
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    cx<T> *Ab = A + (size_t)b * strideA;
+    int *ip = ipiv + (size_t)b * n;
+    const int M = n - k;
+    for (int cs = k; cs < k + jb; cs += SB) {
+        const int w = min(SB, k + jb - cs);
+        for (int i = 0; i < w; ++i)
+            for (int r = t; r < M; r += LBT) Vs[(size_t)i * M + r] = Ab[(size_t)(cs + i) * lda + k + r];
+        __syncthreads();
+        for (int i = 0; i < w; ++i) {
+            const int rc = cs + i - k;
+            cx<T> *col = Vs + (size_t)i * M;
+            // pivot: first maximum of |re| + |im| over rows rc..M-1
+            T best = (T)-1;
+            int bi = M;
+            for (int r = rc + t; r < M; r += LBT) {
+                const T v = cabs1(col[r]);
+                if (v > best) {
+                    best = v;
+                    bi = r;
+                }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const T v2 = __shfl_xor(best, o, 64);
+                const int i2 = __shfl_xor(bi, o, 64);
+                if (v2 > best || (v2 == best && i2 < bi)) {
+                    best = v2;
+                    bi = i2;
+                }
+            }
+            if (lane == 0) {
+                red_v[wv] = best;
+                red_i[wv] = bi;
+            }
+            __syncthreads();
+            if (t == 0) {
+                T bv = red_v[0];
+                int bx = red_i[0];
+                for (int q = 1; q < LNW; ++q)
+                    if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bx)) {
+                        bv = red_v[q];
+                        bx = red_i[q];
+                    }
+                piv_s = bx;
+                ip[cs + i] = k + bx;
+                if (bv == (T)0 && info && info[b] == 0) info[b] = cs + i + 1;
+                // swap rows rc and bx inside the sub-panel (LDS)
+                if (bx != rc)
+                    for (int q = 0; q < w; ++q) {
+                        const cx<T> tmp = Vs[(size_t)q * M + rc];
+                        Vs[(size_t)q * M + rc] = Vs[(size_t)q * M + bx];
+                        Vs[(size_t)q * M + bx] = tmp;
+                    }
+                const cx<T> d = col[rc];
+                pinv_s = (bv == (T)0) ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
+            }
+            __syncthreads();
+            const cx<T> pinv = pinv_s;
+            for (int r = rc + 1 + t; r < M; r += LBT) {
+                const cx<T> l = col[r] * pinv;
+                col[r] = l;
+                for (int q = i + 1; q < w; ++q) {
+                    cx<T> *cq = Vs + (size_t)q * M;
+                    cq[r] = cq[r] - l * cq[rc];
+                }
+            }
+            __syncthreads();
+        }
+        for (int i = 0; i < w; ++i)
+            for (int r = t; r < M; r += LBT) Ab[(size_t)(cs + i) * lda + k + r] = Vs[(size_t)i * M + r];
+        // this sub-panel's interchanges on the panel's other columns
+        if (t < jb) {
+            const int cc = k + t;
+            if (cc < cs || cc >= cs + w) {
+                cx<T> *Ac = Ab + (size_t)cc * lda;
+                for (int i = 0; i < w; ++i) {
+                    const int r1 = cs + i, r2 = ip[cs + i];
+                    if (r2 != r1) {
+                        const cx<T> tmp = Ac[r1];
+                        Ac[r1] = Ac[r2];
+                        Ac[r2] = tmp;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // later panel columns: U12 = L11^-1 a[cs..cs+w), a[cs+w..] -= L21 U12
+        const int r0 = cs - k;
+        for (int j = cs + w + wv; j < k + jb; j += LNW) {
+            cx<T> *Aj = Ab + (size_t)j * lda + k;
+            // tiny forward solve, redundantly in every lane
+            cx<T> x[SB];
+#pragma unroll
+            for (int i = 0; i < SB; ++i) {
+                if (i >= w) break;
+                cx<T> v = Aj[r0 + i];
+                for (int l = 0; l < i; ++l) v = v - Vs[(size_t)l * M + r0 + i] * x[l];
+                x[i] = v;
+            }
+            if (lane < w) {
+#pragma unroll
+                for (int i = 0; i < SB; ++i)
+                    if (i == lane) Aj[r0 + i] = x[i];
+            }
+            for (int r = r0 + w + lane; r < M; r += 64) {
+                cx<T> a = Aj[r];
+#pragma unroll
+                for (int i = 0; i < SB; ++i)
+                    if (i < w) a = a - Vs[(size_t)i * M + r] * x[i];
+                Aj[r] = a;
+            }
+        }
+        __syncthreads();
+    }
+    // the panel's interchanges as one row permutation (for the other columns)
+    if (t == 0) {
+        int *pm = permall + (size_t)b * (1 + 2 * PERM_MAX);
+        int *rows = prow, *src = psrc;
+        int cnt = 0;
+        auto slot = [&](int r) {
+            for (int q = 0; q < cnt; ++q)
+                if (rows[q] == r) return q;
+            rows[cnt] = r;
+            src[cnt] = r;
+            return cnt++;
+        };
+        for (int c = k; c < k + jb; ++c) {
+            const int p = ip[c];
+            if (p == c) continue;
+            const int a1 = slot(c), a2 = slot(p);
+            const int tmp = src[a1];
+            src[a1] = src[a2];
+            src[a2] = tmp;
+        }
+        int m = 0;
+        for (int q = 0; q < cnt; ++q)
+            if (src[q] != rows[q]) {
+                pm[1 + m] = rows[q];
+                pm[1 + PERM_MAX + m] = src[q];
+                ++m;
+            }
+        pm[0] = m;
+    }
+}
+
+// new[dst_q] = old[src_q] for the listed rows, in every column outside
+// [k, k+jb); one wave per column (all loads before the stores).
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp_perm(int n, int k, int jb, cx<T> *A, int lda,
+                                                   size_t strideA, const int *permall) {
+    const int b = blockIdx.y;
+    const int lane = threadIdx.x & 63;
+    int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (col >= n - jb) return;
+    if (col >= k) col += jb;
+    const int *pm = permall + (size_t)b * (1 + 2 * PERM_MAX);
+    const int m = pm[0];
+    if (lane >= m) return;
+    cx<T> *Ac = A + (size_t)b * strideA + (size_t)col * lda;
+    const cx<T> v = Ac[pm[1 + PERM_MAX + lane]];
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    Ac[pm[1 + lane]] = v;
+}
+
 // apply interchanges ipiv[k..k+jb) to all columns outside [k, k+jb)
 template <typename T>
 __global__ void k_laswp(int n, int k, int jb, cx<T> *A, int lda, size_t strideA,
@@ -289,6 +478,12 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
                         hipStream_t s) {
     if (n <= 0 || batch <= 0) return IA_OK;
     const cx<T> mone{-1, 0}, one{1, 0};
+    int *perm = nullptr;  // per matrix: the current panel's row permutation
+    if (hipMallocAsync((void **)&perm, sizeof(int) * (1 + 2 * PERM_MAX) * (size_t)batch, s) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        perm = nullptr;
+    }
     // panels of NBO columns, each factored as NB-wide sub-panels whose
     // updates stay inside the panel; then U12 = L11^-1 A12 and one rank-NBO
     // update of the trailing matrix per panel
@@ -296,11 +491,35 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
         const int pend = min(n, p0 + NBO);
         for (int k = p0; k < pend; k += NB) {
             const int jb = min(NB, pend - k);
-            hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(512), 0, s, n, k, jb, A, lda,
-                               strideA, ipiv, info);
-            if (n - jb > 0) {
-                dim3 g((n - jb + 255) / 256, batch);
-                hipLaunchKernelGGL(k_laswp<T>, g, dim3(256), 0, s, n, k, jb, A, lda, strideA, ipiv);
+            const size_t colb = sizeof(cx<T>) * (size_t)(n - k);
+            const int sb = colb * 8 <= LDS_CAP ? 8 : colb * 4 <= LDS_CAP ? 4 : colb * 2 <= LDS_CAP ? 2 : 0;
+            if (sb > 0 && perm) {
+                auto go = [&](auto kern) {
+                    (void)hipFuncSetAttribute((const void *)kern,
+                                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              (int)(colb * sb));
+                    hipLaunchKernelGGL(kern, dim3(batch), dim3(LBT), colb * sb, s, n, k, jb, A, lda,
+                                       strideA, ipiv, info, perm);
+                };
+                if (sb == 8)
+                    go(k_panel_lds<T, 8>);
+                else if (sb == 4)
+                    go(k_panel_lds<T, 4>);
+                else
+                    go(k_panel_lds<T, 2>);
+                if (n - jb > 0) {
+                    dim3 g((n - jb + 3) / 4, batch);
+                    hipLaunchKernelGGL(k_laswp_perm<T>, g, dim3(256), 0, s, n, k, jb, A, lda,
+                                       strideA, (const int *)perm);
+                }
+            } else {
+                hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(512), 0, s, n, k, jb, A, lda,
+                                   strideA, ipiv, info);
+                if (n - jb > 0) {
+                    dim3 g((n - jb + 255) / 256, batch);
+                    hipLaunchKernelGGL(k_laswp<T>, g, dim3(256), 0, s, n, k, jb, A, lda, strideA,
+                                       ipiv);
+                }
             }
             const int rin = pend - k - jb;
             if (rin > 0) {
@@ -322,6 +541,7 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
                            A + (size_t)pend * lda + pend, lda, strideA, batch, s);
         }
     }
+    if (perm) IA_HIP(hipFreeAsync(perm, s));
     IA_HIP(hipGetLastError());
     return IA_OK;
 }
