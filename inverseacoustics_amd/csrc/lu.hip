@@ -16,6 +16,8 @@
 // blocked triangular solves (diagonal block per column thread, GEMM update).
 #include "ia_internal.hpp"
 
+#include <cstdlib>
+
 namespace ia {
 
 namespace {
@@ -410,16 +412,43 @@ __global__ void k_copy_cols(int n, int nrhs, const cx<T> *src, int lds, size_t s
 // diagonal solves and rank-NB updates confined to the block, then the rest of
 // B gets one rank-NBO GEMM update per block (four times fewer passes over B
 // than rank-NB updates).
+// Left-looking outer blocks (IA_TRSM_RIGHT unset): block O first takes the
+// update from every solved row at once, B_O -= A[O, done] X_done, one
+// product with K = |done| that reads each block of B once, instead of a
+// rank-NBO update of all remaining rows after every block.
+inline bool trsm_left_looking() {
+    const char *e = std::getenv("IA_TRSM_RIGHT");
+    return !(e && *e && *e != '0');
+}
+
 template <typename T, int MODE>
 void trsm_blocked(int n, int nrhs, const cx<T> *A, int lda, size_t sA, cx<T> *B, int ldb,
                   size_t sB, int batch, hipStream_t s) {
     const cx<T> mone{-1, 0}, one{1, 0};
     dim3 dgrid((nrhs + 255) / 256, batch);
     const bool forward = (MODE == 0 || MODE == 1);
+    const bool left = trsm_left_looking();
     const int nob = (n + NBO - 1) / NBO;
     for (int ob = 0; ob < nob; ++ob) {
         const int blkO = forward ? ob : (nob - 1 - ob);
         const int o0 = blkO * NBO, ojb = min(NBO, n - o0), o1 = o0 + ojb;
+        if (left) {
+            if (forward && o0 > 0) {
+                if (MODE == 0)  // B_O -= L[O, :o0] X[:o0]
+                    launch_gemm<T>('N', 'N', ojb, nrhs, o0, mone, A + o0, lda, sA, B, ldb, sB,
+                                   one, B + o0, ldb, sB, batch, s);
+                else  // B_O -= (U[:o0, O])^H X[:o0]
+                    launch_gemm<T>('C', 'N', ojb, nrhs, o0, mone, A + (size_t)o0 * lda, lda, sA,
+                                   B, ldb, sB, one, B + o0, ldb, sB, batch, s);
+            } else if (!forward && o1 < n) {
+                if (MODE == 2)  // B_O -= U[O, o1:] X[o1:]
+                    launch_gemm<T>('N', 'N', ojb, nrhs, n - o1, mone, A + (size_t)o1 * lda + o0,
+                                   lda, sA, B + o1, ldb, sB, one, B + o0, ldb, sB, batch, s);
+                else  // B_O -= (L[o1:, O])^H X[o1:]
+                    launch_gemm<T>('C', 'N', ojb, nrhs, n - o1, mone, A + (size_t)o0 * lda + o1,
+                                   lda, sA, B + o1, ldb, sB, one, B + o0, ldb, sB, batch, s);
+            }
+        }
         const int nblk = (ojb + NB - 1) / NB;
         for (int bi = 0; bi < nblk; ++bi) {
             const int blk = forward ? bi : (nblk - 1 - bi);
@@ -450,6 +479,7 @@ void trsm_blocked(int n, int nrhs, const cx<T> *A, int lda, size_t sA, cx<T> *B,
                 }
             }
         }
+        if (left) continue;
         if (forward) {
             const int rest = n - o1;
             if (rest > 0) {
