@@ -128,7 +128,7 @@ __global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__r
     const double sigma = sall[(size_t)b * n + i];
     double *DD = work;                       // U diagonal
     double *DU = work + (size_t)N * G;       // U first superdiagonal
-    double *DU2 = work + (size_t)2 * N * G;  // U second superdiagonal
+    double *DU2 = work + (size_t)2 * N * G;  // U second superdiagonal (pivoted rows only)
     double *DL = work + (size_t)3 * N * G;   // multipliers
     double *X = work + (size_t)4 * N * G;    // iterate
     unsigned char *PV = reinterpret_cast<unsigned char *>(work + (size_t)5 * N * G);
@@ -138,84 +138,91 @@ __global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__r
     for (int k = 0; k < N - 1; ++k) tnorm = fmax(tnorm, fabs(toff(k)));
     const double eps = 2.220446049250313e-16;
     const double tiny = fmax(eps * tnorm, 1e-300);
-    // ---- factor (LAPACK gttrf)
-    for (int k = 0; k < N; ++k) {
-        AT(DD, k) = -sigma;
-        if (k < N - 1) {
-            AT(DU, k) = toff(k);
-            AT(DL, k) = toff(k);
-        }
-        if (k < N - 2) AT(DU2, k) = 0.0;
-    }
-    for (int k = 0; k < N - 1; ++k) {
-        double dk = AT(DD, k), dlk = AT(DL, k);
-        if (fabs(dk) >= fabs(dlk)) {
-            PV[(size_t)k * G + g] = 0;
-            double fact = (dk != 0.0) ? dlk / dk : 0.0;
-            AT(DL, k) = fact;
-            AT(DD, k + 1) = AT(DD, k + 1) - fact * AT(DU, k);
-        } else {
-            PV[(size_t)k * G + g] = 1;
-            double fact = dk / dlk;
-            AT(DD, k) = dlk;
-            AT(DL, k) = fact;
-            double temp = AT(DU, k);
-            double dk1 = AT(DD, k + 1);
-            AT(DU, k) = dk1;
-            AT(DD, k + 1) = temp - fact * dk1;
-            if (k < N - 2) {
-                double du1 = AT(DU, k + 1);
-                AT(DU2, k) = du1;
-                AT(DU, k + 1) = -fact * du1;
-            }
-        }
-    }
-    for (int k = 0; k < N; ++k)
-        if (fabs(AT(DD, k)) < tiny) AT(DD, k) = (AT(DD, k) < 0) ? -tiny : tiny;
-    // ---- start vector
-    for (int k = 0; k < N; ++k) AT(X, k) = hash01(((uint64_t)b << 40) ^ ((uint64_t)i << 20) ^ k) + 0.25;
-    for (int itn = 0; itn < 3; ++itn) {
-        // forward: L
+    auto clampd = [&](double v) { return fabs(v) < tiny ? (v < 0 ? -tiny : tiny) : v; };
+    auto start = [&](int k) {
+        return hash01(((uint64_t)b << 40) ^ ((uint64_t)i << 20) ^ k) + 0.25;
+    };
+    // ---- one pass: LU of T - sigma I with partial pivoting (LAPACK gttrf),
+    // the random start vector and the first solve with L
+    {
+        double dd = -sigma, du = toff(0), x = start(0);
         for (int k = 0; k < N - 1; ++k) {
-            double xk = AT(X, k), xk1 = AT(X, k + 1);
-            if (PV[(size_t)k * G + g] == 0) {
-                AT(X, k + 1) = xk1 - AT(DL, k) * xk;
+            const double dlk = toff(k), ddn = -sigma, dun = k + 1 < N - 1 ? toff(k + 1) : 0.0;
+            const double xn = start(k + 1);
+            if (fabs(dd) >= fabs(dlk)) {
+                const double fact = dd != 0.0 ? dlk / dd : 0.0;
+                AT(PV, k) = 0;
+                AT(DL, k) = fact;
+                AT(DD, k) = clampd(dd);
+                AT(DU, k) = du;
+                AT(X, k) = x;
+                x = xn - fact * x;
+                dd = ddn - fact * du;
+                du = dun;
             } else {
-                AT(X, k) = xk1;
-                AT(X, k + 1) = xk - AT(DL, k) * xk1;
+                const double fact = dd / dlk;
+                AT(PV, k) = 1;
+                AT(DL, k) = fact;
+                AT(DD, k) = clampd(dlk);
+                AT(DU, k) = ddn;
+                if (k < N - 2) AT(DU2, k) = dun;
+                AT(X, k) = xn;
+                x = x - fact * xn;
+                dd = du - fact * ddn;
+                du = -fact * dun;
             }
         }
-        // backward: U
-        double scale = 0;
-        double x2 = 0, x1 = 0;  // x[k+2], x[k+1]
-        for (int k = N - 1; k >= 0; --k) {
-            double s = AT(X, k);
-            if (k < N - 1) s -= AT(DU, k) * x1;
-            if (k < N - 2) s -= AT(DU2, k) * x2;
-            s /= AT(DD, k);
-            AT(X, k) = s;
-            x2 = x1;
-            x1 = s;
-            scale = fmax(scale, fabs(s));
-        }
-        double inv = (scale > 0) ? 1.0 / scale : 1.0;
-        for (int k = 0; k < N; ++k) AT(X, k) *= inv;
+        AT(DD, N - 1) = clampd(dd);
+        AT(X, N - 1) = x;
     }
+    // backward solve with U; returns 1 / max |x|
+    auto backward = [&]() {
+        double scale = 0, x2 = 0, x1 = 0;  // x[k+2], x[k+1]
+        for (int k = N - 1; k >= 0; --k) {
+            double sv = AT(X, k);
+            if (k < N - 1) sv -= AT(DU, k) * x1;
+            if (k < N - 2 && AT(PV, k)) sv -= AT(DU2, k) * x2;
+            sv /= AT(DD, k);
+            AT(X, k) = sv;
+            x2 = x1;
+            x1 = sv;
+            scale = fmax(scale, fabs(sv));
+        }
+        return scale > 0 ? 1.0 / scale : 1.0;
+    };
+    // forward solve with L of the iterate scaled by inv
+    auto forward = [&](double inv) {
+        double x = AT(X, 0) * inv;
+        for (int k = 0; k < N - 1; ++k) {
+            const double xn = AT(X, k + 1) * inv, fk = AT(DL, k);
+            if (AT(PV, k) == 0) {
+                AT(X, k) = x;
+                x = xn - fk * x;
+            } else {
+                AT(X, k) = xn;
+                x = x - fk * xn;
+            }
+        }
+        AT(X, N - 1) = x;
+    };
+    double inv = backward();
+    forward(inv);
+    inv = backward();
+    forward(inv);
+    inv = backward();
     // ---- split, normalise, fix the sign of p so that p^T B q >= 0
-    double nq = 0, np = 0;
+    double nq = 0, np = 0, sgn = 0;
     for (int k = 0; k < n; ++k) {
-        double q = AT(X, 2 * k), p = AT(X, 2 * k + 1);
+        const double q = AT(X, 2 * k) * inv, p = AT(X, 2 * k + 1) * inv;
+        const double qn = k < n - 1 ? AT(X, 2 * k + 2) * inv : 0.0;
         nq += q * q;
         np += p * p;
+        sgn += p * (d[k] * q + (k < n - 1 ? e[k] * qn : 0.0));
     }
     nq = sqrt(nq);
     np = sqrt(np);
-    double iq = nq > 0 ? 1.0 / nq : 0.0, ip = np > 0 ? 1.0 / np : 0.0;
-    double sgn = 0;
-    for (int k = 0; k < n; ++k) {
-        double bq = d[k] * AT(X, 2 * k) + (k < n - 1 ? e[k] * AT(X, 2 * k + 2) : 0.0);
-        sgn += AT(X, 2 * k + 1) * bq;
-    }
+    const double iq = nq > 0 ? inv / nq : 0.0;
+    double ip = np > 0 ? inv / np : 0.0;
     if (sgn < 0) ip = -ip;
     double *Qb = QBt + (size_t)b * n * n;
     double *Pb = PBt + (size_t)b * n * n;
