@@ -30,14 +30,23 @@ using hh::wave_sum;
 
 // ------------------------------------------------------------ bisection
 // #eigenvalues of the zero-diagonal symmetric tridiagonal with squared
-// off-diagonals t2[0..N-2] that are < x.
+// off-diagonals t2[0..N-2] that are < x.  t2[i] / q is formed from the
+// hardware reciprocal refined by two Newton steps (|q| >= pivmin > 0), a
+// relative error of a few ulps that the count tolerates like rounding.
+__device__ __forceinline__ double recip(double q) {
+    double r = __builtin_amdgcn_rcp(q);
+    r = __builtin_fma(r, __builtin_fma(-q, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-q, r, 1.0), r);
+    return r;
+}
+
 __device__ __forceinline__ int sturm_count(int N, const double *t2, double x, double pivmin) {
     int cnt = 0;
     double q = -x;
     if (q < 0) cnt++;
     for (int i = 0; i < N - 1; ++i) {
         if (fabs(q) < pivmin) q = -pivmin;
-        q = -x - t2[i] / q;
+        q = __builtin_fma(-t2[i], recip(q), -x);
         if (q < 0) cnt++;
     }
     return cnt;
