@@ -132,14 +132,15 @@ def main():
     db_err, idx_match, vel_rel = errors(o_mixed)
     db_c, idx_c, vel_c = errors(o_cert)
 
-    # ---- roofline of the dominant kernel: the panel bidiagonalisation
-    # k_labrd (HBM-bound) of the l x l triangular factor of H_e^H = Q R.  Per
-    # step j it reads the trailing block twice (rows >= j and > j, columns
-    # > j), so a launch over panel columns j0..j0+31 moves
-    # sum_j (2(l-j)-1)(l-1-j) * sizeof(c) bytes per frequency, times the
-    # frequencies of the launch.  The library
-    # brackets every launch with HIP events on the sweep's stream and
-    # reports the summed duration, bytes and launch count.
+    # ---- roofline of the chase kernel k_bulge (HBM-bound): the second stage
+    # of the two-stage bidiagonalisation, our largest single launch of the
+    # sweep (the rocBLAS GEMMs of the factorisations and the band reduction
+    # take more time in aggregate, spread over ~600 calls of many shapes).
+    # Algorithmic bytes per launch: per task of the chase the overlap block
+    # and the block right of it are read and the overlap block and the rows
+    # above it written (bulge_alg_bytes), over all frequencies of the launch.
+    # The library brackets the launch with HIP events on the sweep's stream
+    # and reports its duration, bytes and launch count.
     def roof(st, form, esz):
         n = max(1.0, st["bidiag_kernel_launches"])
         ms = st["bidiag_kernel_ms"] / n
@@ -147,7 +148,7 @@ def main():
         ach = by / (ms / 1e3) / 1e9 if ms > 0 else 0.0
         key = (form, F, m, l, a.chunk)
         tr = HBM_TRAFFIC_PER_LAUNCH.get(key)
-        return {"bound": "hbm", "kernel": f"k_labrd<{'float' if esz == 8 else 'double'}> ({form} form)",
+        return {"bound": "hbm", "kernel": f"k_bulge<{'float' if esz == 8 else 'double'}> ({form} form)",
                 "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
                 "frac": round(ach / 8000.0, 4), "traffic": tr,
                 "algorithmic_bytes_per_launch": round(by), "launch_ms_avg": round(ms, 4),
@@ -197,12 +198,12 @@ def main():
     print(json.dumps(res))
 
 
-# HBM bytes per k_labrd launch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; FETCH scaled by
-# 1.99, the WRITE/FETCH ratio of a plain 1 GiB device copy in the same runs) keyed by
-# (form, frequencies, m, l, chunk): profiles/labrd_esm_sweep_256f_1024x2048_hbm_counters.txt
+# HBM bytes per k_bulge launch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; FETCH scaled by
+# 2.0, the WRITE/FETCH ratio of a plain 1 GiB device copy in the same runs) keyed by
+# (form, frequencies, m, l, chunk): profiles/bulge_esm_sweep_256f_1024x2048_hbm_counters.txt
 HBM_TRAFFIC_PER_LAUNCH = {
-    ("fp64", 256, 2048, 1024, 0): 103207968929,
-    ("mixed", 256, 2048, 1024, 0): 52029598145,
+    ("fp64", 256, 2048, 1024, 0): 297710959119,
+    ("mixed", 256, 2048, 1024, 0): 169788622616,
 }
 
 

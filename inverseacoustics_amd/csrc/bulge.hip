@@ -463,7 +463,7 @@ size_t bulge_refl_elems(int C) { return C > 1 ? (size_t)(C - 1) * ntask_of(C, 0)
 template <typename T>
 ia_status launch_bulge(int C, const cx<T> *Z, int ldz, size_t sZ, cx<T> *Bs, cx<T> *uR,
                        cx<T> *uL, cx<T> *tR, cx<T> *tL, double *d, double *e, int batch,
-                       hipStream_t s) {
+                       hipStream_t s, KernelClock *clk) {
     if (C < 2) return fail(IA_ERR_ARG, "bulge chasing needs C >= 2");
     const size_t nb = (size_t)C * BLD;
     hipLaunchKernelGGL(k_band_extract<T>, dim3((unsigned)((nb + 255) / 256), batch), dim3(256), 0,
@@ -472,8 +472,14 @@ ia_status launch_bulge(int C, const cx<T> *Z, int ldz, size_t sZ, cx<T> *Bs, cx<
     // eight waves; complex64 also holds the right-hand block in registers
     // from the start of a task (two rows of 32 values per lane)
     constexpr bool pre = sizeof(T) == sizeof(float);
+    if (clk) clk->mark(s);
     hipLaunchKernelGGL((k_bulge<T, 8, pre>), dim3(batch), dim3(512), 0, s, C, Bs, uR, uL, tR, tL,
                        tmax, d, e);
+    if (clk) {
+        clk->mark(s);
+        clk->launches += 1;
+        clk->bytes += bulge_alg_bytes(C, sizeof(cx<T>)) * batch;
+    }
     IA_HIP(hipGetLastError());
     return IA_OK;
 }
@@ -481,16 +487,27 @@ ia_status launch_bulge(int C, const cx<T> *Z, int ldz, size_t sZ, cx<T> *Bs, cx<
 size_t bulge_apply_lds(int C) { return sizeof(cx<double>) * (size_t)C; }
 
 double bulge_alg_bytes(int C, size_t elem) {
+    // per task: the overlap block and the block right of it come from
+    // memory; the rows above the overlap come from memory only for a
+    // sweep's first task (later ones receive them in LDS); the rows above and
+    // the overlap go back to memory, the right-hand block only after a
+    // sweep's last task; plus the two reflectors
     double n = 0;
-    for (int sw = 0; sw <= C - 2; ++sw)
-        for (int t = 0; t < ntask_of(C, sw); ++t) {
+    for (int sw = 0; sw <= C - 2; ++sw) {
+        const int nts = ntask_of(C, sw);
+        for (int t = 0; t < nts; ++t) {
             const int r = t == 0 ? sw : sw + 1 + (t - 1) * BNB;
             const int c = t == 0 ? sw + 1 : r + BNB;
             const int L = std::min(BNB, C - c);
-            const int nc = std::min(C, c + L + BNB) - c;
-            n += (double)(c + L - r) * L + (double)L * nc;
+            const int nR = std::min(C, c + L + BNB) - (c + L);
+            const int nt = c - r;
+            n += 2.0 * L * L + (double)L * nR + (double)nt * L;   // OV in/out, RH in, TB out
+            if (t == 0) n += (double)nt * L;                      // TB in
+            if (t + 1 == nts) n += (double)L * nR;                // RH out
+            n += 2.0 * L;                                         // reflectors
         }
-    return 2.0 * n * elem;
+    }
+    return n * elem;
 }
 
 template <typename T>
@@ -513,7 +530,7 @@ void launch_bulge_apply(bool fwd, int C, const cx<T> *u, const cx<T> *tau, cx<do
 #define INST(T)                                                                               \
     template ia_status launch_bulge<T>(int, const cx<T> *, int, size_t, cx<T> *, cx<T> *,      \
                                        cx<T> *, cx<T> *, cx<T> *, double *, double *, int,     \
-                                       hipStream_t);                                          \
+                                       hipStream_t, KernelClock *);                           \
     template void launch_bulge_apply<T>(bool, int, const cx<T> *, const cx<T> *, cx<double> *, \
                                         size_t, int, int, hipStream_t);
 INST(double)

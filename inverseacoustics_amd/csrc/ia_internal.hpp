@@ -186,10 +186,13 @@ size_t bulge_refl_elems(int C);
 size_t bulge_apply_lds(int C);
 // bytes the chase's tasks read and write (each task's two blocks once)
 double bulge_alg_bytes(int C, size_t elem);
+struct KernelClock;
+// clk (optional) brackets the chase kernel with HIP events and adds its
+// bulge_alg_bytes
 template <typename T>
 ia_status launch_bulge(int C, const cx<T> *Z, int ldz, size_t sZ, cx<T> *Bs, cx<T> *uR,
                        cx<T> *uL, cx<T> *tR, cx<T> *tL, double *d, double *e, int batch,
-                       hipStream_t s);
+                       hipStream_t s, KernelClock *clk = nullptr);
 // v <- H v over one side's stage-two reflectors: fwd = true applies
 // P2^H (uR) or Q2^H (uL), fwd = false applies P2 or Q2
 template <typename T>
@@ -423,17 +426,11 @@ inline ia_status svd_reduce(SvdWork<T> &w, cx<T> *Y, hipStream_t s, KernelClock 
         IA_TRY(launch_gebrd_band<T>(Rz, C, Z, Rz, (size_t)Rz * C, w.tauL.template as<cx<T>>(),
                                     w.Pm.template as<cx<T>>(), w.tauP.template as<cx<T>>(),
                                     w.gw.template as<cx<T>>(), batch, s));
-        if (clk) clk->mark(s);
-        IA_TRY(launch_bulge<T>(C, Z, Rz, (size_t)Rz * C, w.Bs.template as<cx<T>>(),
+        return launch_bulge<T>(C, Z, Rz, (size_t)Rz * C, w.Bs.template as<cx<T>>(),
                                w.uR2.template as<cx<T>>(), w.uL2.template as<cx<T>>(),
                                w.tR2.template as<cx<T>>(), w.tL2.template as<cx<T>>(),
-                               w.d.template as<double>(), w.e.template as<double>(), batch, s));
-        if (clk) {
-            clk->mark(s);
-            clk->launches += 1;
-            clk->bytes += bulge_alg_bytes(C, sizeof(cx<T>)) * batch;
-        }
-        return IA_OK;
+                               w.d.template as<double>(), w.e.template as<double>(), batch, s,
+                               clk);
     }
     return launch_gebrd<T>(Rz, C, Z, Rz, (size_t)Rz * C, w.d.template as<double>(),
                            w.e.template as<double>(), w.tauL.template as<cx<T>>(),
