@@ -120,7 +120,6 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
     __shared__ T red_v[LNW];
     __shared__ int red_i[LNW];
     __shared__ cx<T> pinv_s;
-    __shared__ int piv_s;
     __shared__ int prow[PERM_MAX], psrc[PERM_MAX];
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     cx<T> *Ab = A + (size_t)b * strideA;
@@ -166,7 +165,6 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
                         bv = red_v[q];
                         bx = red_i[q];
                     }
-                piv_s = bx;
                 ip[cs + i] = k + bx;
                 if (bv == (T)0 && info && info[b] == 0) info[b] = cs + i + 1;
                 // swap rows rc and bx inside the sub-panel (LDS)
@@ -315,56 +313,144 @@ __global__ void k_laswp(int n, int k, int jb, cx<T> *A, int lda, size_t strideA,
 //   MODE 2: non-unit upper U                           backward
 //   MODE 3: L^H, L unit lower                          backward
 // The block is rows/cols [i0, i0+jb) of A; B rows [i0, i0+jb).
-template <typename T, int MODE>
-__global__ __launch_bounds__(256) void k_trsm_diag(int n, int i0, int jb, const cx<T> *A, int lda,
-                                                  size_t strideA, cx<T> *B, int ldb,
-                                                  size_t strideB, int nrhs) {
-    __shared__ cx<T> Ts[NB][NB + 1];  // Ts[row][col] of the operator T
-    const int b = blockIdx.y;
+// The right-hand-side tile is staged through
+// LDS: the block's rows of NC columns are loaded and stored with coalesced
+// accesses (each column's jb rows are contiguous), every thread solves its
+// column from LDS, and the non-unit diagonal is inverted once per block.
+template <typename T, int MODE, int NC>
+__global__ __launch_bounds__(NC) void k_trsm_diag_lds(int n, int i0, int jb, const cx<T> *A,
+                                                    int lda, size_t strideA, cx<T> *B, int ldb,
+                                                    size_t strideB, int nrhs) {
+    __shared__ cx<T> Ts[NB][NB + 1];
+    __shared__ cx<T> dinv[NB];
+    __shared__ cx<T> tile[NB][NC + 1];
+    const int b = blockIdx.y, t = threadIdx.x;
+    const int col0 = blockIdx.x * NC;
     const cx<T> *Ab = A + (size_t)b * strideA;
-    for (int e = threadIdx.x; e < jb * jb; e += blockDim.x) {
-        int r = e % jb, c = e / jb;
-        cx<T> v = Ab[(size_t)(i0 + c) * lda + (i0 + r)];  // A[r][c]
-        if (MODE == 0 || MODE == 2) {
+    for (int e = t; e < jb * jb; e += NC) {
+        const int r = e % jb, c = e / jb;
+        const cx<T> v = Ab[(size_t)(i0 + c) * lda + (i0 + r)];  // A[r][c]
+        if (MODE == 0 || MODE == 2)
             Ts[r][c] = v;
-        } else {  // conjugate transpose: T[c][r] = conj(A[r][c])
+        else
             Ts[c][r] = conj(v);
-        }
+    }
+    cx<T> *Bb = B + (size_t)b * strideB + i0;
+    for (int e = t; e < NB * NC; e += NC) {
+        const int i = e % NB, c = e / NB;
+        if (i < jb && col0 + c < nrhs) tile[i][c] = Bb[(size_t)(col0 + c) * ldb + i];
     }
     __syncthreads();
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= nrhs) return;
-    cx<T> *Bc = B + (size_t)b * strideB + (size_t)col * ldb + i0;
+    if ((MODE == 1 || MODE == 2) && t < jb) dinv[t] = cdiv(cx<T>{1, 0}, Ts[t][t]);
+    __syncthreads();
     cx<T> x[NB];
 #pragma unroll
-    for (int i = 0; i < NB; ++i)
-        if (i < jb) x[i] = Bc[i];
+    for (int i = 0; i < NB; ++i) x[i] = i < jb ? tile[i][t] : cx<T>{0, 0};
     if (MODE == 0 || MODE == 1) {
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             if (i < jb) {
-                cx<T> s = x[i];
+                cx<T> sacc = x[i];
 #pragma unroll
                 for (int j = 0; j < NB; ++j)
-                    if (j < i) s = s - Ts[i][j] * x[j];
-                x[i] = (MODE == 0) ? s : cdiv(s, Ts[i][i]);
+                    if (j < i) sacc = sacc - Ts[i][j] * x[j];
+                x[i] = (MODE == 0) ? sacc : sacc * dinv[i];
             }
         }
     } else {
 #pragma unroll
         for (int i = NB - 1; i >= 0; --i) {
             if (i < jb) {
-                cx<T> s = x[i];
+                cx<T> sacc = x[i];
 #pragma unroll
                 for (int j = 0; j < NB; ++j)
-                    if (j > i && j < jb) s = s - Ts[i][j] * x[j];
-                x[i] = (MODE == 3) ? s : cdiv(s, Ts[i][i]);
+                    if (j > i && j < jb) sacc = sacc - Ts[i][j] * x[j];
+                x[i] = (MODE == 3) ? sacc : sacc * dinv[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+        if (i < jb) tile[i][t] = x[i];
+    __syncthreads();
+    for (int e = t; e < NB * NC; e += NC) {
+        const int i = e % NB, c = e / NB;
+        if (i < jb && col0 + c < nrhs) Bb[(size_t)(col0 + c) * ldb + i] = tile[i][c];
+    }
+}
+
+// Direct form (complex128): one thread per column reading its jb rows
+// straight from memory (all loads issued before the solve), for the
+// occupancy that the LDS tile would cost at 16 bytes per element.
+template <typename T, int MODE>
+__global__ __launch_bounds__(256) void k_trsm_diag_direct(int n, int i0, int jb, const cx<T> *A,
+                                                         int lda, size_t strideA, cx<T> *B,
+                                                         int ldb, size_t strideB, int nrhs) {
+    __shared__ cx<T> Ts[NB][NB + 1];
+    __shared__ cx<T> dinv[NB];
+    const int b = blockIdx.y, t = threadIdx.x;
+    const cx<T> *Ab = A + (size_t)b * strideA;
+    for (int e = t; e < jb * jb; e += blockDim.x) {
+        const int r = e % jb, c = e / jb;
+        const cx<T> v = Ab[(size_t)(i0 + c) * lda + (i0 + r)];
+        if (MODE == 0 || MODE == 2)
+            Ts[r][c] = v;
+        else
+            Ts[c][r] = conj(v);
+    }
+    __syncthreads();
+    if ((MODE == 1 || MODE == 2) && t < jb) dinv[t] = cdiv(cx<T>{1, 0}, Ts[t][t]);
+    __syncthreads();
+    const int col = blockIdx.x * blockDim.x + t;
+    if (col >= nrhs) return;
+    cx<T> *Bc = B + (size_t)b * strideB + (size_t)col * ldb + i0;
+    cx<T> x[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) x[i] = Bc[i < jb ? i : 0];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+        if (i >= jb) x[i] = cx<T>{0, 0};
+    if (MODE == 0 || MODE == 1) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            if (i < jb) {
+                cx<T> sacc = x[i];
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+                    if (j < i) sacc = sacc - Ts[i][j] * x[j];
+                x[i] = (MODE == 0) ? sacc : sacc * dinv[i];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = NB - 1; i >= 0; --i) {
+            if (i < jb) {
+                cx<T> sacc = x[i];
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+                    if (j > i && j < jb) sacc = sacc - Ts[i][j] * x[j];
+                x[i] = (MODE == 3) ? sacc : sacc * dinv[i];
             }
         }
     }
 #pragma unroll
     for (int i = 0; i < NB; ++i)
         if (i < jb) Bc[i] = x[i];
+}
+
+template <typename T, int MODE>
+void launch_trsm_diag(int n, int i0, int jb, const cx<T> *A, int lda, size_t sA, cx<T> *B,
+                      int ldb, size_t sB, int nrhs, int batch, hipStream_t s) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+        constexpr int NC = 256;
+        dim3 grid((nrhs + NC - 1) / NC, batch);
+        hipLaunchKernelGGL((k_trsm_diag_lds<T, MODE, NC>), grid, dim3(NC), 0, s, n, i0, jb, A,
+                           lda, sA, B, ldb, sB, nrhs);
+    } else {
+        dim3 grid((nrhs + 255) / 256, batch);
+        hipLaunchKernelGGL((k_trsm_diag_direct<T, MODE>), grid, dim3(256), 0, s, n, i0, jb, A,
+                           lda, sA, B, ldb, sB, nrhs);
+    }
 }
 
 // permutation index from interchanges: perm applied as (P B)[r] = B[idx[r]]
@@ -425,7 +511,6 @@ template <typename T, int MODE>
 void trsm_blocked(int n, int nrhs, const cx<T> *A, int lda, size_t sA, cx<T> *B, int ldb,
                   size_t sB, int batch, hipStream_t s) {
     const cx<T> mone{-1, 0}, one{1, 0};
-    dim3 dgrid((nrhs + 255) / 256, batch);
     const bool forward = (MODE == 0 || MODE == 1);
     const bool left = trsm_left_looking();
     const int nob = (n + NBO - 1) / NBO;
@@ -453,8 +538,7 @@ void trsm_blocked(int n, int nrhs, const cx<T> *A, int lda, size_t sA, cx<T> *B,
         for (int bi = 0; bi < nblk; ++bi) {
             const int blk = forward ? bi : (nblk - 1 - bi);
             const int i0 = o0 + blk * NB, jb = min(NB, o1 - i0);
-            hipLaunchKernelGGL((k_trsm_diag<T, MODE>), dgrid, dim3(256), 0, s, n, i0, jb, A, lda,
-                               sA, B, ldb, sB, nrhs);
+            launch_trsm_diag<T, MODE>(n, i0, jb, A, lda, sA, B, ldb, sB, nrhs, batch, s);
             if (forward) {
                 const int rest = o1 - i0 - jb;
                 if (rest > 0) {
@@ -553,10 +637,8 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
             }
             const int rin = pend - k - jb;
             if (rin > 0) {
-                dim3 dgrid((rin + 255) / 256, batch);
-                hipLaunchKernelGGL((k_trsm_diag<T, 0>), dgrid, dim3(256), 0, s, n, k, jb,
-                                   (const cx<T> *)A + 0, lda, strideA,
-                                   A + (size_t)(k + jb) * lda, lda, strideA, rin);
+                launch_trsm_diag<T, 0>(n, k, jb, (const cx<T> *)A, lda, strideA,
+                                       A + (size_t)(k + jb) * lda, lda, strideA, rin, batch, s);
                 launch_gemm<T>('N', 'N', n - k - jb, rin, jb, mone, A + (size_t)k * lda + k + jb,
                                lda, strideA, A + (size_t)(k + jb) * lda + k, lda, strideA, one,
                                A + (size_t)(k + jb) * lda + k + jb, lda, strideA, batch, s);
