@@ -191,16 +191,33 @@ void dispatch_beta(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, si
 
 namespace {
 
-// rocBLAS handle of the current device for this host thread (created once)
-rocblas_handle blas_handle() {
-    static thread_local rocblas_handle h[16] = {};
+// rocBLAS handle of the current device and stream for this host thread
+// (created once per pair; streams that run concurrently never share a
+// handle's workspace)
+rocblas_handle blas_handle(hipStream_t s) {
+    struct Slot {
+        int dev;
+        hipStream_t s;
+        rocblas_handle h;
+    };
+    static thread_local Slot slots[16];
+    static thread_local int used = 0;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) {
+    if (hipGetDevice(&dev) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
     }
-    if (!h[dev] && rocblas_create_handle(&h[dev]) != rocblas_status_success) h[dev] = nullptr;
-    return h[dev];
+    for (int i = 0; i < used; ++i)
+        if (slots[i].dev == dev && slots[i].s == s) return slots[i].h;
+    if (used == 16) return nullptr;  // the caller falls back to the own kernel
+    rocblas_handle h = nullptr;
+    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+    if (rocblas_set_stream(h, s) != rocblas_status_success) {
+        (void)rocblas_destroy_handle(h);
+        return nullptr;
+    }
+    slots[used++] = Slot{dev, s, h};
+    return h;
 }
 
 rocblas_operation blas_op(char op) {
@@ -216,8 +233,8 @@ bool blas_gemm(char opA, char opB, int M, int N, int K, cx<T> alpha, const cx<T>
                size_t sA, const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc,
                size_t sC, int batch, hipStream_t s) {
     if ((double)M * N * K < 32768.0 || getenv("IA_GEMM_OWN")) return false;
-    rocblas_handle h = blas_handle();
-    if (!h || rocblas_set_stream(h, s) != rocblas_status_success) return false;
+    rocblas_handle h = blas_handle(s);
+    if (!h) return false;
     rocblas_status st;
     if (sizeof(T) == sizeof(double))
         st = rocblas_zgemm_strided_batched(
