@@ -6,7 +6,7 @@
 // that traffic by ~(R/C)(3 - C/R)/2, while the QR itself is GEMM work on
 // the matrix cores.
 //
-// Per panel of NB columns (LAPACK geqrf/larft scheme):
+// Per panel of NBQ = 64 columns (LAPACK geqrf/larft scheme):
 //   k_qr_panel   one workgroup per matrix: unblocked Householder QR of the
 //                panel (H_i = I - tau_i v_i v_i^H, LAPACK zlarfg), explicit
 //                copy of the reflectors V (unit diagonal, zeros above);
@@ -27,18 +27,19 @@ using hh::block_sum;
 using hh::larfg;
 using hh::wave_sum;
 
-constexpr int NB = 32;
+constexpr int NB = 32;   // band-reduction panels (the band width)
+constexpr int NBQ = 64;  // QR panels
 
 template <typename T>
 __global__ __launch_bounds__(BT) void k_qr_panel(int R, int C, int j0, int nbp,
                                                 cx<T> *__restrict__ Yall, int ldy, size_t sY,
                                                 cx<T> *__restrict__ tauall,
-                                                cx<T> *__restrict__ Vwall) {
+                                                cx<T> *__restrict__ Vwall, int vcols) {
     __shared__ double red[NW];
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     cx<T> *A = Yall + (size_t)b * sY;
     cx<T> *tau = tauall + (size_t)b * C;
-    cx<T> *Vw = Vwall + (size_t)b * R * NB;
+    cx<T> *Vw = Vwall + (size_t)b * R * vcols;
     for (int i = 0; i < nbp; ++i) {
         const int j = j0 + i;
         cx<T> *Aj = A + (size_t)j * ldy;
@@ -94,7 +95,7 @@ template <typename T, int SB>
 __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int nbp,
                                                     cx<T> *__restrict__ Yall, int ldy, size_t sY,
                                                     cx<T> *__restrict__ tauall,
-                                                    cx<T> *__restrict__ Vwall) {
+                                                    cx<T> *__restrict__ Vwall, int vcols) {
     extern __shared__ unsigned char smem_raw[];
     cx<T> *Vs = reinterpret_cast<cx<T> *>(smem_raw);  // SB columns of M rows
     __shared__ double red[NW];
@@ -104,7 +105,7 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     cx<T> *A = Yall + (size_t)b * sY;
     cx<T> *tau = tauall + (size_t)b * C;
-    cx<T> *Vw = Vwall + (size_t)b * R * NB;
+    cx<T> *Vw = Vwall + (size_t)b * R * vcols;
     const int M = R - j0;
     for (int cs = j0; cs < j0 + nbp; cs += SB) {
         const int w = min(SB, j0 + nbp - cs);
@@ -240,16 +241,18 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
 
 // Panel QR: the LDS-blocked kernel when SB columns of the panel fit in
 // 128 KB of LDS (SB = 8, 4 or 2), else the column-by-column kernel.
+// vcols: columns per matrix of the explicit-reflector buffer Vw.
 template <typename T>
 void launch_qr_panel(int R, int C, int j0, int nbp, cx<T> *Y, int ldy, size_t sY, cx<T> *tau,
-                     cx<T> *Vw, int batch, hipStream_t s) {
+                     cx<T> *Vw, int vcols, int batch, hipStream_t s) {
     const size_t colb = sizeof(cx<T>) * (size_t)(R - j0);
     const size_t cap = 128 * 1024;
     auto go = [&](auto kern, int sb) {
         const size_t lds = colb * sb;
         (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)lds);
-        hipLaunchKernelGGL(kern, dim3(batch), dim3(BT), lds, s, R, C, j0, nbp, Y, ldy, sY, tau, Vw);
+        hipLaunchKernelGGL(kern, dim3(batch), dim3(BT), lds, s, R, C, j0, nbp, Y, ldy, sY, tau, Vw,
+                           vcols);
     };
     if (colb * 8 <= cap)
         go(k_qr_panel_lds<T, 8>, 8);
@@ -259,36 +262,41 @@ void launch_qr_panel(int R, int C, int j0, int nbp, cx<T> *Y, int ldy, size_t sY
         go(k_qr_panel_lds<T, 2>, 2);
     else
         hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, R, C, j0, nbp, Y, ldy, sY,
-                           tau, Vw);
+                           tau, Vw, vcols);
 }
 
 // T (nbp x nbp, upper) with H_0 ... H_{nbp-1} = I - V T V^H from G = V^H V
-// and tau (LAPACK larft, forward columnwise); one wave per matrix.
-template <typename T>
+// and tau (LAPACK larft, forward columnwise); one wave per matrix; G and T
+// are P x P (leading dimension P) per matrix.
+template <typename T, int P>
 __global__ __launch_bounds__(64) void k_qr_tmat(int C, int j0, int nbp, const cx<T> *__restrict__ Gall,
                                                const cx<T> *__restrict__ tauall,
                                                cx<T> *__restrict__ Tall) {
-    __shared__ cx<double> Ts[NB][NB];
-    __shared__ cx<double> z[NB];
+    static_assert(P <= 64, "one lane per row of T");
+    // upper triangle packed by columns: T[r][c] (r <= c) at c (c + 1) / 2 + r
+    __shared__ cx<double> Ts[P * (P + 1) / 2];
+    __shared__ cx<double> z[P];
     const int b = blockIdx.x, t = threadIdx.x;
-    const cx<T> *G = Gall + (size_t)b * NB * NB;
+    const cx<T> *G = Gall + (size_t)b * P * P;
     const cx<T> *tau = tauall + (size_t)b * C + j0;
-    cx<T> *Tm = Tall + (size_t)b * NB * NB;
-    for (int e = t; e < NB * NB; e += 64) Ts[e % NB][e / NB] = cx<double>{0, 0};
-    __syncthreads();
+    cx<T> *Tm = Tall + (size_t)b * P * P;
     for (int i = 0; i < nbp; ++i) {
         const cx<double> ti = cvt<double>(tau[i]);
-        if (t < i) z[t] = cx<double>{0, 0} - ti * cvt<double>(G[(size_t)i * NB + t]);  // G[t, i]
+        if (t < i) z[t] = cx<double>{0, 0} - ti * cvt<double>(G[(size_t)i * P + t]);  // G[t, i]
         __syncthreads();
+        const int ci = i * (i + 1) / 2;
         if (t < i) {
             cx<double> acc{0, 0};
-            for (int l = t; l < i; ++l) acc = acc + Ts[t][l] * z[l];
-            Ts[t][i] = acc;
+            for (int l = t; l < i; ++l) acc = acc + Ts[l * (l + 1) / 2 + t] * z[l];
+            Ts[ci + t] = acc;
         }
-        if (t == 0) Ts[i][i] = ti;
+        if (t == 0) Ts[ci + i] = ti;
         __syncthreads();
     }
-    for (int e = t; e < NB * NB; e += 64) Tm[e] = cvt<T>(Ts[e % NB][e / NB]);
+    for (int e = t; e < P * P; e += 64) {
+        const int r = e % P, c = e / P;
+        Tm[e] = (r <= c && c < nbp) ? cvt<T>(Ts[c * (c + 1) / 2 + r]) : cx<T>{0, 0};
+    }
 }
 
 // Rb (C x C) = upper triangle of Y's leading C x C block, zeros below
@@ -331,34 +339,35 @@ __global__ void k_lq_scatter(int k, int nbp, cx<T> *__restrict__ Zall, int ldz, 
 }  // namespace
 
 size_t geqrf_work_elems(int R, int C) {
-    return (size_t)R * NB + 2 * (size_t)NB * NB + 2 * (size_t)NB * C;
+    return (size_t)R * NBQ + 2 * (size_t)NBQ * NBQ + 2 * (size_t)NBQ * C;
 }
 
 template <typename T>
 ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, cx<T> *work,
                        int batch, hipStream_t s) {
     if (R < C) return fail(IA_ERR_ARG, "QR needs R >= C (%d x %d)", R, C);
+    constexpr int P = NBQ;
     cx<T> *Vw = work;
-    cx<T> *G = Vw + (size_t)batch * R * NB;
-    cx<T> *Tm = G + (size_t)batch * NB * NB;
-    cx<T> *W = Tm + (size_t)batch * NB * NB;
-    cx<T> *W2 = W + (size_t)batch * NB * C;
+    cx<T> *G = Vw + (size_t)batch * R * P;
+    cx<T> *Tm = G + (size_t)batch * P * P;
+    cx<T> *W = Tm + (size_t)batch * P * P;
+    cx<T> *W2 = W + (size_t)batch * P * C;
     const cx<T> one{1, 0}, zero{0, 0}, mone{-1, 0};
-    const size_t sV = (size_t)R * NB, sT = (size_t)NB * NB, sW = (size_t)NB * C;
-    for (int j0 = 0; j0 < C; j0 += NB) {
-        const int nbp = std::min(NB, C - j0);
-        launch_qr_panel<T>(R, C, j0, nbp, Y, ldy, sY, tau, Vw, batch, s);
+    const size_t sV = (size_t)R * P, sT = (size_t)P * P, sW = (size_t)P * C;
+    for (int j0 = 0; j0 < C; j0 += P) {
+        const int nbp = std::min(P, C - j0);
+        launch_qr_panel<T>(R, C, j0, nbp, Y, ldy, sY, tau, Vw, P, batch, s);
         const int N = C - j0 - nbp, M = R - j0;
         if (N <= 0) continue;
         const cx<T> *V0 = Vw + j0;  // rows j0.. of the explicit panel reflectors
         // G = V^H V
-        launch_gemm<T>('C', 'N', nbp, nbp, M, one, V0, R, sV, V0, R, sV, zero, G, NB, sT, batch, s);
-        hipLaunchKernelGGL(k_qr_tmat<T>, dim3(batch), dim3(64), 0, s, C, j0, nbp, G, tau, Tm);
+        launch_gemm<T>('C', 'N', nbp, nbp, M, one, V0, R, sV, V0, R, sV, zero, G, P, sT, batch, s);
+        hipLaunchKernelGGL((k_qr_tmat<T, P>), dim3(batch), dim3(64), 0, s, C, j0, nbp, G, tau, Tm);
         cx<T> *A2 = Y + j0 + (size_t)(j0 + nbp) * ldy;
         // W = V^H A2, W2 = T^H W, A2 -= V W2
-        launch_gemm<T>('C', 'N', nbp, N, M, one, V0, R, sV, A2, ldy, sY, zero, W, NB, sW, batch, s);
-        launch_gemm<T>('C', 'N', nbp, N, nbp, one, Tm, NB, sT, W, NB, sW, zero, W2, NB, sW, batch, s);
-        launch_gemm<T>('N', 'N', M, N, nbp, mone, V0, R, sV, W2, NB, sW, one, A2, ldy, sY, batch, s);
+        launch_gemm<T>('C', 'N', nbp, N, M, one, V0, R, sV, A2, ldy, sY, zero, W, P, sW, batch, s);
+        launch_gemm<T>('C', 'N', nbp, N, nbp, one, Tm, P, sT, W, P, sW, zero, W2, P, sW, batch, s);
+        launch_gemm<T>('N', 'N', M, N, nbp, mone, V0, R, sV, W2, P, sW, one, A2, ldy, sY, batch, s);
     }
     IA_HIP(hipGetLastError());
     return IA_OK;
@@ -394,13 +403,13 @@ ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *t
     for (int k = 0; k < C; k += NB) {
         const int w = std::min(NB, C - k);
         // ---- QR of columns k..k+w-1, left update of columns k+w..
-        launch_qr_panel<T>(R, C, k, w, Z, ldz, sZ, tauQ, Vw, batch, s);
+        launch_qr_panel<T>(R, C, k, w, Z, ldz, sZ, tauQ, Vw, NB, batch, s);
         const size_t sV = (size_t)R * NB;
         const int N = C - k - w, M = R - k;
         if (N > 0) {
             const cx<T> *V0 = Vw + k;
             launch_gemm<T>('C', 'N', w, w, M, one, V0, R, sV, V0, R, sV, zero, G, NB, sT, batch, s);
-            hipLaunchKernelGGL(k_qr_tmat<T>, dim3(batch), dim3(64), 0, s, C, k, w, G, tauQ, Tm);
+            hipLaunchKernelGGL((k_qr_tmat<T, NB>), dim3(batch), dim3(64), 0, s, C, k, w, G, tauQ, Tm);
             cx<T> *A2 = Z + k + (size_t)(k + w) * ldz;
             launch_gemm<T>('C', 'N', w, N, M, one, V0, R, sV, A2, ldz, sZ, zero, W, NB, sW, batch, s);
             launch_gemm<T>('C', 'N', w, N, w, one, Tm, NB, sT, W, NB, sW, zero, W2, NB, sW, batch, s);
@@ -413,7 +422,7 @@ ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *t
         const int nbp = std::min(NB, N2);    // reflectors (rows with columns left)
         hipLaunchKernelGGL(k_lq_gather<T>, dim3((N2 + 255) / 256, nbp, batch), dim3(256), 0, s, k,
                            N2, nbp, Z, ldz, sZ, Pm, Cp, sP);
-        launch_qr_panel<T>(Cp, Cp, k, nbp, Pm, Cp, sP, tauP, Vw, batch, s);
+        launch_qr_panel<T>(Cp, Cp, k, nbp, Pm, Cp, sP, tauP, Vw, NB, batch, s);
         hipLaunchKernelGGL(k_lq_scatter<T>, dim3((nbp * nbp + 255) / 256, batch), dim3(256), 0, s,
                            k, nbp, Z, ldz, sZ, Pm, Cp, sP);
         // rows below the reflected ones: Z2 <- Z2 - ((Z2 V) T) V^H
@@ -424,7 +433,7 @@ ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *t
         cx<T> *Z2 = Z + (k + nbp) + (size_t)(k + NB) * ldz;
         launch_gemm<T>('C', 'N', nbp, nbp, N2, one, V0, Cp, sV2, V0, Cp, sV2, zero, G, NB, sT,
                        batch, s);
-        hipLaunchKernelGGL(k_qr_tmat<T>, dim3(batch), dim3(64), 0, s, Cp, k, nbp, G, tauP, Tm);
+        hipLaunchKernelGGL((k_qr_tmat<T, NB>), dim3(batch), dim3(64), 0, s, Cp, k, nbp, G, tauP, Tm);
         launch_gemm<T>('N', 'N', M2, nbp, N2, one, Z2, ldz, sZ, V0, Cp, sV2, zero, W, Wl, sW, batch,
                        s);
         launch_gemm<T>('N', 'N', M2, nbp, nbp, one, W, Wl, sW, Tm, NB, sT, zero, W2, Wl, sW, batch,
