@@ -91,7 +91,7 @@ __global__ __launch_bounds__(BT) void k_qr_panel(int R, int C, int j0, int nbp,
 // A <- (I - V T V^H)^H A: two passes over each column (V^H a, then
 // a -= V T^H (V^H a)) instead of two passes per reflector.  Same outputs
 // (reflectors below the diagonal, tau, explicit V in Vw) as k_qr_panel.
-template <typename T, int SB>
+template <typename T, int SB, int RPL, int TB>
 __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int nbp,
                                                     cx<T> *__restrict__ Yall, int ldy, size_t sY,
                                                     cx<T> *__restrict__ tauall,
@@ -188,48 +188,121 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
                 }
             }
             __syncthreads();
-            // ---- remaining panel columns: a <- a - V T^H (V^H a), one wave per column
+            // ---- remaining panel columns: a <- a - V T^H (V^H a), one wave per column.
+            // The first 64 RPL rows of the column stay in registers between
+            // the two passes (all their loads in flight at once); rows beyond
+            // are streamed twice, TB rows per lane at a time.
             for (int k = k0 + wv; k < k1; k += NW) {
                 cx<T> *Ak = A + (size_t)k * ldy + j0;
-                cx<double> acc[SB];
+                cx<T> ar[RPL];
 #pragma unroll
-                for (int i = 0; i < SB; ++i) acc[i] = cx<double>{0, 0};
-                for (int r = r0 + lane; r < M; r += 64) {
-                    const cx<double> a = cvt<double>(Ak[r]);
+                for (int q = 0; q < RPL; ++q) {
+                    const int r = r0 + lane + 64 * q;
+                    ar[q] = (r < M) ? Ak[r] : cx<T>{0, 0};
+                }
+                cx<T> acc[SB];
+                // rows r0 .. r0 + 63 hold the unit diagonal and zeros above it
 #pragma unroll
-                    for (int i = 0; i < SB; ++i) {
-                        if (i >= w || r < r0 + i) continue;
-                        const cx<double> u =
-                            (r == r0 + i) ? cx<double>{1, 0} : cvt<double>(Vs[(size_t)i * M + r]);
-                        acc[i] = acc[i] + conj(u) * a;
+                for (int i = 0; i < SB; ++i) {
+                    const int r = r0 + lane;
+                    acc[i] = cx<T>{0, 0};
+                    if (i < w && r >= r0 + i && r < M) {
+                        const cx<T> u = (r == r0 + i) ? cx<T>{1, 0} : Vs[(size_t)i * M + r];
+                        acc[i] = cmulc(u, ar[0]);
+                    }
+                }
+#pragma unroll
+                for (int q = 1; q < RPL; ++q) {
+                    const int r = r0 + lane + 64 * q;
+                    if (r < M) {
+#pragma unroll
+                        for (int i = 0; i < SB; ++i)
+                            if (i < w) acc[i] = acc[i] + cmulc(Vs[(size_t)i * M + r], ar[q]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);  // keep the LDS reads of one row block live at a time
+                }
+                // rows beyond: blocks of TB rows per lane, loads issued together
+                for (int base = r0 + lane + 64 * RPL; base < M; base += 64 * TB) {
+                    cx<T> tb[TB];
+#pragma unroll
+                    for (int q = 0; q < TB; ++q) {
+                        const int r = base + 64 * q;
+                        tb[q] = (r < M) ? Ak[r] : cx<T>{0, 0};
+                    }
+#pragma unroll
+                    for (int q = 0; q < TB; ++q) {
+                        const int r = base + 64 * q;
+                        if (r < M) {
+#pragma unroll
+                            for (int i = 0; i < SB; ++i)
+                                if (i < w) acc[i] = acc[i] + cmulc(Vs[(size_t)i * M + r], tb[q]);
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
                     }
                 }
 #pragma unroll
                 for (int i = 0; i < SB; ++i) {
-                    const cx<double> v{wave_sum(acc[i].re), wave_sum(acc[i].im)};
+                    const cx<double> v{wave_sum((double)acc[i].re), wave_sum((double)acc[i].im)};
                     if (lane == 0) Zs[wv][i] = v;
                 }
                 // z = T^H acc (lane 0), then a <- a - V z with z read from LDS
                 if (lane == 0) {
-                    cx<double> z[SB];
-                    for (int jj = 0; jj < w; ++jj) {
+                    // in place, last entry first: z_jj reads entries 0..jj only
+                    for (int jj = w - 1; jj >= 0; --jj) {
                         cx<double> sacc{0, 0};
                         for (int ii = 0; ii <= jj; ++ii) sacc = sacc + conj(Ts[ii][jj]) * Zs[wv][ii];
-                        z[jj] = sacc;
+                        Zs[wv][jj] = sacc;
                     }
-                    for (int jj = 0; jj < w; ++jj) Zs[wv][jj] = z[jj];
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
-                for (int r = r0 + lane; r < M; r += 64) {
-                    cx<double> a = cvt<double>(Ak[r]);
-                    for (int i = 0; i < w; ++i) {
-                        if (r < r0 + i) break;
-                        const cx<double> u =
-                            (r == r0 + i) ? cx<double>{1, 0} : cvt<double>(Vs[(size_t)i * M + r]);
-                        a = a - u * Zs[wv][i];
+                cx<T> zt[SB];
+#pragma unroll
+                for (int i = 0; i < SB; ++i) zt[i] = cvt<T>(Zs[wv][i]);
+                {
+                    const int r = r0 + lane;
+                    if (r < M) {
+                        cx<T> a = ar[0];
+#pragma unroll
+                        for (int i = 0; i < SB; ++i)
+                            if (i < w && r >= r0 + i) {
+                                const cx<T> u = (r == r0 + i) ? cx<T>{1, 0} : Vs[(size_t)i * M + r];
+                                a = a - u * zt[i];
+                            }
+                        Ak[r] = a;
                     }
-                    Ak[r] = cvt<T>(a);
+                }
+#pragma unroll
+                for (int q = 1; q < RPL; ++q) {
+                    const int r = r0 + lane + 64 * q;
+                    if (r < M) {
+                        cx<T> a = ar[q];
+#pragma unroll
+                        for (int i = 0; i < SB; ++i)
+                            if (i < w) a = a - Vs[(size_t)i * M + r] * zt[i];
+                        Ak[r] = a;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                for (int base = r0 + lane + 64 * RPL; base < M; base += 64 * TB) {
+                    cx<T> tb[TB];
+#pragma unroll
+                    for (int q = 0; q < TB; ++q) {
+                        const int r = base + 64 * q;
+                        tb[q] = (r < M) ? Ak[r] : cx<T>{0, 0};
+                    }
+#pragma unroll
+                    for (int q = 0; q < TB; ++q) {
+                        const int r = base + 64 * q;
+                        if (r < M) {
+                            cx<T> a = tb[q];
+#pragma unroll
+                            for (int i = 0; i < SB; ++i)
+                                if (i < w) a = a - Vs[(size_t)i * M + r] * zt[i];
+                            Ak[r] = a;
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -254,12 +327,15 @@ void launch_qr_panel(int R, int C, int j0, int nbp, cx<T> *Y, int ldy, size_t sY
         hipLaunchKernelGGL(kern, dim3(batch), dim3(BT), lds, s, R, C, j0, nbp, Y, ldy, sY, tau, Vw,
                            vcols);
     };
+    // RPL rows of a column per lane kept in registers, TB per streamed block
+    // (sizes that keep every variant within 128 VGPRs without spills)
+    constexpr bool F = sizeof(T) == 4;
     if (colb * 8 <= cap)
-        go(k_qr_panel_lds<T, 8>, 8);
+        go(k_qr_panel_lds<T, 8, F ? 12 : 6, F ? 3 : 2>, 8);
     else if (colb * 4 <= cap)
-        go(k_qr_panel_lds<T, 4>, 4);
+        go(k_qr_panel_lds<T, 4, F ? 12 : 8, F ? 3 : 2>, 4);
     else if (colb * 2 <= cap)
-        go(k_qr_panel_lds<T, 2>, 2);
+        go(k_qr_panel_lds<T, 2, F ? 12 : 8, F ? 3 : 2>, 2);
     else
         hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, R, C, j0, nbp, Y, ldy, sY,
                            tau, Vw, vcols);
@@ -280,15 +356,26 @@ __global__ __launch_bounds__(64) void k_qr_tmat(int C, int j0, int nbp, const cx
     const cx<T> *G = Gall + (size_t)b * P * P;
     const cx<T> *tau = tauall + (size_t)b * C + j0;
     cx<T> *Tm = Tall + (size_t)b * P * P;
+    const cx<double> tau_t = t < nbp ? cvt<double>(tau[t]) : cx<double>{0, 0};
+    // column i of G (rows t < P), prefetched one step ahead
+    cx<T> gnext = t < P ? G[t] : cx<T>{0, 0};
     for (int i = 0; i < nbp; ++i) {
-        const cx<double> ti = cvt<double>(tau[i]);
-        if (t < i) z[t] = cx<double>{0, 0} - ti * cvt<double>(G[(size_t)i * P + t]);  // G[t, i]
+        const cx<T> gcur = gnext;
+        if (i + 1 < nbp && t < P) gnext = G[(size_t)(i + 1) * P + t];
+        const cx<double> ti{__shfl(tau_t.re, i, 64), __shfl(tau_t.im, i, 64)};
+        if (t < i) z[t] = cx<double>{0, 0} - ti * cvt<double>(gcur);  // G[t, i]
         __syncthreads();
         const int ci = i * (i + 1) / 2;
         if (t < i) {
-            cx<double> acc{0, 0};
-            for (int l = t; l < i; ++l) acc = acc + Ts[l * (l + 1) / 2 + t] * z[l];
-            Ts[ci + t] = acc;
+            // two interleaved partial sums shorten the dependent FMA chain
+            cx<double> a0{0, 0}, a1{0, 0};
+            int l = t;
+            for (; l + 1 < i; l += 2) {
+                a0 = a0 + Ts[l * (l + 1) / 2 + t] * z[l];
+                a1 = a1 + Ts[(l + 1) * (l + 2) / 2 + t] * z[l + 1];
+            }
+            if (l < i) a0 = a0 + Ts[l * (l + 1) / 2 + t] * z[l];
+            Ts[ci + t] = a0 + a1;
         }
         if (t == 0) Ts[ci + i] = ti;
         __syncthreads();

@@ -453,8 +453,39 @@ void launch_trsm_diag(int n, int i0, int jb, const cx<T> *A, int lda, size_t sA,
     }
 }
 
-// permutation index from interchanges: perm applied as (P B)[r] = B[idx[r]]
-__global__ void k_perm_index(int n, const int *ipiv, int *idx, int *inv, int batch) {
+// permutation index from interchanges: perm applied as (P B)[r] = B[idx[r]];
+// one workgroup per matrix, the interchanges replayed in order in LDS
+__global__ __launch_bounds__(256) void k_perm_index(int n, const int *ipiv, int *idx, int *inv) {
+    extern __shared__ int pl[];  // ipiv (n), index (n)
+    int *ps = pl, *xs = pl + n;
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int *ip = ipiv + (size_t)b * n;
+    int *ix = idx + (size_t)b * n;
+    int *iv = inv + (size_t)b * n;
+    for (int r = t; r < n; r += 256) {
+        ps[r] = ip[r];
+        xs[r] = r;
+    }
+    __syncthreads();
+    if (t == 0)
+        for (int c = 0; c < n; ++c) {
+            const int p = ps[c];
+            if (p != c) {
+                const int tmp = xs[c];
+                xs[c] = xs[p];
+                xs[p] = tmp;
+            }
+        }
+    __syncthreads();
+    for (int r = t; r < n; r += 256) {
+        const int v = xs[r];
+        ix[r] = v;
+        iv[v] = r;
+    }
+}
+
+// the same for orders whose index lists exceed LDS: one thread per matrix
+__global__ void k_perm_index_global(int n, const int *ipiv, int *idx, int *inv, int batch) {
     int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= batch) return;
     const int *ip = ipiv + (size_t)b * n;
@@ -585,6 +616,117 @@ void trsm_blocked(int n, int nrhs, const cx<T> *A, int lda, size_t sA, cx<T> *B,
     }
 }
 
+// One right-hand side, op 'N' (LAPACK getrs): b <- U^-1 L^-1 P b in a single
+// launch, one workgroup per matrix with b resident in LDS throughout.  The
+// interchanges are applied in order by one thread in LDS; each triangle is
+// walked in diagonal blocks of G1 rows: the block is staged in LDS and solved
+// by one wave (lane l owns x_l, the solved entry is broadcast by a lane
+// shuffle), then every thread updates its rows of the rest of b with the
+// block's columns (x_r -= A[r, blk] x_blk), so each element of L and U is
+// read from memory once.
+constexpr int G1 = 64;
+constexpr int G1T = 1024;
+
+// lane shuffle of a float or double, as 32-bit halves (no stack round trip)
+__device__ __forceinline__ float shfl_r(float v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ double shfl_r(double v, int src) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = __shfl((int)(unsigned)u, src, 64), hi = __shfl((int)(unsigned)(u >> 32), src, 64);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <typename T>
+__device__ __forceinline__ cx<T> shfl_cx(cx<T> v, int src) {
+    return cx<T>{shfl_r(v.re, src), shfl_r(v.im, src)};
+}
+
+template <typename T>
+__global__ __launch_bounds__(G1T) void k_getrs1(int n, const cx<T> *__restrict__ Aall, int lda,
+                                               size_t sA, const int *__restrict__ ipivall,
+                                               cx<T> *__restrict__ Ball, size_t sB) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *xs = reinterpret_cast<cx<T> *>(smem_raw);  // n
+    cx<T> *Ds = xs + n;                                // G1 x G1 block, column-major, ld G1 + 1
+    int *ps = reinterpret_cast<int *>(Ds + (size_t)G1 * (G1 + 1));  // n pivots
+    constexpr int LD = G1 + 1;
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const cx<T> *A = Aall + (size_t)b * sA;
+    cx<T> *B = Ball + (size_t)b * sB;
+    const int *ip = ipivall + (size_t)b * n;
+    for (int r = t; r < n; r += G1T) {
+        xs[r] = B[r];
+        ps[r] = ip[r];
+    }
+    __syncthreads();
+    if (t == 0)
+        for (int c = 0; c < n; ++c) {
+            const int p = ps[c];
+            if (p != c) {
+                const cx<T> tmp = xs[c];
+                xs[c] = xs[p];
+                xs[p] = tmp;
+            }
+        }
+    __syncthreads();
+    // ---- L (unit lower), top to bottom
+    for (int i0 = 0; i0 < n; i0 += G1) {
+        const int jb = min(G1, n - i0);
+        for (int e = t; e < jb * jb; e += G1T) {
+            const int l = e % jb, j = e / jb;
+            Ds[j * LD + l] = A[(size_t)(i0 + j) * lda + i0 + l];
+        }
+        __syncthreads();
+        if (wv == 0) {
+            cx<T> x = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
+            for (int j = 0; j < jb; ++j) {
+                const cx<T> xj = shfl_cx(x, j);
+                if (lane > j && lane < jb) x = x - Ds[j * LD + lane] * xj;
+            }
+            if (lane < jb) xs[i0 + lane] = x;
+        }
+        __syncthreads();
+        for (int r = i0 + jb + t; r < n; r += G1T) {
+            cx<T> acc{0, 0};
+#pragma unroll 16
+            for (int j = 0; j < jb; ++j) acc = acc + A[(size_t)(i0 + j) * lda + r] * xs[i0 + j];
+            xs[r] = xs[r] - acc;
+        }
+        __syncthreads();
+    }
+    // ---- U (upper, non-unit), bottom to top
+    const int nblk = (n + G1 - 1) / G1;
+    for (int bk = nblk - 1; bk >= 0; --bk) {
+        const int i0 = bk * G1, jb = min(G1, n - i0);
+        for (int e = t; e < jb * jb; e += G1T) {
+            const int l = e % jb, j = e / jb;
+            Ds[j * LD + l] = A[(size_t)(i0 + j) * lda + i0 + l];
+        }
+        __syncthreads();
+        if (wv == 0) {
+            cx<T> x = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
+            for (int j = jb - 1; j >= 0; --j) {
+                if (lane == j) x = x * cdiv(cx<T>{1, 0}, Ds[j * LD + j]);
+                const cx<T> xj = shfl_cx(x, j);
+                if (lane < j) x = x - Ds[j * LD + lane] * xj;
+            }
+            if (lane < jb) xs[i0 + lane] = x;
+        }
+        __syncthreads();
+        for (int r = t; r < i0; r += G1T) {
+            cx<T> acc{0, 0};
+#pragma unroll 16
+            for (int j = 0; j < jb; ++j) acc = acc + A[(size_t)(i0 + j) * lda + r] * xs[i0 + j];
+            xs[r] = xs[r] - acc;
+        }
+        __syncthreads();
+    }
+    for (int r = t; r < n; r += G1T) B[r] = xs[r];
+}
+
+template <typename T>
+size_t getrs1_lds(int n) {
+    return sizeof(cx<T>) * ((size_t)n + (size_t)G1 * (G1 + 1)) + sizeof(int) * (size_t)n;
+}
+
 }  // namespace
 
 template <typename T>
@@ -663,13 +805,29 @@ ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_
                         const int *ipiv, cx<T> *B, int ldb, size_t strideB, int batch,
                         hipStream_t s) {
     if (n <= 0 || nrhs <= 0 || batch <= 0) return IA_OK;
+    if (op == 'N' && nrhs == 1 && getrs1_lds<T>(n) <= 160 * 1024) {
+        const size_t lds = getrs1_lds<T>(n);
+        (void)hipFuncSetAttribute((const void *)k_getrs1<T>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_getrs1<T>, dim3(batch), dim3(G1T), lds, s, n, A, lda, strideA, ipiv, B,
+                           strideB);
+        IA_HIP(hipGetLastError());
+        return IA_OK;
+    }
     int *idx = nullptr;
     cx<T> *tmp = nullptr;
     IA_HIP(hipMallocAsync((void **)&idx, sizeof(int) * 2 * (size_t)n * batch, s));
     IA_HIP(hipMallocAsync((void **)&tmp, sizeof(cx<T>) * (size_t)n * nrhs * batch, s));
     int *inv = idx + (size_t)n * batch;
-    hipLaunchKernelGGL(k_perm_index, dim3((batch + 63) / 64), dim3(64), 0, s, n, ipiv, idx, inv,
-                       batch);
+    const size_t plds = 2 * sizeof(int) * (size_t)n;
+    if (plds <= 160 * 1024) {
+        (void)hipFuncSetAttribute((const void *)k_perm_index,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds);
+        hipLaunchKernelGGL(k_perm_index, dim3(batch), dim3(256), plds, s, n, ipiv, idx, inv);
+    } else {
+        hipLaunchKernelGGL(k_perm_index_global, dim3((batch + 63) / 64), dim3(64), 0, s, n, ipiv,
+                           idx, inv, batch);
+    }
     dim3 pg((n + 255) / 256, nrhs, batch);
     const size_t sT = (size_t)n * nrhs;
     if (op == 'N') {

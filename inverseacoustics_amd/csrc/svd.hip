@@ -110,9 +110,11 @@ __global__ __launch_bounds__(256) void k_bisect(int n, const double *__restrict_
 // ----------------------------------------------------- inverse iteration
 // One thread per singular value: pivoted LU of T - sigma I (N = 2n), three
 // solves from a deterministic pseudo-random start, split into q (even) and
-// p (odd).  Scratch is interleaved across threads (element k of thread g at
-// k*G + g) so that every step is a coalesced access.  Outputs QBt, PBt hold
-// the vectors as ROWS: QBt[i + k n] = q_i[k].
+// p (odd).  Scratch is one slab per wave with the 64 lanes interleaved
+// (element k of lane l at (wave N + k) 64 + l): every step is one 512-byte
+// coalesced access and each wave streams through consecutive memory.
+// G is the scratch column count (64 per wave).  Outputs QBt, PBt hold the
+// vectors as ROWS: QBt[i + k n] = q_i[k].
 __device__ __forceinline__ double hash01(uint64_t x) {
     x += 0x9E3779B97F4A7C15ull;
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -121,7 +123,16 @@ __device__ __forceinline__ double hash01(uint64_t x) {
     return (double)(x >> 11) * (1.0 / 9007199254740992.0) - 0.5;
 }
 
-__global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__restrict__ dall,
+// Every pass walks the N steps in blocks of IU: the block's operands are
+// loaded into registers first (all loads of a block in flight together),
+// then the serial recurrence runs on registers; the pivot flags of the
+// backward pass are fetched one block ahead so that the conditional loads of
+// the second superdiagonal need no extra round trip.
+constexpr int IU = 6;
+
+// Four waves per SIMD (at most 128 VGPRs): 256 frequencies x 1024 singular
+// values are 4096 single-wave workgroups, one round on 1024 SIMDs.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_inviter(int n, int b0, const double *__restrict__ dall,
                                                const double *__restrict__ eall,
                                                const double *__restrict__ sall,
                                                double *__restrict__ QBt, double *__restrict__ PBt,
@@ -130,7 +141,8 @@ __global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__r
     const int bl = blockIdx.y;       // local batch index within this call
     const int b = b0 + bl;
     if (i >= n) return;
-    const int g = bl * n + i;        // scratch column
+    const int lane = threadIdx.x;
+    const size_t wid = (size_t)bl * gridDim.x + blockIdx.x;  // this wave's scratch slab
     const int N = 2 * n;
     const double *d = dall + (size_t)b * n;
     const double *e = eall + (size_t)b * n;
@@ -141,7 +153,7 @@ __global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__r
     double *DL = work + (size_t)3 * N * G;   // multipliers
     double *X = work + (size_t)4 * N * G;    // iterate
     unsigned char *PV = reinterpret_cast<unsigned char *>(work + (size_t)5 * N * G);
-#define AT(arr, k) arr[(size_t)(k) * G + g]
+#define AT(arr, k) arr[(wid * N + (k)) * 64 + lane]
     auto toff = [&](int k) { return (k & 1) ? e[k >> 1] : d[k >> 1]; };
     double tnorm = 0;
     for (int k = 0; k < N - 1; ++k) tnorm = fmax(tnorm, fabs(toff(k)));
@@ -151,34 +163,43 @@ __global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__r
     auto start = [&](int k) {
         return hash01(((uint64_t)b << 40) ^ ((uint64_t)i << 20) ^ k) + 0.25;
     };
+    const int Np = (N + IU - 1) / IU * IU;
     // ---- one pass: LU of T - sigma I with partial pivoting (LAPACK gttrf),
     // the random start vector and the first solve with L
     {
         double dd = -sigma, du = toff(0), x = start(0);
-        for (int k = 0; k < N - 1; ++k) {
-            const double dlk = toff(k), ddn = -sigma, dun = k + 1 < N - 1 ? toff(k + 1) : 0.0;
-            const double xn = start(k + 1);
-            if (fabs(dd) >= fabs(dlk)) {
-                const double fact = dd != 0.0 ? dlk / dd : 0.0;
-                AT(PV, k) = 0;
-                AT(DL, k) = fact;
-                AT(DD, k) = clampd(dd);
-                AT(DU, k) = du;
-                AT(X, k) = x;
-                x = xn - fact * x;
-                dd = ddn - fact * du;
-                du = dun;
-            } else {
-                const double fact = dd / dlk;
-                AT(PV, k) = 1;
-                AT(DL, k) = fact;
-                AT(DD, k) = clampd(dlk);
-                AT(DU, k) = ddn;
-                if (k < N - 2) AT(DU2, k) = dun;
-                AT(X, k) = xn;
-                x = x - fact * xn;
-                dd = du - fact * ddn;
-                du = -fact * dun;
+        for (int kb = 0; kb < N - 1; kb += IU) {
+            double tv[IU + 1];  // off-diagonals k = kb .. kb + IU (0 from N - 1 on)
+#pragma unroll
+            for (int u = 0; u <= IU; ++u) tv[u] = (kb + u < N - 1) ? toff(kb + u) : 0.0;
+#pragma unroll
+            for (int u = 0; u < IU; ++u) {
+                const int k = kb + u;
+                if (k >= N - 1) break;
+                const double dlk = tv[u], ddn = -sigma, dun = tv[u + 1];
+                const double xn = start(k + 1);
+                if (fabs(dd) >= fabs(dlk)) {
+                    const double fact = dd != 0.0 ? dlk / dd : 0.0;
+                    AT(PV, k) = 0;
+                    AT(DL, k) = fact;
+                    AT(DD, k) = clampd(dd);
+                    AT(DU, k) = du;
+                    AT(X, k) = x;
+                    x = xn - fact * x;
+                    dd = ddn - fact * du;
+                    du = dun;
+                } else {
+                    const double fact = dd / dlk;
+                    AT(PV, k) = 1;
+                    AT(DL, k) = fact;
+                    AT(DD, k) = clampd(dlk);
+                    AT(DU, k) = ddn;
+                    if (k < N - 2) AT(DU2, k) = dun;
+                    AT(X, k) = xn;
+                    x = x - fact * xn;
+                    dd = du - fact * ddn;
+                    du = -fact * dun;
+                }
             }
         }
         AT(DD, N - 1) = clampd(dd);
@@ -187,29 +208,72 @@ __global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__r
     // backward solve with U; returns 1 / max |x|
     auto backward = [&]() {
         double scale = 0, x2 = 0, x1 = 0;  // x[k+2], x[k+1]
-        for (int k = N - 1; k >= 0; --k) {
-            double sv = AT(X, k);
-            if (k < N - 1) sv -= AT(DU, k) * x1;
-            if (k < N - 2 && AT(PV, k)) sv -= AT(DU2, k) * x2;
-            sv /= AT(DD, k);
-            AT(X, k) = sv;
-            x2 = x1;
-            x1 = sv;
-            scale = fmax(scale, fabs(sv));
+        int pvc[IU];
+#pragma unroll
+        for (int u = 0; u < IU; ++u) {
+            const int k = Np - IU + u;
+            pvc[u] = (k < N - 2) ? AT(PV, k) : 0;
+        }
+        for (int kb = Np - IU; kb >= 0; kb -= IU) {
+            double xv[IU], du[IU], du2[IU], dd[IU];
+            int pvn[IU];
+#pragma unroll
+            for (int u = 0; u < IU; ++u) {
+                const int k = kb + u;
+                if (k < N) {
+                    xv[u] = AT(X, k);
+                    du[u] = (k < N - 1) ? AT(DU, k) : 0.0;
+                    dd[u] = AT(DD, k);
+                    du2[u] = pvc[u] ? AT(DU2, k) : 0.0;
+                }
+                const int kn = k - IU;  // next (lower) block
+                pvn[u] = (kn >= 0 && kn < N - 2) ? AT(PV, kn) : 0;
+            }
+#pragma unroll
+            for (int u = IU - 1; u >= 0; --u) {
+                const int k = kb + u;
+                if (k >= N) continue;
+                double sv = xv[u];
+                if (k < N - 1) sv -= du[u] * x1;
+                if (pvc[u]) sv -= du2[u] * x2;
+                sv /= dd[u];
+                AT(X, k) = sv;
+                x2 = x1;
+                x1 = sv;
+                scale = fmax(scale, fabs(sv));
+            }
+#pragma unroll
+            for (int u = 0; u < IU; ++u) pvc[u] = pvn[u];
         }
         return scale > 0 ? 1.0 / scale : 1.0;
     };
     // forward solve with L of the iterate scaled by inv
     auto forward = [&](double inv) {
         double x = AT(X, 0) * inv;
-        for (int k = 0; k < N - 1; ++k) {
-            const double xn = AT(X, k + 1) * inv, fk = AT(DL, k);
-            if (AT(PV, k) == 0) {
-                AT(X, k) = x;
-                x = xn - fk * x;
-            } else {
-                AT(X, k) = xn;
-                x = x - fk * xn;
+        for (int kb = 0; kb < N - 1; kb += IU) {
+            double xn[IU], fl[IU];
+            int pv[IU];
+#pragma unroll
+            for (int u = 0; u < IU; ++u) {
+                const int k = kb + u;
+                if (k < N - 1) {
+                    xn[u] = AT(X, k + 1);
+                    fl[u] = AT(DL, k);
+                    pv[u] = AT(PV, k);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < IU; ++u) {
+                const int k = kb + u;
+                if (k >= N - 1) break;
+                const double xk = xn[u] * inv, fk = fl[u];
+                if (pv[u] == 0) {
+                    AT(X, k) = x;
+                    x = xk - fk * x;
+                } else {
+                    AT(X, k) = xk;
+                    x = x - fk * xk;
+                }
             }
         }
         AT(X, N - 1) = x;
@@ -220,13 +284,27 @@ __global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__r
     forward(inv);
     inv = backward();
     // ---- split, normalise, fix the sign of p so that p^T B q >= 0
+    constexpr int HU = IU / 2;  // singular-vector entries per block
     double nq = 0, np = 0, sgn = 0;
-    for (int k = 0; k < n; ++k) {
-        const double q = AT(X, 2 * k) * inv, p = AT(X, 2 * k + 1) * inv;
-        const double qn = k < n - 1 ? AT(X, 2 * k + 2) * inv : 0.0;
-        nq += q * q;
-        np += p * p;
-        sgn += p * (d[k] * q + (k < n - 1 ? e[k] * qn : 0.0));
+    for (int kb = 0; kb < n; kb += HU) {
+        double xv[2 * HU + 1], dv[HU], ev[HU];
+#pragma unroll
+        for (int u = 0; u <= 2 * HU; ++u) xv[u] = (2 * kb + u < N) ? AT(X, 2 * kb + u) : 0.0;
+#pragma unroll
+        for (int u = 0; u < HU; ++u) {
+            dv[u] = (kb + u < n) ? d[kb + u] : 0.0;
+            ev[u] = (kb + u < n - 1) ? e[kb + u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < HU; ++u) {
+            const int k = kb + u;
+            if (k >= n) break;
+            const double q = xv[2 * u] * inv, p = xv[2 * u + 1] * inv;
+            const double qn = k < n - 1 ? xv[2 * u + 2] * inv : 0.0;
+            nq += q * q;
+            np += p * p;
+            sgn += p * (dv[u] * q + (k < n - 1 ? ev[u] * qn : 0.0));
+        }
     }
     nq = sqrt(nq);
     np = sqrt(np);
@@ -235,9 +313,17 @@ __global__ __launch_bounds__(64) void k_inviter(int n, int b0, const double *__r
     if (sgn < 0) ip = -ip;
     double *Qb = QBt + (size_t)b * n * n;
     double *Pb = PBt + (size_t)b * n * n;
-    for (int k = 0; k < n; ++k) {
-        Qb[(size_t)k * n + i] = AT(X, 2 * k) * iq;
-        Pb[(size_t)k * n + i] = AT(X, 2 * k + 1) * ip;
+    for (int kb = 0; kb < n; kb += HU) {
+        double xv[2 * HU];
+#pragma unroll
+        for (int u = 0; u < 2 * HU; ++u) xv[u] = (2 * kb + u < N) ? AT(X, 2 * kb + u) : 0.0;
+#pragma unroll
+        for (int u = 0; u < HU; ++u) {
+            const int k = kb + u;
+            if (k >= n) break;
+            Qb[(size_t)k * n + i] = xv[2 * u] * iq;
+            Pb[(size_t)k * n + i] = xv[2 * u + 1] * ip;
+        }
     }
 #undef AT
 }
@@ -285,6 +371,95 @@ __global__ __launch_bounds__(256) void k_apply_left(int R, int C, const cx<T> *_
             v[r] = v[r] - uu * f;
         }
         __syncthreads();  // the next reflector's dot product reads rows other waves wrote
+    }
+}
+
+// Same product with the vector held in registers (row r = t + 256 q in
+// x[q], R <= 256 RPT) and the reflector columns streamed through a ring of
+// PD register slots, each loaded PD reflectors ahead of its use, so that the
+// memory latency overlaps the preceding reflectors' arithmetic.  One barrier
+// per reflector: the four waves' partial dot products alternate between two
+// LDS slots.
+template <typename T, bool HERM, int RPT, int PD>
+__global__ __launch_bounds__(256) void k_apply_left_reg(int R, int C,
+                                                       const cx<T> *__restrict__ Yall, int ldy,
+                                                       size_t sY, const cx<T> *__restrict__ tauall,
+                                                       cx<double> *__restrict__ vall, size_t sv,
+                                                       int nvec) {
+    __shared__ double red[2][2][4];
+    const int vec = blockIdx.x, b = blockIdx.y;
+    const cx<T> *Y = Yall + (size_t)b * sY;
+    const cx<T> *tau = tauall + (size_t)b * C;
+    cx<double> *v = vall + (size_t)b * nvec * sv + (size_t)vec * sv;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    cx<double> x[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = t + 256 * q;
+        x[q] = r < R ? v[r] : cx<double>{0, 0};
+    }
+    auto refl = [&](int jj) { return HERM ? jj : (C - 1 - jj); };
+    cx<T> ring[PD][RPT];
+    cx<T> rtau[PD];
+    auto fetch = [&](int jj, cx<T> *u, cx<T> &tj) {
+        if (jj < C) {
+            const int j = refl(jj);
+            const cx<T> *uc = Y + (size_t)j * ldy;
+            tj = tau[j];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int r = t + 256 * q;
+                u[q] = (r > j && r < R) ? uc[r] : cx<T>{0, 0};
+            }
+        }
+    };
+#pragma unroll
+    for (int sl = 0; sl < PD; ++sl) fetch(sl, ring[sl], rtau[sl]);
+    for (int jj0 = 0; jj0 < C; jj0 += PD) {
+#pragma unroll
+        for (int sl = 0; sl < PD; ++sl) {
+            const int jj = jj0 + sl;
+            if (jj >= C) break;
+            const int j = refl(jj);
+            cx<T> u[RPT];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) u[q] = ring[sl][q];
+            const cx<double> tj = cvt<double>(rtau[sl]);
+            fetch(jj + PD, ring[sl], rtau[sl]);
+            if (tj.re == 0.0 && tj.im == 0.0) continue;  // uniform: every thread read tau[j]
+            double ar = 0, ai = 0;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int r = t + 256 * q;
+                const cx<double> uu = (r == j) ? cx<double>{1, 0} : cvt<double>(u[q]);
+                if (r >= j && r < R) {
+                    ar += uu.re * x[q].re + uu.im * x[q].im;
+                    ai += uu.re * x[q].im - uu.im * x[q].re;
+                }
+            }
+            ar = wave_sum(ar);
+            ai = wave_sum(ai);
+            const int pb = jj & 1;
+            if (lane == 0) {
+                red[pb][0][w] = ar;
+                red[pb][1][w] = ai;
+            }
+            __syncthreads();
+            const cx<double> dot{red[pb][0][0] + red[pb][0][1] + red[pb][0][2] + red[pb][0][3],
+                                 red[pb][1][0] + red[pb][1][1] + red[pb][1][2] + red[pb][1][3]};
+            const cx<double> f = (HERM ? conj(tj) : tj) * dot;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int r = t + 256 * q;
+                const cx<double> uu = (r == j) ? cx<double>{1, 0} : cvt<double>(u[q]);
+                if (r >= j && r < R) x[q] = x[q] - uu * f;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int r = t + 256 * q;
+        if (r < R) v[r] = x[q];
     }
 }
 
@@ -406,8 +581,11 @@ void launch_bisect(int n, const double *d, const double *e, double *sigma, int b
     hipLaunchKernelGGL(k_bisect, grid, dim3(256), lds, s, n, d, e, sigma);
 }
 
+// scratch columns: 64 per wave, (n + 63) / 64 waves per matrix
+static size_t inviter_cols(int n, int sub) { return (size_t)(n + 63) / 64 * 64 * sub; }
+
 size_t inviter_work_bytes(int n, int sub) {
-    size_t G = (size_t)n * sub, N = 2 * (size_t)n;
+    size_t G = inviter_cols(n, sub), N = 2 * (size_t)n;
     return 5 * N * G * sizeof(double) + N * G + 256;
 }
 
@@ -417,7 +595,7 @@ void launch_inviter(int n, const double *d, const double *e, const double *sigma
         int nb = min(sub, batch - b0);
         dim3 grid((n + 63) / 64, nb);
         hipLaunchKernelGGL(k_inviter, grid, dim3(64), 0, s, n, b0, d, e, sigma, QBt, PBt, work,
-                           n * sub);
+                           (int)inviter_cols(n, sub));
     }
 }
 
@@ -437,7 +615,20 @@ void launch_apply_left(bool herm, int R, int C, const cx<T> *Y, int ldy, size_t 
                        const cx<T> *tauL, cx<double> *v, size_t stridev, int nvec, int batch,
                        hipStream_t s) {
     dim3 grid(nvec, batch);
-    if (herm)
+    constexpr int PD = sizeof(T) == 4 ? 8 : 4;
+    auto reg = [&](auto k_h, auto k_n) {
+        if (herm)
+            hipLaunchKernelGGL(k_h, grid, dim3(256), 0, s, R, C, Y, ldy, strideY, tauL, v, stridev,
+                               nvec);
+        else
+            hipLaunchKernelGGL(k_n, grid, dim3(256), 0, s, R, C, Y, ldy, strideY, tauL, v, stridev,
+                               nvec);
+    };
+    if (R <= 256 * 4)
+        reg(k_apply_left_reg<T, true, 4, PD>, k_apply_left_reg<T, false, 4, PD>);
+    else if (R <= 256 * 8)
+        reg(k_apply_left_reg<T, true, 8, PD>, k_apply_left_reg<T, false, 8, PD>);
+    else if (herm)
         hipLaunchKernelGGL((k_apply_left<T, true>), grid, dim3(256), 0, s, R, C, Y, ldy, strideY,
                            tauL, v, stridev, nvec);
     else

@@ -66,6 +66,22 @@ def test_zgesv(ia, n, nrhs, op):
     np.testing.assert_allclose(Dop @ X, B, atol=1e-11 * np.abs(B).max())
 
 
+@pytest.mark.parametrize("n", [2, 63, 64, 65, 130, 1000, 3000])
+def test_zgesv_single_rhs_general(ia, n):
+    # one right-hand side (the in-LDS solve), no diagonal dominance so that
+    # rows are interchanged throughout; backward-error bound of LU with
+    # partial pivoting, and agreement with the multi-RHS path
+    rng = np.random.default_rng(n)
+    D = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    B = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    X = ia.zgesv(D, B)
+    assert X.shape == (n,)
+    res = np.abs(D @ X - B).max()
+    assert res <= 1e-13 * n * np.abs(D).max() * np.abs(X).max()
+    X2 = ia.zgesv(D, np.stack([B, B], axis=1))
+    np.testing.assert_allclose(X, X2[:, 0], rtol=0, atol=1e-9 * np.abs(X).max())
+
+
 def test_zgesv_pivoting_needed(ia):
     # zero leading entry forces a row interchange in the first panel
     D = np.array([[0, 2, 1], [1, 1, 0], [3, 0, 1]], dtype=complex)
