@@ -1,6 +1,6 @@
-// Device helpers shared by the Householder kernels (svd.hip, gebrd.hip):
-// wave/block reductions for 1024-thread workgroups and the complex
-// reflector generator.
+// Device helpers shared by the Householder kernels (svd.hip, gebrd.hip,
+// geqrf.hip, bulge.hip): wave/block reductions for 1024-thread workgroups
+// and the complex reflector generator.
 #pragma once
 #include "ia_internal.hpp"
 
@@ -9,12 +9,6 @@ namespace hh {
 
 constexpr int BT = 1024;  // threads of a Householder workgroup
 constexpr int NW = BT / 64;
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // Wave sums on the DPP lane network (no LDS traffic): pairs and quads by
 // quad permutes, 8 and 16 lanes by the half-row and row mirrors, then
@@ -55,6 +49,9 @@ __device__ __forceinline__ R wave_sum_dpp(R v) {
         return __builtin_bit_cast(R, ((long long)hi << 32) | (unsigned)lo);
     }
 }
+
+// sum over the 64 lanes of a wave (every lane active), in every lane
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_dpp(v); }
 
 // block sum over BT threads; red must hold NW doubles; all threads get result
 __device__ __forceinline__ double block_sum(double v, double *red) {
