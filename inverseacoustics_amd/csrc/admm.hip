@@ -49,7 +49,7 @@ __global__ __launch_bounds__(BT) void k_admm(int n, const cx<double> *__restrict
                                             cx<double> *__restrict__ uall,
                                             cx<double> *__restrict__ call, AdmmDev prm,
                                             int *__restrict__ iters) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<double> *r = reinterpret_cast<cx<double> *>(smem_raw);  // n
     __shared__ double red[NW];
     const int b = blockIdx.x, t = threadIdx.x;

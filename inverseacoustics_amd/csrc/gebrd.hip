@@ -206,7 +206,7 @@ __global__ __launch_bounds__(BT) void k_labrd(int R, int C, int j0, int nbp,
                                              cx<T> *__restrict__ Vrall, cx<T> *__restrict__ Xwall,
                                              cx<T> *__restrict__ Ywall,
                                              cx<T> *__restrict__ Vwall) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<double> *part = reinterpret_cast<cx<double> *>(smem_raw);  // NW * 64 * 2
     cx<double> *pu = part + NW * 64 * 2;                            // NB: u^H U_k
     cx<double> *pq = pu + NB;                                       // NB: u^H Xw_k

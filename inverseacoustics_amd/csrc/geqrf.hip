@@ -112,7 +112,7 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
     // for q < i - 1, x^H a_q for q > i
     constexpr int NRED = 1 + 2 * SB;
     static_assert(NRED <= 64, "one lane per reduced value");
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *Vs = reinterpret_cast<cx<T> *>(smem_raw);  // SB columns of M rows
     __shared__ double red[NW][NRED];
     __shared__ cx<T> fs[SB];       // update factors of the current reflector
@@ -224,7 +224,6 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
                             prm[0] = rf.scale;
                             prm[1] = cx<double>{rf.beta, 0};
                             taus[i] = rf.tau;
-                            tau[cs + i] = cvt<T>(rf.tau);
                         }
                     }
                 }
@@ -244,7 +243,7 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
                 for (int r = rc + 1 + t; r < M; r += BT) col[r] = col[r] * sc;
                 if (t == 0) {
                     col[rc] = cx<T>{(T)rf.beta, 0};
-                    tau[cs + i] = cvt<T>(rf.tau);
+                    taus[i] = rf.tau;
                 }
                 __syncthreads();
                 const cx<double> ctau = conj(rf.tau);
@@ -267,6 +266,9 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
             }
         }
         // ---- write back: factored columns and explicit reflectors
+        // (tau leaves LDS here: a global store inside the column loop would
+        // make every barrier there wait for it)
+        if (t < w) tau[cs + t] = cvt<T>(taus[t]);
         for (int i = 0; i < w; ++i) {
             const int rc = cs + i - j0;
             for (int r = t; r < M; r += BT) {
@@ -298,7 +300,7 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
                 __syncthreads();
                 if (t == 0) {
                     for (int ii = 0; ii < w; ++ii) {
-                        const cx<double> ti = cvt<double>(tau[cs + ii]);
+                        const cx<double> ti = taus[ii];
                         for (int jj = 0; jj < ii; ++jj) {
                             cx<double> acc{0, 0};
                             for (int l = jj; l < ii; ++l) acc = acc + Ts[jj][l] * Gs[l][ii];

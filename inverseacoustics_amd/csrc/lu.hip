@@ -111,47 +111,90 @@ constexpr size_t LDS_CAP = 128 * 1024;
 constexpr int LNW = LBT / 64;
 constexpr int PERM_MAX = 2 * NB;
 
+// (value, row) of the first maximum over the wave, in every lane: the DPP
+// reduction pattern of the wave sums (quad permutes, half-row and row
+// mirrors, row broadcasts), masked-off lanes keeping their own pair.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ int dpp_keep(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_keep(float v) {
+    return __builtin_bit_cast(float, dpp_keep<CTRL, ROWS>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dpp_keep(double v) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = dpp_keep<CTRL, ROWS>((int)u), hi = dpp_keep<CTRL, ROWS>((int)(u >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float lane63(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+__device__ __forceinline__ double lane63(double v) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)u, 63), hi = __builtin_amdgcn_readlane((int)(u >> 32), 63);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, int ROWS = 0xF, typename T>
+__device__ __forceinline__ void argmax_step(T &best, int &bi) {
+    const T v2 = dpp_keep<CTRL, ROWS>(best);
+    const int i2 = dpp_keep<CTRL, ROWS>(bi);
+    if (v2 > best || (v2 == best && i2 < bi)) {
+        best = v2;
+        bi = i2;
+    }
+}
+template <typename T>
+__device__ __forceinline__ void argmax_dpp(T &best, int &bi) {
+    argmax_step<0xB1>(best, bi);
+    argmax_step<0x4E>(best, bi);
+    argmax_step<0x141>(best, bi);
+    argmax_step<0x140>(best, bi);
+    argmax_step<0x142, 0xA>(best, bi);
+    argmax_step<0x143, 0xC>(best, bi);
+    best = lane63(best);
+    bi = __builtin_amdgcn_readlane(bi, 63);
+}
+
 template <typename T, int SB>
 __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *A, int lda,
                                                   size_t strideA, int *ipiv, int *info,
                                                   int *permall) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *Vs = reinterpret_cast<cx<T> *>(smem_raw);  // SB columns of M rows
     __shared__ T red_v[LNW];
     __shared__ int red_i[LNW];
     __shared__ cx<T> pinv_s;
     __shared__ int prow[PERM_MAX], psrc[PERM_MAX];
+    __shared__ int ipl[NB];  // the panel's pivot rows (to global ipiv after each sub-panel)
+    __shared__ int zpiv;     // first zero pivot column + 1 of this panel, 0 if none
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     cx<T> *Ab = A + (size_t)b * strideA;
     int *ip = ipiv + (size_t)b * n;
     const int M = n - k;
+    if (t == 0) zpiv = 0;
     for (int cs = k; cs < k + jb; cs += SB) {
         const int w = min(SB, k + jb - cs);
         for (int i = 0; i < w; ++i)
             for (int r = t; r < M; r += LBT) Vs[(size_t)i * M + r] = Ab[(size_t)(cs + i) * lda + k + r];
         __syncthreads();
+        // pivot candidates of the sub-panel's first column; later columns'
+        // candidates are gathered by the update pass before them
+        T best = (T)-1;
+        int bi = M;
+        for (int r = cs - k + t; r < M; r += LBT) {
+            const T v = cabs1(Vs[r]);
+            if (v > best) {
+                best = v;
+                bi = r;
+            }
+        }
         for (int i = 0; i < w; ++i) {
             const int rc = cs + i - k;
             cx<T> *col = Vs + (size_t)i * M;
             // pivot: first maximum of |re| + |im| over rows rc..M-1
-            T best = (T)-1;
-            int bi = M;
-            for (int r = rc + t; r < M; r += LBT) {
-                const T v = cabs1(col[r]);
-                if (v > best) {
-                    best = v;
-                    bi = r;
-                }
-            }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const T v2 = __shfl_xor(best, o, 64);
-                const int i2 = __shfl_xor(bi, o, 64);
-                if (v2 > best || (v2 == best && i2 < bi)) {
-                    best = v2;
-                    bi = i2;
-                }
-            }
+            argmax_dpp(best, bi);
             if (lane == 0) {
                 red_v[wv] = best;
                 red_i[wv] = bi;
@@ -165,8 +208,8 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
                         bv = red_v[q];
                         bx = red_i[q];
                     }
-                ip[cs + i] = k + bx;
-                if (bv == (T)0 && info && info[b] == 0) info[b] = cs + i + 1;
+                ipl[cs - k + i] = k + bx;
+                if (bv == (T)0 && zpiv == 0) zpiv = cs + i + 1;
                 // swap rows rc and bx inside the sub-panel (LDS)
                 if (bx != rc)
                     for (int q = 0; q < w; ++q) {
@@ -179,6 +222,9 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
             }
             __syncthreads();
             const cx<T> pinv = pinv_s;
+            best = (T)-1;
+            bi = M;
+            cx<T> *nxt = col + M;  // column i + 1
             for (int r = rc + 1 + t; r < M; r += LBT) {
                 const cx<T> l = col[r] * pinv;
                 col[r] = l;
@@ -186,9 +232,17 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
                     cx<T> *cq = Vs + (size_t)q * M;
                     cq[r] = cq[r] - l * cq[rc];
                 }
+                if (i + 1 < w) {
+                    const T v = cabs1(nxt[r]);
+                    if (v > best) {
+                        best = v;
+                        bi = r;
+                    }
+                }
             }
             __syncthreads();
         }
+        if (t < w) ip[cs + t] = ipl[cs - k + t];
         for (int i = 0; i < w; ++i)
             for (int r = t; r < M; r += LBT) Ab[(size_t)(cs + i) * lda + k + r] = Vs[(size_t)i * M + r];
         // this sub-panel's interchanges on the panel's other columns
@@ -197,7 +251,7 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
             if (cc < cs || cc >= cs + w) {
                 cx<T> *Ac = Ab + (size_t)cc * lda;
                 for (int i = 0; i < w; ++i) {
-                    const int r1 = cs + i, r2 = ip[cs + i];
+                    const int r1 = cs + i, r2 = ipl[cs - k + i];
                     if (r2 != r1) {
                         const cx<T> tmp = Ac[r1];
                         Ac[r1] = Ac[r2];
@@ -248,7 +302,7 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
             return cnt++;
         };
         for (int c = k; c < k + jb; ++c) {
-            const int p = ip[c];
+            const int p = ipl[c - k];
             if (p == c) continue;
             const int a1 = slot(c), a2 = slot(p);
             const int tmp = src[a1];
@@ -263,6 +317,7 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
                 ++m;
             }
         pm[0] = m;
+        if (zpiv && info && info[b] == 0) info[b] = zpiv;
     }
 }
 
@@ -643,7 +698,7 @@ template <typename T>
 __global__ __launch_bounds__(G1T) void k_getrs1(int n, const cx<T> *__restrict__ Aall, int lda,
                                                size_t sA, const int *__restrict__ ipivall,
                                                cx<T> *__restrict__ Ball, size_t sB) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *xs = reinterpret_cast<cx<T> *>(smem_raw);  // n
     cx<T> *Ds = xs + n;                                // G1 x G1 block, column-major, ld G1 + 1
     int *ps = reinterpret_cast<int *>(Ds + (size_t)G1 * (G1 + 1));  // n pivots

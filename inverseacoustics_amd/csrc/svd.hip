@@ -515,7 +515,7 @@ __global__ __launch_bounds__(256) void k_apply_right(int C, const cx<T> *__restr
 template <typename T>
 __global__ __launch_bounds__(BT) void k_geqr2(int R, int C, cx<T> *__restrict__ Aall, int lda,
                                              size_t sA, cx<T> *__restrict__ tauall) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *u = reinterpret_cast<cx<T> *>(smem_raw);
     cx<T> *w = u + R;
     double *red = reinterpret_cast<double *>(w + C);
