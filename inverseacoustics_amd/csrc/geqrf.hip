@@ -99,11 +99,8 @@ __global__ __launch_bounds__(BT) void k_qr_panel(int R, int C, int j0, int nbp,
 // and the raw products x^H a of the later columns); one block reduction
 // then lets wave 0 form the reflector (LAPACK zlarfg), the update factors
 // f = conj(tau) u^H a with u^H a = a[rc] + conj(scale) x^H a, and the next
-// column of T (LAPACK larft, forward).  Two barriers per column.  FUSED
-// selects this form (complex64); the complex128 instantiation keeps the
-// two-phase form (norm, reflector, then one wave per later column; T from
-// the Gram matrix afterwards), which measured faster there.
-template <typename T, int SB, int RPL, int TB, bool FUSED>
+// column of T (LAPACK larft, forward).  Two barriers per column.
+template <typename T, int SB, int RPL, int TB>
 __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int nbp,
                                                     cx<T> *__restrict__ Yall, int ldy, size_t sY,
                                                     cx<T> *__restrict__ tauall,
@@ -121,8 +118,6 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
     __shared__ cx<double> Gc[SB];  // Gram column of the reflector just finished
     __shared__ cx<double> Ts[SB][SB];
     __shared__ cx<double> Zs[NW][SB];
-    __shared__ cx<double> Gs[FUSED ? 1 : SB][SB];  // Gram matrix (unfused form)
-    __shared__ double red1[NW];                     // block sums (unfused form)
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     cx<T> *A = Yall + (size_t)b * sY;
     cx<T> *tau = tauall + (size_t)b * C;
@@ -134,136 +129,99 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
         for (int i = 0; i < w; ++i)
             for (int r = t; r < M; r += BT) Vs[(size_t)i * M + r] = A[(size_t)(cs + i) * ldy + j0 + r];
         __syncthreads();
-        if constexpr (FUSED) {
-            // ---- unblocked QR of the sub-panel in LDS; pass i applies
-            // reflector i - 1 and gathers the sums of column i
-            for (int i = 0; i <= w; ++i) {
-                const int rp = r0 + i - 1;  // diagonal row of reflector i - 1
-                const int rc = rp + 1;      // diagonal row of column i
-                cx<T> scT{0, 0};
-                T beta = 0;
+        // ---- unblocked QR of the sub-panel in LDS; pass i applies
+        // reflector i - 1 and gathers the sums of column i
+        for (int i = 0; i <= w; ++i) {
+            const int rp = r0 + i - 1;  // diagonal row of reflector i - 1
+            const int rc = rp + 1;      // diagonal row of column i
+            cx<T> scT{0, 0};
+            T beta = 0;
+            if (i > 0) {
+                scT = cvt<T>(prm[0]);
+                beta = (T)prm[1].re;
+            }
+            T p = 0;
+            cx<T> sq[SB];
+#pragma unroll
+            for (int q = 0; q < SB; ++q) sq[q] = cx<T>{0, 0};
+            for (int r = t; r < M; r += BT) {
+                if (i > 0 && r >= rp) {
+                    cx<T> u{1, 0};
+                    if (r == rp) {
+                        Vs[(size_t)(i - 1) * M + r] = cx<T>{beta, 0};
+                    } else {
+                        u = Vs[(size_t)(i - 1) * M + r] * scT;
+                        Vs[(size_t)(i - 1) * M + r] = u;
+                    }
+#pragma unroll
+                    for (int q = 0; q < SB; ++q) {
+                        if (q >= i && q < w) {  // later columns: a -= u f
+                            cx<T> &a = Vs[(size_t)q * M + r];
+                            a = a - u * fs[q];
+                        }
+                        if (q < i - 1)  // Gram column: conj(u_q) u (u_q final below its unit)
+                            sq[q] = sq[q] + cmulc(Vs[(size_t)q * M + r], u);
+                    }
+                }
+                if (i < w && r > rc) {
+                    const cx<T> x = Vs[(size_t)i * M + r];
+                    p += abs2(x);
+#pragma unroll
+                    for (int q = 0; q < SB; ++q)
+                        if (q > i && q < w) sq[q] = sq[q] + cmulc(x, Vs[(size_t)q * M + r]);
+                }
+            }
+            const double pd = (double)hh::wave_sum_dpp(p);
+#pragma unroll
+            for (int q = 0; q < SB; ++q) {
+                double sr = 0, si = 0;
+                if (q < i - 1 || (q > i && q < w)) {
+                    sr = (double)hh::wave_sum_dpp(sq[q].re);
+                    si = (double)hh::wave_sum_dpp(sq[q].im);
+                }
+                if (lane == 0) {
+                    red[wv][1 + 2 * q] = sr;
+                    red[wv][2 + 2 * q] = si;
+                }
+            }
+            if (lane == 0) red[wv][0] = pd;
+            __syncthreads();
+            if (wv == 0) {
+                double v = 0;
+                if (lane < NRED)
+                    for (int q = 0; q < NW; ++q) v += red[q][lane];
+                const int ql = min(lane, SB - 1);
+                const double sre = __shfl(v, 1 + 2 * ql, 64), sim = __shfl(v, 2 + 2 * ql, 64);
+                // T column i - 1: T[j][i-1] = -tau_{i-1} sum_{l=j}^{i-2} T[j][l] G[l][i-1]
                 if (i > 0) {
-                    scT = cvt<T>(prm[0]);
-                    beta = (T)prm[1].re;
-                }
-                T p = 0;
-                cx<T> sq[SB];
-    #pragma unroll
-                for (int q = 0; q < SB; ++q) sq[q] = cx<T>{0, 0};
-                for (int r = t; r < M; r += BT) {
-                    if (i > 0 && r >= rp) {
-                        cx<T> u{1, 0};
-                        if (r == rp) {
-                            Vs[(size_t)(i - 1) * M + r] = cx<T>{beta, 0};
-                        } else {
-                            u = Vs[(size_t)(i - 1) * M + r] * scT;
-                            Vs[(size_t)(i - 1) * M + r] = u;
-                        }
-    #pragma unroll
-                        for (int q = 0; q < SB; ++q) {
-                            if (q >= i && q < w) {  // later columns: a -= u f
-                                cx<T> &a = Vs[(size_t)q * M + r];
-                                a = a - u * fs[q];
-                            }
-                            if (q < i - 1)  // Gram column: conj(u_q) u (u_q final below its unit)
-                                sq[q] = sq[q] + cmulc(Vs[(size_t)q * M + r], u);
-                        }
+                    if (lane < i - 1) Gc[lane] = cx<double>{sre, sim};
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const cx<double> tp = taus[i - 1];
+                    if (lane < i - 1) {
+                        cx<double> acc{0, 0};
+                        for (int l = lane; l < i - 1; ++l) acc = acc + Ts[lane][l] * Gc[l];
+                        Ts[lane][i - 1] = cx<double>{0, 0} - tp * acc;
                     }
-                    if (i < w && r > rc) {
-                        const cx<T> x = Vs[(size_t)i * M + r];
-                        p += abs2(x);
-    #pragma unroll
-                        for (int q = 0; q < SB; ++q)
-                            if (q > i && q < w) sq[q] = sq[q] + cmulc(x, Vs[(size_t)q * M + r]);
-                    }
+                    if (lane == i - 1) Ts[lane][lane] = tp;
                 }
-                const double pd = (double)hh::wave_sum_dpp(p);
-    #pragma unroll
-                for (int q = 0; q < SB; ++q) {
-                    double sr = 0, si = 0;
-                    if (q < i - 1 || (q > i && q < w)) {
-                        sr = (double)hh::wave_sum_dpp(sq[q].re);
-                        si = (double)hh::wave_sum_dpp(sq[q].im);
+                if (i < w) {
+                    const double xn2 = __shfl(v, 0, 64);
+                    const Refl rf = larfg(cvt<double>(Vs[(size_t)i * M + rc]), xn2);
+                    if (lane > i && lane < w) {
+                        const cx<double> d = cvt<double>(Vs[(size_t)lane * M + rc]) +
+                                             conj(rf.scale) * cx<double>{sre, sim};
+                        fs[lane] = cvt<T>(conj(rf.tau) * d);
                     }
                     if (lane == 0) {
-                        red[wv][1 + 2 * q] = sr;
-                        red[wv][2 + 2 * q] = si;
+                        prm[0] = rf.scale;
+                        prm[1] = cx<double>{rf.beta, 0};
+                        taus[i] = rf.tau;
                     }
                 }
-                if (lane == 0) red[wv][0] = pd;
-                __syncthreads();
-                if (wv == 0) {
-                    double v = 0;
-                    if (lane < NRED)
-                        for (int q = 0; q < NW; ++q) v += red[q][lane];
-                    const int ql = min(lane, SB - 1);
-                    const double sre = __shfl(v, 1 + 2 * ql, 64), sim = __shfl(v, 2 + 2 * ql, 64);
-                    // T column i - 1: T[j][i-1] = -tau_{i-1} sum_{l=j}^{i-2} T[j][l] G[l][i-1]
-                    if (i > 0) {
-                        if (lane < i - 1) Gc[lane] = cx<double>{sre, sim};
-                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                        const cx<double> tp = taus[i - 1];
-                        if (lane < i - 1) {
-                            cx<double> acc{0, 0};
-                            for (int l = lane; l < i - 1; ++l) acc = acc + Ts[lane][l] * Gc[l];
-                            Ts[lane][i - 1] = cx<double>{0, 0} - tp * acc;
-                        }
-                        if (lane == i - 1) Ts[lane][lane] = tp;
-                    }
-                    if (i < w) {
-                        const double xn2 = __shfl(v, 0, 64);
-                        const Refl rf = larfg(cvt<double>(Vs[(size_t)i * M + rc]), xn2);
-                        if (lane > i && lane < w) {
-                            const cx<double> d = cvt<double>(Vs[(size_t)lane * M + rc]) +
-                                                 conj(rf.scale) * cx<double>{sre, sim};
-                            fs[lane] = cvt<T>(conj(rf.tau) * d);
-                        }
-                        if (lane == 0) {
-                            prm[0] = rf.scale;
-                            prm[1] = cx<double>{rf.beta, 0};
-                            taus[i] = rf.tau;
-                        }
-                    }
-                }
-                __syncthreads();
             }
-        } else {
-            // ---- unblocked QR of the sub-panel in LDS
-            for (int i = 0; i < w; ++i) {
-                const int rc = cs + i - j0;  // local row of the diagonal
-                cx<T> *col = Vs + (size_t)i * M;
-                double p = 0;
-                for (int r = rc + 1 + t; r < M; r += BT) p += (double)abs2(col[r]);
-                const double xn2 = block_sum(p, red1);
-                const Refl rf = larfg(cvt<double>(col[rc]), xn2);
-                __syncthreads();  // col[rc] read by all before it is overwritten
-                const cx<T> sc = cvt<T>(rf.scale);
-                for (int r = rc + 1 + t; r < M; r += BT) col[r] = col[r] * sc;
-                if (t == 0) {
-                    col[rc] = cx<T>{(T)rf.beta, 0};
-                    taus[i] = rf.tau;
-                }
-                __syncthreads();
-                const cx<double> ctau = conj(rf.tau);
-                for (int ip = i + 1 + wv; ip < w; ip += NW) {
-                    cx<T> *cp = Vs + (size_t)ip * M;
-                    double ar = 0, ai = 0;
-                    for (int r = rc + lane; r < M; r += 64) {
-                        const cx<T> uu = (r == rc) ? cx<T>{1, 0} : col[r];
-                        const cx<T> y = cp[r];
-                        ar += (double)uu.re * y.re + (double)uu.im * y.im;
-                        ai += (double)uu.re * y.im - (double)uu.im * y.re;
-                    }
-                    const cx<double> f = ctau * cx<double>{wave_sum(ar), wave_sum(ai)};
-                    for (int r = rc + lane; r < M; r += 64) {
-                        const cx<double> uu = (r == rc) ? cx<double>{1, 0} : cvt<double>(col[r]);
-                        cp[r] = cvt<T>(cvt<double>(cp[r]) - uu * f);
-                    }
-                }
-                __syncthreads();
-            }
+            __syncthreads();
         }
         // ---- write back: factored columns and explicit reflectors
         // (tau leaves LDS here: a global store inside the column loop would
@@ -280,37 +238,6 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
         }
         const int k0 = cs + w, k1 = j0 + nbp;
         if (k0 < k1) {
-            if constexpr (!FUSED) {
-                // ---- T of the sub-panel (LAPACK larft, forward): G(j, i) = v_j^H v_i
-                for (int pr = wv; pr < w * w; pr += NW) {
-                    const int jj = pr / w, ii = pr % w;
-                    if (jj >= ii) continue;
-                    double gr = 0, gi = 0;
-                    for (int r = r0 + ii + lane; r < M; r += 64) {
-                        const cx<T> vi = (r == r0 + ii) ? cx<T>{1, 0} : Vs[(size_t)ii * M + r];
-                        const cx<T> vj = (r < r0 + jj) ? cx<T>{0, 0}
-                                         : (r == r0 + jj) ? cx<T>{1, 0} : Vs[(size_t)jj * M + r];
-                        gr += (double)vj.re * vi.re + (double)vj.im * vi.im;
-                        gi += (double)vj.re * vi.im - (double)vj.im * vi.re;
-                    }
-                    gr = wave_sum(gr);
-                    gi = wave_sum(gi);
-                    if (lane == 0) Gs[jj][ii] = cx<double>{gr, gi};
-                }
-                __syncthreads();
-                if (t == 0) {
-                    for (int ii = 0; ii < w; ++ii) {
-                        const cx<double> ti = taus[ii];
-                        for (int jj = 0; jj < ii; ++jj) {
-                            cx<double> acc{0, 0};
-                            for (int l = jj; l < ii; ++l) acc = acc + Ts[jj][l] * Gs[l][ii];
-                            Ts[jj][ii] = cx<double>{0, 0} - ti * acc;
-                        }
-                        Ts[ii][ii] = ti;
-                    }
-                }
-                __syncthreads();
-            }
             // ---- remaining panel columns: a <- a - V T^H (V^H a), one wave per column.
             // The first 64 RPL rows of the column stay in registers between
             // the two passes (all their loads in flight at once); rows beyond
@@ -454,11 +381,11 @@ void launch_qr_panel(int R, int C, int j0, int nbp, cx<T> *Y, int ldy, size_t sY
     // (sizes that keep every variant within 128 VGPRs without spills)
     constexpr bool F = sizeof(T) == 4;
     if (colb * 8 <= cap)
-        go(k_qr_panel_lds<T, 8, F ? 12 : 6, F ? 3 : 2, F>, 8);
+        go(k_qr_panel_lds<T, 8, F ? 12 : 6, F ? 3 : 2>, 8);
     else if (colb * 4 <= cap)
-        go(k_qr_panel_lds<T, 4, F ? 12 : 8, F ? 3 : 2, F>, 4);
+        go(k_qr_panel_lds<T, 4, F ? 12 : 8, F ? 3 : 2>, 4);
     else if (colb * 2 <= cap)
-        go(k_qr_panel_lds<T, 2, F ? 12 : 8, F ? 3 : 2, F>, 2);
+        go(k_qr_panel_lds<T, 2, F ? 12 : 8, F ? 3 : 2>, 2);
     else
         hipLaunchKernelGGL(k_qr_panel<T>, dim3(batch), dim3(BT), 0, s, R, C, j0, nbp, Y, ldy, sY,
                            tau, Vw, vcols);
