@@ -52,10 +52,13 @@ __device__ __forceinline__ int sturm_count(int N, const double *t2, double x, do
     return cnt;
 }
 
-__global__ __launch_bounds__(256) void k_bisect(int n, const double *__restrict__ dall,
-                                               const double *__restrict__ eall,
-                                               double *__restrict__ sall) {
+// One workgroup per matrix (1024 threads, one value each for n <= 1024).
+constexpr int GL = 10, GPTS = 1 << GL;
+__global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict__ dall,
+                                                const double *__restrict__ eall,
+                                                double *__restrict__ sall) {
     extern __shared__ double t2[];
+    __shared__ int cgrid[GPTS];
     __shared__ double s_bound, s_pivmin;
     const int b = blockIdx.y;
     const double *d = dall + (size_t)b * n;
@@ -73,11 +76,11 @@ __global__ __launch_bounds__(256) void k_bisect(int n, const double *__restrict_
         double g2 = fabs(d[i]) + (i < n - 1 ? fabs(e[i]) : 0.0);
         gmax = fmax(gmax, fmax(g, g2));
     }
-    __shared__ double red[2][256];
+    __shared__ double red[2][1024];
     red[0][t] = lmax;
     red[1][t] = gmax;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = 512; s > 0; s >>= 1) {
         if (t < s) {
             red[0][t] = fmax(red[0][t], red[0][t + s]);
             red[1][t] = fmax(red[1][t], red[1][t + s]);
@@ -89,22 +92,36 @@ __global__ __launch_bounds__(256) void k_bisect(int n, const double *__restrict_
         s_pivmin = fmax(red[0][0] * 2.2250738585072014e-308, 1e-300);
     }
     __syncthreads();
-    const int i = blockIdx.x * blockDim.x + t;
-    if (i >= n) return;
     const double bound = s_bound, pivmin = s_pivmin;
+    // the first GL levels of every value's bisection in one step: counts at
+    // the interior points bound j / 2^GL of the dyadic grid, one per thread
+    for (int j = t + 1; j < GPTS; j += blockDim.x)
+        cgrid[j] = sturm_count(N, t2, bound * ((double)j / GPTS), pivmin) - n;
+    __syncthreads();
     const double eps = 2.220446049250313e-16;
-    const int target = n - 1 - i;  // sigma_i (descending) has target values below it
-    double lo = 0.0, hi = bound;
-    for (int it = 0; it < 128; ++it) {
-        if (hi - lo <= fmax(2.0 * eps * hi, 1e-3 * eps * bound)) break;
-        double mid = 0.5 * (lo + hi);
-        int c = sturm_count(N, t2, mid, pivmin) - n;
-        if (c > target)
-            hi = mid;
-        else
-            lo = mid;
+    for (int i = t; i < n; i += blockDim.x) {
+        const int target = n - 1 - i;  // sigma_i (descending) has target values below it
+        // first grid point with more than target values below it (GPTS if none)
+        int jl = 0, jh = GPTS;  // cgrid(jl) <= target < cgrid(jh), cgrid(0) = 0, cgrid(GPTS) = n
+        while (jh - jl > 1) {
+            const int jm = (jl + jh) >> 1;
+            if (cgrid[jm] > target)
+                jh = jm;
+            else
+                jl = jm;
+        }
+        double lo = bound * ((double)jl / GPTS), hi = jh == GPTS ? bound : bound * ((double)jh / GPTS);
+        for (int it = 0; it < 128; ++it) {
+            if (hi - lo <= fmax(2.0 * eps * hi, 1e-3 * eps * bound)) break;
+            double mid = 0.5 * (lo + hi);
+            int c = sturm_count(N, t2, mid, pivmin) - n;
+            if (c > target)
+                hi = mid;
+            else
+                lo = mid;
+        }
+        sall[(size_t)b * n + i] = 0.5 * (lo + hi);
     }
-    sall[(size_t)b * n + i] = 0.5 * (lo + hi);
 }
 
 // ----------------------------------------------------- inverse iteration
@@ -577,8 +594,7 @@ void launch_bisect(int n, const double *d, const double *e, double *sigma, int b
     size_t lds = sizeof(double) * (2 * (size_t)n);
     (void)hipFuncSetAttribute((const void *)k_bisect, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
-    dim3 grid((n + 255) / 256, batch);
-    hipLaunchKernelGGL(k_bisect, grid, dim3(256), lds, s, n, d, e, sigma);
+    hipLaunchKernelGGL(k_bisect, dim3(1, batch), dim3(1024), lds, s, n, d, e, sigma);
 }
 
 // scratch columns: 64 per wave, (n + 63) / 64 waves per matrix
