@@ -28,48 +28,55 @@ __device__ __forceinline__ void sincos_t<double>(double x, double *s, double *c)
 template <>
 __device__ __forceinline__ void sincos_t<float>(float x, float *s, float *c) { sincosf(x, s, c); }
 
-template <typename T>
+template <typename T, bool HERM>
 __global__ __launch_bounds__(256) void k_esm_D(int m, const double *__restrict__ kv,
                                               const double *__restrict__ r_s,
                                               const double *__restrict__ src,
                                               const double *__restrict__ nrm,
                                               cx<T> *__restrict__ D, size_t strideD) {
-    __shared__ T sq[3][TILE];
+    // D[i][j] (surface point i, source j) at D[j m + i]; HERM stores the
+    // conjugate transpose, D^H[j][i] at D[i m + j].  The 64 lanes of a wave
+    // walk the storage-contiguous index (i, or j for HERM) so that every
+    // store is coalesced; the other index's coordinates are staged in LDS.
+    __shared__ T sq[6][TILE];
     const int f = blockIdx.z;
     const int i0 = blockIdx.x * TILE, j0 = blockIdx.y * TILE;
     const int t = threadIdx.x;
     if (t < TILE) {
-        int j = j0 + t;
-        if (j < m) {
-            sq[0][t] = (T)src[j];
-            sq[1][t] = (T)src[j + m];
-            sq[2][t] = (T)src[j + 2 * m];
-        } else {
-            sq[0][t] = sq[1][t] = sq[2][t] = (T)0;
+        if (!HERM) {  // sources of the tile
+            const int j = j0 + t;
+            for (int q = 0; q < 3; ++q) sq[q][t] = j < m ? (T)src[j + q * m] : (T)0;
+        } else {  // surface points and normals of the tile
+            const int i = i0 + t;
+            T x = 0, y = 0, z = 0, nx = 0, ny = 0, nz = 0;
+            if (i < m) {
+                x = (T)r_s[i];
+                y = (T)r_s[i + m];
+                z = (T)r_s[i + 2 * m];
+                if (nrm) {
+                    nx = (T)nrm[i];
+                    ny = (T)nrm[i + m];
+                    nz = (T)nrm[i + 2 * m];
+                } else {
+                    const T rn = sqrt(x * x + y * y + z * z);
+                    nx = x / rn;
+                    ny = y / rn;
+                    nz = z / rn;
+                }
+            }
+            sq[0][t] = x;
+            sq[1][t] = y;
+            sq[2][t] = z;
+            sq[3][t] = nx;
+            sq[4][t] = ny;
+            sq[5][t] = nz;
         }
     }
     __syncthreads();
-    const int i = i0 + (t & (TILE - 1));
-    if (i >= m) return;
     const T k = (T)kv[f];
-    const T x = (T)r_s[i], y = (T)r_s[i + m], z = (T)r_s[i + 2 * m];
-    T nx, ny, nz;
-    if (nrm) {
-        nx = (T)nrm[i];
-        ny = (T)nrm[i + m];
-        nz = (T)nrm[i + 2 * m];
-    } else {
-        T rn = sqrt(x * x + y * y + z * z);
-        nx = x / rn;
-        ny = y / rn;
-        nz = z / rn;
-    }
     cx<T> *Df = D + (size_t)f * strideD;
-    const int jb = (t >> 6) * 16;
-    for (int jj = 0; jj < 16; ++jj) {
-        const int jl = jb + jj, j = j0 + jl;
-        if (j >= m) break;
-        T dx = x - sq[0][jl], dy = y - sq[1][jl], dz = z - sq[2][jl];
+    auto entry = [&](T x, T y, T z, T nx, T ny, T nz, T sx, T sy, T sz) {
+        T dx = x - sx, dy = y - sy, dz = z - sz;
         T R2 = dx * dx + dy * dy + dz * dz;
         T R = sqrt(R2);
         T cth = (nx * dx + ny * dy + nz * dz) / R;
@@ -78,8 +85,39 @@ __global__ __launch_bounds__(256) void k_esm_D(int m, const double *__restrict__
         // ((ikR - 1) e^{ikR}) = (-1 + i kR)(cs + i sn) = (-cs - kR sn) + i(kR cs - sn)
         T kR = k * R;
         T scale = (T)INV4PI * cth / R2;
-        cx<T> v{(-cs - kR * sn) * scale, (kR * cs - sn) * scale};
-        Df[(size_t)j * m + i] = v;
+        return cx<T>{(-cs - kR * sn) * scale, (kR * cs - sn) * scale};
+    };
+    const int lb = (t >> 6) * 16;
+    if (!HERM) {
+        const int i = i0 + (t & (TILE - 1));
+        if (i >= m) return;
+        const T x = (T)r_s[i], y = (T)r_s[i + m], z = (T)r_s[i + 2 * m];
+        T nx, ny, nz;
+        if (nrm) {
+            nx = (T)nrm[i];
+            ny = (T)nrm[i + m];
+            nz = (T)nrm[i + 2 * m];
+        } else {
+            T rn = sqrt(x * x + y * y + z * z);
+            nx = x / rn;
+            ny = y / rn;
+            nz = z / rn;
+        }
+        for (int jj = 0; jj < 16; ++jj) {
+            const int jl = lb + jj, j = j0 + jl;
+            if (j >= m) break;
+            Df[(size_t)j * m + i] = entry(x, y, z, nx, ny, nz, sq[0][jl], sq[1][jl], sq[2][jl]);
+        }
+    } else {
+        const int j = j0 + (t & (TILE - 1));
+        if (j >= m) return;
+        const T sx = (T)src[j], sy = (T)src[j + m], sz = (T)src[j + 2 * m];
+        for (int ii = 0; ii < 16; ++ii) {
+            const int il = lb + ii, i = i0 + il;
+            if (i >= m) break;
+            Df[(size_t)i * m + j] = conj(entry(sq[0][il], sq[1][il], sq[2][il], sq[3][il], sq[4][il],
+                                               sq[5][il], sx, sy, sz));
+        }
     }
 }
 
@@ -190,9 +228,14 @@ __global__ __launch_bounds__(256) void k_esm_msq_apply(int m, const double *__re
 
 template <typename T>
 void launch_esm_D(int m, int nf, const double *k, const double *r_s, const double *src,
-                  const double *nrm, cx<T> *D, size_t strideD, hipStream_t s) {
+                  const double *nrm, cx<T> *D, size_t strideD, hipStream_t s, bool herm) {
     dim3 grid((m + TILE - 1) / TILE, (m + TILE - 1) / TILE, nf);
-    hipLaunchKernelGGL(k_esm_D<T>, grid, dim3(256), 0, s, m, k, r_s, src, nrm, D, strideD);
+    if (herm)
+        hipLaunchKernelGGL((k_esm_D<T, true>), grid, dim3(256), 0, s, m, k, r_s, src, nrm, D,
+                           strideD);
+    else
+        hipLaunchKernelGGL((k_esm_D<T, false>), grid, dim3(256), 0, s, m, k, r_s, src, nrm, D,
+                           strideD);
 }
 
 template <typename T>
@@ -220,9 +263,9 @@ void launch_esm_msq_apply(int m, int nf, const double *k, double rho_c, const do
 }
 
 template void launch_esm_D<double>(int, int, const double *, const double *, const double *,
-                                   const double *, cx<double> *, size_t, hipStream_t);
+                                   const double *, cx<double> *, size_t, hipStream_t, bool);
 template void launch_esm_D<float>(int, int, const double *, const double *, const double *,
-                                  const double *, cx<float> *, size_t, hipStream_t);
+                                  const double *, cx<float> *, size_t, hipStream_t, bool);
 template void launch_esm_M<double>(int, int, int, const double *, double, const double *,
                                    const double *, cx<double> *, size_t, bool, hipStream_t);
 template void launch_esm_M<float>(int, int, int, const double *, double, const double *,

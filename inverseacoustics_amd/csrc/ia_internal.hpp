@@ -119,9 +119,10 @@ ia_status to_device(DevBuf &dst, const void *src, size_t n, hipStream_t s);
 // batched with element strides.  T is double or float.
 
 // ESM assembly (esm.hip)
+// herm: store D^H instead of D (same column-major layout)
 template <typename T>
 void launch_esm_D(int m, int nf, const double *k, const double *r_s, const double *src,
-                  const double *nrm, cx<T> *D, size_t strideD, hipStream_t s);
+                  const double *nrm, cx<T> *D, size_t strideD, hipStream_t s, bool herm = false);
 template <typename T>
 void launch_esm_M(int rows, int m, int nf, const double *k, double rho_c, const double *r_rows,
                   const double *src, cx<T> *M, size_t strideM, bool conj_transpose, hipStream_t s);

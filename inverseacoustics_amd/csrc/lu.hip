@@ -671,6 +671,41 @@ void trsm_blocked(int n, int nrhs, const cx<T> *A, int lda, size_t sA, cx<T> *B,
     }
 }
 
+// wave sum (every lane active) over the DPP lane network; result in every lane
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ int dpp_zero(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dppz_f(float v) {
+    return __builtin_bit_cast(float, dpp_zero<CTRL, ROWS>(__builtin_bit_cast(int, v)));
+}
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ double dppz_d(double v) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = dpp_zero<CTRL, ROWS>((int)u), hi = dpp_zero<CTRL, ROWS>((int)(u >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+template <typename R>
+__device__ __forceinline__ R hh_wave_sum(R v) {
+    if constexpr (sizeof(R) == 4) {
+        v += dppz_f<0xB1>(v);
+        v += dppz_f<0x4E>(v);
+        v += dppz_f<0x141>(v);
+        v += dppz_f<0x140>(v);
+        v += dppz_f<0x142, 0xA>(v);
+        v += dppz_f<0x143, 0xC>(v);
+    } else {
+        v += dppz_d<0xB1>(v);
+        v += dppz_d<0x4E>(v);
+        v += dppz_d<0x141>(v);
+        v += dppz_d<0x140>(v);
+        v += dppz_d<0x142, 0xA>(v);
+        v += dppz_d<0x143, 0xC>(v);
+    }
+    return lane63(v);
+}
+
 // One right-hand side, op 'N' (LAPACK getrs): b <- U^-1 L^-1 P b in a single
 // launch, one workgroup per matrix with b resident in LDS throughout.  The
 // interchanges are applied in order by one thread in LDS; each triangle is
@@ -777,6 +812,103 @@ __global__ __launch_bounds__(G1T) void k_getrs1(int n, const cx<T> *__restrict__
     for (int r = t; r < n; r += G1T) B[r] = xs[r];
 }
 
+// op 'C', one right-hand side: x <- D^-1 b for A = P D^H factored as L U,
+// D = U^H L^H P: forward with U^H (lower, non-unit), backward with L^H
+// (upper, unit), then the interchanges in reverse order.  Row r of U^H is
+// column r of U (contiguous), so the update of the rest of b after a block
+// gives each row to one wave: lanes read the block's 64 entries of that
+// column in one coalesced access and a DPP sum forms x_r's correction.
+template <typename T>
+__device__ __forceinline__ cx<T> dot_row_c(const cx<T> *__restrict__ a, const cx<T> xl, int lane,
+                                           int jb) {
+    const cx<T> v = lane < jb ? cmulc(a[lane], xl) : cx<T>{0, 0};
+    return cx<T>{hh_wave_sum(v.re), hh_wave_sum(v.im)};
+}
+
+template <typename T>
+__global__ __launch_bounds__(G1T) void k_getrs1c(int n, const cx<T> *__restrict__ Aall, int lda,
+                                                size_t sA, const int *__restrict__ ipivall,
+                                                cx<T> *__restrict__ Ball, size_t sB) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *xs = reinterpret_cast<cx<T> *>(smem_raw);  // n
+    cx<T> *Ds = xs + n;                                // op(A) block: Ds[j LD + l] = op[l][j]
+    int *ps = reinterpret_cast<int *>(Ds + (size_t)G1 * (G1 + 1));  // n pivots
+    constexpr int LD = G1 + 1, NWV = G1T / 64;
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const cx<T> *A = Aall + (size_t)b * sA;
+    cx<T> *B = Ball + (size_t)b * sB;
+    const int *ip = ipivall + (size_t)b * n;
+    for (int r = t; r < n; r += G1T) {
+        xs[r] = B[r];
+        ps[r] = ip[r];
+    }
+    // op(A)[i0 + l][i0 + j] = conj(A[i0 + j][i0 + l]), read along j
+    auto stage = [&](int i0, int jb) {
+        for (int e = t; e < jb * jb; e += G1T) {
+            const int j = e % jb, l = e / jb;
+            Ds[j * LD + l] = conj(A[(size_t)(i0 + l) * lda + i0 + j]);
+        }
+    };
+    // rows [r0, r1) of b lose op(A)[r, blk] x_blk
+    auto update = [&](int i0, int jb, int r0, int r1) {
+        const cx<T> xl = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
+#pragma unroll 4
+        for (int r = r0 + wv; r < r1; r += NWV) {
+            const cx<T> d = dot_row_c(A + (size_t)r * lda + i0, xl, lane, jb);
+            if (lane == 0) xs[r] = xs[r] - d;
+        }
+    };
+    __syncthreads();
+    // ---- U^H (lower, non-unit), top to bottom
+    for (int i0 = 0; i0 < n; i0 += G1) {
+        const int jb = min(G1, n - i0);
+        stage(i0, jb);
+        __syncthreads();
+        if (wv == 0) {
+            cx<T> x = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
+            for (int j = 0; j < jb; ++j) {
+                if (lane == j) x = x * cdiv(cx<T>{1, 0}, Ds[j * LD + j]);
+                const cx<T> xj = shfl_cx(x, j);
+                if (lane > j && lane < jb) x = x - Ds[j * LD + lane] * xj;
+            }
+            if (lane < jb) xs[i0 + lane] = x;
+        }
+        __syncthreads();
+        update(i0, jb, i0 + jb, n);
+        __syncthreads();
+    }
+    // ---- L^H (upper, unit), bottom to top
+    const int nblk = (n + G1 - 1) / G1;
+    for (int bk = nblk - 1; bk >= 0; --bk) {
+        const int i0 = bk * G1, jb = min(G1, n - i0);
+        stage(i0, jb);
+        __syncthreads();
+        if (wv == 0) {
+            cx<T> x = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
+            for (int j = jb - 1; j >= 0; --j) {
+                const cx<T> xj = shfl_cx(x, j);
+                if (lane < j) x = x - Ds[j * LD + lane] * xj;
+            }
+            if (lane < jb) xs[i0 + lane] = x;
+        }
+        __syncthreads();
+        update(i0, jb, 0, i0);
+        __syncthreads();
+    }
+    // x = P^T z: the interchanges undone in reverse order
+    if (t == 0)
+        for (int c = n - 1; c >= 0; --c) {
+            const int p = ps[c];
+            if (p != c) {
+                const cx<T> tmp = xs[c];
+                xs[c] = xs[p];
+                xs[p] = tmp;
+            }
+        }
+    __syncthreads();
+    for (int r = t; r < n; r += G1T) B[r] = xs[r];
+}
+
 template <typename T>
 size_t getrs1_lds(int n) {
     return sizeof(cx<T>) * ((size_t)n + (size_t)G1 * (G1 + 1)) + sizeof(int) * (size_t)n;
@@ -860,12 +992,16 @@ ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_
                         const int *ipiv, cx<T> *B, int ldb, size_t strideB, int batch,
                         hipStream_t s) {
     if (n <= 0 || nrhs <= 0 || batch <= 0) return IA_OK;
-    if (op == 'N' && nrhs == 1 && getrs1_lds<T>(n) <= 160 * 1024) {
+    if (nrhs == 1 && getrs1_lds<T>(n) <= 160 * 1024) {
         const size_t lds = getrs1_lds<T>(n);
-        (void)hipFuncSetAttribute((const void *)k_getrs1<T>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_getrs1<T>, dim3(batch), dim3(G1T), lds, s, n, A, lda, strideA, ipiv, B,
-                           strideB);
+        const void *kern = op == 'N' ? (const void *)k_getrs1<T> : (const void *)k_getrs1c<T>;
+        (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (op == 'N')
+            hipLaunchKernelGGL(k_getrs1<T>, dim3(batch), dim3(G1T), lds, s, n, A, lda, strideA,
+                               ipiv, B, strideB);
+        else
+            hipLaunchKernelGGL(k_getrs1c<T>, dim3(batch), dim3(G1T), lds, s, n, A, lda, strideA,
+                               ipiv, B, strideB);
         IA_HIP(hipGetLastError());
         return IA_OK;
     }

@@ -67,18 +67,20 @@ def test_zgesv(ia, n, nrhs, op):
 
 
 @pytest.mark.parametrize("n", [2, 63, 64, 65, 130, 1000, 3000])
-def test_zgesv_single_rhs_general(ia, n):
-    # one right-hand side (the in-LDS solve), no diagonal dominance so that
-    # rows are interchanged throughout; backward-error bound of LU with
-    # partial pivoting, and agreement with the multi-RHS path
+@pytest.mark.parametrize("op", ["N", "C"])
+def test_zgesv_single_rhs_general(ia, n, op):
+    # one right-hand side (the in-LDS solves of op N and op C), no diagonal
+    # dominance so that rows are interchanged throughout; backward-error
+    # bound of LU with partial pivoting, and agreement with the multi-RHS path
     rng = np.random.default_rng(n)
     D = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
     B = rng.standard_normal(n) + 1j * rng.standard_normal(n)
-    X = ia.zgesv(D, B)
+    X = ia.zgesv(D, B, op)
     assert X.shape == (n,)
-    res = np.abs(D @ X - B).max()
+    Dop = D if op == "N" else D.conj().T
+    res = np.abs(Dop @ X - B).max()
     assert res <= 1e-13 * n * np.abs(D).max() * np.abs(X).max()
-    X2 = ia.zgesv(D, np.stack([B, B], axis=1))
+    X2 = ia.zgesv(D, np.stack([B, B], axis=1), op)
     np.testing.assert_allclose(X, X2[:, 0], rtol=0, atol=1e-9 * np.abs(X).max())
 
 
