@@ -173,7 +173,8 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
         launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(), nf, s);
         tm.mark();  // 5
         launch_inviter(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(),
-                       sw.QBt.template as<double>(), sw.PBt.template as<double>(), sw.work.template as<double>(), sw.sub, nf, s);
+                       sw.QBt.template as<double>(), sw.PBt.template as<double>(), sw.work.p, sw.sub, nf, s,
+                       sizeof(T) == sizeof(float));
         tm.mark();  // 6
         // beta = Q_B^T V_b^H b
         const cx<double> *bf = b + (size_t)f0 * l;

@@ -240,10 +240,12 @@ ia_status launch_gebrd(int R, int C, cx<T> *Y, int ldy, size_t sY, double *d, do
                        hipStream_t s, KernelClock *clk = nullptr);
 void launch_bisect(int n, const double *d, const double *e, double *sigma, int batch,
                    hipStream_t s);
-size_t inviter_work_bytes(int n, int sub);
+// f32: the scratch holds the factorization and iterate in float (the
+// complex64 form), else in double
+size_t inviter_work_bytes(int n, int sub, bool f32 = false);
 // QBt/PBt: n x n per batch item, vector i stored as row i (QBt[i + k n]).
 void launch_inviter(int n, const double *d, const double *e, const double *sigma, double *QBt,
-                    double *PBt, double *work, int sub, int batch, hipStream_t s);
+                    double *PBt, void *work, int sub, int batch, hipStream_t s, bool f32 = false);
 // v <- U_b v (herm=false) or U_b^H v, length-R vectors, nvec per batch item
 template <typename T>
 void launch_apply_left(bool herm, int R, int C, const cx<T> *Y, int ldy, size_t strideY,
@@ -392,8 +394,9 @@ struct SvdWork {
         // quarter of the free device memory holds (1..32 GiB), so one launch
         // covers a whole 256-frequency batch (one thread per singular value
         // needs >= 1024 * 256 threads to fill the device)
-        size_t per = inviter_work_bytes(C, 1);
-        if (work.bytes >= inviter_work_bytes(C, batch)) {
+        constexpr bool f32 = sizeof(T) == sizeof(float);
+        size_t per = inviter_work_bytes(C, 1, f32);
+        if (work.bytes >= inviter_work_bytes(C, batch, f32)) {
             sub = batch;  // a kept buffer already holds the whole batch
         } else {
             size_t fr = 0, tot = 0;
@@ -405,7 +408,7 @@ struct SvdWork {
                                                 std::max<size_t>((size_t)1 << 30, fr / 4));
             sub = (int)std::max<size_t>(1, std::min<size_t>((size_t)batch, cap / per));
         }
-        IA_TRY(work.alloc(inviter_work_bytes(C, sub)));
+        IA_TRY(work.alloc(inviter_work_bytes(C, sub, f32)));
         return IA_OK;
     }
 };
@@ -448,7 +451,8 @@ inline ia_status svd_factor(SvdWork<T> &w, cx<T> *Y, hipStream_t s) {
     IA_HIP(hipGetLastError());
     launch_inviter(C, w.d.template as<double>(), w.e.template as<double>(),
                    w.sig.template as<double>(), w.QBt.template as<double>(),
-                   w.PBt.template as<double>(), w.work.template as<double>(), w.sub, batch, s);
+                   w.PBt.template as<double>(), w.work.p, w.sub, batch, s,
+                   sizeof(T) == sizeof(float));
     IA_HIP(hipGetLastError());
     return IA_OK;
 }

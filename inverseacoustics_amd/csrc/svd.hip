@@ -149,11 +149,16 @@ constexpr int IU = 6;
 
 // Four waves per SIMD (at most 128 VGPRs): 256 frequencies x 1024 singular
 // values are 4096 single-wave workgroups, one round on 1024 SIMDs.
+// S: storage type of the factorization and the iterate (the arithmetic is
+// FP64 either way).  float halves the scratch traffic, which bounds the
+// kernel; the mixed form uses it, its bidiagonal carrying complex64
+// rounding of the same order already.
+template <typename S>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_inviter(int n, int b0, const double *__restrict__ dall,
                                                const double *__restrict__ eall,
                                                const double *__restrict__ sall,
                                                double *__restrict__ QBt, double *__restrict__ PBt,
-                                               double *__restrict__ work, int G) {
+                                               void *__restrict__ workv, int G) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int bl = blockIdx.y;       // local batch index within this call
     const int b = b0 + bl;
@@ -164,18 +169,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const double *d = dall + (size_t)b * n;
     const double *e = eall + (size_t)b * n;
     const double sigma = sall[(size_t)b * n + i];
-    double *DD = work;                       // U diagonal
-    double *DU = work + (size_t)N * G;       // U first superdiagonal
-    double *DU2 = work + (size_t)2 * N * G;  // U second superdiagonal (pivoted rows only)
-    double *DL = work + (size_t)3 * N * G;   // multipliers
-    double *X = work + (size_t)4 * N * G;    // iterate
+    S *work = reinterpret_cast<S *>(workv);
+    S *DD = work;                       // U diagonal
+    S *DU = work + (size_t)N * G;       // U first superdiagonal
+    S *DU2 = work + (size_t)2 * N * G;  // U second superdiagonal (pivoted rows only)
+    S *DL = work + (size_t)3 * N * G;   // multipliers
+    S *X = work + (size_t)4 * N * G;    // iterate
     unsigned char *PV = reinterpret_cast<unsigned char *>(work + (size_t)5 * N * G);
 #define AT(arr, k) arr[(wid * N + (k)) * 64 + lane]
     auto toff = [&](int k) { return (k & 1) ? e[k >> 1] : d[k >> 1]; };
     double tnorm = 0;
     for (int k = 0; k < N - 1; ++k) tnorm = fmax(tnorm, fabs(toff(k)));
     const double eps = 2.220446049250313e-16;
-    const double tiny = fmax(eps * tnorm, 1e-300);
+    // (never below the storage type's smallest normal: a stored pivot must
+    // stay invertible)
+    const double tiny = fmax(eps * tnorm, sizeof(S) == 4 ? 1e-37 : 1e-300);
     auto clampd = [&](double v) { return fabs(v) < tiny ? (v < 0 ? -tiny : tiny) : v; };
     auto start = [&](int k) {
         return hash01(((uint64_t)b << 40) ^ ((uint64_t)i << 20) ^ k) + 0.25;
@@ -600,18 +608,22 @@ void launch_bisect(int n, const double *d, const double *e, double *sigma, int b
 // scratch columns: 64 per wave, (n + 63) / 64 waves per matrix
 static size_t inviter_cols(int n, int sub) { return (size_t)(n + 63) / 64 * 64 * sub; }
 
-size_t inviter_work_bytes(int n, int sub) {
+size_t inviter_work_bytes(int n, int sub, bool f32) {
     size_t G = inviter_cols(n, sub), N = 2 * (size_t)n;
-    return 5 * N * G * sizeof(double) + N * G + 256;
+    return 5 * N * G * (f32 ? sizeof(float) : sizeof(double)) + N * G + 256;
 }
 
 void launch_inviter(int n, const double *d, const double *e, const double *sigma, double *QBt,
-                    double *PBt, double *work, int sub, int batch, hipStream_t s) {
+                    double *PBt, void *work, int sub, int batch, hipStream_t s, bool f32) {
     for (int b0 = 0; b0 < batch; b0 += sub) {
         int nb = min(sub, batch - b0);
         dim3 grid((n + 63) / 64, nb);
-        hipLaunchKernelGGL(k_inviter, grid, dim3(64), 0, s, n, b0, d, e, sigma, QBt, PBt, work,
-                           (int)inviter_cols(n, sub));
+        if (f32)
+            hipLaunchKernelGGL(k_inviter<float>, grid, dim3(64), 0, s, n, b0, d, e, sigma, QBt, PBt,
+                               work, (int)inviter_cols(n, sub));
+        else
+            hipLaunchKernelGGL(k_inviter<double>, grid, dim3(64), 0, s, n, b0, d, e, sigma, QBt,
+                               PBt, work, (int)inviter_cols(n, sub));
     }
 }
 
