@@ -170,7 +170,8 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
         sw.batch = nf;
         IA_TRY(svd_reduce<T>(sw, Y, s, &clk));
         tm.mark();  // 4
-        launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(), nf, s);
+        launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(), nf, s,
+                      sizeof(T) == sizeof(float));
         tm.mark();  // 5
         launch_inviter(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(),
                        sw.QBt.template as<double>(), sw.PBt.template as<double>(), sw.work.p, sw.sub, nf, s,

@@ -238,8 +238,9 @@ template <typename T>
 ia_status launch_gebrd(int R, int C, cx<T> *Y, int ldy, size_t sY, double *d, double *e,
                        cx<T> *tauL, cx<T> *tauR, cx<T> *Vr, cx<T> *work, int batch,
                        hipStream_t s, KernelClock *clk = nullptr);
+// f32: the bidiagonal comes from the complex64 reduction (coarser stop)
 void launch_bisect(int n, const double *d, const double *e, double *sigma, int batch,
-                   hipStream_t s);
+                   hipStream_t s, bool f32 = false);
 // f32: the scratch holds the factorization and iterate in float (the
 // complex64 form), else in double
 size_t inviter_work_bytes(int n, int sub, bool f32 = false);
@@ -447,7 +448,7 @@ inline ia_status svd_factor(SvdWork<T> &w, cx<T> *Y, hipStream_t s) {
     const int C = w.C, batch = w.batch;
     IA_TRY(svd_reduce<T>(w, Y, s));
     launch_bisect(C, w.d.template as<double>(), w.e.template as<double>(), w.sig.template as<double>(),
-                  batch, s);
+                  batch, s, sizeof(T) == sizeof(float));
     IA_HIP(hipGetLastError());
     launch_inviter(C, w.d.template as<double>(), w.e.template as<double>(),
                    w.sig.template as<double>(), w.QBt.template as<double>(),

@@ -54,9 +54,11 @@ __device__ __forceinline__ int sturm_count(int N, const double *t2, double x, do
 
 // One workgroup per matrix (1024 threads, one value each for n <= 1024).
 constexpr int GL = 10, GPTS = 1 << GL;
+// Stopping width max(rtol hi, atol bound).
 __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict__ dall,
                                                 const double *__restrict__ eall,
-                                                double *__restrict__ sall) {
+                                                double *__restrict__ sall, double rtol,
+                                                double atol) {
     extern __shared__ double t2[];
     __shared__ int cgrid[GPTS];
     __shared__ double s_bound, s_pivmin;
@@ -98,7 +100,6 @@ __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict
     for (int j = t + 1; j < GPTS; j += blockDim.x)
         cgrid[j] = sturm_count(N, t2, bound * ((double)j / GPTS), pivmin) - n;
     __syncthreads();
-    const double eps = 2.220446049250313e-16;
     for (int i = t; i < n; i += blockDim.x) {
         const int target = n - 1 - i;  // sigma_i (descending) has target values below it
         // first grid point with more than target values below it (GPTS if none)
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict
         }
         double lo = bound * ((double)jl / GPTS), hi = jh == GPTS ? bound : bound * ((double)jh / GPTS);
         for (int it = 0; it < 128; ++it) {
-            if (hi - lo <= fmax(2.0 * eps * hi, 1e-3 * eps * bound)) break;
+            if (hi - lo <= fmax(rtol * hi, atol * bound)) break;
             double mid = 0.5 * (lo + hi);
             int c = sturm_count(N, t2, mid, pivmin) - n;
             if (c > target)
@@ -598,11 +599,18 @@ __global__ __launch_bounds__(BT) void k_geqr2(int R, int C, cx<T> *__restrict__ 
 
 
 void launch_bisect(int n, const double *d, const double *e, double *sigma, int batch,
-                   hipStream_t s) {
+                   hipStream_t s, bool f32) {
     size_t lds = sizeof(double) * (2 * (size_t)n);
     (void)hipFuncSetAttribute((const void *)k_bisect, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
-    hipLaunchKernelGGL(k_bisect, dim3(1, batch), dim3(1024), lds, s, n, d, e, sigma);
+    // FP64 form: full relative precision (down to 1e-3 eps of the bound).
+    // Complex64 form: 1e-12 relative or 1e-13 of the bound, a quarter fewer
+    // steps; the selection and the inverse-iteration shifts still see the
+    // values of the (complex64-rounded) bidiagonal to well below its own
+    // rounding (1e-9 relative measurably moved nearly tied GCV choices)
+    const double eps = 2.220446049250313e-16;
+    const double rtol = f32 ? 1e-12 : 2.0 * eps, atol = f32 ? 1e-13 : 1e-3 * eps;
+    hipLaunchKernelGGL(k_bisect, dim3(1, batch), dim3(1024), lds, s, n, d, e, sigma, rtol, atol);
 }
 
 // scratch columns: 64 per wave, (n + 63) / 64 waves per matrix
