@@ -717,16 +717,18 @@ __device__ __forceinline__ R hh_wave_sum(R v) {
 constexpr int G1 = 64;
 constexpr int G1T = 1024;
 
-// lane shuffle of a float or double, as 32-bit halves (no stack round trip)
-__device__ __forceinline__ float shfl_r(float v, int src) { return __shfl(v, src, 64); }
-__device__ __forceinline__ double shfl_r(double v, int src) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const int lo = __shfl((int)(unsigned)u, src, 64), hi = __shfl((int)(unsigned)(u >> 32), src, 64);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+// broadcast from a wave-uniform lane through the scalar unit (no LDS)
+__device__ __forceinline__ float bcast_r(float v, int src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+}
+__device__ __forceinline__ double bcast_r(double v, int src) {
+    const long long u = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)u, src), hi = __builtin_amdgcn_readlane((int)(u >> 32), src);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
 }
 template <typename T>
-__device__ __forceinline__ cx<T> shfl_cx(cx<T> v, int src) {
-    return cx<T>{shfl_r(v.re, src), shfl_r(v.im, src)};
+__device__ __forceinline__ cx<T> bcast_cx(cx<T> v, int src) {
+    return cx<T>{bcast_r(v.re, src), bcast_r(v.im, src)};
 }
 
 template <typename T>
@@ -768,7 +770,7 @@ __global__ __launch_bounds__(G1T) void k_getrs1(int n, const cx<T> *__restrict__
         if (wv == 0) {
             cx<T> x = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
             for (int j = 0; j < jb; ++j) {
-                const cx<T> xj = shfl_cx(x, j);
+                const cx<T> xj = bcast_cx(x, j);
                 if (lane > j && lane < jb) x = x - Ds[j * LD + lane] * xj;
             }
             if (lane < jb) xs[i0 + lane] = x;
@@ -793,9 +795,10 @@ __global__ __launch_bounds__(G1T) void k_getrs1(int n, const cx<T> *__restrict__
         __syncthreads();
         if (wv == 0) {
             cx<T> x = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
+            const cx<T> dl = lane < jb ? cdiv(cx<T>{1, 0}, Ds[lane * LD + lane]) : cx<T>{0, 0};
             for (int j = jb - 1; j >= 0; --j) {
-                if (lane == j) x = x * cdiv(cx<T>{1, 0}, Ds[j * LD + j]);
-                const cx<T> xj = shfl_cx(x, j);
+                if (lane == j) x = x * dl;
+                const cx<T> xj = bcast_cx(x, j);
                 if (lane < j) x = x - Ds[j * LD + lane] * xj;
             }
             if (lane < jb) xs[i0 + lane] = x;
@@ -818,13 +821,6 @@ __global__ __launch_bounds__(G1T) void k_getrs1(int n, const cx<T> *__restrict__
 // column r of U (contiguous), so the update of the rest of b after a block
 // gives each row to one wave: lanes read the block's 64 entries of that
 // column in one coalesced access and a DPP sum forms x_r's correction.
-template <typename T>
-__device__ __forceinline__ cx<T> dot_row_c(const cx<T> *__restrict__ a, const cx<T> xl, int lane,
-                                           int jb) {
-    const cx<T> v = lane < jb ? cmulc(a[lane], xl) : cx<T>{0, 0};
-    return cx<T>{hh_wave_sum(v.re), hh_wave_sum(v.im)};
-}
-
 template <typename T>
 __global__ __launch_bounds__(G1T) void k_getrs1c(int n, const cx<T> *__restrict__ Aall, int lda,
                                                 size_t sA, const int *__restrict__ ipivall,
@@ -849,13 +845,25 @@ __global__ __launch_bounds__(G1T) void k_getrs1c(int n, const cx<T> *__restrict_
             Ds[j * LD + l] = conj(A[(size_t)(i0 + l) * lda + i0 + j]);
         }
     };
-    // rows [r0, r1) of b lose op(A)[r, blk] x_blk
+    // rows [r0, r1) of b lose op(A)[r, blk] x_blk; each wave takes RB rows
+    // at a time, their loads issued together
     auto update = [&](int i0, int jb, int r0, int r1) {
+        constexpr int RB = 8;
         const cx<T> xl = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
-#pragma unroll 4
-        for (int r = r0 + wv; r < r1; r += NWV) {
-            const cx<T> d = dot_row_c(A + (size_t)r * lda + i0, xl, lane, jb);
-            if (lane == 0) xs[r] = xs[r] - d;
+        for (int rb = r0 + wv; rb < r1; rb += NWV * RB) {
+            cx<T> a[RB];
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int r = rb + q * NWV;
+                a[q] = (r < r1 && lane < jb) ? A[(size_t)r * lda + i0 + lane] : cx<T>{0, 0};
+            }
+#pragma unroll
+            for (int q = 0; q < RB; ++q) {
+                const int r = rb + q * NWV;
+                const cx<T> v = cmulc(a[q], xl);
+                const cx<T> d{hh_wave_sum(v.re), hh_wave_sum(v.im)};
+                if (lane == 0 && r < r1) xs[r] = xs[r] - d;
+            }
         }
     };
     __syncthreads();
@@ -866,9 +874,10 @@ __global__ __launch_bounds__(G1T) void k_getrs1c(int n, const cx<T> *__restrict_
         __syncthreads();
         if (wv == 0) {
             cx<T> x = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
+            const cx<T> dl = lane < jb ? cdiv(cx<T>{1, 0}, Ds[lane * LD + lane]) : cx<T>{0, 0};
             for (int j = 0; j < jb; ++j) {
-                if (lane == j) x = x * cdiv(cx<T>{1, 0}, Ds[j * LD + j]);
-                const cx<T> xj = shfl_cx(x, j);
+                if (lane == j) x = x * dl;
+                const cx<T> xj = bcast_cx(x, j);
                 if (lane > j && lane < jb) x = x - Ds[j * LD + lane] * xj;
             }
             if (lane < jb) xs[i0 + lane] = x;
@@ -886,7 +895,7 @@ __global__ __launch_bounds__(G1T) void k_getrs1c(int n, const cx<T> *__restrict_
         if (wv == 0) {
             cx<T> x = lane < jb ? xs[i0 + lane] : cx<T>{0, 0};
             for (int j = jb - 1; j >= 0; --j) {
-                const cx<T> xj = shfl_cx(x, j);
+                const cx<T> xj = bcast_cx(x, j);
                 if (lane < j) x = x - Ds[j * LD + lane] * xj;
             }
             if (lane < jb) xs[i0 + lane] = x;
