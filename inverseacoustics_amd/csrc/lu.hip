@@ -265,26 +265,39 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
         const int r0 = cs - k;
         for (int j = cs + w + wv; j < k + jb; j += LNW) {
             cx<T> *Aj = Ab + (size_t)j * lda + k;
-            // tiny forward solve, redundantly in every lane
+            // tiny forward solve, redundantly in every lane (its w loads
+            // issued together)
             cx<T> x[SB];
 #pragma unroll
-            for (int i = 0; i < SB; ++i) {
-                if (i >= w) break;
-                cx<T> v = Aj[r0 + i];
-                for (int l = 0; l < i; ++l) v = v - Vs[(size_t)l * M + r0 + i] * x[l];
-                x[i] = v;
-            }
+            for (int i = 0; i < SB; ++i) x[i] = i < w ? Aj[r0 + i] : cx<T>{0, 0};
+#pragma unroll
+            for (int i = 0; i < SB; ++i)
+                if (i < w)
+                    for (int l = 0; l < i; ++l) x[i] = x[i] - Vs[(size_t)l * M + r0 + i] * x[l];
             if (lane < w) {
 #pragma unroll
                 for (int i = 0; i < SB; ++i)
                     if (i == lane) Aj[r0 + i] = x[i];
             }
-            for (int r = r0 + w + lane; r < M; r += 64) {
-                cx<T> a = Aj[r];
+            // rank-w update, CH rows per lane at a time with their loads in flight together
+            constexpr int CH = 8;
+            for (int rb = r0 + w + lane; rb < M; rb += 64 * CH) {
+                cx<T> a[CH];
 #pragma unroll
-                for (int i = 0; i < SB; ++i)
-                    if (i < w) a = a - Vs[(size_t)i * M + r] * x[i];
-                Aj[r] = a;
+                for (int q = 0; q < CH; ++q) {
+                    const int r = rb + 64 * q;
+                    a[q] = r < M ? Aj[r] : cx<T>{0, 0};
+                }
+#pragma unroll
+                for (int q = 0; q < CH; ++q) {
+                    const int r = rb + 64 * q;
+                    if (r < M) {
+#pragma unroll
+                        for (int i = 0; i < SB; ++i)
+                            if (i < w) a[q] = a[q] - Vs[(size_t)i * M + r] * x[i];
+                        Aj[r] = a[q];
+                    }
+                }
             }
         }
         __syncthreads();
