@@ -166,7 +166,6 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
     __shared__ T red_v[LNW];
     __shared__ int red_i[LNW];
     __shared__ cx<T> pinv_s;
-    __shared__ int prow[PERM_MAX], psrc[PERM_MAX];
     __shared__ int ipl[NB];  // the panel's pivot rows (to global ipiv after each sub-panel)
     __shared__ int zpiv;     // first zero pivot column + 1 of this panel, 0 if none
     const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -200,25 +199,26 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
                 red_i[wv] = bi;
             }
             __syncthreads();
-            if (t == 0) {
-                T bv = red_v[0];
-                int bx = red_i[0];
-                for (int q = 1; q < LNW; ++q)
-                    if (red_v[q] > bv || (red_v[q] == bv && red_i[q] < bx)) {
-                        bv = red_v[q];
-                        bx = red_i[q];
-                    }
-                ipl[cs - k + i] = k + bx;
-                if (bv == (T)0 && zpiv == 0) zpiv = cs + i + 1;
-                // swap rows rc and bx inside the sub-panel (LDS)
-                if (bx != rc)
-                    for (int q = 0; q < w; ++q) {
-                        const cx<T> tmp = Vs[(size_t)q * M + rc];
-                        Vs[(size_t)q * M + rc] = Vs[(size_t)q * M + bx];
-                        Vs[(size_t)q * M + bx] = tmp;
-                    }
-                const cx<T> d = col[rc];
-                pinv_s = (bv == (T)0) ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
+            if (wv == 0) {
+                // the waves' candidates (lanes >= LNW hold none), then lane q
+                // swaps rows rc and bx of sub-panel column q
+                T bv = lane < LNW ? red_v[lane] : (T)-1;
+                int bx = lane < LNW ? red_i[lane] : M;
+                argmax_dpp(bv, bx);
+                if (bx != rc && lane < w) {
+                    const cx<T> tmp = Vs[(size_t)lane * M + rc];
+                    Vs[(size_t)lane * M + rc] = Vs[(size_t)lane * M + bx];
+                    Vs[(size_t)lane * M + bx] = tmp;
+                }
+                if (lane == 0) {
+                    ipl[cs - k + i] = k + bx;
+                    if (bv == (T)0 && zpiv == 0) zpiv = cs + i + 1;
+                }
+                // lane i swapped column i (the pivot column); its new diagonal
+                if (lane == i) {
+                    const cx<T> d = col[rc];
+                    pinv_s = (bv == (T)0) ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
+                }
             }
             __syncthreads();
             const cx<T> pinv = pinv_s;
@@ -302,34 +302,50 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
         }
         __syncthreads();
     }
-    // the panel's interchanges as one row permutation (for the other columns)
-    if (t == 0) {
-        int *pm = permall + (size_t)b * (1 + 2 * PERM_MAX);
-        int *rows = prow, *src = psrc;
-        int cnt = 0;
-        auto slot = [&](int r) {
-            for (int q = 0; q < cnt; ++q)
-                if (rows[q] == r) return q;
-            rows[cnt] = r;
-            src[cnt] = r;
-            return cnt++;
-        };
-        for (int c = k; c < k + jb; ++c) {
-            const int p = ipl[c - k];
-            if (p == c) continue;
-            const int a1 = slot(c), a2 = slot(p);
-            const int tmp = src[a1];
-            src[a1] = src[a2];
-            src[a2] = tmp;
-        }
-        int m = 0;
-        for (int q = 0; q < cnt; ++q)
-            if (src[q] != rows[q]) {
-                pm[1 + m] = rows[q];
-                pm[1 + PERM_MAX + m] = src[q];
-                ++m;
+    // the panel's interchanges as one row permutation (for the other
+    // columns): the swaps replayed on a row map in LDS (the sub-panel buffer,
+    // no longer needed), then every touched row lists itself once
+    int *pmap = reinterpret_cast<int *>(smem_raw);  // pmap[r - k], rows k..n-1
+    __shared__ int pcnt;
+    if (t < jb) {
+        pmap[t] = k + t;
+        pmap[ipl[t] - k] = ipl[t];
+    }
+    if (t == 0) pcnt = 0;
+    __syncthreads();
+    if (t == 0)
+        for (int c = 0; c < jb; ++c) {
+            const int p = ipl[c] - k;
+            if (p != c) {
+                const int tmp = pmap[c];
+                pmap[c] = pmap[p];
+                pmap[p] = tmp;
             }
-        pm[0] = m;
+        }
+    __syncthreads();
+    int *pm = permall + (size_t)b * (1 + 2 * PERM_MAX);
+    if (t < 2 * jb) {
+        // candidate rows: k + t (t < jb), or the pivot row ipl[t - jb] when it
+        // lies outside the panel rows and is its first occurrence
+        int r = -1;
+        if (t < jb) {
+            r = k + t;
+        } else {
+            const int q = t - jb, p = ipl[q];
+            bool first = p >= k + jb;
+            for (int q2 = 0; q2 < q && first; ++q2)
+                if (ipl[q2] == p) first = false;
+            if (first) r = p;
+        }
+        if (r >= 0 && pmap[r - k] != r) {
+            const int slot = atomicAdd(&pcnt, 1);
+            pm[1 + slot] = r;
+            pm[1 + PERM_MAX + slot] = pmap[r - k];
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        pm[0] = pcnt;
         if (zpiv && info && info[b] == 0) info[b] = zpiv;
     }
 }
