@@ -447,16 +447,43 @@ __global__ void k_copy_upper(int C, const cx<T> *__restrict__ Yall, int ldy, siz
     Rball[(size_t)b * C * C + (size_t)c * C + r] = (r <= c) ? v : cx<T>{0, 0};
 }
 
-// LQ panel of the band reduction, as a QR of the conjugate transpose:
-// P[(k + q) + (k + i) ldp] = conj(Z[k + i, k + NB + q]), q < N2, i < nbp
+// LQ panel of the band reduction, as a QR of the conjugate transpose, with
+// the left update of the panel's rows folded in:
+// P[(k + q) + (k + i) ldp] = conj(Z[k + i, k + NB + q] - sum_j V1[k + i, j] W1[j, q])
+// for i < nbp, q < N2, j < w (nbp, w <= 32).  One workgroup per 32 columns
+// q; the three 32 x 32 tiles are read with consecutive lanes on consecutive
+// addresses and P is written the same way.
 template <typename T>
-__global__ void k_lq_gather(int k, int N2, int nbp, const cx<T> *__restrict__ Zall, int ldz,
-                            size_t sZ, cx<T> *__restrict__ Pall, int ldp, size_t sP) {
-    const int b = blockIdx.z, i = blockIdx.y;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= N2 || i >= nbp) return;
-    const cx<T> z = Zall[(size_t)b * sZ + (size_t)(k + NB + q) * ldz + (k + i)];
-    Pall[(size_t)b * sP + (size_t)(k + i) * ldp + (k + q)] = conj(z);
+__global__ __launch_bounds__(256) void k_lq_gather(int k, int N2, int nbp, int w,
+                                                   const cx<T> *__restrict__ Zall, int ldz,
+                                                   size_t sZ, const cx<T> *__restrict__ V1all,
+                                                   int ldv, size_t sV,
+                                                   const cx<T> *__restrict__ W1all, int ldw,
+                                                   size_t sW, cx<T> *__restrict__ Pall, int ldp,
+                                                   size_t sP) {
+    __shared__ cx<T> Vt[32][33];  // V1[k + i, j]
+    __shared__ cx<T> Wt[32][33];  // W1[j, q0 + c]
+    __shared__ cx<T> At[32][33];  // Z[k + i, k + NB + q0 + c]
+    const int b = blockIdx.y, q0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const cx<T> *Z = Zall + (size_t)b * sZ;
+    const cx<T> *V1 = V1all + (size_t)b * sV;
+    const cx<T> *W1 = W1all + (size_t)b * sW;
+    const cx<T> zero{0, 0};
+    for (int c = ty; c < 32; c += 8) {
+        Vt[tx][c] = (c < w && tx < nbp) ? V1[(size_t)c * ldv + k + tx] : zero;
+        const bool qv = q0 + c < N2;
+        Wt[tx][c] = (qv && tx < w) ? W1[(size_t)(q0 + c) * ldw + tx] : zero;
+        At[tx][c] = (qv && tx < nbp) ? Z[(size_t)(k + NB + q0 + c) * ldz + k + tx] : zero;
+    }
+    __syncthreads();
+    if (q0 + tx >= N2) return;
+    cx<T> *P = Pall + (size_t)b * sP;
+    for (int i = ty; i < nbp; i += 8) {
+        cx<T> acc = At[i][tx];
+        for (int j = 0; j < w; ++j) acc = acc - Vt[i][j] * Wt[j][tx];
+        P[(size_t)(k + i) * ldp + k + q0 + tx] = conj(acc);
+    }
 }
 
 // rows k..k+nbp-1 of Z after the LQ panel: Z[k + i, k + NB + q] =
@@ -471,6 +498,26 @@ __global__ void k_lq_scatter(int k, int nbp, cx<T> *__restrict__ Zall, int ldz, 
     const cx<T> r = Pall[(size_t)b * sP + (size_t)(k + i) * ldp + (k + q)];
     Zall[(size_t)b * sZ + (size_t)(k + NB + q) * ldz + (k + i)] =
         (q <= i) ? conj(r) : cx<T>{0, 0};
+}
+
+// B[r0 + j, q] = conj(V[k + q, j]) for j < nbp <= 32, q < N2: the LQ
+// reflectors as rows of the band update's right operand, transposed
+// through a 32 x 32 LDS tile so that reads and writes are both contiguous
+template <typename T>
+__global__ __launch_bounds__(256) void k_conj_rows(int k, int N2, int nbp,
+                                                   const cx<T> *__restrict__ Vall, int ldv,
+                                                   size_t sV, cx<T> *__restrict__ Ball, int ldb,
+                                                   size_t sB, int r0) {
+    __shared__ cx<T> tile[32][33];
+    const int b = blockIdx.y, q0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const cx<T> *V = Vall + (size_t)b * sV;
+    cx<T> *B = Ball + (size_t)b * sB;
+    for (int j = ty; j < nbp; j += 8)
+        if (q0 + tx < N2) tile[j][tx] = V[(size_t)j * ldv + k + q0 + tx];
+    __syncthreads();
+    for (int qq = ty; qq < 32; qq += 8)
+        if (q0 + qq < N2 && tx < nbp) B[(size_t)(q0 + qq) * ldb + r0 + tx] = conj(tile[tx][qq]);
 }
 
 }  // namespace
@@ -513,69 +560,111 @@ ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, c
 // Stage one of the two-stage bidiagonalisation: Z (R x C, R >= C) ->
 // Q1^H Z P1, upper band of width NB in the leading C x C block.  Per panel
 // k: QR of the NB columns k.. (reflectors below the diagonal of Z, tau in
-// tauQ) and its trailing update from the left; then, while columns remain
-// beyond k + NB, the LQ of rows k..k+NB-1 over columns k+NB.. computed as
-// a QR of their conjugate transpose in Pm (C-NB x C-NB per matrix; the
-// reflectors stay there, tau in tauP) and applied from the right to the
-// rows below, Z2 <- Z2 (I - V T V^H).  All trailing updates are GEMMs.
+// tauQ, explicit V1 in the first NB columns of Vw1); then, while columns
+// remain beyond k + NB, the LQ of rows k..k+NB-1 over columns k+NB..
+// computed as a QR of their conjugate transpose in Pm (C-NB x C-NB per
+// matrix; the reflectors stay there, explicit V2 in Vw2, tau in tauP).
+//
+// The two trailing updates of the rows below the LQ rows are applied as
+// one rank-2NB GEMM.  With A the trailing block before either update,
+// W1 = T1^H V1^H A, A' = A - V1 W1 (left) and A'' = A' - A' V2 T2 V2^H
+// (right); since A' V2 = A V2 - V1 (W1 V2),
+//     A'' = A - [V1, X] [W1; V2^H],   X = (A V2 - V1 (W1 V2)) T2,
+// so A is read by V1^H A and A V2 and read and written once by the update,
+// instead of twice each by two rank-NB updates.  The LQ rows themselves
+// take the left update inside k_lq_gather, on their way into Pm (what they
+// hold beyond the band afterwards is never read).
 // tauP has C - NB entries per matrix (the reflectors of Pm's columns).
 size_t band_work_elems(int R, int C) {
-    return (size_t)R * NB + 2 * (size_t)NB * NB + 2 * (size_t)NB * std::max(R, C);
+    const size_t Wl = std::max(R, C);
+    return 2 * (size_t)NB * R + (size_t)NB * C + 3 * (size_t)NB * NB + 3 * (size_t)NB * Wl;
 }
 
 template <typename T>
 ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *tauQ, cx<T> *Pm,
                             cx<T> *tauP, cx<T> *work, int batch, hipStream_t s) {
     if (R < C) return fail(IA_ERR_ARG, "band reduction needs R >= C (%d x %d)", R, C);
-    cx<T> *Vw = work;
-    cx<T> *G = Vw + (size_t)batch * R * NB;
-    cx<T> *Tm = G + (size_t)batch * NB * NB;
+    constexpr int NB2 = 2 * NB;
     const int Wl = std::max(R, C);
-    cx<T> *W = Tm + (size_t)batch * NB * NB;
-    cx<T> *W2 = W + (size_t)batch * NB * Wl;
+    cx<T> *Vw1 = work;                             // R x 2NB: [V1, X]
+    cx<T> *Vw2 = Vw1 + (size_t)batch * R * NB2;    // C x NB: V2
+    cx<T> *G = Vw2 + (size_t)batch * C * NB;       // NB x NB
+    cx<T> *Tm = G + (size_t)batch * NB * NB;       // NB x NB: T1, then T2
+    cx<T> *U = Tm + (size_t)batch * NB * NB;       // NB x NB: W1 V2
+    cx<T> *Wt = U + (size_t)batch * NB * NB;       // NB x Wl: V1^H A, then A V2
+    cx<T> *Bm = Wt + (size_t)batch * NB * Wl;      // 2NB x C: [W1; V2^H]
     const cx<T> one{1, 0}, zero{0, 0}, mone{-1, 0};
-    const size_t sT = (size_t)NB * NB, sW = (size_t)NB * Wl;
+    const size_t sV1 = (size_t)R * NB2, sT = (size_t)NB * NB, sW = (size_t)NB * Wl,
+                 sB = (size_t)NB2 * C;
     const int Cp = C - NB;
     const size_t sP = (size_t)std::max(Cp, 1) * std::max(Cp, 1);
+    const size_t sV2 = (size_t)std::max(Cp, 1) * NB;
+    // complex64: A V2 as its conjugate transpose V2^H A^H, the faster form
+    // of that product in single precision (complex128 prefers A V2)
+    constexpr bool F = sizeof(T) == sizeof(float);
     for (int k = 0; k < C; k += NB) {
         const int w = std::min(NB, C - k);
-        // ---- QR of columns k..k+w-1, left update of columns k+w..
-        launch_qr_panel<T>(R, C, k, w, Z, ldz, sZ, tauQ, Vw, NB, batch, s);
-        const size_t sV = (size_t)R * NB;
+        // ---- QR of columns k..k+w-1
+        launch_qr_panel<T>(R, C, k, w, Z, ldz, sZ, tauQ, Vw1, NB2, batch, s);
         const int N = C - k - w, M = R - k;
-        if (N > 0) {
-            const cx<T> *V0 = Vw + k;
-            launch_gemm<T>('C', 'N', w, w, M, one, V0, R, sV, V0, R, sV, zero, G, NB, sT, batch, s);
-            hipLaunchKernelGGL((k_qr_tmat<T, NB>), dim3(batch), dim3(64), 0, s, C, k, w, G, tauQ, Tm);
-            cx<T> *A2 = Z + k + (size_t)(k + w) * ldz;
-            launch_gemm<T>('C', 'N', w, N, M, one, V0, R, sV, A2, ldz, sZ, zero, W, NB, sW, batch, s);
-            launch_gemm<T>('C', 'N', w, N, w, one, Tm, NB, sT, W, NB, sW, zero, W2, NB, sW, batch, s);
-            launch_gemm<T>('N', 'N', M, N, w, mone, V0, R, sV, W2, NB, sW, one, A2, ldz, sZ, batch,
+        if (N <= 0) continue;
+        const cx<T> *V1 = Vw1 + k;
+        launch_gemm<T>('C', 'N', w, w, M, one, V1, R, sV1, V1, R, sV1, zero, G, NB, sT, batch, s);
+        hipLaunchKernelGGL((k_qr_tmat<T, NB>), dim3(batch), dim3(64), 0, s, C, k, w, G, tauQ, Tm);
+        cx<T> *A2 = Z + k + (size_t)(k + w) * ldz;
+        // W1 = T1^H V1^H A (rows 0..w-1 of Bm)
+        launch_gemm<T>('C', 'N', w, N, M, one, V1, R, sV1, A2, ldz, sZ, zero, Wt, NB, sW, batch, s);
+        launch_gemm<T>('C', 'N', w, N, w, one, Tm, NB, sT, Wt, NB, sW, zero, Bm, NB2, sB, batch, s);
+        if (k + NB >= C) {
+            // last panel: left update only
+            launch_gemm<T>('N', 'N', M, N, w, mone, V1, R, sV1, Bm, NB2, sB, one, A2, ldz, sZ, batch,
                            s);
+            continue;
         }
-        if (k + NB >= C) continue;
-        // ---- LQ of rows k..k+NB-1 over columns k+NB..C-1
-        const int N2 = C - k - NB;           // columns of the row panel
-        const int nbp = std::min(NB, N2);    // reflectors (rows with columns left)
-        hipLaunchKernelGGL(k_lq_gather<T>, dim3((N2 + 255) / 256, nbp, batch), dim3(256), 0, s, k,
-                           N2, nbp, Z, ldz, sZ, Pm, Cp, sP);
-        launch_qr_panel<T>(Cp, Cp, k, nbp, Pm, Cp, sP, tauP, Vw, NB, batch, s);
+        // ---- LQ of rows k..k+NB-1 over columns k+NB..C-1 (here w == NB, N == N2)
+        const int N2 = N;
+        const int nbp = std::min(NB, N2);  // reflectors (rows with columns left)
+        // the LQ rows take the left update on their way into Pm
+        hipLaunchKernelGGL(k_lq_gather<T>, dim3((N2 + 31) / 32, batch), dim3(256), 0, s, k, N2, nbp,
+                           w, Z, ldz, sZ, Vw1, R, sV1, Bm, NB2, sB, Pm, Cp, sP);
+        launch_qr_panel<T>(Cp, Cp, k, nbp, Pm, Cp, sP, tauP, Vw2, NB, batch, s);
         hipLaunchKernelGGL(k_lq_scatter<T>, dim3((nbp * nbp + 255) / 256, batch), dim3(256), 0, s,
                            k, nbp, Z, ldz, sZ, Pm, Cp, sP);
-        // rows below the reflected ones: Z2 <- Z2 - ((Z2 V) T) V^H
-        const int M2 = R - k - nbp;
+        const int M2 = R - k - nbp;  // rows below the LQ rows
         if (M2 <= 0) continue;
-        const size_t sV2 = (size_t)Cp * NB;
-        const cx<T> *V0 = Vw + k;
-        cx<T> *Z2 = Z + (k + nbp) + (size_t)(k + NB) * ldz;
-        launch_gemm<T>('C', 'N', nbp, nbp, N2, one, V0, Cp, sV2, V0, Cp, sV2, zero, G, NB, sT,
+        const cx<T> *V2 = Vw2 + k;
+        launch_gemm<T>('C', 'N', nbp, nbp, N2, one, V2, Cp, sV2, V2, Cp, sV2, zero, G, NB, sT,
                        batch, s);
         hipLaunchKernelGGL((k_qr_tmat<T, NB>), dim3(batch), dim3(64), 0, s, Cp, k, nbp, G, tauP, Tm);
-        launch_gemm<T>('N', 'N', M2, nbp, N2, one, Z2, ldz, sZ, V0, Cp, sV2, zero, W, Wl, sW, batch,
-                       s);
-        launch_gemm<T>('N', 'N', M2, nbp, nbp, one, W, Wl, sW, Tm, NB, sT, zero, W2, Wl, sW, batch,
-                       s);
-        launch_gemm<T>('N', 'C', M2, N2, nbp, mone, W2, Wl, sW, V0, Cp, sV2, one, Z2, ldz, sZ,
+        // V2^H as rows w..w+nbp-1 of Bm
+        hipLaunchKernelGGL(k_conj_rows<T>, dim3((N2 + 31) / 32, batch), dim3(256), 0, s, k, N2, nbp,
+                           Vw2, Cp, sV2, Bm, NB2, sB, w);
+        cx<T> *Z2 = A2 + nbp;
+        const cx<T> *V1b = V1 + nbp;           // rows k+nbp.. of V1
+        cx<T> *X = Vw1 + (size_t)NB * R + k + nbp;  // column w.. of [V1, X]
+        if (F) {
+            // U^H = V2^H W1^H, Wt = (A V2)^H - U^H V1^H, X = Wt^H T2
+            launch_gemm<T>('C', 'C', nbp, w, N2, one, V2, Cp, sV2, Bm, NB2, sB, zero, U, NB, sT,
+                           batch, s);
+            launch_gemm<T>('C', 'C', nbp, M2, N2, one, V2, Cp, sV2, Z2, ldz, sZ, zero, Wt, NB, sW,
+                           batch, s);
+            launch_gemm<T>('N', 'C', nbp, M2, w, mone, U, NB, sT, V1b, R, sV1, one, Wt, NB, sW,
+                           batch, s);
+            launch_gemm<T>('C', 'N', M2, nbp, nbp, one, Wt, NB, sW, Tm, NB, sT, zero, X, R, sV1,
+                           batch, s);
+        } else {
+            // U = W1 V2, Wt = A V2 - V1 U, X = Wt T2
+            launch_gemm<T>('N', 'N', w, nbp, N2, one, Bm, NB2, sB, V2, Cp, sV2, zero, U, NB, sT,
+                           batch, s);
+            launch_gemm<T>('N', 'N', M2, nbp, N2, one, Z2, ldz, sZ, V2, Cp, sV2, zero, Wt, Wl, sW,
+                           batch, s);
+            launch_gemm<T>('N', 'N', M2, nbp, w, mone, V1b, R, sV1, U, NB, sT, one, Wt, Wl, sW,
+                           batch, s);
+            launch_gemm<T>('N', 'N', M2, nbp, nbp, one, Wt, Wl, sW, Tm, NB, sT, zero, X, R, sV1,
+                           batch, s);
+        }
+        // A'' = A - [V1, X] [W1; V2^H]
+        launch_gemm<T>('N', 'N', M2, N2, w + nbp, mone, V1b, R, sV1, Bm, NB2, sB, one, Z2, ldz, sZ,
                        batch, s);
     }
     IA_HIP(hipGetLastError());
