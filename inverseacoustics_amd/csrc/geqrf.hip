@@ -392,51 +392,62 @@ void launch_qr_panel(int R, int C, int j0, int nbp, cx<T> *Y, int ldy, size_t sY
 }
 
 // T (nbp x nbp, upper) with H_0 ... H_{nbp-1} = I - V T V^H from G = V^H V
-// and tau (LAPACK larft, forward columnwise); one wave per matrix; G and T
-// are P x P (leading dimension P) per matrix.
+// and tau (the T of LAPACK larft, forward columnwise), built by doubling
+// instead of column by column: for adjacent blocks of h reflectors,
+// (I - V1 T11 V1^H)(I - V2 T22 V2^H) = I - [V1 V2] [T11 T12; 0 T22] [V1 V2]^H
+// with T12 = -T11 (V1^H V2) T22, so log2(P) levels of small independent
+// products, starting from T_ii = tau_i, replace the P dependent
+// triangular products of the column recurrence.  One workgroup of 256
+// threads per matrix; G, tau and T are P x P (leading dimension P).
 template <typename T, int P>
-__global__ __launch_bounds__(64) void k_qr_tmat(int C, int j0, int nbp, const cx<T> *__restrict__ Gall,
-                                               const cx<T> *__restrict__ tauall,
-                                               cx<T> *__restrict__ Tall) {
-    static_assert(P <= 64, "one lane per row of T");
-    // upper triangle packed by columns: T[r][c] (r <= c) at c (c + 1) / 2 + r
-    __shared__ cx<double> Ts[P * (P + 1) / 2];
-    __shared__ cx<double> z[P];
+__global__ __launch_bounds__(256) void k_qr_tmat(int C, int j0, int nbp, const cx<T> *__restrict__ Gall,
+                                                const cx<T> *__restrict__ tauall,
+                                                cx<T> *__restrict__ Tall) {
+    constexpr int NT = 256;
+    // strictly upper part: G until the level that replaces the block by T12;
+    // diagonal: tau; zero outside nbp (so padded reflectors contribute nothing)
+    __shared__ cx<double> Ms[P][P + 1];
+    __shared__ cx<double> tmp[P * P / 4];  // G12 T22 of every pair of a level
     const int b = blockIdx.x, t = threadIdx.x;
     const cx<T> *G = Gall + (size_t)b * P * P;
     const cx<T> *tau = tauall + (size_t)b * C + j0;
     cx<T> *Tm = Tall + (size_t)b * P * P;
-    const cx<double> tau_t = t < nbp ? cvt<double>(tau[t]) : cx<double>{0, 0};
-    // column i of G (rows t < P), prefetched one step ahead
-    cx<T> gnext = t < P ? G[t] : cx<T>{0, 0};
-    for (int i = 0; i < nbp; ++i) {
-        const cx<T> gcur = gnext;
-        if (i + 1 < nbp && t < P) gnext = G[(size_t)(i + 1) * P + t];
-        const cx<double> ti{__shfl(tau_t.re, i, 64), __shfl(tau_t.im, i, 64)};
-        if (t < i) z[t] = cx<double>{0, 0} - ti * cvt<double>(gcur);  // G[t, i]
-        __syncthreads();
-        const int ci = i * (i + 1) / 2;
-        if (t < i) {
-            // two interleaved partial sums shorten the dependent FMA chain
-            cx<double> a0{0, 0}, a1{0, 0};
-            int l = t;
-            for (; l + 1 < i; l += 2) {
-                a0 = a0 + Ts[l * (l + 1) / 2 + t] * z[l];
-                a1 = a1 + Ts[(l + 1) * (l + 2) / 2 + t] * z[l + 1];
-            }
-            if (l < i) a0 = a0 + Ts[l * (l + 1) / 2 + t] * z[l];
-            Ts[ci + t] = a0 + a1;
+    for (int e = t; e < P * P; e += NT) {
+        const int r = e % P, c = e / P;
+        cx<double> v{0, 0};
+        if (c < nbp) {
+            if (r < c) v = cvt<double>(G[e]);
+            else if (r == c) v = cvt<double>(tau[r]);
         }
-        if (t == 0) Ts[ci + i] = ti;
+        Ms[r][c] = v;
+    }
+    __syncthreads();
+    for (int h = 1; h < P; h *= 2) {
+        const int n = (P / (2 * h)) * h * h;  // outputs of the level
+        // tmp[r, c] = sum_{l <= c} G12[r, l] T22[l, c]
+        for (int e = t; e < n; e += NT) {
+            const int pr = e / (h * h), r = e % h, c = (e / h) % h, o = pr * 2 * h;
+            cx<double> acc{0, 0};
+            for (int l = 0; l <= c; ++l) acc = acc + Ms[o + r][o + h + l] * Ms[o + h + l][o + h + c];
+            tmp[e] = acc;
+        }
+        __syncthreads();
+        // T12[r, c] = -sum_{l >= r} T11[r, l] tmp[l, c]
+        for (int e = t; e < n; e += NT) {
+            const int pr = e / (h * h), r = e % h, c = (e / h) % h, o = pr * 2 * h;
+            const cx<double> *tc = tmp + (size_t)pr * h * h + (size_t)c * h;
+            cx<double> acc{0, 0};
+            for (int l = r; l < h; ++l) acc = acc + Ms[o + r][o + l] * tc[l];
+            Ms[o + r][o + h + c] = cx<double>{0, 0} - acc;
+        }
         __syncthreads();
     }
-    for (int e = t; e < P * P; e += 64) {
+    for (int e = t; e < P * P; e += NT) {
         const int r = e % P, c = e / P;
-        Tm[e] = (r <= c && c < nbp) ? cvt<T>(Ts[c * (c + 1) / 2 + r]) : cx<T>{0, 0};
+        Tm[e] = (r <= c && c < nbp) ? cvt<T>(Ms[r][c]) : cx<T>{0, 0};
     }
 }
 
-// Rb (C x C) = upper triangle of Y's leading C x C block, zeros below
 template <typename T>
 __global__ void k_copy_upper(int C, const cx<T> *__restrict__ Yall, int ldy, size_t sY,
                              cx<T> *__restrict__ Rball) {
@@ -546,7 +557,7 @@ ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, c
         const cx<T> *V0 = Vw + j0;  // rows j0.. of the explicit panel reflectors
         // G = V^H V
         launch_gemm<T>('C', 'N', nbp, nbp, M, one, V0, R, sV, V0, R, sV, zero, G, P, sT, batch, s);
-        hipLaunchKernelGGL((k_qr_tmat<T, P>), dim3(batch), dim3(64), 0, s, C, j0, nbp, G, tau, Tm);
+        hipLaunchKernelGGL((k_qr_tmat<T, P>), dim3(batch), dim3(256), 0, s, C, j0, nbp, G, tau, Tm);
         cx<T> *A2 = Y + j0 + (size_t)(j0 + nbp) * ldy;
         // W = V^H A2, W2 = T^H W, A2 -= V W2
         launch_gemm<T>('C', 'N', nbp, N, M, one, V0, R, sV, A2, ldy, sY, zero, W, P, sW, batch, s);
@@ -610,7 +621,7 @@ ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *t
         if (N <= 0) continue;
         const cx<T> *V1 = Vw1 + k;
         launch_gemm<T>('C', 'N', w, w, M, one, V1, R, sV1, V1, R, sV1, zero, G, NB, sT, batch, s);
-        hipLaunchKernelGGL((k_qr_tmat<T, NB>), dim3(batch), dim3(64), 0, s, C, k, w, G, tauQ, Tm);
+        hipLaunchKernelGGL((k_qr_tmat<T, NB>), dim3(batch), dim3(256), 0, s, C, k, w, G, tauQ, Tm);
         cx<T> *A2 = Z + k + (size_t)(k + w) * ldz;
         // W1 = T1^H V1^H A (rows 0..w-1 of Bm)
         launch_gemm<T>('C', 'N', w, N, M, one, V1, R, sV1, A2, ldz, sZ, zero, Wt, NB, sW, batch, s);
@@ -635,7 +646,7 @@ ia_status launch_gebrd_band(int R, int C, cx<T> *Z, int ldz, size_t sZ, cx<T> *t
         const cx<T> *V2 = Vw2 + k;
         launch_gemm<T>('C', 'N', nbp, nbp, N2, one, V2, Cp, sV2, V2, Cp, sV2, zero, G, NB, sT,
                        batch, s);
-        hipLaunchKernelGGL((k_qr_tmat<T, NB>), dim3(batch), dim3(64), 0, s, Cp, k, nbp, G, tauP, Tm);
+        hipLaunchKernelGGL((k_qr_tmat<T, NB>), dim3(batch), dim3(256), 0, s, Cp, k, nbp, G, tauP, Tm);
         // V2^H as rows w..w+nbp-1 of Bm
         hipLaunchKernelGGL(k_conj_rows<T>, dim3((N2 + 31) / 32, batch), dim3(256), 0, s, k, N2, nbp,
                            Vw2, Cp, sV2, Bm, NB2, sB, w);
