@@ -391,6 +391,37 @@ __global__ void k_laswp(int n, int k, int jb, cx<T> *A, int lda, size_t strideA,
     }
 }
 
+// x <- op(block)^-1 x for one right-hand side held in registers; JB > 0
+// fixes the block size at compile time, JB == 0 takes it from jb
+template <typename T, int MODE, int JB>
+__device__ __forceinline__ void trsm_diag_solve(cx<T> (&x)[NB], const cx<T> (&Ts)[NB][NB + 1],
+                                                const cx<T> (&dinv)[NB], int jb = NB) {
+    const int n = JB > 0 ? JB : jb;
+    if (MODE == 0 || MODE == 1) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            if (i < n) {
+                cx<T> sacc = x[i];
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+                    if (j < i) sacc = sacc - Ts[i][j] * x[j];
+                x[i] = (MODE == 0) ? sacc : sacc * dinv[i];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = NB - 1; i >= 0; --i) {
+            if (i < n) {
+                cx<T> sacc = x[i];
+#pragma unroll
+                for (int j = 0; j < NB; ++j)
+                    if (j > i && j < n) sacc = sacc - Ts[i][j] * x[j];
+                x[i] = (MODE == 3) ? sacc : sacc * dinv[i];
+            }
+        }
+    }
+}
+
 // Diagonal-block triangular solve, one thread per right-hand-side column.
 //   MODE 0: unit lower L (from A's lower part)        forward
 //   MODE 1: U^H, U non-unit upper (A's upper part)     forward
@@ -411,18 +442,37 @@ __global__ __launch_bounds__(NC) void k_trsm_diag_lds(int n, int i0, int jb, con
     const int b = blockIdx.y, t = threadIdx.x;
     const int col0 = blockIdx.x * NC;
     const cx<T> *Ab = A + (size_t)b * strideA;
-    for (int e = t; e < jb * jb; e += NC) {
-        const int r = e % jb, c = e / jb;
-        const cx<T> v = Ab[(size_t)(i0 + c) * lda + (i0 + r)];  // A[r][c]
-        if (MODE == 0 || MODE == 2)
-            Ts[r][c] = v;
-        else
-            Ts[c][r] = conj(v);
-    }
     cx<T> *Bb = B + (size_t)b * strideB + i0;
-    for (int e = t; e < NB * NC; e += NC) {
-        const int i = e % NB, c = e / NB;
-        if (i < jb && col0 + c < nrhs) tile[i][c] = Bb[(size_t)(col0 + c) * ldb + i];
+    // every load of the block and of the tile is issued before the first
+    // LDS store (clamped addresses, out-of-range values dropped), so the
+    // staging costs one memory latency rather than one per element
+    static_assert((NB * NB) % NC == 0 && (NB * NC) % NC == 0, "whole rounds");
+    constexpr int LA = NB * NB / NC;
+    cx<T> va[LA], vb[NB];
+#pragma unroll
+    for (int m = 0; m < LA; ++m) {
+        const int e = t + m * NC, r = min(e % NB, jb - 1), c = min(e / NB, jb - 1);
+        va[m] = Ab[(size_t)(i0 + c) * lda + (i0 + r)];  // A[r][c]
+    }
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        const int e = t + m * NC, i = min(e % NB, jb - 1), c = min(col0 + e / NB, nrhs - 1);
+        vb[m] = Bb[(size_t)c * ldb + i];
+    }
+#pragma unroll
+    for (int m = 0; m < LA; ++m) {
+        const int e = t + m * NC, r = e % NB, c = e / NB;
+        if (r < jb && c < jb) {
+            if (MODE == 0 || MODE == 2)
+                Ts[r][c] = va[m];
+            else
+                Ts[c][r] = conj(va[m]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+        const int e = t + m * NC, i = e % NB, c = e / NB;
+        if (i < jb && col0 + c < nrhs) tile[i][c] = vb[m];
     }
     __syncthreads();
     if ((MODE == 1 || MODE == 2) && t < jb) dinv[t] = cdiv(cx<T>{1, 0}, Ts[t][t]);
@@ -430,29 +480,12 @@ __global__ __launch_bounds__(NC) void k_trsm_diag_lds(int n, int i0, int jb, con
     cx<T> x[NB];
 #pragma unroll
     for (int i = 0; i < NB; ++i) x[i] = i < jb ? tile[i][t] : cx<T>{0, 0};
-    if (MODE == 0 || MODE == 1) {
-#pragma unroll
-        for (int i = 0; i < NB; ++i) {
-            if (i < jb) {
-                cx<T> sacc = x[i];
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-                    if (j < i) sacc = sacc - Ts[i][j] * x[j];
-                x[i] = (MODE == 0) ? sacc : sacc * dinv[i];
-            }
-        }
-    } else {
-#pragma unroll
-        for (int i = NB - 1; i >= 0; --i) {
-            if (i < jb) {
-                cx<T> sacc = x[i];
-#pragma unroll
-                for (int j = 0; j < NB; ++j)
-                    if (j > i && j < jb) sacc = sacc - Ts[i][j] * x[j];
-                x[i] = (MODE == 3) ? sacc : sacc * dinv[i];
-            }
-        }
-    }
+    // full blocks take a branch-free copy of the solve, whose LDS reads the
+    // compiler can issue ahead of the dependent products
+    if (jb == NB)
+        trsm_diag_solve<T, MODE, NB>(x, Ts, dinv);
+    else
+        trsm_diag_solve<T, MODE, 0>(x, Ts, dinv, jb);
 #pragma unroll
     for (int i = 0; i < NB; ++i)
         if (i < jb) tile[i][t] = x[i];
