@@ -126,8 +126,25 @@ __global__ __launch_bounds__(BT) void k_qr_panel_lds(int R, int C, int j0, int n
     for (int cs = j0; cs < j0 + nbp; cs += SB) {
         const int w = min(SB, j0 + nbp - cs);
         const int r0 = cs - j0;  // local row of the sub-panel's first diagonal
-        for (int i = 0; i < w; ++i)
-            for (int r = t; r < M; r += BT) Vs[(size_t)i * M + r] = A[(size_t)(cs + i) * ldy + j0 + r];
+        {
+            // all of a thread's loads in flight before its LDS stores (the
+            // launcher sizes the LDS to at most 128 KB, so M <= LROWS * BT)
+            constexpr int LROWS = (128 * 1024 / ((int)sizeof(cx<T>) * SB) + BT - 1) / BT;
+            cx<T> v[LROWS][SB];
+#pragma unroll
+            for (int q = 0; q < LROWS; ++q) {
+                const int r = min(t + q * BT, M - 1);
+#pragma unroll
+                for (int i = 0; i < SB; ++i) v[q][i] = A[(size_t)(cs + min(i, w - 1)) * ldy + j0 + r];
+            }
+#pragma unroll
+            for (int q = 0; q < LROWS; ++q) {
+                const int r = t + q * BT;
+#pragma unroll
+                for (int i = 0; i < SB; ++i)
+                    if (i < w && r < M) Vs[(size_t)i * M + r] = v[q][i];
+            }
+        }
         __syncthreads();
         // ---- unblocked QR of the sub-panel in LDS; pass i applies
         // reflector i - 1 and gathers the sums of column i
