@@ -193,7 +193,7 @@ typedef struct {
     /* IA_PREC_MIXED only: a frequency whose selection is nearly tied (the
      * GCV / L-curve value of the runner-up parameter within this relative
      * gap of the chosen one) is solved again in FP64 and its outputs
-     * replaced.  0 = 2e-5, < 0 = never. */
+     * replaced.  0 = 2e-4, < 0 = never. */
     double recheck_gap;
 } ia_sweep_desc;
 

@@ -457,7 +457,7 @@ def esm_sweep(k, r_s, src, r_h, b, area, normals=None, precision="fp64", filter=
     H_e v = b[f] with the selected regularisation, reconstruct the surface
     pressure and sound power.  Returns a ``SweepResult``.  In the mixed form a
     frequency whose parameter choice is within `recheck_gap` (relative; 0 =
-    2e-5, negative = never) of a tie is solved again in FP64."""
+    2e-4, negative = never) of a tie is solved again in FP64."""
     k = np.ascontiguousarray(k, dtype=np.float64).ravel()
     F = len(k)
     rs, sq, rh = soa(r_s), soa(src), soa(r_h)
