@@ -376,21 +376,27 @@ __global__ __launch_bounds__(64 * NWB) void k_bulge(int C, cx<T> *__restrict__ B
 
 // v <- H v for the stage-two reflectors of one side, in sequential order
 // (FWD, with H^H: P2^H v, Q2^H v) or reverse order (P2 v, Q2 v).  Reflector
-// (s, t) acts on entries c..c+L-1 of the vector; at time step 2 s + t the
-// reflectors in use act on disjoint ranges and any two whose ranges meet
-// run in sequential order, so the result is the sequential product.  One
-// workgroup (NWA waves) per (vector, matrix), the vector in LDS; wave w
-// takes the sweeps s = w mod NWA and loads the reflector of its next step
-// while the current one is applied.
+// (s, t) acts on entries c..c+L-1 of the vector, c = s + 1 + 32 t: the
+// reflectors of one sweep touch disjoint ranges, and (s, t) overlaps only
+// (s + 1, t - 1) and (s + 1, t) of the next sweep.  One workgroup (NWA
+// waves) per (vector, matrix) with the vector in LDS; the sweeps are taken
+// in application order p = 0, 1, ... (s = p forward, s = C - 2 - p in
+// reverse), wave w taking p = w mod NWA, and reflector t of sweep p starts
+// once sweep p - 1 has finished the reflectors it overlaps (t and t + 1
+// forward, t - 1 and t in reverse), signalled through LDS progress keys as
+// in the chase; reflectors that run together touch disjoint entries, so
+// the result is the sequential product.  The next reflector of a sweep is
+// loaded while the current one is applied.
 constexpr int NWA = 16;
 
 template <typename T, bool FWD>
 __global__ __launch_bounds__(64 * NWA) void k_bulge_apply(int C, const cx<T> *__restrict__ uall,
                                                           const cx<T> *__restrict__ tall,
-                                                          int tmax, int nsteps,
-                                                          cx<double> *__restrict__ vall,
+                                                          int tmax, cx<double> *__restrict__ vall,
                                                           size_t sv, int nvec) {
     extern __shared__ cx<double> vs[];          // C entries
+    __shared__ int prog[NWA];
+    __shared__ int bad;
     const int vec = blockIdx.x, b = blockIdx.y;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const size_t nslot = (size_t)(C - 1) * tmax;
@@ -398,57 +404,55 @@ __global__ __launch_bounds__(64 * NWA) void k_bulge_apply(int C, const cx<T> *__
     const cx<T> *tau = tall + (size_t)b * nslot;
     cx<double> *v = vall + (size_t)b * nvec * sv + (size_t)vec * sv;
     for (int i = threadIdx.x; i < C; i += blockDim.x) vs[i] = v[i];
-    // first task of this wave at a step (or -1): s = slo.. with s = wv mod NWA
-    auto first = [&](int ts) -> int {
-        if (ts < 0 || ts >= nsteps) return -1;
-        const int shi = min(C - 2, ts / 2);
-        const int lo = ts - tmax + 1;
-        const int slo = lo <= 0 ? 0 : (lo + 1) / 2;
-        for (int sw = slo + ((wv - slo % NWA) + NWA) % NWA; sw <= shi; sw += NWA) {
-            const int t = ts - 2 * sw;
-            if (t >= 0 && t < ntask_of(C, sw)) return sw;
-        }
-        return -1;
-    };
-    auto fetch = [&](int sw, int t, cx<T> &uu, cx<T> &tt) {
-        const size_t slot = (size_t)sw * tmax + t;
-        uu = lane < BNB ? u[slot * BNB + lane] : cx<T>{0, 0};
-        tt = tau[slot];
-    };
-    const int step0 = FWD ? 0 : nsteps - 1, dstep = FWD ? 1 : -1;
-    cx<T> nu{0, 0}, nt{0, 0};
-    int ns = first(step0);
-    if (ns >= 0) fetch(ns, step0 - 2 * ns, nu, nt);
+    if (lane == 0) prog[wv] = -1;
+    if (threadIdx.x == 0) bad = 0;
     __syncthreads();
-    for (int k = 0, ts = step0; k < nsteps; ++k, ts += dstep) {
-        const int s0 = ns;
-        const cx<T> cu = nu, ct = nt;
-        // prefetch the next step's first task of this wave
-        ns = first(ts + dstep);
-        if (ns >= 0) fetch(ns, ts + dstep - 2 * ns, nu, nt);
-        if (s0 >= 0) {
-            const int shi = min(C - 2, ts / 2);
-            for (int sw = s0; sw <= shi; sw += NWA) {
-                const int t = ts - 2 * sw;
-                if (t < 0 || t >= ntask_of(C, sw)) continue;
-                cx<T> uT = cu, tT = ct;
-                if (sw != s0) fetch(sw, t, uT, tT);
-                const int c = t == 0 ? sw + 1 : sw + 1 + t * BNB;
-                const int L = min(BNB, C - c);
-                const cx<double> tk = FWD ? conj(cvt<double>(tT)) : cvt<double>(tT);
-                cx<double> x{0, 0}, uu{0, 0};
-                if (lane < L) {
-                    x = vs[c + lane];
-                    uu = cvt<double>(uT);
-                }
-                const cx<double> pr = conj(uu) * x;
-                const cx<double> dot{wave_sum(pr.re), wave_sum(pr.im)};
-                if (lane < L) vs[c + lane] = x - uu * (tk * dot);
+    const int np = C - 1;  // sweeps
+    bool ok = true;
+    for (int p = wv; p < np && ok; p += NWA) {
+        const int s = FWD ? p : np - 1 - p;
+        const int nts = ntask_of(C, s);
+        // reflectors of the sweep applied before this one (p - 1)
+        const int sp = FWD ? s - 1 : s + 1;
+        const int ntp = p > 0 ? ntask_of(C, sp) : 0;
+        const size_t slot0 = (size_t)s * tmax;
+        cx<T> nu = lane < BNB ? u[slot0 * BNB + lane] : cx<T>{0, 0};
+        cx<T> nt = tau[slot0];
+        for (int t = 0; t < nts; ++t) {
+            const cx<T> cu = nu, ct = nt;
+            if (t + 1 < nts) {
+                nu = lane < BNB ? u[(slot0 + t + 1) * BNB + lane] : cx<T>{0, 0};
+                nt = tau[slot0 + t + 1];
             }
+            if (p > 0 && !wait_key(&prog[(p - 1) % NWA],
+                                   (p - 1) * KEYM + min(FWD ? t + 2 : t + 1, ntp))) {
+                ok = false;
+                break;
+            }
+            const int c = s + 1 + t * BNB;
+            const int L = min(BNB, C - c);
+            const cx<double> tk = FWD ? conj(cvt<double>(ct)) : cvt<double>(ct);
+            cx<double> x{0, 0}, uu{0, 0};
+            if (lane < L) {
+                x = vs[c + lane];
+                uu = cvt<double>(cu);
+            }
+            const cx<double> pr = conj(uu) * x;
+            const cx<double> dot{wave_sum(pr.re), wave_sum(pr.im)};
+            if (lane < L) vs[c + lane] = x - uu * (tk * dot);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (lane == 0)
+                __hip_atomic_store(&prog[wv], p * KEYM + t + 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-        __syncthreads();
     }
-    for (int i = threadIdx.x; i < C; i += blockDim.x) v[i] = vs[i];
+    if (!ok && lane == 0) bad = 1;
+    // a wave that gave up publishes completion so no other wave waits on it
+    if (lane == 0)
+        __hip_atomic_store(&prog[wv], 0x7fffffff, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
+    const double nan = __builtin_nan("");
+    for (int i = threadIdx.x; i < C; i += blockDim.x) v[i] = bad ? cx<double>{nan, nan} : vs[i];
 }
 
 }  // namespace
@@ -515,16 +519,14 @@ void launch_bulge_apply(bool fwd, int C, const cx<T> *u, const cx<T> *tau, cx<do
                         size_t sv, int nvec, int batch, hipStream_t s) {
     if (C < 2) return;
     const int tmax = ntask_of(C, 0);
-    int nsteps = 0;
-    for (int sw = 0; sw <= C - 2; ++sw) nsteps = std::max(nsteps, 2 * sw + ntask_of(C, sw));
     dim3 grid(nvec, batch);
     const size_t lds = bulge_apply_lds(C);
     if (fwd)
         hipLaunchKernelGGL((k_bulge_apply<T, true>), grid, dim3(64 * NWA), lds, s, C, u, tau, tmax,
-                           nsteps, v, sv, nvec);
+                           v, sv, nvec);
     else
         hipLaunchKernelGGL((k_bulge_apply<T, false>), grid, dim3(64 * NWA), lds, s, C, u, tau,
-                           tmax, nsteps, v, sv, nvec);
+                           tmax, v, sv, nvec);
 }
 
 #define INST(T)                                                                               \
