@@ -388,6 +388,25 @@ __global__ __launch_bounds__(64 * NWB) void k_bulge(int C, cx<T> *__restrict__ B
 // the result is the sequential product.  The next reflector of a sweep is
 // loaded while the current one is applied.
 constexpr int NWA = 16;
+static_assert(BNB == 32, "one reflector entry per lane of each half-wave");
+
+// (sum of v over lanes 0..31, sum over lanes 32..63), in every lane: the
+// row sums of the DPP wave sum, then row_bcast:15 into rows 1 and 3 only
+__device__ __forceinline__ cx<double> half_sums(double v) {
+    using namespace hh::dpp;
+    v += mv<QP_1032>(v);
+    v += mv<QP_2301>(v);
+    v += mv<HALF_MIRROR>(v);
+    v += mv<MIRROR>(v);
+    v += mv<BCAST15, 0xA>(v);
+    const long long w = __builtin_bit_cast(long long, v);
+    auto lane_val = [&](int l) {
+        const int lo = __builtin_amdgcn_readlane((int)w, l);
+        const int hi = __builtin_amdgcn_readlane((int)(w >> 32), l);
+        return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+    };
+    return {lane_val(31), lane_val(63)};
+}
 
 template <typename T, bool FWD>
 __global__ __launch_bounds__(64 * NWA) void k_bulge_apply(int C, const cx<T> *__restrict__ uall,
@@ -416,12 +435,12 @@ __global__ __launch_bounds__(64 * NWA) void k_bulge_apply(int C, const cx<T> *__
         const int sp = FWD ? s - 1 : s + 1;
         const int ntp = p > 0 ? ntask_of(C, sp) : 0;
         const size_t slot0 = (size_t)s * tmax;
-        cx<T> nu = lane < BNB ? u[slot0 * BNB + lane] : cx<T>{0, 0};
+        cx<T> nu = u[slot0 * BNB + (lane & 31)];  // both halves hold the reflector
         cx<T> nt = tau[slot0];
         for (int t = 0; t < nts; ++t) {
             const cx<T> cu = nu, ct = nt;
             if (t + 1 < nts) {
-                nu = lane < BNB ? u[(slot0 + t + 1) * BNB + lane] : cx<T>{0, 0};
+                nu = u[(slot0 + t + 1) * BNB + (lane & 31)];
                 nt = tau[slot0 + t + 1];
             }
             if (p > 0 && !wait_key(&prog[(p - 1) % NWA],
@@ -432,13 +451,17 @@ __global__ __launch_bounds__(64 * NWA) void k_bulge_apply(int C, const cx<T> *__
             const int c = s + 1 + t * BNB;
             const int L = min(BNB, C - c);
             const cx<double> tk = FWD ? conj(cvt<double>(ct)) : cvt<double>(ct);
+            // lane q and lane 32 + q both hold entry q: the low half sums the
+            // real parts of conj(u) x, the high half the imaginary parts, so
+            // one 32-lane reduction serves both
+            const int q = lane & 31;
             cx<double> x{0, 0}, uu{0, 0};
-            if (lane < L) {
-                x = vs[c + lane];
+            if (q < L) {
+                x = vs[c + q];
                 uu = cvt<double>(cu);
             }
             const cx<double> pr = conj(uu) * x;
-            const cx<double> dot{wave_sum(pr.re), wave_sum(pr.im)};
+            const cx<double> dot = half_sums(lane < 32 ? pr.re : pr.im);
             if (lane < L) vs[c + lane] = x - uu * (tk * dot);
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             if (lane == 0)
