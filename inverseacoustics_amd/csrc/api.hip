@@ -89,10 +89,10 @@ struct Timer {
 // runtime has shut down.
 template <typename T>
 struct SweepCache {
-    DevBuf bD, bY, bipiv, binfo, bc, bbeta, bphi, bx, bq, bps, bpsd, bn2, bcn2;
+    DevBuf bD, bY, bipiv, bperm, binfo, bc, bbeta, bphi, bx, bq, bps, bpsd, bn2, bcn2;
     SvdWork<T> sw;
     void release() {
-        for (DevBuf *b : {&bD, &bY, &bipiv, &binfo, &bc, &bbeta, &bphi, &bx, &bq, &bps, &bpsd,
+        for (DevBuf *b : {&bD, &bY, &bipiv, &bperm, &binfo, &bc, &bbeta, &bphi, &bx, &bq, &bps, &bpsd,
                           &bn2, &bcn2, &sw.d, &sw.e, &sw.sig, &sw.tauL, &sw.tauR, &sw.Vr, &sw.QBt,
                           &sw.PBt, &sw.work, &sw.gw, &sw.Rb, &sw.tauQ, &sw.qw, &sw.Bs, &sw.uR2,
                           &sw.uL2, &sw.tR2, &sw.tL2, &sw.Pm, &sw.tauP})
@@ -124,10 +124,11 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
     SweepCache<T> &W = sweep_cache<T>();
     DevBuf &bD = W.bD, &bY = W.bY, &bipiv = W.bipiv, &binfo = W.binfo, &bc = W.bc,
            &bbeta = W.bbeta, &bphi = W.bphi, &bx = W.bx, &bq = W.bq, &bps = W.bps, &bpsd = W.bpsd,
-           &bn2 = W.bn2, &bcn2 = W.bcn2;
+           &bn2 = W.bn2, &bcn2 = W.bcn2, &bperm = W.bperm;
     IA_TRY(bD.alloc(sizeof(cx<T>) * (size_t)chunk * m * m));
     IA_TRY(bY.alloc(sizeof(cx<T>) * (size_t)chunk * m * l));
     IA_TRY(bipiv.alloc(sizeof(int) * (size_t)chunk * m));
+    IA_TRY(bperm.alloc(2 * sizeof(int) * (size_t)chunk * m));
     IA_TRY(binfo.alloc(sizeof(int) * (size_t)chunk));
     IA_TRY(bc.alloc(sizeof(cx<double>) * (size_t)chunk * C));
     IA_TRY(bbeta.alloc(sizeof(cx<double>) * (size_t)chunk * C));
@@ -158,14 +159,24 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
         tm.n = 0;
         tm.mark();  // 0
         launch_esm_D<T>(m, nf, kf, r_s, src, nrm, Dm, (size_t)m * m, s, herm_D);
-        launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s);
+        if (!herm_D) launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s);
         IA_HIP(hipGetLastError());
         tm.mark();  // 1
         IA_HIP(hipMemsetAsync(binfo.p, 0, sizeof(int) * nf, s));
         IA_TRY(getrf_batched<T>(m, Dm, m, (size_t)m * m, bipiv.as<int>(), binfo.as<int>(), nf, s));
         tm.mark();  // 2
-        IA_TRY(getrs_batched<T>(herm_D ? 'N' : 'C', m, l, Dm, m, (size_t)m * m, bipiv.as<int>(),
-                                Y, m, (size_t)m * l, nf, s));
+        if (herm_D) {
+            // P D^H = L U: M^H is assembled with its rows already in the
+            // pivoted order P M^H, so the solve is the two triangular
+            // solves in place (no row gather or copy of the right-hand sides)
+            int *pidx = bperm.as<int>();
+            IA_TRY(lu_perm_index(m, bipiv.as<int>(), pidx, pidx + (size_t)m * nf, nf, s));
+            launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s, pidx);
+            IA_TRY(getrs_prepermuted<T>(m, l, Dm, m, (size_t)m * m, Y, m, (size_t)m * l, nf, s));
+        } else {
+            IA_TRY(getrs_batched<T>('C', m, l, Dm, m, (size_t)m * m, bipiv.as<int>(), Y, m,
+                                    (size_t)m * l, nf, s));
+        }
         tm.mark();  // 3
         sw.batch = nf;
         IA_TRY(svd_reduce<T>(sw, Y, s, &clk));

@@ -122,11 +122,14 @@ __global__ __launch_bounds__(256) void k_esm_D(int m, const double *__restrict__
 }
 
 // M (rows x m) or M^H (m x rows): i rho c k e^{ikR}/(4 pi R), R = |p_i - q_j|.
+// With rowperm (CT only, m entries per frequency) output row r of M^H is
+// source rowperm[r], i.e. the rows come out in a pivoted order P M^H.
 template <typename T, bool CT>
 __global__ __launch_bounds__(256) void k_esm_M(int rows, int m, const double *__restrict__ kv,
                                               double rho_c, const double *__restrict__ pr,
                                               const double *__restrict__ src,
-                                              cx<T> *__restrict__ M, size_t strideM) {
+                                              cx<T> *__restrict__ M, size_t strideM,
+                                              const int *__restrict__ rowperm) {
     // CT=false: tile over (i rows of M, j sources); thread owns row i.
     // CT=true : output is m x rows; tile over (j rows of M^H, i columns);
     //           thread owns row j, walks columns i (mics) staged in LDS.
@@ -155,7 +158,8 @@ __global__ __launch_bounds__(256) void k_esm_M(int rows, int m, const double *__
     if (r >= nr_out) return;
     const T k = (T)kv[f];
     const T amp = (T)(rho_c * kv[f] * INV4PI);
-    const T x = (T)rowpts[r], y = (T)rowpts[r + nrp], z = (T)rowpts[r + 2 * nrp];
+    const int rp = (CT && rowperm) ? rowperm[(size_t)f * m + r] : r;
+    const T x = (T)rowpts[rp], y = (T)rowpts[rp + nrp], z = (T)rowpts[rp + 2 * nrp];
     cx<T> *Mf = M + (size_t)f * strideM;
     const int cb = (t >> 6) * 16;
     for (int cc = 0; cc < 16; ++cc) {
@@ -241,15 +245,15 @@ void launch_esm_D(int m, int nf, const double *k, const double *r_s, const doubl
 template <typename T>
 void launch_esm_M(int rows, int m, int nf, const double *k, double rho_c, const double *r_rows,
                   const double *src, cx<T> *M, size_t strideM, bool conj_transpose,
-                  hipStream_t s) {
+                  hipStream_t s, const int *rowperm) {
     if (conj_transpose) {
         dim3 grid((m + TILE - 1) / TILE, (rows + TILE - 1) / TILE, nf);
         hipLaunchKernelGGL((k_esm_M<T, true>), grid, dim3(256), 0, s, rows, m, k, rho_c, r_rows,
-                           src, M, strideM);
+                           src, M, strideM, rowperm);
     } else {
         dim3 grid((rows + TILE - 1) / TILE, (m + TILE - 1) / TILE, nf);
         hipLaunchKernelGGL((k_esm_M<T, false>), grid, dim3(256), 0, s, rows, m, k, rho_c, r_rows,
-                           src, M, strideM);
+                           src, M, strideM, nullptr);
     }
 }
 
@@ -267,9 +271,11 @@ template void launch_esm_D<double>(int, int, const double *, const double *, con
 template void launch_esm_D<float>(int, int, const double *, const double *, const double *,
                                   const double *, cx<float> *, size_t, hipStream_t, bool);
 template void launch_esm_M<double>(int, int, int, const double *, double, const double *,
-                                   const double *, cx<double> *, size_t, bool, hipStream_t);
+                                   const double *, cx<double> *, size_t, bool, hipStream_t,
+                                   const int *);
 template void launch_esm_M<float>(int, int, int, const double *, double, const double *,
-                                  const double *, cx<float> *, size_t, bool, hipStream_t);
+                                  const double *, cx<float> *, size_t, bool, hipStream_t,
+                                  const int *);
 template void launch_esm_msq_apply<double>(int, int, const double *, double, const double *,
                                            const double *, const cx<double> *, size_t,
                                            cx<double> *, size_t, hipStream_t);

@@ -125,7 +125,8 @@ void launch_esm_D(int m, int nf, const double *k, const double *r_s, const doubl
                   const double *nrm, cx<T> *D, size_t strideD, hipStream_t s, bool herm = false);
 template <typename T>
 void launch_esm_M(int rows, int m, int nf, const double *k, double rho_c, const double *r_rows,
-                  const double *src, cx<T> *M, size_t strideM, bool conj_transpose, hipStream_t s);
+                  const double *src, cx<T> *M, size_t strideM, bool conj_transpose, hipStream_t s,
+                  const int *rowperm = nullptr);
 template <typename T>
 void launch_esm_msq_apply(int m, int nf, const double *k, double rho_c, const double *r_s,
                           const double *src, const cx<T> *q, size_t strideq, cx<T> *p,
@@ -147,6 +148,14 @@ ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_
                         const int *ipiv, cx<T> *B, int ldb, size_t strideB, int batch,
                         hipStream_t s);
 
+// getrf's interchanges as a row index per matrix: (P B)[r] = B[idx[r]],
+// inv the inverse permutation (n ints each per matrix)
+ia_status lu_perm_index(int n, const int *ipiv, int *idx, int *inv, int batch, hipStream_t s);
+// A X = B for B given already in pivoted row order (B = P B0): the two
+// triangular solves of getrs op 'N', in place
+template <typename T>
+ia_status getrs_prepermuted(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
+                            int ldb, size_t strideB, int batch, hipStream_t s);
 // upper-triangular solve A X = B with the upper triangle of A (non-unit)
 template <typename T>
 ia_status trsm_upper_batched(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,

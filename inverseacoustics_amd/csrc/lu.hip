@@ -1116,6 +1116,31 @@ ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_
     return IA_OK;
 }
 
+ia_status lu_perm_index(int n, const int *ipiv, int *idx, int *inv, int batch, hipStream_t s) {
+    if (n <= 0 || batch <= 0) return IA_OK;
+    const size_t plds = 2 * sizeof(int) * (size_t)n;
+    if (plds <= 160 * 1024) {
+        (void)hipFuncSetAttribute((const void *)k_perm_index,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds);
+        hipLaunchKernelGGL(k_perm_index, dim3(batch), dim3(256), plds, s, n, ipiv, idx, inv);
+    } else {
+        hipLaunchKernelGGL(k_perm_index_global, dim3((batch + 63) / 64), dim3(64), 0, s, n, ipiv,
+                           idx, inv, batch);
+    }
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
+template <typename T>
+ia_status getrs_prepermuted(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
+                            int ldb, size_t strideB, int batch, hipStream_t s) {
+    if (n <= 0 || nrhs <= 0 || batch <= 0) return IA_OK;
+    trsm_blocked<T, 0>(n, nrhs, A, lda, strideA, B, ldb, strideB, batch, s);
+    trsm_blocked<T, 2>(n, nrhs, A, lda, strideA, B, ldb, strideB, batch, s);
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
 template <typename T>
 ia_status trsm_upper_batched(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
                              int ldb, size_t strideB, int batch, hipStream_t s) {
@@ -1127,6 +1152,10 @@ ia_status trsm_upper_batched(int n, int nrhs, const cx<T> *A, int lda, size_t st
 
 template ia_status trsm_upper_batched<double>(int, int, const cx<double> *, int, size_t,
                                               cx<double> *, int, size_t, int, hipStream_t);
+template ia_status getrs_prepermuted<double>(int, int, const cx<double> *, int, size_t,
+                                             cx<double> *, int, size_t, int, hipStream_t);
+template ia_status getrs_prepermuted<float>(int, int, const cx<float> *, int, size_t, cx<float> *,
+                                            int, size_t, int, hipStream_t);
 template ia_status getrf_batched<double>(int, cx<double> *, int, size_t, int *, int *, int,
                                          hipStream_t);
 template ia_status getrf_batched<float>(int, cx<float> *, int, size_t, int *, int *, int,
