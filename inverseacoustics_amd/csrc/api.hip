@@ -174,8 +174,13 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
             launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s, pidx);
             IA_TRY(getrs_prepermuted<T>(m, l, Dm, m, (size_t)m * m, Y, m, (size_t)m * l, nf, s));
         } else {
-            IA_TRY(getrs_batched<T>('C', m, l, Dm, m, (size_t)m * m, bipiv.as<int>(), Y, m,
-                                    (size_t)m * l, nf, s));
+            // D = P^T L U: U^H L^H V = M^H in place leaves V = P H_e^H in Y.
+            // The SVD route needs no more: the QR of P X has the R factor of
+            // X and Q_X = P^T Q_V, so only the filtered solution x is
+            // permuted back (after the left vectors are applied)
+            IA_TRY(getrs_c_permuted<T>(m, l, Dm, m, (size_t)m * m, Y, m, (size_t)m * l, nf, s));
+            int *pidx = bperm.as<int>();
+            IA_TRY(lu_perm_index(m, bipiv.as<int>(), pidx, pidx + (size_t)m * nf, nf, s));
         }
         tm.mark();  // 3
         sw.batch = nf;
@@ -208,10 +213,13 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
         IA_HIP(hipGetLastError());
         tm.mark();  // 7
         // post: q = D^-1 x, p_s = M_sq q, W = 1/2 Re(x^H p_s) S/m
-        if (sizeof(T) == sizeof(double))
-            IA_HIP(hipMemcpyAsync(bq.p, bx.p, sizeof(cx<double>) * (size_t)nf * m,
+        if (sizeof(T) == sizeof(double)) {
+            // Y held P X: x = P^T x_Y, x[r] = x_Y[inv[r]]
+            launch_row_gather<double>(m, 1, bperm.as<int>() + (size_t)m * nf, bx.as<cx<double>>(),
+                                      m, (size_t)m, bq.as<cx<double>>(), m, (size_t)m, nf, s);
+            IA_HIP(hipMemcpyAsync(bx.p, bq.p, sizeof(cx<double>) * (size_t)nf * m,
                                   hipMemcpyDeviceToDevice, s));
-        else
+        } else
             launch_convert<float, double>((size_t)nf * m, bx.as<cx<double>>(),
                                           reinterpret_cast<cx<float> *>(bq.p), s);
         IA_TRY(getrs_batched<T>(herm_D ? 'C' : 'N', m, 1, Dm, m, (size_t)m * m, bipiv.as<int>(),

@@ -156,6 +156,15 @@ ia_status lu_perm_index(int n, const int *ipiv, int *idx, int *inv, int batch, h
 template <typename T>
 ia_status getrs_prepermuted(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
                             int ldb, size_t strideB, int batch, hipStream_t s);
+// A^H V = B with A = P^T L U, leaving V = P X (X the solution) in B: the two
+// triangular solves of getrs op 'C' without the final row permutation
+template <typename T>
+ia_status getrs_c_permuted(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
+                           int ldb, size_t strideB, int batch, hipStream_t s);
+// dst[r, c] = src[perm[r], c] per matrix (perm: n ints per matrix)
+template <typename T>
+void launch_row_gather(int n, int nrhs, const int *perm, const cx<T> *src, int lds, size_t sS,
+                       cx<T> *dst, int ldd, size_t sD, int batch, hipStream_t s);
 // upper-triangular solve A X = B with the upper triangle of A (non-unit)
 template <typename T>
 ia_status trsm_upper_batched(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,

@@ -1142,6 +1142,25 @@ ia_status getrs_prepermuted(int n, int nrhs, const cx<T> *A, int lda, size_t str
 }
 
 template <typename T>
+ia_status getrs_c_permuted(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
+                           int ldb, size_t strideB, int batch, hipStream_t s) {
+    if (n <= 0 || nrhs <= 0 || batch <= 0) return IA_OK;
+    trsm_blocked<T, 1>(n, nrhs, A, lda, strideA, B, ldb, strideB, batch, s);
+    trsm_blocked<T, 3>(n, nrhs, A, lda, strideA, B, ldb, strideB, batch, s);
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
+template <typename T>
+void launch_row_gather(int n, int nrhs, const int *perm, const cx<T> *src, int lds, size_t sS,
+                       cx<T> *dst, int ldd, size_t sD, int batch, hipStream_t s) {
+    if (n <= 0 || nrhs <= 0 || batch <= 0) return;
+    dim3 g((n + 255) / 256, nrhs, batch);
+    hipLaunchKernelGGL(k_row_gather<T>, g, dim3(256), 0, s, n, nrhs, perm, src, lds, sS, dst, ldd,
+                       sD);
+}
+
+template <typename T>
 ia_status trsm_upper_batched(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
                              int ldb, size_t strideB, int batch, hipStream_t s) {
     if (n <= 0 || nrhs <= 0 || batch <= 0) return IA_OK;
@@ -1152,6 +1171,14 @@ ia_status trsm_upper_batched(int n, int nrhs, const cx<T> *A, int lda, size_t st
 
 template ia_status trsm_upper_batched<double>(int, int, const cx<double> *, int, size_t,
                                               cx<double> *, int, size_t, int, hipStream_t);
+template ia_status getrs_c_permuted<double>(int, int, const cx<double> *, int, size_t,
+                                            cx<double> *, int, size_t, int, hipStream_t);
+template ia_status getrs_c_permuted<float>(int, int, const cx<float> *, int, size_t, cx<float> *,
+                                           int, size_t, int, hipStream_t);
+template void launch_row_gather<double>(int, int, const int *, const cx<double> *, int, size_t,
+                                        cx<double> *, int, size_t, int, hipStream_t);
+template void launch_row_gather<float>(int, int, const int *, const cx<float> *, int, size_t,
+                                       cx<float> *, int, size_t, int, hipStream_t);
 template ia_status getrs_prepermuted<double>(int, int, const cx<double> *, int, size_t,
                                              cx<double> *, int, size_t, int, hipStream_t);
 template ia_status getrs_prepermuted<float>(int, int, const cx<float> *, int, size_t, cx<float> *,
