@@ -1081,14 +1081,11 @@ ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_
     IA_HIP(hipMallocAsync((void **)&idx, sizeof(int) * 2 * (size_t)n * batch, s));
     IA_HIP(hipMallocAsync((void **)&tmp, sizeof(cx<T>) * (size_t)n * nrhs * batch, s));
     int *inv = idx + (size_t)n * batch;
-    const size_t plds = 2 * sizeof(int) * (size_t)n;
-    if (plds <= 160 * 1024) {
-        (void)hipFuncSetAttribute((const void *)k_perm_index,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)plds);
-        hipLaunchKernelGGL(k_perm_index, dim3(batch), dim3(256), plds, s, n, ipiv, idx, inv);
-    } else {
-        hipLaunchKernelGGL(k_perm_index_global, dim3((batch + 63) / 64), dim3(64), 0, s, n, ipiv,
-                           idx, inv, batch);
+    const ia_status ps = lu_perm_index(n, ipiv, idx, inv, batch, s);
+    if (ps != IA_OK) {
+        (void)hipFreeAsync(tmp, s);
+        (void)hipFreeAsync(idx, s);
+        return ps;
     }
     dim3 pg((n + 255) / 256, nrhs, batch);
     const size_t sT = (size_t)n * nrhs;
