@@ -215,10 +215,20 @@ __global__ __launch_bounds__(256) void k_esm_msq_apply(int m, const double *__re
         const int jn = min(256, m - j0);
         for (int jj = 0; jj < jn; ++jj) {
             T dx = x - sq[0][jj], dy = y - sq[1][jj], dz = z - sq[2][jj];
-            T R = sqrt(dx * dx + dy * dy + dz * dz);
-            T sn, cs;
-            sincos_t<T>(k * R, &sn, &cs);
-            T a = amp / R;
+            T sn, cs, a;
+            if constexpr (sizeof(T) == sizeof(float)) {
+                // the complex64 form's surface pressure only feeds the sound
+                // power: hardware reciprocal square root and sine / cosine
+                // (absolute error ~1e-6 per entry, far below the 0.1 dB budget)
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float ri = rsqrtf(r2);
+                __sincosf(k * (r2 * ri), &sn, &cs);
+                a = amp * ri;
+            } else {
+                const T R = sqrt(dx * dx + dy * dy + dz * dz);
+                sincos_t<T>(k * R, &sn, &cs);
+                a = amp / R;
+            }
             T mre = -a * sn, mim = a * cs;
             cx<T> qv = sv[jj];
             acc_re += mre * qv.re - mim * qv.im;
