@@ -175,8 +175,25 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
     if (t == 0) zpiv = 0;
     for (int cs = k; cs < k + jb; cs += SB) {
         const int w = min(SB, k + jb - cs);
-        for (int i = 0; i < w; ++i)
-            for (int r = t; r < M; r += LBT) Vs[(size_t)i * M + r] = Ab[(size_t)(cs + i) * lda + k + r];
+        {
+            // all of a thread's loads in flight before its LDS stores (the
+            // launcher sizes the LDS to at most LDS_CAP, so M <= LROWS * LBT)
+            constexpr int LROWS = (int)((LDS_CAP / (sizeof(cx<T>) * SB) + LBT - 1) / LBT);
+            cx<T> v[LROWS][SB];
+#pragma unroll
+            for (int q = 0; q < LROWS; ++q) {
+                const int r = min(t + q * LBT, M - 1);
+#pragma unroll
+                for (int i = 0; i < SB; ++i) v[q][i] = Ab[(size_t)(cs + min(i, w - 1)) * lda + k + r];
+            }
+#pragma unroll
+            for (int q = 0; q < LROWS; ++q) {
+                const int r = t + q * LBT;
+#pragma unroll
+                for (int i = 0; i < SB; ++i)
+                    if (i < w && r < M) Vs[(size_t)i * M + r] = v[q][i];
+            }
+        }
         __syncthreads();
         // pivot candidates of the sub-panel's first column; later columns'
         // candidates are gathered by the update pass before them
