@@ -367,15 +367,17 @@ __global__ __launch_bounds__(LBT) void k_panel_lds(int n, int k, int jb, cx<T> *
     }
 }
 
-// new[dst_q] = old[src_q] for the listed rows, in every column outside
-// [k, k+jb); one wave per column (all loads before the stores).
+// new[dst_q] = old[src_q] for the listed rows, in every column of [c0, n)
+// outside [k, k+jb) (the columns left of c0 take their interchanges later,
+// in k_laswp_deferred); one wave per column (all loads before the stores).
 template <typename T>
-__global__ __launch_bounds__(256) void k_laswp_perm(int n, int k, int jb, cx<T> *A, int lda,
-                                                   size_t strideA, const int *permall) {
+__global__ __launch_bounds__(256) void k_laswp_perm(int n, int k, int jb, int c0, cx<T> *A,
+                                                   int lda, size_t strideA, const int *permall) {
     const int b = blockIdx.y;
     const int lane = threadIdx.x & 63;
     int col = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (col >= n - jb) return;
+    if (col >= n - jb - c0) return;
+    col += c0;
     if (col >= k) col += jb;
     const int *pm = permall + (size_t)b * (1 + 2 * PERM_MAX);
     const int m = pm[0];
@@ -387,14 +389,15 @@ __global__ __launch_bounds__(256) void k_laswp_perm(int n, int k, int jb, cx<T> 
     Ac[pm[1 + lane]] = v;
 }
 
-// apply interchanges ipiv[k..k+jb) to all columns outside [k, k+jb)
+// apply interchanges ipiv[k..k+jb) to the columns of [c0, n) outside [k, k+jb)
 template <typename T>
-__global__ void k_laswp(int n, int k, int jb, cx<T> *A, int lda, size_t strideA,
+__global__ void k_laswp(int n, int k, int jb, int c0, cx<T> *A, int lda, size_t strideA,
                         const int *ipiv) {
     const int b = blockIdx.y;
     int col = blockIdx.x * blockDim.x + threadIdx.x;
-    int nout = n - jb;
+    int nout = n - jb - c0;
     if (col >= nout) return;
+    col += c0;
     if (col >= k) col += jb;
     cx<T> *Ac = A + (size_t)b * strideA + (size_t)col * lda;
     const int *ip = ipiv + (size_t)b * n;
@@ -405,6 +408,71 @@ __global__ void k_laswp(int n, int k, int jb, cx<T> *A, int lda, size_t strideA,
             Ac[c] = Ac[p];
             Ac[p] = tmp;
         }
+    }
+}
+
+// Deferred interchanges of the columns left of each outer panel.  For the
+// outer-panel boundary a = (J + 1) NBO, idx_a gathers rows a..n-1 after the
+// interchanges a..n-1 in order: new[r] = old[idx_a[r]].  Going down from
+// a = n (identity), idx_a[r] = sigma_a(idx_{a+1}[r]) with sigma_a the
+// transposition (a, ipiv[a]), so every thread composes its own rows with
+// no exchange between threads.  One workgroup per matrix; idxall holds
+// (n / NBO) * n ints per matrix, boundary J at J * n.
+constexpr int DEF_ROWS = 16;  // rows per thread: deferral for n <= 256 * 16
+__global__ __launch_bounds__(256) void k_perm_suffix(int n, const int *ipiv, int *idxall) {
+    extern __shared__ int ips[];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int *ip = ipiv + (size_t)b * n;
+    for (int r = t; r < n; r += 256) ips[r] = ip[r];
+    __syncthreads();
+    int v[DEF_ROWS];
+#pragma unroll
+    for (int q = 0; q < DEF_ROWS; ++q) v[q] = t + 256 * q;
+    int *idx = idxall + (size_t)b * (n / NBO) * n;
+    const int amin = NBO;
+    for (int a = n - 1; a >= amin; --a) {
+        const int pa = ips[a];
+#pragma unroll
+        for (int q = 0; q < DEF_ROWS; ++q) v[q] = v[q] == a ? pa : v[q] == pa ? a : v[q];
+        if (a % NBO == 0) {
+            int *ia = idx + (size_t)(a / NBO - 1) * n;
+#pragma unroll
+            for (int q = 0; q < DEF_ROWS; ++q) {
+                const int r = t + 256 * q;
+                if (r >= a && r < n) ia[r] = v[q];
+            }
+        }
+    }
+}
+
+// columns [J NBO, (J + 1) NBO) with a = (J + 1) NBO < n: rows a..n-1
+// gathered through idx_a; every thread reads its rows' sources before the
+// barrier and writes after it.  Grid (columns / 8, batch), 8 columns per
+// workgroup, all in one outer panel.
+template <typename T>
+__global__ __launch_bounds__(256) void k_laswp_deferred(int n, int ncols, const int *idxall,
+                                                        cx<T> *A, int lda, size_t strideA) {
+    const int b = blockIdx.y, t = threadIdx.x;
+    const int c0 = blockIdx.x * 8;
+    if (c0 >= ncols) return;
+    const int J = c0 / NBO, a = (J + 1) * NBO;
+    const int *ia = idxall + (size_t)b * (n / NBO) * n + (size_t)J * n;
+    int src[DEF_ROWS];
+#pragma unroll
+    for (int q = 0; q < DEF_ROWS; ++q) {
+        const int r = t + 256 * q;
+        src[q] = (r >= a && r < n) ? ia[r] : -1;
+    }
+    cx<T> *Ab = A + (size_t)b * strideA;
+    for (int c = c0; c < min(c0 + 8, ncols); ++c) {
+        cx<T> *col = Ab + (size_t)c * lda;
+        cx<T> val[DEF_ROWS];
+#pragma unroll
+        for (int q = 0; q < DEF_ROWS; ++q) val[q] = src[q] >= 0 ? col[src[q]] : cx<T>{0, 0};
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < DEF_ROWS; ++q)
+            if (src[q] >= 0) col[t + 256 * q] = val[q];
     }
 }
 
@@ -1018,8 +1086,12 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
     // panels of NBO columns, each factored as NB-wide sub-panels whose
     // updates stay inside the panel; then U12 = L11^-1 A12 and one rank-NBO
     // update of the trailing matrix per panel
+    // interchanges of columns left of the current outer panel are deferred
+    // to one gather per column at the end (n <= 256 DEF_ROWS)
+    const bool defer = n > NBO && n <= 256 * DEF_ROWS;
     for (int p0 = 0; p0 < n; p0 += NBO) {
         const int pend = min(n, p0 + NBO);
+        const int c0 = defer ? p0 : 0;
         for (int k = p0; k < pend; k += NB) {
             const int jb = min(NB, pend - k);
             const size_t colb = sizeof(cx<T>) * (size_t)(n - k);
@@ -1038,18 +1110,18 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
                     go(k_panel_lds<T, 4>);
                 else
                     go(k_panel_lds<T, 2>);
-                if (n - jb > 0) {
-                    dim3 g((n - jb + 3) / 4, batch);
-                    hipLaunchKernelGGL(k_laswp_perm<T>, g, dim3(256), 0, s, n, k, jb, A, lda,
+                if (n - jb - c0 > 0) {
+                    dim3 g((n - jb - c0 + 3) / 4, batch);
+                    hipLaunchKernelGGL(k_laswp_perm<T>, g, dim3(256), 0, s, n, k, jb, c0, A, lda,
                                        strideA, (const int *)perm);
                 }
             } else {
                 hipLaunchKernelGGL(k_panel<T>, dim3(batch), dim3(512), 0, s, n, k, jb, A, lda,
                                    strideA, ipiv, info);
-                if (n - jb > 0) {
-                    dim3 g((n - jb + 255) / 256, batch);
-                    hipLaunchKernelGGL(k_laswp<T>, g, dim3(256), 0, s, n, k, jb, A, lda, strideA,
-                                       ipiv);
+                if (n - jb - c0 > 0) {
+                    dim3 g((n - jb - c0 + 255) / 256, batch);
+                    hipLaunchKernelGGL(k_laswp<T>, g, dim3(256), 0, s, n, k, jb, c0, A, lda,
+                                       strideA, ipiv);
                 }
             }
             const int rin = pend - k - jb;
@@ -1069,6 +1141,20 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
                            strideA, A + (size_t)pend * lda + p0, lda, strideA, one,
                            A + (size_t)pend * lda + pend, lda, strideA, batch, s);
         }
+    }
+    if (defer) {
+        // columns 0 .. (last outer panel start) - 1
+        const int ncols = ((n - 1) / NBO) * NBO;
+        int *idx = nullptr;
+        IA_HIP(hipMallocAsync((void **)&idx, sizeof(int) * (size_t)(n / NBO) * n * batch, s));
+        const size_t lds = sizeof(int) * (size_t)n;
+        (void)hipFuncSetAttribute((const void *)k_perm_suffix,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_perm_suffix, dim3(batch), dim3(256), lds, s, n, (const int *)ipiv,
+                           idx);
+        hipLaunchKernelGGL(k_laswp_deferred<T>, dim3((ncols + 7) / 8, batch), dim3(256), 0, s, n,
+                           ncols, (const int *)idx, A, lda, strideA);
+        IA_HIP(hipFreeAsync(idx, s));
     }
     if (perm) IA_HIP(hipFreeAsync(perm, s));
     IA_HIP(hipGetLastError());
