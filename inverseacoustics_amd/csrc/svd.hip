@@ -54,6 +54,7 @@ __device__ __forceinline__ int sturm_count(int N, const double *t2, double x, do
 
 // One workgroup per matrix (1024 threads, one value each for n <= 1024).
 constexpr int GL = 10, GPTS = 1 << GL;
+constexpr int GGEO = GPTS / 2, GLIN = GPTS - GGEO;  // geometric / linear grid points
 // Stopping width max(rtol hi, atol bound).
 __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict__ dall,
                                                 const double *__restrict__ eall,
@@ -95,10 +96,19 @@ __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict
     }
     __syncthreads();
     const double bound = s_bound, pivmin = s_pivmin;
-    // the first GL levels of every value's bisection in one step: counts at
-    // the interior points bound j / 2^GL of the dyadic grid, one per thread
+    // every value's first bisection levels in one step: Sturm counts on a
+    // shared grid, one point per thread.  Above bound / GLIN the grid is
+    // linear (GLIN points); below it geometric, 8 points per octave over 64
+    // octaves, so that small values also start from an interval of ~9 %
+    // relative width instead of [0, bound / GLIN]
+    auto gpt = [&](int j) -> double {
+        if (j <= 0) return 0.0;
+        if (j >= GPTS) return bound;
+        if (j > GGEO) return bound * ((double)(j - GGEO) / GLIN);
+        return (bound / GLIN) * exp2(-(double)(GGEO + 1 - j) / 8.0);
+    };
     for (int j = t + 1; j < GPTS; j += blockDim.x)
-        cgrid[j] = sturm_count(N, t2, bound * ((double)j / GPTS), pivmin) - n;
+        cgrid[j] = sturm_count(N, t2, gpt(j), pivmin) - n;
     __syncthreads();
     for (int i = t; i < n; i += blockDim.x) {
         const int target = n - 1 - i;  // sigma_i (descending) has target values below it
@@ -111,7 +121,7 @@ __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict
             else
                 jl = jm;
         }
-        double lo = bound * ((double)jl / GPTS), hi = jh == GPTS ? bound : bound * ((double)jh / GPTS);
+        double lo = gpt(jl), hi = gpt(jh);
         for (int it = 0; it < 128; ++it) {
             if (hi - lo <= fmax(rtol * hi, atol * bound)) break;
             double mid = 0.5 * (lo + hi);
