@@ -228,6 +228,44 @@ ia_status ia_esm_sweep_device(const ia_sweep_desc *desc, const double *k,
                               const double *nrm, const double *r_h,
                               const ia_z *b, ia_sweep_out *out, void *stream);
 
+/* ------------------------------- gradients of the ESM solve (adjoint) */
+/* No counterpart in the reference, which has no derivative code; the chain
+ * differentiated is its forward path esm.cuh:1-92 (assembly, H_e = M_hq D^-1),
+ * tikhonovG.m:29-30 (filtered solution) and utils.cuh:199-289 (surface
+ * pressure, sound power).  For each frequency f: sources q = r_s - d nrm
+ * (retreat d along the unit normals nrm), the Tikhonov solution v of
+ * H_e v = b_f at lambda_f (held fixed), the misfit J = ||H_e v - b_f||^2 and
+ * W = 1/2 Re(v^H M_sq D^-1 v) S/m, with their gradients with respect to d,
+ * the microphone coordinates r_h and log lambda_f, by a hand-written adjoint
+ * that recomputes the assembled matrices in the backward pass.  FP64,
+ * l <= m.  lambda: [n_freq] values, or NULL to select each lambda_f first by
+ * desc->reg (Tikhonov filter; the chosen value is then held fixed).  Inputs
+ * host or device pointers, outputs host or device pointers (any NULL). */
+typedef struct {
+    int m, l, n_freq;
+    ia_reg_params reg;        /* used when lambda is NULL                  */
+    double area;              /* S in W                                    */
+    int chunk;                /* frequencies per device batch (0 = auto)   */
+} ia_grad_desc;
+
+typedef struct {
+    double *lambda;           /* [n_freq] lambda used                      */
+    double *misfit;           /* [n_freq] J                                */
+    double *power;            /* [n_freq] W                                */
+    double *dmisfit_dd;       /* [n_freq] dJ/dd                            */
+    double *dpower_dd;        /* [n_freq] dW/dd                            */
+    double *dmisfit_dmic;     /* [n_freq * 3l] dJ/dr_h, SoA (x, y, z blocks)
+                                 per frequency                             */
+    double *dpower_dmic;      /* [n_freq * 3l] dW/dr_h                     */
+    double *dmisfit_dloglam;  /* [n_freq] dJ/dlog(lambda)                  */
+    double *dpower_dloglam;   /* [n_freq] dW/dlog(lambda)                  */
+    ia_z *velocity;           /* [n_freq * m] v                            */
+} ia_grad_out;
+
+ia_status ia_esm_gradient(const ia_grad_desc *desc, const double *k, const double *r_s,
+                          const double *nrm, double d, const double *r_h, const ia_z *b,
+                          const double *lambda, ia_grad_out *out);
+
 /* Batched HELS sweep with a per-frequency expansion order (uneven system
  * sizes are grouped by order): H (l x n_t) regularised solve for the
  * coefficients c, then v = Q_e c (to_velocity) and p = M_e c (to_pressure)

@@ -22,7 +22,7 @@ __all__ = [
     "tsvd", "tikhonov", "dsvd", "regularize", "to_pressure", "to_velocity", "sound_power",
     "sphere", "cartesian_2_spherical", "geometry", "esm_sweep", "SweepResult", "zgemm", "cgemm", "zgesv", "hels", "hels_sweep", "regsolve",
     "quadrature_load", "set_quadrature", "ibem", "ibem_sweep", "invr", "radiation_modes", "admm",
-    "linesource", "write_file_complex", "write_file_double", "write_file_string", "write_matrix",
+    "linesource", "esm_gradient", "write_file_complex", "write_file_double", "write_file_string", "write_matrix",
     "FILTERS", "SELECTS",
 ]
 
@@ -478,6 +478,41 @@ def esm_sweep(k, r_s, src, r_h, b, area, normals=None, precision="fp64", filter=
     return SweepResult(out["lam"], out["lam_index"], out["k_tsvd"], out["power"],
                        out["sigma_max"], out["velocity"], out["pressure"], out["status"], stats,
                        out["gap"])
+
+
+# ------------------------------------------------------------ gradients
+def esm_gradient(k, r_s, normals, d, r_h, b, area, lam=None, n_lambda=50, decades=8.0,
+                 select="gcv", chunk=0):
+    """Gradients of the ESM solve (ia_esm_gradient): sources at
+    r_s - d * normals, Tikhonov solution at lam[f] (or at the lambda selected
+    by `select` when lam is None, then held fixed).  Returns a dict with
+    lambda, misfit, power, dmisfit_dd, dpower_dd (F,), dmisfit_dmic,
+    dpower_dmic (F, l, 3), dmisfit_dloglam, dpower_dloglam (F,) and
+    velocity (F, m)."""
+    k = np.ascontiguousarray(k, dtype=np.float64).ravel()
+    F = len(k)
+    rs, nr, rh = soa(r_s), soa(normals), soa(r_h)
+    m, l = len(rs) // 3, len(rh) // 3
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).reshape(F, l))
+    desc = _lib.ia_grad_desc(m, l, F, _reg("tikhonov", select, n_lambda, decades), float(area),
+                             int(chunk))
+    out = {n: np.zeros(F) for n in ("lambda", "misfit", "power", "dmisfit_dd", "dpower_dd",
+                                     "dmisfit_dloglam", "dpower_dloglam")}
+    out["dmisfit_dmic"] = np.zeros((F, 3, l))
+    out["dpower_dmic"] = np.zeros((F, 3, l))
+    out["velocity"] = np.zeros((F, m), np.complex128)
+    o = _lib.ia_grad_out(*(out[n].ctypes.data for n in
+                           ("lambda", "misfit", "power", "dmisfit_dd", "dpower_dd",
+                            "dmisfit_dmic", "dpower_dmic", "dmisfit_dloglam", "dpower_dloglam",
+                            "velocity")))
+    lam_arr = None if lam is None else np.ascontiguousarray(
+        np.broadcast_to(np.asarray(lam, dtype=np.float64), (F,)))
+    check(lib().ia_esm_gradient(C.byref(desc), _lib.dptr(k), _lib.dptr(rs), _lib.dptr(nr),
+                                float(d), _lib.dptr(rh), _lib.zptr(b),
+                                _lib.dptr(lam_arr) if lam_arr is not None else None, C.byref(o)))
+    out["dmisfit_dmic"] = out["dmisfit_dmic"].transpose(0, 2, 1).copy()
+    out["dpower_dmic"] = out["dpower_dmic"].transpose(0, 2, 1).copy()
+    return out
 
 
 # ------------------------------------------------------------ scenarios

@@ -51,6 +51,17 @@ class ia_ibem_desc(C.Structure):
                 ("area", C.c_double), ("chunk", C.c_int)]
 
 
+class ia_grad_desc(C.Structure):
+    _fields_ = [("m", C.c_int), ("l", C.c_int), ("n_freq", C.c_int), ("reg", ia_reg_params),
+                ("area", C.c_double), ("chunk", C.c_int)]
+
+
+class ia_grad_out(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in
+                ("lambda_", "misfit", "power", "dmisfit_dd", "dpower_dd", "dmisfit_dmic",
+                 "dpower_dmic", "dmisfit_dloglam", "dpower_dloglam", "velocity")]
+
+
 class ia_admm_params(C.Structure):
     _fields_ = [("lambda0", C.c_double), ("rho", C.c_double), ("gamma", C.c_double),
                 ("epsilon", C.c_double), ("max_iter", C.c_int), ("norm", C.c_int),
@@ -114,6 +125,8 @@ SIGNATURES = {
                                 C.POINTER(ia_sweep_out)]),
     "ia_regsolve": (C.c_int, [C.c_int, C.c_int, C.c_int, _zp, _zp, C.POINTER(ia_reg_params), C.c_int,
                               _zp, _dp, _ip, _ip, _dp]),
+    "ia_esm_gradient": (C.c_int, [C.POINTER(ia_grad_desc), _dp, _dp, _dp, C.c_double, _dp, _zp,
+                                  _dp, C.POINTER(ia_grad_out)]),
     "ia_quadrature_load": (C.c_int, [C.c_char_p, C.c_char_p, _ip, _dpp, _ip, _dpp]),
     "ia_set_quadrature": (C.c_int, [C.c_int, _dp, C.c_int, _dp]),
     "ibem": (None, [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, _zpp, _zpp, C.c_double, C.c_int,

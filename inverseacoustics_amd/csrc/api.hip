@@ -303,6 +303,7 @@ void ia_release(void) {
     (void)hipDeviceSynchronize();
     sweep_cache<double>().release();
     sweep_cache<float>().release();
+    release_grad_cache();
 }
 
 ia_status ia_last_stats(ia_stats *st) {
@@ -592,6 +593,8 @@ ia_status ia_csvd(int m, int n, const ia_z *a, ia_z **u, double **s_out, ia_z **
     SvdWork<double> sw;
     IA_TRY(sw.alloc(R, C, 1));
     IA_TRY(svd_factor<double>(sw, bY.as<cx<double>>(), s));
+    launch_orthonormalise_tail(C, sw.sig.as<double>(), sw.QBt.as<double>(), sw.PBt.as<double>(), 1,
+                               s);
     // U_Y = U_b [P_B; 0] (R x C), V_Y = V_b Q_B (C x C)
     IA_TRY(bUY.alloc(sizeof(cx<double>) * (size_t)R * C));
     IA_TRY(bVY.alloc(sizeof(cx<double>) * (size_t)C * C));
@@ -628,9 +631,12 @@ static ia_status filtered_solve(int m, int n, int p, const ia_z *u, const double
     if (m <= 0 || n <= 0 || p <= 0 || !u || !s || !v || !b || !x)
         return fail(IA_ERR_ARG, "bad arguments");
     hipStream_t st = nullptr;
+    // TSVD reads only V(:, 1:k), as tsvd.cuh:145-146 copies n*k entries: a
+    // caller may pass V truncated to k columns
+    const int pv = (kind == 0) ? kk : p;
     DevBuf dU, dV, db, dbeta, dx;
     IA_TRY(to_device(dU, u, sizeof(ia_z) * (size_t)m * p, st));
-    IA_TRY(to_device(dV, v, sizeof(ia_z) * (size_t)n * p, st));
+    IA_TRY(to_device(dV, v, sizeof(ia_z) * (size_t)n * pv, st));
     IA_TRY(to_device(db, b, sizeof(ia_z) * (size_t)m, st));
     IA_TRY(dbeta.alloc(sizeof(ia_z) * p));
     IA_TRY(dx.alloc(sizeof(ia_z) * n));
@@ -651,7 +657,7 @@ static ia_status filtered_solve(int m, int n, int p, const ia_z *u, const double
         beta[i].im *= g;
     }
     IA_HIP(hipMemcpy(dbeta.p, beta.data(), sizeof(ia_z) * p, hipMemcpyHostToDevice));
-    launch_gemm<double>('N', 'N', n, 1, p, one, dV.as<cx<double>>(), n, 0,
+    launch_gemm<double>('N', 'N', n, 1, pv, one, dV.as<cx<double>>(), n, 0,
                         dbeta.as<cx<double>>(), p, 0, zero, dx.as<cx<double>>(), n, 0, 1, st);
     IA_HIP(hipGetLastError());
     IA_HIP(hipDeviceSynchronize());

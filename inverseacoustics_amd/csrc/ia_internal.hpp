@@ -218,6 +218,9 @@ template <typename T>
 void launch_bulge_apply(bool fwd, int C, const cx<T> *u, const cx<T> *tau, cx<double> *v,
                         size_t sv, int nvec, int batch, hipStream_t s);
 
+// frees the device workspace the gradient entry point keeps per host thread
+void release_grad_cache();
+
 // ADMM (admm.hip): A (rows x cols, batch stride rows*cols), b, c on the device
 ia_status admm_batched(int rows, int cols, int batch, const cx<double> *A, const cx<double> *b,
                        const ia_admm_params &P, cx<double> *c, int *iters, hipStream_t s);
@@ -265,6 +268,10 @@ size_t inviter_work_bytes(int n, int sub, bool f32 = false);
 // QBt/PBt: n x n per batch item, vector i stored as row i (QBt[i + k n]).
 void launch_inviter(int n, const double *d, const double *e, const double *sigma, double *QBt,
                     double *PBt, void *work, int sub, int batch, hipStream_t s, bool f32 = false);
+// When the full singular bases are formed (csvd): the q and p vectors of
+// every run of values closer than 1e-4 s_1 orthonormalised each on its own
+void launch_orthonormalise_tail(int n, const double *sigma, double *QBt, double *PBt, int batch,
+                                hipStream_t s);
 // v <- U_b v (herm=false) or U_b^H v, length-R vectors, nvec per batch item
 template <typename T>
 void launch_apply_left(bool herm, int R, int C, const cx<T> *Y, int ldy, size_t strideY,

@@ -364,6 +364,83 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #undef AT
 }
 
+// --------------------------------------------- clusters of multiple values
+// Inverse iteration resolves each vector to an angle of about eps s_1 / gap.
+// For an exact multiple singular value (split only by rounding, a gap of a
+// few eps s_1) every vector of the multiplet comes out as a different random
+// member of its invariant subspace, so the vectors are independent but not
+// orthogonal.  Members of such a cluster -- consecutive values closer than
+// tau_abs s_1 and tau_rel s_i, at or above floor_rel s_1 -- are made
+// orthonormal by two passes of modified Gram-Schmidt on the Golub-Kahan
+// vectors (q_i, p_i) (LAPACK's stein reorthogonalises within clusters in
+// the same way).  Any orthonormal basis of the multiplet's subspace serves:
+// the filtered solution weights all of its vectors alike.  One workgroup
+// per matrix; the cluster walk is uniform over the workgroup.
+// SEP: q and p are orthonormalised each on its own (used when the full
+// singular bases are returned: for tiny values the Golub-Kahan vectors also
+// carry components of the -s partners, so orthogonal (q, p) pairs need not
+// have orthogonal q and p parts).
+template <bool SEP>
+__global__ __launch_bounds__(BT) void k_cluster_orth(int n, const double *__restrict__ sall,
+                                                     double *__restrict__ QBt,
+                                                     double *__restrict__ PBt, double tau_abs,
+                                                     double tau_rel, double floor_rel) {
+    __shared__ double red[NW];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const double *sg = sall + (size_t)b * n;
+    double *Q = QBt + (size_t)b * n * n;
+    double *P = PBt + (size_t)b * n * n;
+    const double s1 = sg[0];
+    if (!(s1 > 0)) return;
+    const double thr = tau_abs * s1, lo = floor_rel * s1;
+    int i0 = 0;
+    while (i0 < n) {
+        int i1 = i0 + 1;
+        while (i1 < n && sg[i1] >= lo && sg[i1 - 1] - sg[i1] <= thr &&
+               sg[i1 - 1] - sg[i1] <= tau_rel * sg[i1])
+            ++i1;
+        if (i1 - i0 >= 2 && sg[i0] >= lo) {
+            for (int j = i0 + 1; j < i1; ++j) {
+                for (int pass = 0; pass < 2; ++pass) {
+                    for (int i = i0; i < j; ++i) {
+                        double dq = 0, dp = 0;
+                        for (int k = t; k < n; k += BT) {
+                            dq += Q[i + (size_t)k * n] * Q[j + (size_t)k * n];
+                            dp += P[i + (size_t)k * n] * P[j + (size_t)k * n];
+                        }
+                        double cq, cp;
+                        if (SEP) {
+                            cq = block_sum(dq, red);
+                            cp = block_sum(dp, red);
+                        } else {
+                            // |(q_i, p_i)|^2 = 2: the projection coefficient is d / 2
+                            cq = cp = 0.5 * block_sum(dq + dp, red);
+                        }
+                        for (int k = t; k < n; k += BT) {
+                            Q[j + (size_t)k * n] -= cq * Q[i + (size_t)k * n];
+                            P[j + (size_t)k * n] -= cp * P[i + (size_t)k * n];
+                        }
+                    }
+                }
+                double nq = 0, np = 0;
+                for (int k = t; k < n; k += BT) {
+                    const double q = Q[j + (size_t)k * n], pp = P[j + (size_t)k * n];
+                    nq += q * q;
+                    np += pp * pp;
+                }
+                nq = block_sum(nq, red);
+                np = block_sum(np, red);
+                const double iq = nq > 0 ? 1.0 / sqrt(nq) : 0.0, ip = np > 0 ? 1.0 / sqrt(np) : 0.0;
+                for (int k = t; k < n; k += BT) {
+                    Q[j + (size_t)k * n] *= iq;
+                    P[j + (size_t)k * n] *= ip;
+                }
+            }
+        }
+        i0 = i1;
+    }
+}
+
 // --------------------------------------------------- reflector application
 // v <- U_b v (FWD=false: H_0 H_1 ... H_{C-1} v) or U_b^H v, length-R vectors;
 // reflector j lives in column j of Y below the diagonal (unit at row j).
@@ -643,6 +720,18 @@ void launch_inviter(int n, const double *d, const double *e, const double *sigma
             hipLaunchKernelGGL(k_inviter<double>, grid, dim3(64), 0, s, n, b0, d, e, sigma, QBt,
                                PBt, work, (int)inviter_cols(n, sub));
     }
+    // multiplets: values within 1e-12 s_1 (and 1e-3 relative) of each other,
+    // at or above the pooling floor of the precision form
+    hipLaunchKernelGGL(k_cluster_orth<false>, dim3(batch), dim3(BT), 0, s, n, sigma, QBt, PBt,
+                       1e-12, 1e-3, f32 ? 1e-6 : 1e-12);
+}
+
+void launch_orthonormalise_tail(int n, const double *sigma, double *QBt, double *PBt, int batch,
+                                hipStream_t s) {
+    // every run of values closer than 1e-4 s_1 to its neighbour: beyond such
+    // gaps inverse iteration leaves orthogonality errors below ~1e-12
+    hipLaunchKernelGGL(k_cluster_orth<true>, dim3(batch), dim3(BT), 0, s, n, sigma, QBt, PBt, 1e-4,
+                       1e300, 0.0);
 }
 
 template <typename T>

@@ -32,12 +32,7 @@ def esm_solve(k, r_s, r_h, src, b, area, normals=None, rho=1.21, c=343.0, filter
             if filter == "tikhonov":
                 idx, grid, _ = rt.gcv_tikhonov_grid(s, beta, rows, delta0, n_lambda, decades)
             else:
-                grid = rt.lambda_grid(s[0], n_lambda, decades)
-                Gs = []
-                for la in grid:
-                    f = la / (s + la)
-                    Gs.append((np.sum(f * f * np.abs(beta[:p]) ** 2) + delta0) / ((rows - p) + f.sum()) ** 2)
-                idx = int(np.argmin(Gs))
+                idx, grid, _ = rt.gcv_dsvd_grid(s, beta, rows, delta0, n_lambda, decades)
             lam = grid[idx]
         elif select == "lcurve":
             idx, grid, _ = rt.l_curve_grid(s, beta, rows, "t" if filter == "tikhonov" else "d",

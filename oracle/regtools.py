@@ -162,23 +162,17 @@ def reg_grid_regtools(s, npoints=200):
     return (ratio ** np.arange(npoints - 1, -1, -1)) * rpm
 
 
-def gcv_regtools(U, s, b, method="t"):
-    """Literal restatement of gcvG.m:19-52 ('regtools-GPU' mode).
-
-    Grid: f = lam/(s + lam) for BOTH methods (gcvG.m:38).  Refinement by
-    fminbnd of 'd': f = lam/(s+lam), 't': f = lam^2/(s + lam^2) (gcvG.m:43-48)
-    between the grid neighbours of the argmin (gcvG.m:51-52).  Note this is not
-    Hansen's Tikhonov GCV; see ``gcv_tikhonov_grid`` for that.
-    Returns (reg_min, reg_param, G, minGi)."""
+def gcv_regtools_fun(U, s, b, method="t"):
+    """The scalar function gcvG.m minimises (gcvG.m:32-48):
+    G(lam) = (||f .* beta||^2 + delta0) / ((m - n) + sum f)^2 with
+    'd': f = lam/(s + lam) (also the grid filter of both methods, gcvG.m:38)
+    and 't': f = lam^2/(s + lam^2), i.e. G_t(lam) = G_d(lam^2); (m, n) is the
+    shape of U and delta0 = ||b||^2 - ||beta||^2 when m > n and positive."""
     m, n = U.shape
     p = s.shape[0]
     beta = U.conj().T @ b
     beta2 = np.linalg.norm(b) ** 2 - np.linalg.norm(beta) ** 2
-    reg_param = reg_grid_regtools(s)
     delta0 = beta2 if (m > n and beta2 > 0) else 0.0
-    f = reg_param[None, :] / (s[:, None] + reg_param[None, :])
-    G = (np.sum(np.abs(f * beta[:p, None]) ** 2, axis=0) + delta0) / ((m - n) + np.sum(f, axis=0)) ** 2
-    minGi = int(np.argmin(G))
     bp = beta[:p]
     if method == "d":
         def fun(la):
@@ -190,6 +184,23 @@ def gcv_regtools(U, s, b, method="t"):
             return (np.linalg.norm(ff * bp) ** 2 + delta0) / ((m - n) + np.sum(ff)) ** 2
     else:
         raise ValueError("method must be 't' or 'd'")
+    return fun
+
+
+def gcv_regtools(U, s, b, method="t"):
+    """Literal restatement of gcvG.m:19-52 ('regtools-GPU' mode).
+
+    Grid: f = lam/(s + lam) for BOTH methods (gcvG.m:38).  Refinement by
+    fminbnd of 'd': f = lam/(s+lam), 't': f = lam^2/(s + lam^2) (gcvG.m:43-48)
+    between the grid neighbours of the argmin (gcvG.m:51-52).  Note this is not
+    Hansen's Tikhonov GCV; see ``gcv_tikhonov_grid`` for that.
+    Returns (reg_min, reg_param, G, minGi)."""
+    p = s.shape[0]
+    reg_param = reg_grid_regtools(s)
+    fun_d = gcv_regtools_fun(U, s, b, "d")
+    G = np.array([fun_d(la) for la in reg_param])
+    minGi = int(np.argmin(G))
+    fun = gcv_regtools_fun(U, s, b, method)
     npts = len(reg_param)
     lo = reg_param[min(minGi + 1, npts - 1)]
     hi = reg_param[max(minGi - 1, 0)]
@@ -213,6 +224,20 @@ def gcv_tikhonov_grid(s, beta, rows, delta0=0.0, npoints=50, decades=8.0):
     lam = lambda_grid(s[0], npoints, decades)
     s2 = (s * s)[:, None]
     f = lam[None, :] ** 2 / (s2 + lam[None, :] ** 2)
+    num = np.sum(f * f * (np.abs(beta[:p]) ** 2)[:, None], axis=0) + delta0
+    den = ((rows - p) + np.sum(f, axis=0)) ** 2
+    G = num / den
+    return int(np.argmin(G)), lam, G
+
+
+def gcv_dsvd_grid(s, beta, rows, delta0=0.0, npoints=50, decades=8.0):
+    """GCV of the damped SVD (dsvdG.m:26, filter f = s/(s+lam)) on the
+    configured grid: G(lam) = (sum (lam/(s+lam))^2 |beta|^2 + delta0)
+    / ((rows - p) + sum lam/(s+lam))^2.  Returns (index, lam_grid, G); ties
+    -> lowest index."""
+    p = s.shape[0]
+    lam = lambda_grid(s[0], npoints, decades)
+    f = lam[None, :] / (s[:, None] + lam[None, :])
     num = np.sum(f * f * (np.abs(beta[:p]) ** 2)[:, None], axis=0) + delta0
     den = ((rows - p) + np.sum(f, axis=0)) ** 2
     G = num / den

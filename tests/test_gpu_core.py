@@ -159,6 +159,17 @@ def test_filters_and_tsvd_match_oracle(ia):
     np.testing.assert_allclose(ia.dsvd(U, s, V, b, lam), rt.dsvd(U, s, V, b, lam), rtol=1e-12)
 
 
+def test_tsvd_reads_only_k_columns_of_v(ia):
+    # tsvd.cuh:145-146 copies n*k entries of V: V truncated to k columns is valid
+    A = _cr((40, 70), 16)
+    b = _cr(40, 17)
+    U, s, V = rt.csvd(A)
+    for k in (1, 9, 40):
+        Vk = np.asfortranarray(V[:, :k])
+        np.testing.assert_allclose(ia.tsvd(U, s, Vk, b, k), rt.tsvd(U, s, V, b, k), rtol=1e-12,
+                                   atol=1e-13)
+
+
 def test_tsvd_k_out_of_range_raises(ia):
     A = _cr((10, 8), 12)
     U, s, V = rt.csvd(A)

@@ -157,3 +157,75 @@ def test_gcv_regtools_literal_grid_and_refinement():
     assert G[j] == pytest.approx((np.linalg.norm(f * beta) ** 2 + d0) / ((24 - 16) + f.sum()) ** 2, rel=1e-12)
     lo, hi = grid[min(gi + 1, 199)], grid[max(gi - 1, 0)]
     assert lo <= reg_min <= hi
+
+
+def _brute_gcv_damped(A, b, lam):
+    """GCV of the damped SVD from its explicit influence matrix A A#,
+    A# = V diag(1/(s + lam)) U^H, with an independent SVD (LAPACK gesvd)."""
+    m, n = A.shape
+    U, s, Vh = sl.svd(A, full_matrices=False, lapack_driver="gesvd")
+    Ash = Vh.conj().T @ np.diag(1.0 / (s + lam)) @ U.conj().T
+    x = Ash @ b
+    infl = A @ Ash
+    return np.linalg.norm(A @ x - b) ** 2 / (m - np.trace(infl).real) ** 2
+
+
+@pytest.mark.parametrize("shape", [(24, 16), (16, 24), (18, 18)])
+def test_gcv_regtools_functions_brute_force(shape):
+    """gcvG.m's 'd' function is the damped-SVD GCV of the influence matrix,
+    and its 't' function is that same GCV at lam^2 (G_t(lam) = G_d(lam^2)),
+    so the literal grid values and both refinement functions are pinned."""
+    A, b = _rand(*shape, seed=12, decay=5)
+    U, s, V = rt.csvd(A)
+    fd = rt.gcv_regtools_fun(U, s, b, "d")
+    ft = rt.gcv_regtools_fun(U, s, b, "t")
+    reg_min, grid, G, gi = rt.gcv_regtools(U, s, b, "t")
+    for j in (0, 17, 60, 123, 199):
+        assert G[j] == pytest.approx(_brute_gcv_damped(A, b, grid[j]), rel=1e-9)
+    for la in (1e-4 * s[0], 3e-2 * s[0], 0.7 * s[0]):
+        assert fd(la) == pytest.approx(_brute_gcv_damped(A, b, la), rel=1e-9)
+        assert ft(np.sqrt(la)) == pytest.approx(_brute_gcv_damped(A, b, la), rel=1e-9)
+    # refinement (gcvG.m:51-52): the bracket is the grid neighbours of the
+    # brute-force argmin of the damped GCV, and the minimiser is the bounded
+    # Brent minimum of the brute-force G_d(lam^2) over it (scipy fminbound)
+    Gb = np.array([_brute_gcv_damped(A, b, la) for la in grid])
+    assert gi == int(np.argmin(Gb))
+    lo, hi = grid[min(gi + 1, 199)], grid[max(gi - 1, 0)]
+    ref = fminbound(lambda la: _brute_gcv_damped(A, b, la * la), lo, hi, xtol=1e-4)
+    assert lo <= reg_min <= hi
+    assert reg_min == pytest.approx(ref, abs=2e-4)
+
+
+@pytest.mark.parametrize("shape", [(24, 16), (16, 24)])
+def test_gcv_dsvd_grid_brute_force(shape):
+    A, b = _rand(*shape, seed=13, decay=8)
+    U, s, V = rt.csvd(A)
+    beta = U.conj().T @ b
+    rows = A.shape[0]
+    delta0 = max(np.linalg.norm(b) ** 2 - np.linalg.norm(beta) ** 2, 0) if rows > len(s) else 0.0
+    idx, lam, G = rt.gcv_dsvd_grid(s, beta, rows, delta0, npoints=15, decades=6)
+    Gb = np.array([_brute_gcv_damped(A, b, l) for l in lam])
+    np.testing.assert_allclose(G, Gb, rtol=1e-8)
+    assert idx == int(np.argmin(Gb))
+
+
+def test_esm_pipeline_dsvd_branch():
+    """pipeline.esm_solve with the damped filter: the grid index is the
+    argmin of the brute-force damped GCV and x the damped solution
+    V diag(1/(s+lam)) U^H b from an independent SVD."""
+    from inverseacoustics_amd import scenarios as sc
+    from oracle import esm as oesm
+    from oracle import pipeline
+
+    S = sc.sphere_scenario()
+    o = pipeline.esm_solve(S.k[0], S.r_s, S.r_h, S.src, S.b[0], S.area, None, S.rho, S.c,
+                           filter="dsvd")
+    D, M_sq, M_hq = oesm.esm_assemble(S.k[0], S.r_s, S.r_h, S.src, None, S.rho, S.c)
+    H = np.linalg.solve(D.T, M_hq.T).T
+    lam = rt.lambda_grid(np.linalg.norm(H, 2), 50, 8.0)
+    Gb = np.array([_brute_gcv_damped(H, S.b[0], l) for l in lam])
+    assert o["idx"] == int(np.argmin(Gb))
+    assert o["lam"] == pytest.approx(lam[o["idx"]], rel=1e-12)
+    U, s, Vh = sl.svd(H, full_matrices=False, lapack_driver="gesvd")
+    x = Vh.conj().T @ ((U.conj().T @ S.b[0]) / (s + o["lam"]))
+    assert np.linalg.norm(o["x"] - x) <= 1e-9 * np.linalg.norm(x)
