@@ -344,6 +344,46 @@ ia_status ia_invr(int m, int l, int n, const double *r_ms, const int *e_n,
 ia_status ia_radiation_modes(int np, const double *r_p, double a_e, double k,
                              int n_modes, double **eigval, double **modes);
 
+/* Batched radiation-mode sweep on elemental radiators (the reference's
+ * radiation-mode path MATLAB/invrS.m:157-170, CUDA/invr.cuh:426-433, with
+ * the elemental-radiator resistance instead of the BEM one): np patches of
+ * equal area A on a baffled plane (r_p: 3 np SoA), l microphones (r_h: 3 l),
+ * pressures b (n_freq columns of l).  Per frequency: the n_modes leading
+ * FP64 eigenpairs (e, Q) of R_ij = (w^2 rho A^2/(4 pi c)) sin(k r_ij)/(k r_ij)
+ * (assembled on the device), H_r = G Q with the Rayleigh transfer
+ * G_ij = -(i w rho A/(2 pi)) e^{ik R_ij}/R_ij, the regularised solution y of
+ * H_r y = b_f (IA_PREC_MIXED: complex64 SVD of H_r, FP64 parameter choice),
+ * W = sum_i e_i |y_i|^2 (= v^H R v) and v = Q y.  Modes with
+ * e_i < eig_floor e_1 are dropped (zero columns of Q and H_r, zero
+ * amplitudes): their shapes are not resolved in FP64.  np <= 4096.  Inputs
+ * host or device pointers; outputs host or device pointers, any NULL. */
+typedef struct {
+    int np, l, n_freq, n_modes;
+    ia_precision precision;
+    ia_reg_params reg;
+    double area;              /* patch area A                              */
+    int chunk;                /* frequencies per device batch (0 = auto)   */
+    double eig_floor;         /* relative eigenvalue floor (0 = 1e-8,
+                                 < 0 = keep all n_modes)                   */
+} ia_radiation_desc;
+
+typedef struct {
+    double *power;            /* [n_freq] W                                */
+    double *lambda;           /* [n_freq] selected lambda                  */
+    int *lambda_index;        /* [n_freq] grid index                       */
+    int *k_tsvd;              /* [n_freq] discrete-GCV TSVD index          */
+    double *eigval;           /* [n_freq * n_modes] e, descending          */
+    double *modes;            /* [n_freq * np * n_modes] Q, column-major per
+                                 frequency, largest entry of each positive */
+    ia_z *amplitude;          /* [n_freq * n_modes] y                      */
+    ia_z *velocity;           /* [n_freq * np] v                           */
+    int *status;              /* [n_freq]                                  */
+} ia_radiation_out;
+
+ia_status ia_radiation_sweep(const ia_radiation_desc *desc, const double *k,
+                             const double *r_p, const double *r_h, const ia_z *b,
+                             ia_radiation_out *out);
+
 /* Batched regularised solve of A_f x_f = b_f for any transfer matrices
  * (rows x cols each, column-major, batch stride rows*cols; host or device
  * pointers): SVD, parameter choice, filtered solution.  lambda, index

@@ -21,7 +21,7 @@ __all__ = [
     "IAError", "set_air", "get_air", "device_count", "esm", "esmgpu", "esm_matrices", "csvd",
     "tsvd", "tikhonov", "dsvd", "regularize", "to_pressure", "to_velocity", "sound_power",
     "sphere", "cartesian_2_spherical", "geometry", "esm_sweep", "SweepResult", "zgemm", "cgemm", "zgesv", "hels", "hels_sweep", "regsolve",
-    "quadrature_load", "set_quadrature", "ibem", "ibem_sweep", "invr", "radiation_modes", "admm",
+    "quadrature_load", "set_quadrature", "ibem", "ibem_sweep", "invr", "radiation_modes", "radiation_sweep", "admm",
     "linesource", "esm_gradient", "write_file_complex", "write_file_double", "write_file_string", "write_matrix",
     "FILTERS", "SELECTS",
 ]
@@ -420,7 +420,42 @@ def radiation_modes(pts, area, k, n_modes):
     return _lib.take_d(ev, nm), _lib.take_d(md, npt * nm).reshape(nm, npt).T.copy()
 
 
+def radiation_sweep(k, r_p, area, r_h, b, n_modes, precision="fp64", filter="tikhonov",
+                    select="gcv", n_lambda=50, decades=8.0, lam=0.0, kfix=0, chunk=0,
+                    want_modes=False, eig_floor=0.0):
+    """Batched radiation-mode sweep (ia_radiation_sweep): patch centres r_p
+    (np, 3) of equal area, microphones r_h (l, 3), pressures b (F, l); modes
+    with eigenvalues below eig_floor e_1 (0: 1e-8) are dropped.
+    Returns a dict with power, lambda, lambda_index, k_tsvd (F,), eigval
+    (F, n_modes), amplitude (F, n_modes), velocity (F, np) and, with
+    want_modes, modes (F, np, n_modes)."""
+    k = np.ascontiguousarray(k, dtype=np.float64).ravel()
+    F = len(k)
+    rp, rh = soa(r_p), soa(r_h)
+    npt, l = len(rp) // 3, len(rh) // 3
+    n = int(n_modes)
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.complex128).reshape(F, l))
+    desc = _lib.ia_radiation_desc(npt, l, F, n, 1 if precision == "mixed" else 0,
+                                  _reg(filter, select, n_lambda, decades, lam, kfix), float(area),
+                                  int(chunk), float(eig_floor))
+    out = dict(power=np.zeros(F), lam=np.zeros(F), lambda_index=np.zeros(F, np.int32),
+               k_tsvd=np.zeros(F, np.int32), eigval=np.zeros((F, n)),
+               modes=np.zeros((F, n, npt)) if want_modes else None,
+               amplitude=np.zeros((F, n), np.complex128), velocity=np.zeros((F, npt), np.complex128),
+               status=np.zeros(F, np.int32))
+    o = _lib.ia_radiation_out(*((out[q].ctypes.data if out[q] is not None else None) for q in
+                                ("power", "lam", "lambda_index", "k_tsvd", "eigval", "modes",
+                                 "amplitude", "velocity", "status")))
+    check(lib().ia_radiation_sweep(C.byref(desc), _lib.dptr(k), _lib.dptr(rp), _lib.dptr(rh),
+                                   _lib.zptr(b), C.byref(o)))
+    if want_modes:
+        out["modes"] = out["modes"].transpose(0, 2, 1).copy()
+    out["lambda"] = out.pop("lam")
+    return out
+
+
 # ------------------------------------------------------------ ESM sweep
+@dataclass# ------------------------------------------------------------ ESM sweep
 @dataclass
 class SweepResult:
     lam: np.ndarray

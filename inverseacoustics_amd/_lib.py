@@ -62,6 +62,18 @@ class ia_grad_out(C.Structure):
                  "dpower_dmic", "dmisfit_dloglam", "dpower_dloglam", "velocity")]
 
 
+class ia_radiation_desc(C.Structure):
+    _fields_ = [("np_", C.c_int), ("l", C.c_int), ("n_freq", C.c_int), ("n_modes", C.c_int),
+                ("precision", C.c_int), ("reg", ia_reg_params), ("area", C.c_double),
+                ("chunk", C.c_int), ("eig_floor", C.c_double)]
+
+
+class ia_radiation_out(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in
+                ("power", "lambda_", "lambda_index", "k_tsvd", "eigval", "modes", "amplitude",
+                 "velocity", "status")]
+
+
 class ia_admm_params(C.Structure):
     _fields_ = [("lambda0", C.c_double), ("rho", C.c_double), ("gamma", C.c_double),
                 ("epsilon", C.c_double), ("max_iter", C.c_int), ("norm", C.c_int),
@@ -127,6 +139,8 @@ SIGNATURES = {
                               _zp, _dp, _ip, _ip, _dp]),
     "ia_esm_gradient": (C.c_int, [C.POINTER(ia_grad_desc), _dp, _dp, _dp, C.c_double, _dp, _zp,
                                   _dp, C.POINTER(ia_grad_out)]),
+    "ia_radiation_sweep": (C.c_int, [C.POINTER(ia_radiation_desc), _dp, _dp, _dp, _zp,
+                                     C.POINTER(ia_radiation_out)]),
     "ia_quadrature_load": (C.c_int, [C.c_char_p, C.c_char_p, _ip, _dpp, _ip, _dpp]),
     "ia_set_quadrature": (C.c_int, [C.c_int, _dp, C.c_int, _dp]),
     "ibem": (None, [C.c_int, C.c_int, C.c_int, _dp, _ip, _dp, _zpp, _zpp, C.c_double, C.c_int,

@@ -304,6 +304,7 @@ void ia_release(void) {
     sweep_cache<double>().release();
     sweep_cache<float>().release();
     release_grad_cache();
+    release_rad_cache();
 }
 
 ia_status ia_last_stats(ia_stats *st) {

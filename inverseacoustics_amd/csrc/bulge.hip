@@ -544,6 +544,11 @@ void launch_bulge_apply(bool fwd, int C, const cx<T> *u, const cx<T> *tau, cx<do
     const int tmax = ntask_of(C, 0);
     dim3 grid(nvec, batch);
     const size_t lds = bulge_apply_lds(C);
+    // C = 4096 needs the whole 64 KB of dynamic LDS
+    (void)hipFuncSetAttribute((const void *)k_bulge_apply<T, true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    (void)hipFuncSetAttribute((const void *)k_bulge_apply<T, false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (fwd)
         hipLaunchKernelGGL((k_bulge_apply<T, true>), grid, dim3(64 * NWA), lds, s, C, u, tau, tmax,
                            v, sv, nvec);

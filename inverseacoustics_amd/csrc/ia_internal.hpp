@@ -218,8 +218,10 @@ template <typename T>
 void launch_bulge_apply(bool fwd, int C, const cx<T> *u, const cx<T> *tau, cx<double> *v,
                         size_t sv, int nvec, int batch, hipStream_t s);
 
-// frees the device workspace the gradient entry point keeps per host thread
+// free the device workspace the gradient and radiation-mode entry points
+// keep per host thread
 void release_grad_cache();
+void release_rad_cache();
 
 // ADMM (admm.hip): A (rows x cols, batch stride rows*cols), b, c on the device
 ia_status admm_batched(int rows, int cols, int batch, const cx<double> *A, const cx<double> *b,

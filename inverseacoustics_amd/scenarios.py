@@ -15,6 +15,8 @@ on, so that the GPU path and the CPU checks see byte-identical inputs:
   z = 0.1 m, 64 x 32 equivalent sources at z = -d, 64 x 32 evaluation points
   at z = 0 with normal +z; pressures from the Rayleigh integral over a finer
   patch grid (avoids the inverse crime).
+* ``radiation_scenario`` - the same plate as 64 x 64 elemental radiators for
+  the radiation-mode sweep (50-2500 Hz, 20 dB SNR).
 
 Conventions follow the reference: time dependence e^{-i w t}, outgoing
 e^{+ikR} (``CUDA/esm.cuh:16-17``), spherical Hankel h_n = j_n + i y_n
@@ -238,6 +240,56 @@ def plate_scenario(freqs=None, n_freq=256, fmin=200.0, fmax=1500.0, d=0.02,
         v_true[f] = plate_velocity(spec, w, x0, y0, r_s[:, 0], r_s[:, 1])
     return Scenario("plate", freqs, k, r_s, r_h, src, normals, None, b, spec.Lx * spec.Ly,
                     rho, c, truth=dict(v_s=v_true, force_at=(x0, y0)))
+
+
+@dataclass
+class RadiationScenario:
+    """Inputs of one radiation-mode sweep: patch centres (np, 3) of equal
+    area, microphones (l, 3), pressures b (F, l)."""
+    freqs: np.ndarray
+    k: np.ndarray
+    r_p: np.ndarray
+    r_h: np.ndarray
+    b: np.ndarray
+    patch_area: float
+    rho: float = RHO_DEFAULT
+    c: float = C_DEFAULT
+    truth: dict = field(default_factory=dict)
+
+
+def radiation_scenario(freqs=None, n_freq=256, fmin=50.0, fmax=2500.0, patches=(64, 64),
+                       mic_n=(32, 32), mic_pitch=0.03, mic_z=0.1, ref_n=(128, 100), snr_db=20.0,
+                       seed=1005, spec: PlateSpec | None = None, rho=RHO_DEFAULT, c=C_DEFAULT,
+                       device="cpu"):
+    """Baffled plate of patches x elemental radiators (the plate of
+    ``plate_scenario``), microphones on a grid above it, pressures from the
+    Rayleigh integral over a finer grid plus seeded noise at snr_db; the
+    patch velocities of the modal model are returned as truth."""
+    spec = spec or PlateSpec()
+    if freqs is None:
+        freqs = np.linspace(fmin, fmax, n_freq)
+    freqs = np.atleast_1d(np.asarray(freqs, dtype=np.float64))
+    k = 2 * math.pi * freqs / c
+    rng = np.random.default_rng(seed)
+    x0 = float(rng.uniform(0.1, 0.9)) * spec.Lx
+    y0 = float(rng.uniform(0.1, 0.9)) * spec.Ly
+    mx = (np.arange(mic_n[0]) - (mic_n[0] - 1) / 2) * mic_pitch + spec.Lx / 2
+    my = (np.arange(mic_n[1]) - (mic_n[1] - 1) / 2) * mic_pitch + spec.Ly / 2
+    MX, MY = np.meshgrid(mx, my, indexing="xy")
+    r_h = np.stack([MX.ravel(), MY.ravel(), np.full(MX.size, mic_z)], axis=1)
+    r_p = grid_points(patches[0], patches[1], spec.Lx, spec.Ly, 0.0)
+    ref = grid_points(ref_n[0], ref_n[1], spec.Lx, spec.Ly, 0.0)
+    dA = spec.Lx * spec.Ly / len(ref)
+    b = np.empty((len(freqs), len(r_h)), dtype=np.complex128)
+    v_p = np.empty((len(freqs), len(r_p)), dtype=np.complex128)
+    for f, kf in enumerate(k):
+        w = 2 * math.pi * freqs[f]
+        v_ref = plate_velocity(spec, w, x0, y0, ref[:, 0], ref[:, 1])
+        p = rayleigh_pressure(kf, w, rho, ref, v_ref, dA, r_h, device)
+        b[f] = add_noise(p, snr_db, rng)
+        v_p[f] = plate_velocity(spec, w, x0, y0, r_p[:, 0], r_p[:, 1])
+    return RadiationScenario(freqs, k, r_p, r_h, b, spec.Lx * spec.Ly / len(r_p), rho, c,
+                             truth=dict(v_p=v_p, force_at=(x0, y0)))
 
 
 def soa(pts: np.ndarray) -> np.ndarray:
