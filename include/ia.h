@@ -180,6 +180,19 @@ ia_status ia_regularize(int m, int n, int p, const ia_z *u, const double *s,
                         const ia_z *v, const ia_z *b, const ia_reg_params *prm,
                         ia_z **x, double *lambda, int *index);
 
+/* The solve of ia_regularize for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Every pointer is a device
+ * pointer: u (m x p), s (p), v (n x p), b (m) in; x (n), lambda (1 double)
+ * and index (1 int, as ia_regularize returns it) out, lambda and index may
+ * be NULL.  The work is enqueued on `stream` (a hipStream_t; NULL = default
+ * stream) and nothing is allocated: the intermediates live in a fixed
+ * 128 KiB device array per GPU, so p <= 4096 (larger p returns IA_ERR_ARG),
+ * and calls that overlap on one GPU race. */
+ia_status ia_regularize_noalloc(int m, int n, int p, const ia_z *u, const double *s,
+                                const ia_z *v, const ia_z *b, const ia_reg_params *prm,
+                                ia_z *x, double *lambda, int *index, void *stream);
+
 /* -------------------------------------------------- batched ESM sweep */
 typedef enum { IA_PREC_FP64 = 0, IA_PREC_MIXED = 1 } ia_precision;
 

@@ -121,6 +121,9 @@ SIGNATURES = {
     "cartesian_2_spherical": (None, [C.c_int, _dp, _dpp]),
     "ia_regularize": (C.c_int, [C.c_int, C.c_int, C.c_int, _zp, _dp, _zp, _zp,
                                 C.POINTER(ia_reg_params), _zpp, _dp, _ip]),
+    "ia_regularize_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.POINTER(ia_reg_params),
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "ia_esm_sweep": (C.c_int, [C.POINTER(ia_sweep_desc), _dp, _dp, _dp, _dp, _dp, _zp,
                                C.POINTER(ia_sweep_out)]),
     "ia_esm_sweep_device": (C.c_int, [C.POINTER(ia_sweep_desc), C.c_void_p, C.c_void_p,
