@@ -4,7 +4,11 @@ ESM sweep (plate geometry, 1024 microphones, 2048 equivalent sources,
 
 One step = one full sweep (assembly, LU, H_e, SVD, GCV selection over 50
 lambdas, Tikhonov solution, surface pressure and sound power for all 256
-frequencies) with every input already resident in HBM.  Prints ONE JSON line.
+frequencies) with every input already resident in HBM.  The headline is the
+certified mixed form (the library default: complex64 bulk, nearly tied
+selections solved again in FP64), which keeps the FP64 form's lambda index at
+every frequency; the uncertified mixed form and the FP64 form are reported
+beside it.  Prints ONE JSON line.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--freqs 256]
 """
@@ -18,6 +22,9 @@ import time
 
 import numpy as np
 
+# MI355X peaks (AMD datasheet, dense): FP32 / FP64 matrix and vector, HBM3E
+PEAK_FP32_TF, PEAK_FP64_TF, PEAK_HBM_GBS = 157.3, 78.6, 8000.0
+
 
 def _parse():
     ap = argparse.ArgumentParser()
@@ -27,12 +34,41 @@ def _parse():
     ap.add_argument("--freqs", type=int, default=256)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=3, help="frequencies in the CPU sample")
-    ap.add_argument("--only", choices=("both", "fp64", "mixed"), default="both",
+    ap.add_argument("--cpu-sample", type=int, default=8, help="frequencies in the CPU sample")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the IBEM / HELS / radiation-mode / gradient throughputs")
+    ap.add_argument("--only", choices=("both", "fp64", "mixed", "certified"), default="both",
                     help="profile runs: time one form only")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
+
+
+# ------------------------------------------------------------ algorithm work
+def sweep_flops(m, l):
+    """Real flops of the executed ESM pipeline per frequency (a complex MAC
+    is 8 flops): LU of D, the two triangular solves giving Y = H_e^H (m x l),
+    the Householder QR of Y (m >= 3l/2) and the two-stage bidiagonalisation of
+    its l x l R factor.  Forming U and V is never done (only the vectors a
+    solve needs are pushed through the reflectors), so those terms are absent."""
+    lu = 8.0 / 3.0 * m ** 3
+    trsm = 8.0 * m * m * l
+    qr = 8.0 * (m * l * l - l ** 3 / 3.0) if 2 * m >= 3 * l else 0.0
+    band = 32.0 / 3.0 * l ** 3
+    return {"factor": lu, "solve_he": trsm, "bidiag": qr + band}
+
+
+def assembly_flops(m, l):
+    """Assembly work per frequency (sincos, sqrt, reciprocal count 1 each)."""
+    return 20.0 * (l * m + m * m) + 35.0 * m * m
+
+
+def bulge_compulsory_bytes(C, elem):
+    """Bytes the chase k_bulge cannot avoid per matrix: the band in (C x 33
+    entries), the stage-two reflectors out (two of 33 entries per task) and
+    the real bidiagonal out."""
+    tasks = sum(1 + (C - 3 - s) // 32 if C - 3 - s >= 0 else 1 for s in range(C - 1))
+    return C * 33 * elem + tasks * 2 * 33 * elem + 2 * C * 8
 
 
 def main():
@@ -108,14 +144,15 @@ def main():
         torch.cuda.synchronize()
         del src, dst
     if a.only != "both":
-        dt, o, st = timed(1 if a.only == "mixed" else 0, -1.0 if a.only == "mixed" else 0.0)
+        prec, rc = {"fp64": (0, 0.0), "mixed": (1, -1.0), "certified": (1, 0.0)}[a.only]
+        dt, o, st = timed(prec, rc)
         print(json.dumps({"profile_run": a.only, "ms_per_step": round(dt * 1e3, 2),
                           "phases_ms": {k: round(v, 2) for k, v in st.items()}}))
         return
-    # mixed form as is (the headline), the certified mixed form (nearly tied
-    # GCV choices solved again in FP64, the library default) and plain FP64
-    t_mixed, o_mixed, st_mixed = timed(1, -1.0)
+    # the certified mixed form (the library default: the headline), the mixed
+    # form without the FP64 recheck, and plain FP64
     t_cert, o_cert, st_cert = timed(1, 0.0)
+    t_mixed, o_mixed, st_mixed = timed(1, -1.0)
     t_fp64, o_fp64, st_fp64 = timed(0)
 
     W64 = o_fp64["W"].cpu().numpy()
@@ -129,101 +166,213 @@ def main():
                 float(np.mean(idx64 == o["idx"].cpu().numpy())),
                 float(np.max(np.linalg.norm(v - v64, axis=1) / np.linalg.norm(v64, axis=1))))
 
-    db_err, idx_match, vel_rel = errors(o_mixed)
     db_c, idx_c, vel_c = errors(o_cert)
+    db_m, idx_m, vel_m = errors(o_mixed)
 
-    # ---- roofline of the chase kernel k_bulge (HBM-bound): the second stage
-    # of the two-stage bidiagonalisation, our largest single launch of the
-    # sweep (the rocBLAS GEMMs of the factorisations and the band reduction
-    # take more time in aggregate, spread over ~600 calls of many shapes).
-    # Algorithmic bytes per launch: per task of the chase the overlap block
-    # and the block right of it are read and the overlap block and the rows
-    # above it written (bulge_alg_bytes), over all frequencies of the launch.
-    # The library brackets the launch with HIP events on the sweep's stream
-    # and reports its duration, bytes and launch count.
-    def roof(st, form, esz):
-        n = max(1.0, st["bidiag_kernel_launches"])
-        ms = st["bidiag_kernel_ms"] / n
-        by = st["bidiag_kernel_bytes"] / n
-        ach = by / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    # ---- roofline of the sweep: the executed algorithm's flops over the
+    # sweep's duration, timed by HIP events on the sweep's stream inside the
+    # library (ia_stats), against the matrix-core peak of the form's
+    # arithmetic: FP32 for the complex64 bulk, FP64 for the FP64 form.  The
+    # sweep is matrix-core bound: its HBM traffic (profiles/) moves in a few
+    # percent of the sweep time.  Per-phase fractions beside it.
+    fl = sweep_flops(m, l)
+    total_fl = sum(fl.values()) * F
+
+    def roof(st, peak, form):
+        ms = st["total"]
+        ach = total_fl / (ms / 1e3) / 1e12
+        phases = {}
+        for ph in ("factor", "solve_he", "bidiag"):
+            t_ph = st[ph]
+            phases[ph] = {"flops": fl[ph] * F, "ms": round(t_ph, 3),
+                          "tflops": round(fl[ph] * F / (t_ph / 1e3) / 1e12, 2),
+                          "frac": round(fl[ph] * F / (t_ph / 1e3) / 1e12 / peak, 4)}
+        asm = assembly_flops(m, l) * F
+        phases["assemble"] = {"flops": asm, "ms": round(st["assemble"], 3),
+                              "tflops": round(asm / (st["assemble"] / 1e3) / 1e12, 2),
+                              "frac_valu": round(asm / (st["assemble"] / 1e3) / 1e12 / peak, 4)}
         key = (form, F, m, l, a.chunk)
-        tr = HBM_TRAFFIC_PER_LAUNCH.get(key)
-        return {"bound": "hbm", "kernel": f"k_bulge<{'float' if esz == 8 else 'double'}> ({form} form)",
-                "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(ach / 8000.0, 4), "traffic": tr,
-                "algorithmic_bytes_per_launch": round(by), "launch_ms_avg": round(ms, 4),
-                "launches": int(n)}
+        return {"bound": "mfma", "kernel": f"ESM sweep, {form} form (all kernels of one sweep)",
+                "achieved": round(ach, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(ach / peak, 4), "traffic": SWEEP_HBM_BYTES.get(key),
+                "algorithmic_flops_per_sweep": total_fl, "sweep_ms": round(ms, 3),
+                "phases": phases}
 
-    roofline = roof(st_mixed, "mixed", 8)
-    roofline_fp64 = roof(st_fp64, "fp64", 16)
+    roofline = roof(st_mixed, PEAK_FP32_TF, "mixed")
+    roofline_fp64 = roof(st_fp64, PEAK_FP64_TF, "fp64")
+
+    # the chase k_bulge, the largest single launch of the hand-written
+    # kernels, against the bytes it cannot avoid (it is latency-bound)
+    def chase(st, elem):
+        nl = max(1.0, st["bidiag_kernel_launches"])
+        ms = st["bidiag_kernel_ms"] / nl
+        cb = bulge_compulsory_bytes(l, elem) * F
+        return {"kernel": f"k_bulge<{'float' if elem == 8 else 'double'}>",
+                "launch_ms_avg": round(ms, 3), "compulsory_bytes": cb,
+                "achieved_gbs": round(cb / (ms / 1e3) / 1e9, 1),
+                "frac_hbm": round(cb / (ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4)}
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(P, a.cpu_sample)
+        cpu = cpu_baseline(P, a.cpu_sample, o_fp64)
+
+    workloads = None if a.no_workloads else other_workloads(ia, sc)
 
     res = {
         "metric": "frequencies solved/sec, batched mixed-precision vs plain FP64 on GPU; dB error",
-        "value": round(F / t_mixed, 3),
+        "value": round(F / t_cert, 3),
         "unit": "frequencies/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(t_mixed * 1e3, 2),
+        "ms_per_step": round(t_cert * 1e3, 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64+f64",
         "data": "synthetic (plate, Rayleigh-integral pressures, seeded 30 dB SNR noise)",
         "config": {"workload": "ESM sweep 256 frequencies 200-1500 Hz, 1024 mics x 2048 sources "
-                               "(0.6x0.5 m steel plate, d=0.02 m), Tikhonov + GCV over 50 lambdas",
+                               "(0.6x0.5 m steel plate, d=0.02 m), Tikhonov + GCV over 50 lambdas, "
+                               "certified mixed precision",
                    "frequencies": F, "l": l, "m": m},
+        "speedup_certified_vs_fp64": round(t_fp64 / t_cert, 3),
+        "db_error_max_vs_fp64": round(db_c, 5),
+        "lambda_index_match_vs_fp64": idx_c,
+        "velocity_rel_l2_max_vs_fp64": vel_c,
+        "rechecked_in_fp64": int(st_cert["rechecked"]),
+        "recheck_ms": round(st_cert["recheck_ms"], 2),
+        "target_speedup": 5.0,
         "fp64_form": {"value": round(F / t_fp64, 3), "ms_per_step": round(t_fp64 * 1e3, 2),
                       "phases_ms": {k: round(v, 2) for k, v in st_fp64.items()}},
-        "mixed_phases_ms": {k: round(v, 2) for k, v in st_mixed.items()},
-        "mixed_certified": {"value": round(F / t_cert, 3), "ms_per_step": round(t_cert * 1e3, 2),
-                            "rechecked_in_fp64": int(st_cert["rechecked"]),
-                            "recheck_ms": round(st_cert["recheck_ms"], 2),
-                            "db_error_max_vs_fp64": round(db_c, 5),
-                            "lambda_index_match_vs_fp64": idx_c,
-                            "velocity_rel_l2_max_vs_fp64": vel_c},
-        "speedup_mixed_vs_fp64": round(t_fp64 / t_mixed, 3),
-        "db_error_max_mixed_vs_fp64": round(db_err, 5),
-        "lambda_index_match_mixed_vs_fp64": idx_match,
-        "velocity_rel_l2_max_mixed_vs_fp64": vel_rel,
+        "mixed_uncertified": {"value": round(F / t_mixed, 3), "ms_per_step": round(t_mixed * 1e3, 2),
+                              "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
+                              "db_error_max_vs_fp64": round(db_m, 5),
+                              "lambda_index_match_vs_fp64": idx_m,
+                              "velocity_rel_l2_max_vs_fp64": vel_m,
+                              "phases_ms": {k: round(v, 2) for k, v in st_mixed.items()}},
         "roofline": roofline,
         "roofline_fp64_form": roofline_fp64,
+        "chase_kernel": {"mixed": chase(st_mixed, 8), "fp64": chase(st_fp64, 16)},
         "cpu_baseline": cpu,
+        "other_workloads": workloads,
         "input_generation_s": round(gen_s, 2),
     }
     print(json.dumps(res))
 
 
-# HBM bytes per k_bulge launch (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; FETCH scaled by
-# 2.0, the WRITE/FETCH ratio of a plain 1 GiB device copy in the same runs) keyed by
-# (form, frequencies, m, l, chunk): profiles/bulge_esm_sweep_256f_1024x2048_hbm_counters.txt
-HBM_TRAFFIC_PER_LAUNCH = {
-    ("fp64", 256, 2048, 1024, 0): 297710959119,
-    ("mixed", 256, 2048, 1024, 0): 169788622616,
-}
+# HBM bytes of one mixed / FP64 sweep (every kernel's FETCH_SIZE + WRITE_SIZE
+# from rocprofv3 --pmc, FETCH scaled by the WRITE/FETCH ratio of a plain
+# device copy measured in the same runs), keyed by (form, frequencies, m, l,
+# chunk): profiles/esm_sweep_256f_1024x2048_hbm_traffic.txt
+SWEEP_HBM_BYTES = {}
 
 
-def cpu_baseline(P, nf):
-    """The oracle's numpy/LAPACK FP64 pipeline on host cores, on a bounded
-    sample of nf frequencies of the same workload."""
-    from threadpoolctl import threadpool_info
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(P, nf, o_fp64):
+    """The oracle's numpy/LAPACK FP64 pipeline (the vectorised esmV + csvd +
+    GCV + Tikhonov formulation) on the host cores, on a bounded sample of nf
+    frequencies spread over the sweep (1 warm-up frequency first), with all
+    threads and with one thread; each sampled frequency is also checked
+    against the GPU FP64 sweep (same lambda index, W and velocity within
+    1e-10)."""
+    import torch
+    from threadpoolctl import threadpool_info, threadpool_limits
 
     from oracle import pipeline
 
     idx = np.linspace(0, len(P.k) - 1, nf).round().astype(int)
-    pipeline.esm_solve(P.k[idx[0]], P.r_s, P.r_h, P.src, P.b[idx[0]], P.area, P.normals, P.rho, P.c)
-    t = time.perf_counter()
-    for f in idx:
-        pipeline.esm_solve(P.k[f], P.r_s, P.r_h, P.src, P.b[f], P.area, P.normals, P.rho, P.c)
-    dt = time.perf_counter() - t
+    W64 = o_fp64["W"].cpu().numpy()
+    I64 = o_fp64["idx"].cpu().numpy()
+    V64 = torch.view_as_complex(o_fp64["v"]).cpu().numpy()
+
+    def solve(f):
+        return pipeline.esm_solve(P.k[f], P.r_s, P.r_h, P.src, P.b[f], P.area, P.normals, P.rho,
+                                  P.c)
+
     threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    return {"value": round(nf / dt, 4), "unit": "frequencies/s", "cores": threads, "kind": "port",
-            "sample": f"{nf} of the 256 frequencies (1024x2048, FP64 numpy/LAPACK oracle), "
-                      f"after 1 warm-up frequency"}
+    solve(idx[0])
+    t = time.perf_counter()
+    outs = [solve(f) for f in idx]
+    dt_all = time.perf_counter() - t
+    with threadpool_limits(limits=1):
+        solve(idx[0])
+        n1 = max(2, nf // 2)
+        t = time.perf_counter()
+        for f in idx[:n1]:
+            solve(f)
+        dt_1 = time.perf_counter() - t
+    match = [int(I64[f] == o["idx"]) for f, o in zip(idx, outs)]
+    w_err = [abs(W64[f] / o["W"] - 1) for f, o in zip(idx, outs)]
+    v_err = [float(np.linalg.norm(V64[f] - o["x"]) / np.linalg.norm(o["x"])) for f, o in zip(idx, outs)]
+    return {"value": round(nf / dt_all, 4), "unit": "frequencies/s", "cores": threads,
+            "kind": "port", "cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count(),
+            "sample": f"{nf} of the {len(P.k)} frequencies (indices {idx.tolist()}), 1024 x 2048, "
+                      f"FP64 numpy/LAPACK oracle, after 1 warm-up frequency",
+            "one_thread": {"value": round(n1 / dt_1, 4), "cores": 1,
+                           "sample": f"the first {n1} of those frequencies"},
+            "check_vs_gpu_fp64": {"lambda_index_match": f"{sum(match)}/{nf}",
+                                  "W_rel_err_max": float(max(w_err)),
+                                  "velocity_rel_l2_max": max(v_err)}}
+
+
+def other_workloads(ia, sc):
+    """Throughput of the other solves at their full sizes (host inputs and
+    outputs, one warm-up call then one timed call each)."""
+    import torch
+
+    out = {}
+
+    def clock(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    S = sc.ibem_scenario()
+    ia.set_air(S.c, S.rho)
+    ia.set_quadrature()
+    for prec in ("fp64", "mixed"):
+        dt = clock(lambda: ia.ibem_sweep(S.k, S.X, S.E, S.r_h, S.b, S.area, precision=prec))
+        out[f"ibem_{prec}"] = {"workload": "IBEM icosphere(3) 642 nodes, 128 mics, 64 frequencies "
+                                           "100-2000 Hz, 7-point rule",
+                               "frequencies_per_s": round(len(S.k) / dt, 2), "s": round(dt, 4)}
+    H = sc.hels_scenario()
+    ia.set_air(H.c, H.rho)
+    for prec in ("fp64", "mixed"):
+        dt = clock(lambda: ia.hels_sweep(H.k, H.order, H.azelr_s, H.azelr_h, H.b, H.area,
+                                         precision=prec))
+        out[f"hels_{prec}"] = {"workload": "HELS 256 mics, 256 surface points, 128 frequencies "
+                                           "50-3000 Hz, N = ceil(ka) + 4 (5..11)",
+                               "frequencies_per_s": round(len(H.k) / dt, 2), "s": round(dt, 4)}
+    R = sc.radiation_scenario(device="cuda")
+    ia.set_air(R.c, R.rho)
+    for prec in ("fp64", "mixed"):
+        dt = clock(lambda: ia.radiation_sweep(R.k, R.r_p, R.patch_area, R.r_h, R.b, 128,
+                                              precision=prec))
+        out[f"radiation_{prec}"] = {
+            "workload": "radiation modes, 64 x 64 patches (4096), 1024 mics, 256 frequencies "
+                        "50-2500 Hz, 128 modes, FP64 eigendecomposition",
+            "frequencies_per_s": round(len(R.k) / dt, 2), "s": round(dt, 4)}
+    G = sc.plate_scenario(n_freq=32, device="cuda")
+    ia.set_air(G.c, G.rho)
+    dt = clock(lambda: ia.esm_gradient(G.k, G.r_s, G.normals, 0.02, G.r_h, G.b, G.area))
+    out["esm_gradient_fp64"] = {"workload": "ESM adjoint: misfit and power gradients w.r.t. d, "
+                                            "1024 mic positions and log lambda, 32 frequencies, "
+                                            "1024 x 2048",
+                                "frequencies_per_s": round(len(G.k) / dt, 2), "s": round(dt, 4)}
+    return out
 
 
 if __name__ == "__main__":
