@@ -263,8 +263,13 @@ def main():
 # HBM bytes of one mixed / FP64 sweep (every kernel's FETCH_SIZE + WRITE_SIZE
 # from rocprofv3 --pmc, FETCH scaled by the WRITE/FETCH ratio of a plain
 # device copy measured in the same runs), keyed by (form, frequencies, m, l,
-# chunk): profiles/esm_sweep_256f_1024x2048_hbm_traffic.txt
-SWEEP_HBM_BYTES = {}
+# chunk): profiles/esm_sweep_256f_1024x2048_hbm_traffic.txt.  Well above the
+# algorithmic bytes: the rank-128 / rank-64 trailing updates re-read their
+# operand panels per output tile.
+SWEEP_HBM_BYTES = {
+    ("mixed", 256, 2048, 1024, 0): 1106368285587,
+    ("fp64", 256, 2048, 1024, 0): 2347504694265,
+}
 
 
 def _cpu_model():
