@@ -106,13 +106,126 @@ static SweepCache<T> &sweep_cache() {
 }
 
 template <typename T>
+ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, size_t sA,
+                           const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
+                           size_t sx, double *lam, int *idx, int *kts, double *smax,
+                           hipStream_t s, double *gap);
+
+// More microphones than sources (l > m, the case the reference's csvd
+// serves by its m >= n branch, tsvd.cuh:39): H_e = M_hq D^-1 (l x m) is
+// tall, so it is formed explicitly (H_e^H = D^-H M_hq^H by the LU of D) and
+// solved by the batched regularised solve, which takes the SVD of the tall
+// matrix itself; selection, surface pressure and sound power as in the
+// main path.
+template <typename T>
+static ia_status esm_sweep_tall(const ia_sweep_desc &D, const double *k, const double *r_s,
+                                const double *src, const double *nrm, const double *r_h,
+                                const cx<double> *b, double *lam, int *lidx, int *ktsvd,
+                                double *power, double *smax, cx<double> *vel, cx<double> *pres,
+                                int *status, double *gap, hipStream_t s) {
+    const int m = D.m, l = D.l, F = D.n_freq;
+    int chunk = D.chunk > 0 ? D.chunk : std::min(F, 256);
+    chunk = std::min(chunk, F);
+    const double rho_c = air().rho * air().c;
+    RegDev prm{(int)D.reg.filter, (int)D.reg.select, D.reg.n_lambda > 0 ? D.reg.n_lambda : 50,
+               D.reg.k, D.reg.decades > 0 ? D.reg.decades : 8.0, D.reg.lambda};
+    if (prm.n_lambda > 512) return fail(IA_ERR_ARG, "n_lambda must be <= 512");
+    DevBuf bD, bY, bYt, bH, bipiv, binfo, bx, bq, bps, bpsd;
+    IA_TRY(bD.alloc(sizeof(cx<T>) * (size_t)chunk * m * m));
+    IA_TRY(bY.alloc(sizeof(cx<T>) * (size_t)chunk * m * l));
+    IA_TRY(bH.alloc(sizeof(cx<double>) * (size_t)chunk * m * l));
+    if (sizeof(T) != sizeof(double)) IA_TRY(bYt.alloc(sizeof(cx<T>) * (size_t)chunk * m * l));
+    IA_TRY(bipiv.alloc(sizeof(int) * (size_t)chunk * m));
+    IA_TRY(binfo.alloc(sizeof(int) * (size_t)chunk));
+    IA_TRY(bx.alloc(sizeof(cx<double>) * (size_t)chunk * m));
+    IA_TRY(bq.alloc(sizeof(cx<T>) * (size_t)chunk * m));
+    IA_TRY(bps.alloc(sizeof(cx<T>) * (size_t)chunk * m));
+    IA_TRY(bpsd.alloc(sizeof(cx<double>) * (size_t)chunk * m));
+    Timer tm(s);
+    ia_stats st{};
+    for (int f0 = 0; f0 < F; f0 += chunk) {
+        const int nf = std::min(chunk, F - f0);
+        const double *kf = k + f0;
+        cx<T> *Dm = bD.as<cx<T>>(), *Y = bY.as<cx<T>>();
+        tm.n = 0;
+        tm.mark();  // 0
+        launch_esm_D<T>(m, nf, kf, r_s, src, nrm, Dm, (size_t)m * m, s, false);
+        launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s);
+        IA_HIP(hipGetLastError());
+        tm.mark();  // 1
+        IA_HIP(hipMemsetAsync(binfo.p, 0, sizeof(int) * nf, s));
+        IA_TRY(getrf_batched<T>(m, Dm, m, (size_t)m * m, bipiv.as<int>(), binfo.as<int>(), nf, s));
+        tm.mark();  // 2
+        // Y = D^-H M_hq^H = H_e^H (m x l), then H_e = Y^H in complex128
+        IA_TRY(getrs_batched<T>('C', m, l, Dm, m, (size_t)m * m, bipiv.as<int>(), Y, m,
+                                (size_t)m * l, nf, s));
+        if (sizeof(T) == sizeof(double)) {
+            launch_conj_transpose<double>(m, l, reinterpret_cast<cx<double> *>(Y), (size_t)m * l,
+                                          bH.as<cx<double>>(), (size_t)l * m, nf, s);
+        } else {
+            launch_conj_transpose<T>(m, l, Y, (size_t)m * l, bYt.as<cx<T>>(), (size_t)l * m, nf, s);
+            launch_convert<double, float>((size_t)nf * l * m,
+                                          reinterpret_cast<cx<float> *>(bYt.p),
+                                          bH.as<cx<double>>(), s);
+        }
+        tm.mark();  // 3
+        IA_TRY(regsolve_batched<T>(l, m, nf, bH.as<cx<double>>(), (size_t)l * m, b + (size_t)f0 * l, l,
+                                   prm, bx.as<cx<double>>(), m, lam ? lam + f0 : nullptr,
+                                   lidx ? lidx + f0 : nullptr, ktsvd ? ktsvd + f0 : nullptr,
+                                   smax ? smax + f0 : nullptr, s, gap ? gap + f0 : nullptr));
+        tm.mark();  // 4
+        // post: q = D^-1 x, p_s = M_sq q, W = 1/2 Re(x^H p_s) S/m
+        if (sizeof(T) == sizeof(double))
+            IA_HIP(hipMemcpyAsync(bq.p, bx.p, sizeof(cx<double>) * (size_t)nf * m,
+                                  hipMemcpyDeviceToDevice, s));
+        else
+            launch_convert<float, double>((size_t)nf * m, bx.as<cx<double>>(),
+                                          reinterpret_cast<cx<float> *>(bq.p), s);
+        IA_TRY(getrs_batched<T>('N', m, 1, Dm, m, (size_t)m * m, bipiv.as<int>(), bq.as<cx<T>>(), m,
+                                (size_t)m, nf, s));
+        launch_esm_msq_apply<T>(m, nf, kf, rho_c, r_s, src, bq.as<cx<T>>(), m, bps.as<cx<T>>(), m,
+                                s);
+        if (sizeof(T) == sizeof(double))
+            IA_HIP(hipMemcpyAsync(bpsd.p, bps.p, sizeof(cx<double>) * (size_t)nf * m,
+                                  hipMemcpyDeviceToDevice, s));
+        else
+            launch_convert<double, float>((size_t)nf * m, reinterpret_cast<cx<float> *>(bps.p),
+                                          bpsd.as<cx<double>>(), s);
+        if (power)
+            launch_power<double>(m, bx.as<cx<double>>(), m, bpsd.as<cx<double>>(), m, D.area / m,
+                                 power + f0, nf, s);
+        if (vel)
+            IA_HIP(hipMemcpyAsync(vel + (size_t)f0 * m, bx.p, sizeof(cx<double>) * (size_t)nf * m,
+                                  hipMemcpyDefault, s));
+        if (pres)
+            IA_HIP(hipMemcpyAsync(pres + (size_t)f0 * m, bpsd.p,
+                                  sizeof(cx<double>) * (size_t)nf * m, hipMemcpyDefault, s));
+        if (status)
+            IA_HIP(hipMemcpyAsync(status + f0, binfo.p, sizeof(int) * nf, hipMemcpyDefault, s));
+        IA_HIP(hipGetLastError());
+        tm.mark();  // 5
+        IA_HIP(hipStreamSynchronize(s));
+        st.assemble += tm.ms(0, 1);
+        st.factor += tm.ms(1, 2);
+        st.solve_he += tm.ms(2, 3);
+        st.select += tm.ms(3, 4);
+        st.post += tm.ms(4, 5);
+        st.total += tm.ms(0, 5);
+    }
+    g_stats = st;
+    return IA_OK;
+}
+
+template <typename T>
 static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const double *r_s,
                                 const double *src, const double *nrm, const double *r_h,
                                 const cx<double> *b, double *lam, int *lidx, int *ktsvd,
                                 double *power, double *smax, cx<double> *vel, cx<double> *pres,
                                 int *status, double *gap, hipStream_t s) {
     const int m = D.m, l = D.l, F = D.n_freq;
-    if (l > m) return fail(IA_ERR_ARG, "ia_esm_sweep: l (%d) > m (%d) is not supported", l, m);
+    if (l > m)
+        return esm_sweep_tall<T>(D, k, r_s, src, nrm, r_h, b, lam, lidx, ktsvd, power, smax, vel,
+                                 pres, status, gap, s);
     int chunk = D.chunk > 0 ? D.chunk : std::min(F, 256);
     chunk = std::min(chunk, F);
     const int R = m, C = l;  // Y = H_e^H (m x l), A = Y^H

@@ -130,3 +130,23 @@ def test_full_size_plate_sweep_matches_oracle(ia):
     rm = _run(ia, P, precision="mixed")
     assert np.max(np.abs(post.power_db(rm.power) - post.power_db(sub.power))) < 0.1
     np.testing.assert_array_equal(rm.lam_index, sub.lam_index)
+
+
+@pytest.mark.parametrize("filter,select", [("tikhonov", "gcv"), ("tsvd", "gcv"), ("tikhonov", "lcurve")])
+def test_more_microphones_than_sources(ia, filter, select):
+    """l > m (256 microphones, 100 sources): H_e is tall, the case the
+    reference's csvd serves by its m >= n branch (tsvd.cuh:39)."""
+    S = sc.sphere_scenario(freqs=np.linspace(400.0, 1800.0, 5), m=100, l=256)
+    r = _run(ia, S, filter=filter, select=select)
+    for f in range(len(S.k)):
+        o = _oracle(S, f, filter=filter, select=select)
+        assert r.k_tsvd[f] == o["k_tsvd"]
+        if filter != "tsvd":
+            assert r.lam_index[f] == o["idx"]
+        assert np.linalg.norm(r.velocity[f] - o["x"]) <= 1e-10 * np.linalg.norm(o["x"])
+        assert np.linalg.norm(r.pressure[f] - o["p_s"]) <= 1e-10 * np.linalg.norm(o["p_s"])
+        assert r.power[f] == pytest.approx(o["W"], rel=1e-10)
+    rm = _run(ia, S, filter=filter, select=select, precision="mixed")
+    assert np.max(np.abs(post.power_db(rm.power) - post.power_db(r.power))) < 0.1
+    if filter != "tsvd":
+        np.testing.assert_array_equal(rm.lam_index, r.lam_index)
