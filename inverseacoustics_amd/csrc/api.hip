@@ -216,6 +216,152 @@ static ia_status esm_sweep_tall(const ia_sweep_desc &D, const double *k, const d
     return IA_OK;
 }
 
+// Everything one chunk of frequencies [f0, f0 + nf) needs, enqueued on
+// stream s with the chunk's workspace W; tm brackets the phases.
+template <typename T>
+static ia_status enqueue_chunk(SweepCache<T> &W, const ia_sweep_desc &D, const RegDev &prm,
+                               int f0, int nf, const double *k, const double *r_s,
+                               const double *src, const double *nrm, const double *r_h,
+                               const cx<double> *b, double *lam, int *lidx, int *ktsvd,
+                               double *power, double *smax, cx<double> *vel, cx<double> *pres,
+                               int *status, double *gap, hipStream_t s, Timer &tm,
+                               KernelClock &clk) {
+    const int m = D.m, l = D.l;
+    const int R = m, C = l;  // Y = H_e^H (m x l), A = Y^H
+    const double rho_c = air().rho * air().c;
+    DevBuf &bD = W.bD, &bY = W.bY, &bipiv = W.bipiv, &binfo = W.binfo, &bc = W.bc,
+           &bbeta = W.bbeta, &bphi = W.bphi, &bx = W.bx, &bq = W.bq, &bps = W.bps, &bpsd = W.bpsd,
+           &bn2 = W.bn2, &bcn2 = W.bcn2, &bperm = W.bperm;
+    SvdWork<T> &sw = W.sw;
+    // complex64: D^H is assembled and factored (P D^H = L U), so that
+    // H_e^H = D^-H M^H takes its two triangular solves with the factors as
+    // stored (untransposed GEMM operands, ~8 % faster in rocBLAS complex64)
+    // and q = D^-1 x solves with the conjugate-transposed factors; complex128
+    // factors D, whose transposed-operand products are the faster ones there
+    const bool herm_D = sizeof(T) == sizeof(float);
+    const double *kf = k + f0;
+    cx<T> *Dm = bD.as<cx<T>>();
+    cx<T> *Y = bY.as<cx<T>>();
+    tm.n = 0;
+    tm.mark();  // 0
+    launch_esm_D<T>(m, nf, kf, r_s, src, nrm, Dm, (size_t)m * m, s, herm_D);
+    if (!herm_D) launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s);
+    IA_HIP(hipGetLastError());
+    tm.mark();  // 1
+    IA_HIP(hipMemsetAsync(binfo.p, 0, sizeof(int) * nf, s));
+    IA_TRY(getrf_batched<T>(m, Dm, m, (size_t)m * m, bipiv.as<int>(), binfo.as<int>(), nf, s));
+    tm.mark();  // 2
+    if (herm_D) {
+        // P D^H = L U: M^H is assembled with its rows already in the
+        // pivoted order P M^H, so the solve is the two triangular
+        // solves in place (no row gather or copy of the right-hand sides)
+        int *pidx = bperm.as<int>();
+        IA_TRY(lu_perm_index(m, bipiv.as<int>(), pidx, pidx + (size_t)m * nf, nf, s));
+        launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s, pidx);
+        IA_TRY(getrs_prepermuted<T>(m, l, Dm, m, (size_t)m * m, Y, m, (size_t)m * l, nf, s));
+    } else {
+        // D = P^T L U: U^H L^H V = M^H in place leaves V = P H_e^H in Y.
+        // The SVD route needs no more: the QR of P X has the R factor of
+        // X and Q_X = P^T Q_V, so only the filtered solution x is
+        // permuted back (after the left vectors are applied)
+        IA_TRY(getrs_c_permuted<T>(m, l, Dm, m, (size_t)m * m, Y, m, (size_t)m * l, nf, s));
+        int *pidx = bperm.as<int>();
+        IA_TRY(lu_perm_index(m, bipiv.as<int>(), pidx, pidx + (size_t)m * nf, nf, s));
+    }
+    tm.mark();  // 3
+    sw.batch = nf;
+    IA_TRY(svd_reduce<T>(sw, Y, s, &clk));
+    tm.mark();  // 4
+    launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(),
+                  sw.sig.template as<double>(), nf, s, sizeof(T) == sizeof(float));
+    tm.mark();  // 5
+    launch_inviter(C, sw.d.template as<double>(), sw.e.template as<double>(),
+                   sw.sig.template as<double>(), sw.QBt.template as<double>(),
+                   sw.PBt.template as<double>(), sw.work.p, sw.sub, nf, s,
+                   sizeof(T) == sizeof(float));
+    tm.mark();  // 6
+    // beta = Q_B^T V_b^H b
+    const cx<double> *bf = b + (size_t)f0 * l;
+    IA_HIP(hipMemcpyAsync(bc.p, bf, sizeof(cx<double>) * (size_t)nf * l, hipMemcpyDefault, s));
+    launch_norm2<double>(l, bc.as<cx<double>>(), l, bn2.as<double>(), nf, s);
+    svd_apply_v<T>(sw, true, bc.as<cx<double>>(), C, 1, s);
+    launch_norm2<double>(C, bc.as<cx<double>>(), C, bcn2.as<double>(), nf, s);
+    launch_real_matvec(C, sw.QBt.template as<double>(), 0, bc.as<cx<double>>(), C,
+                       bbeta.as<cx<double>>(), C, nf, s);
+    launch_select(C, l, sw.sig.template as<double>(), bbeta.as<cx<double>>(), bn2.as<double>(),
+                  prm, lam ? lam + f0 : nullptr, lidx ? lidx + f0 : nullptr,
+                  ktsvd ? ktsvd + f0 : nullptr, bphi.as<cx<double>>(), nf, s,
+                  gap ? gap + f0 : nullptr, bcn2.as<double>(), svd_floor_rel<T>());
+    // x = U_b [P_B phi; 0]
+    launch_real_matvec(C, sw.PBt.template as<double>(), 1, bphi.as<cx<double>>(), C,
+                       bc.as<cx<double>>(), C, nf, s);
+    launch_pad_copy(C, R, bc.as<cx<double>>(), C, bx.as<cx<double>>(), R, nf, s);
+    svd_apply_u<T>(sw, Y, bx.as<cx<double>>(), R, 1, s);
+    IA_HIP(hipGetLastError());
+    tm.mark();  // 7
+    // post: q = D^-1 x, p_s = M_sq q, W = 1/2 Re(x^H p_s) S/m
+    if (sizeof(T) == sizeof(double)) {
+        // Y held P X: x = P^T x_Y, x[r] = x_Y[inv[r]]
+        launch_row_gather<double>(m, 1, bperm.as<int>() + (size_t)m * nf, bx.as<cx<double>>(), m,
+                                  (size_t)m, bq.as<cx<double>>(), m, (size_t)m, nf, s);
+        IA_HIP(hipMemcpyAsync(bx.p, bq.p, sizeof(cx<double>) * (size_t)nf * m,
+                              hipMemcpyDeviceToDevice, s));
+    } else
+        launch_convert<float, double>((size_t)nf * m, bx.as<cx<double>>(),
+                                      reinterpret_cast<cx<float> *>(bq.p), s);
+    IA_TRY(getrs_batched<T>(herm_D ? 'C' : 'N', m, 1, Dm, m, (size_t)m * m, bipiv.as<int>(),
+                            bq.as<cx<T>>(), m, (size_t)m, nf, s));
+    launch_esm_msq_apply<T>(m, nf, kf, rho_c, r_s, src, bq.as<cx<T>>(), m, bps.as<cx<T>>(), m, s);
+    if (sizeof(T) == sizeof(double))
+        IA_HIP(hipMemcpyAsync(bpsd.p, bps.p, sizeof(cx<double>) * (size_t)nf * m,
+                              hipMemcpyDeviceToDevice, s));
+    else
+        launch_convert<double, float>((size_t)nf * m, reinterpret_cast<cx<float> *>(bps.p),
+                                      bpsd.as<cx<double>>(), s);
+    if (power)
+        launch_power<double>(m, bx.as<cx<double>>(), m, bpsd.as<cx<double>>(), m, D.area / m,
+                             power + f0, nf, s);
+    if (vel)
+        IA_HIP(hipMemcpyAsync(vel + (size_t)f0 * m, bx.p, sizeof(cx<double>) * (size_t)nf * m,
+                              hipMemcpyDefault, s));
+    if (pres)
+        IA_HIP(hipMemcpyAsync(pres + (size_t)f0 * m, bpsd.p, sizeof(cx<double>) * (size_t)nf * m,
+                              hipMemcpyDefault, s));
+    if (smax)
+        IA_HIP(hipMemcpy2DAsync(smax + f0, sizeof(double), sw.sig.p, sizeof(double) * C,
+                                sizeof(double), nf, hipMemcpyDefault, s));
+    if (status) IA_HIP(hipMemcpyAsync(status + f0, binfo.p, sizeof(int) * nf, hipMemcpyDefault, s));
+    IA_HIP(hipGetLastError());
+    tm.mark();  // 8
+    return IA_OK;
+}
+
+template <typename T>
+static ia_status alloc_chunk(SweepCache<T> &W, int m, int l, int chunk) {
+    const int R = m, C = l;
+    IA_TRY(W.bD.alloc(sizeof(cx<T>) * (size_t)chunk * m * m));
+    IA_TRY(W.bY.alloc(sizeof(cx<T>) * (size_t)chunk * m * l));
+    IA_TRY(W.bipiv.alloc(sizeof(int) * (size_t)chunk * m));
+    IA_TRY(W.bperm.alloc(2 * sizeof(int) * (size_t)chunk * m));
+    IA_TRY(W.binfo.alloc(sizeof(int) * (size_t)chunk));
+    IA_TRY(W.bc.alloc(sizeof(cx<double>) * (size_t)chunk * C));
+    IA_TRY(W.bbeta.alloc(sizeof(cx<double>) * (size_t)chunk * C));
+    IA_TRY(W.bphi.alloc(sizeof(cx<double>) * (size_t)chunk * C));
+    IA_TRY(W.bx.alloc(sizeof(cx<double>) * (size_t)chunk * R));
+    IA_TRY(W.bq.alloc(sizeof(cx<T>) * (size_t)chunk * m));
+    IA_TRY(W.bps.alloc(sizeof(cx<T>) * (size_t)chunk * m));
+    IA_TRY(W.bpsd.alloc(sizeof(cx<double>) * (size_t)chunk * m));
+    IA_TRY(W.bn2.alloc(sizeof(double) * (size_t)chunk));
+    IA_TRY(W.bcn2.alloc(sizeof(double) * (size_t)chunk));
+    return W.sw.alloc(R, C, chunk);
+}
+
+// Sweep driver: chunks of at most 256 frequencies (or the caller's chunk
+// size) in order on the caller's stream, one workspace.  Splitting the sweep
+// over two streams with two workspaces, in lock-step or with the second
+// chunk started when the first reaches its SVD phases, measured no faster:
+// the matrix-core phases of one chunk slow the latency-bound ones of the
+// other as much as they overlap them.
 template <typename T>
 static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const double *r_s,
                                 const double *src, const double *nrm, const double *r_h,
@@ -226,141 +372,19 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
     if (l > m)
         return esm_sweep_tall<T>(D, k, r_s, src, nrm, r_h, b, lam, lidx, ktsvd, power, smax, vel,
                                  pres, status, gap, s);
-    int chunk = D.chunk > 0 ? D.chunk : std::min(F, 256);
-    chunk = std::min(chunk, F);
-    const int R = m, C = l;  // Y = H_e^H (m x l), A = Y^H
-    const double rho_c = air().rho * air().c;
+    const int chunk = std::min(D.chunk > 0 ? D.chunk : 256, F);
     RegDev prm{(int)D.reg.filter, (int)D.reg.select, D.reg.n_lambda > 0 ? D.reg.n_lambda : 50,
                D.reg.k, D.reg.decades > 0 ? D.reg.decades : 8.0, D.reg.lambda};
     if (prm.n_lambda > 512) return fail(IA_ERR_ARG, "n_lambda must be <= 512");
-
     SweepCache<T> &W = sweep_cache<T>();
-    DevBuf &bD = W.bD, &bY = W.bY, &bipiv = W.bipiv, &binfo = W.binfo, &bc = W.bc,
-           &bbeta = W.bbeta, &bphi = W.bphi, &bx = W.bx, &bq = W.bq, &bps = W.bps, &bpsd = W.bpsd,
-           &bn2 = W.bn2, &bcn2 = W.bcn2, &bperm = W.bperm;
-    IA_TRY(bD.alloc(sizeof(cx<T>) * (size_t)chunk * m * m));
-    IA_TRY(bY.alloc(sizeof(cx<T>) * (size_t)chunk * m * l));
-    IA_TRY(bipiv.alloc(sizeof(int) * (size_t)chunk * m));
-    IA_TRY(bperm.alloc(2 * sizeof(int) * (size_t)chunk * m));
-    IA_TRY(binfo.alloc(sizeof(int) * (size_t)chunk));
-    IA_TRY(bc.alloc(sizeof(cx<double>) * (size_t)chunk * C));
-    IA_TRY(bbeta.alloc(sizeof(cx<double>) * (size_t)chunk * C));
-    IA_TRY(bphi.alloc(sizeof(cx<double>) * (size_t)chunk * C));
-    IA_TRY(bx.alloc(sizeof(cx<double>) * (size_t)chunk * R));
-    IA_TRY(bq.alloc(sizeof(cx<T>) * (size_t)chunk * m));
-    IA_TRY(bps.alloc(sizeof(cx<T>) * (size_t)chunk * m));
-    IA_TRY(bpsd.alloc(sizeof(cx<double>) * (size_t)chunk * m));
-    IA_TRY(bn2.alloc(sizeof(double) * (size_t)chunk));
-    IA_TRY(bcn2.alloc(sizeof(double) * (size_t)chunk));
-    SvdWork<T> &sw = W.sw;
-    IA_TRY(sw.alloc(R, C, chunk));
-
-    // complex64: D^H is assembled and factored (P D^H = L U), so that
-    // H_e^H = D^-H M^H takes its two triangular solves with the factors as
-    // stored (untransposed GEMM operands, ~8 % faster in rocBLAS complex64)
-    // and q = D^-1 x solves with the conjugate-transposed factors; complex128
-    // factors D, whose transposed-operand products are the faster ones there
-    const bool herm_D = sizeof(T) == sizeof(float);
+    IA_TRY(alloc_chunk<T>(W, m, l, chunk));
     Timer tm(s);
     KernelClock clk;
     ia_stats st{};
     for (int f0 = 0; f0 < F; f0 += chunk) {
         const int nf = std::min(chunk, F - f0);
-        const double *kf = k + f0;
-        cx<T> *Dm = bD.as<cx<T>>();
-        cx<T> *Y = bY.as<cx<T>>();
-        tm.n = 0;
-        tm.mark();  // 0
-        launch_esm_D<T>(m, nf, kf, r_s, src, nrm, Dm, (size_t)m * m, s, herm_D);
-        if (!herm_D) launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s);
-        IA_HIP(hipGetLastError());
-        tm.mark();  // 1
-        IA_HIP(hipMemsetAsync(binfo.p, 0, sizeof(int) * nf, s));
-        IA_TRY(getrf_batched<T>(m, Dm, m, (size_t)m * m, bipiv.as<int>(), binfo.as<int>(), nf, s));
-        tm.mark();  // 2
-        if (herm_D) {
-            // P D^H = L U: M^H is assembled with its rows already in the
-            // pivoted order P M^H, so the solve is the two triangular
-            // solves in place (no row gather or copy of the right-hand sides)
-            int *pidx = bperm.as<int>();
-            IA_TRY(lu_perm_index(m, bipiv.as<int>(), pidx, pidx + (size_t)m * nf, nf, s));
-            launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s, pidx);
-            IA_TRY(getrs_prepermuted<T>(m, l, Dm, m, (size_t)m * m, Y, m, (size_t)m * l, nf, s));
-        } else {
-            // D = P^T L U: U^H L^H V = M^H in place leaves V = P H_e^H in Y.
-            // The SVD route needs no more: the QR of P X has the R factor of
-            // X and Q_X = P^T Q_V, so only the filtered solution x is
-            // permuted back (after the left vectors are applied)
-            IA_TRY(getrs_c_permuted<T>(m, l, Dm, m, (size_t)m * m, Y, m, (size_t)m * l, nf, s));
-            int *pidx = bperm.as<int>();
-            IA_TRY(lu_perm_index(m, bipiv.as<int>(), pidx, pidx + (size_t)m * nf, nf, s));
-        }
-        tm.mark();  // 3
-        sw.batch = nf;
-        IA_TRY(svd_reduce<T>(sw, Y, s, &clk));
-        tm.mark();  // 4
-        launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(), nf, s,
-                      sizeof(T) == sizeof(float));
-        tm.mark();  // 5
-        launch_inviter(C, sw.d.template as<double>(), sw.e.template as<double>(), sw.sig.template as<double>(),
-                       sw.QBt.template as<double>(), sw.PBt.template as<double>(), sw.work.p, sw.sub, nf, s,
-                       sizeof(T) == sizeof(float));
-        tm.mark();  // 6
-        // beta = Q_B^T V_b^H b
-        const cx<double> *bf = b + (size_t)f0 * l;
-        IA_HIP(hipMemcpyAsync(bc.p, bf, sizeof(cx<double>) * (size_t)nf * l, hipMemcpyDefault, s));
-        launch_norm2<double>(l, bc.as<cx<double>>(), l, bn2.as<double>(), nf, s);
-        svd_apply_v<T>(sw, true, bc.as<cx<double>>(), C, 1, s);
-        launch_norm2<double>(C, bc.as<cx<double>>(), C, bcn2.as<double>(), nf, s);
-        launch_real_matvec(C, sw.QBt.template as<double>(), 0, bc.as<cx<double>>(), C,
-                           bbeta.as<cx<double>>(), C, nf, s);
-        launch_select(C, l, sw.sig.template as<double>(), bbeta.as<cx<double>>(), bn2.as<double>(), prm,
-                      lam ? lam + f0 : nullptr, lidx ? lidx + f0 : nullptr,
-                      ktsvd ? ktsvd + f0 : nullptr, bphi.as<cx<double>>(), nf, s,
-                      gap ? gap + f0 : nullptr, bcn2.as<double>(), svd_floor_rel<T>());
-        // x = U_b [P_B phi; 0]
-        launch_real_matvec(C, sw.PBt.template as<double>(), 1, bphi.as<cx<double>>(), C,
-                           bc.as<cx<double>>(), C, nf, s);
-        launch_pad_copy(C, R, bc.as<cx<double>>(), C, bx.as<cx<double>>(), R, nf, s);
-        svd_apply_u<T>(sw, Y, bx.as<cx<double>>(), R, 1, s);
-        IA_HIP(hipGetLastError());
-        tm.mark();  // 7
-        // post: q = D^-1 x, p_s = M_sq q, W = 1/2 Re(x^H p_s) S/m
-        if (sizeof(T) == sizeof(double)) {
-            // Y held P X: x = P^T x_Y, x[r] = x_Y[inv[r]]
-            launch_row_gather<double>(m, 1, bperm.as<int>() + (size_t)m * nf, bx.as<cx<double>>(),
-                                      m, (size_t)m, bq.as<cx<double>>(), m, (size_t)m, nf, s);
-            IA_HIP(hipMemcpyAsync(bx.p, bq.p, sizeof(cx<double>) * (size_t)nf * m,
-                                  hipMemcpyDeviceToDevice, s));
-        } else
-            launch_convert<float, double>((size_t)nf * m, bx.as<cx<double>>(),
-                                          reinterpret_cast<cx<float> *>(bq.p), s);
-        IA_TRY(getrs_batched<T>(herm_D ? 'C' : 'N', m, 1, Dm, m, (size_t)m * m, bipiv.as<int>(),
-                                bq.as<cx<T>>(), m, (size_t)m, nf, s));
-        launch_esm_msq_apply<T>(m, nf, kf, rho_c, r_s, src, bq.as<cx<T>>(), m, bps.as<cx<T>>(), m,
-                                s);
-        if (sizeof(T) == sizeof(double))
-            IA_HIP(hipMemcpyAsync(bpsd.p, bps.p, sizeof(cx<double>) * (size_t)nf * m,
-                                  hipMemcpyDeviceToDevice, s));
-        else
-            launch_convert<double, float>((size_t)nf * m, reinterpret_cast<cx<float> *>(bps.p),
-                                          bpsd.as<cx<double>>(), s);
-        if (power)
-            launch_power<double>(m, bx.as<cx<double>>(), m, bpsd.as<cx<double>>(), m, D.area / m,
-                                 power + f0, nf, s);
-        if (vel)
-            IA_HIP(hipMemcpyAsync(vel + (size_t)f0 * m, bx.p, sizeof(cx<double>) * (size_t)nf * m,
-                                  hipMemcpyDefault, s));
-        if (pres)
-            IA_HIP(hipMemcpyAsync(pres + (size_t)f0 * m, bpsd.p,
-                                  sizeof(cx<double>) * (size_t)nf * m, hipMemcpyDefault, s));
-        if (smax)
-            IA_HIP(hipMemcpy2DAsync(smax + f0, sizeof(double), sw.sig.p, sizeof(double) * C,
-                                    sizeof(double), nf, hipMemcpyDefault, s));
-        if (status)
-            IA_HIP(hipMemcpyAsync(status + f0, binfo.p, sizeof(int) * nf, hipMemcpyDefault, s));
-        IA_HIP(hipGetLastError());
-        tm.mark();  // 8
+        IA_TRY(enqueue_chunk<T>(W, D, prm, f0, nf, k, r_s, src, nrm, r_h, b, lam, lidx, ktsvd,
+                                power, smax, vel, pres, status, gap, s, tm, clk));
         IA_HIP(hipStreamSynchronize(s));
         st.assemble += tm.ms(0, 1);
         st.factor += tm.ms(1, 2);

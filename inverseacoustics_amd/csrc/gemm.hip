@@ -3,22 +3,26 @@
 //   C = alpha op(A) op(B) + beta C,   op in {'N', 'C'} (conjugate transpose)
 //
 // Replaces the cuBLAS Zgemm/Zgeam call sites of the reference
-// (CUDA/ibem.cuh:398, CUDA/invr.cuh:402-433) and provides the trailing
-// updates of the LU factorisation and the triangular solves.
+// (CUDA/ibem.cuh:398, CUDA/invr.cuh:402-433) and carries every product of
+// the sweep: the trailing updates of the LU, triangular-solve, QR and band
+// reductions.
 //
-// Complex products are four real MFMAs (re*re, -im*im, re*im, im*re) of
+// Complex products by the 3M (Gauss) method: three real MFMAs per complex
+// multiply-add, re*re, im*im and (re+im)*(re+im), into three accumulators;
+// real part = S1 - S2, imaginary part = S3 - S1 - S2 (normwise stable,
+// Higham, "Stability of a method for multiplying complex matrices with
+// three real matrix multiplications", SIMAX 13, 1992).  The sums re+im are
+// formed once per element when the tile is written to LDS.  MFMAs are
 // v_mfma_f64_16x16x4_f64 (FP64 form) or v_mfma_f32_16x16x4_f32 (mixed
-// form) with real accumulators.  Each wave computes the TRANSPOSED 32x32
-// product op(B)^T op(A)^T so that the 16x16 MFMA result rows land on
-// consecutive lanes along the column-major C rows: one store instruction
-// writes 4 columns x 16 contiguous rows.
+// form).  Each wave computes the TRANSPOSED product op(B)^T op(A)^T so that
+// the 16x16 MFMA result rows land on consecutive lanes along the
+// column-major C rows: one store instruction writes 4 columns x 16
+// contiguous rows.
 //
-// Workgroup tile 64x64, K step 16, 4 waves (one per SIMD), operands staged
-// in LDS as split real/imaginary planes in k-major order so that the MFMA
-// operand reads of 16 consecutive lanes hit 16 consecutive words.
-#include <rocblas/rocblas.h>
-
-#include <cstdlib>
+// Workgroup: 4 waves in a 2 x 2 grid; operands staged in LDS as split
+// planes in k-major order so that the MFMA operand reads of 16 consecutive
+// lanes hit 16 consecutive words; two LDS buffers, the next stage loaded
+// from global memory into registers while the current one is multiplied.
 
 #include "ia_internal.hpp"
 
@@ -50,122 +54,202 @@ struct mfma_t<float> {
     }
 };
 
-constexpr int BM = 64, BN = 64, BK = 16, PAD = 4;
+constexpr int PAD = 4;
 
-template <typename T, char OPA, char OPB, bool BETA0>
-__global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, cx<T> alpha,
+// Tile shape: each of the 4 waves (2 x 2) computes (16 WMB) x (16 WNB) of C,
+// the workgroup (32 WMB) x (32 WNB); BK = depth of one LDS stage; OCC =
+// waves per SIMD the register allocation must allow.
+
+// Batch-major workgroup order with the batch dealt round-robin over the 8
+// XCDs (workgroup h runs on XCD h mod 8): every tile of one matrix runs on
+// one XCD, so the panels it shares stay in that XCD's L2.
+__device__ __forceinline__ void tile_of(int h, int tiles, int batch, int &b, int &t) {
+    if ((batch & 7) == 0) {
+        const int x = h & 7, slot = h >> 3;
+        b = (slot / tiles) * 8 + x;
+        t = slot % tiles;
+    } else {
+        b = h / tiles;
+        t = h % tiles;
+    }
+}
+
+// One stage of an R x BK operand tile (R = rows of the tile) from global
+// memory into registers.  KC: element (r, k) is X[r ldx + k] (contiguous
+// along k), else X[k ldx + r] (contiguous along r); consecutive threads take
+// consecutive addresses either way.  CONJ: the tile holds the conjugates.
+template <typename T, bool KC, bool CONJ, int R, int BK>
+struct TileLoader {
+    static constexpr int NE = R * BK / 256;
+    static_assert(NE * 256 == R * BK, "tile must split evenly over 256 threads");
+    static __device__ __forceinline__ void coords(int t, int i, int &r, int &k) {
+        if (KC) {
+            k = t % BK;
+            r = t / BK + i * (256 / BK);
+        } else {
+            r = t % R;
+            k = t / R + i * (256 / R);
+        }
+    }
+    static __device__ __forceinline__ void load(const cx<T> *__restrict__ X, int ldx, int r0,
+                                                int nr, int k0, int K, int t, cx<T> (&v)[NE]) {
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            int r, k;
+            coords(t, i, r, k);
+            const int gr = r0 + r, gk = k0 + k;
+            v[i] = cx<T>{0, 0};
+            if (gr < nr && gk < K) v[i] = KC ? X[(size_t)gr * ldx + gk] : X[(size_t)gk * ldx + gr];
+        }
+    }
+    // split planes, k-major: S[0][k][r] = re, S[1][k][r] = im and, with
+    // three planes, S[2][k][r] = re + im
+    template <int NP>
+    static __device__ __forceinline__ void store(T (*S)[BK][R + PAD], int t,
+                                                 const cx<T> (&v)[NE]) {
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            int r, k;
+            coords(t, i, r, k);
+            const T re = v[i].re, im = CONJ ? -v[i].im : v[i].im;
+            S[0][k][r] = re;
+            S[1][k][r] = im;
+            if (NP == 3) S[2][k][r] = re + im;
+        }
+    }
+};
+
+// C = alpha op(A) op(B) + beta C over a batch (strides sA, sB, sC).
+// Double-buffered: the next stage's tiles are loaded into registers while the
+// matrix cores work on the current one, then written to the other LDS buffer;
+// one barrier per stage.
+template <typename T, char OPA, char OPB, int WMB, int WNB, int BK, int OCC, bool M3, bool BETA0>
+__global__ __launch_bounds__(256, OCC) void k_zgemm(int M, int N, int K, cx<T> alpha,
                                               const cx<T> *__restrict__ A, int lda, size_t sA,
                                               const cx<T> *__restrict__ B, int ldb, size_t sB,
                                               cx<T> beta, cx<T> *__restrict__ C, int ldc,
-                                              size_t sC) {
+                                              size_t sC, int tiles_m, int tiles_n, int batch) {
     using acc_t = typename mfma_t<T>::acc;
-    __shared__ T As_re[BK][BM + PAD], As_im[BK][BM + PAD];
-    __shared__ T Bs_re[BK][BN + PAD], Bs_im[BK][BN + PAD];
+    constexpr int BM = 32 * WMB, BN = 32 * WNB;
+    // op(A) (BM x BK): A[k lda + m] or conj(A[m lda + k]);
+    // op(B)^T (BN x BK): B[n ldb + k] or conj(B[k ldb + n])
+    using LA = TileLoader<T, OPA == 'C', OPA == 'C', BM, BK>;
+    using LB = TileLoader<T, OPB == 'N', OPB == 'C', BN, BK>;
+    constexpr int NP = M3 ? 3 : 2;  // LDS planes per operand
+    __shared__ T As[2][NP][BK][BM + PAD];
+    __shared__ T Bs[2][NP][BK][BN + PAD];
 
-    const int bz = blockIdx.z;
+    int bz, tile;
+    tile_of(blockIdx.x, tiles_m * tiles_n, batch, bz, tile);
     A += (size_t)bz * sA;
     B += (size_t)bz * sB;
     C += (size_t)bz * sC;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int m0 = (tile % tiles_m) * BM, n0 = (tile / tiles_m) * BN;
     const int t = threadIdx.x;
     const int lane = t & 63, w = t >> 6;
-    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    const int wm = (w & 1) * 16 * WMB, wn = (w >> 1) * 16 * WNB;
     const int l16 = lane & 15, lk = lane >> 4;
 
-    acc_t cre[2][2], cim[2][2];
+    // 4M: cre = sum(re re - im im), cim = sum(im re + re im).  3M (Gauss):
+    // cre = sum(re re), cim = sum(im im), c3 = sum((re + im)(re + im)); the
+    // real part is cre - cim and the imaginary part c3 - cre - cim.
+    acc_t cre[WMB][WNB], cim[WMB][WNB], c3[M3 ? WMB : 1][M3 ? WNB : 1];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < WMB; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
+        for (int b = 0; b < WNB; ++b) {
             cre[a][b] = acc_t{0, 0, 0, 0};
             cim[a][b] = acc_t{0, 0, 0, 0};
+            if (M3) c3[M3 ? a : 0][M3 ? b : 0] = acc_t{0, 0, 0, 0};
         }
 
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        // ---- stage op(A) tile (BM x BK) as As[k][m]
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            int r, kk;
-            cx<T> v{0, 0};
-            if (OPA == 'N') {
-                r = t & 63;
-                kk = (t >> 6) + 4 * it;
-                int gr = m0 + r, gk = k0 + kk;
-                if (gr < M && gk < K) v = A[(size_t)gk * lda + gr];
-            } else {
-                kk = t & 15;
-                r = (t >> 4) + 16 * it;
-                int gr = m0 + r, gk = k0 + kk;
-                if (gr < M && gk < K) {
-                    v = A[(size_t)gr * lda + gk];
-                    v.im = -v.im;
-                }
-            }
-            As_re[kk][r] = v.re;
-            As_im[kk][r] = v.im;
+    cx<T> ra[LA::NE], rb[LB::NE];
+    const int nst = (K + BK - 1) / BK;
+    if (nst > 0) {
+        LA::load(A, lda, m0, M, 0, K, t, ra);
+        LB::load(B, ldb, n0, N, 0, K, t, rb);
+        LA::template store<NP>(As[0], t, ra);
+        LB::template store<NP>(Bs[0], t, rb);
+    }
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int cur = st & 1;
+        const bool more = st + 1 < nst;
+        if (more) {
+            LA::load(A, lda, m0, M, (st + 1) * BK, K, t, ra);
+            LB::load(B, ldb, n0, N, (st + 1) * BK, K, t, rb);
         }
-        // ---- stage op(B) tile (BK x BN) as Bs[k][n]
-#pragma unroll
-        for (int it = 0; it < 4; ++it) {
-            int c, kk;
-            cx<T> v{0, 0};
-            if (OPB == 'N') {
-                kk = t & 15;
-                c = (t >> 4) + 16 * it;
-                int gc = n0 + c, gk = k0 + kk;
-                if (gc < N && gk < K) v = B[(size_t)gc * ldb + gk];
-            } else {
-                c = t & 63;
-                kk = (t >> 6) + 4 * it;
-                int gc = n0 + c, gk = k0 + kk;
-                if (gc < N && gk < K) {
-                    v = B[(size_t)gk * ldb + gc];
-                    v.im = -v.im;
-                }
-            }
-            Bs_re[kk][c] = v.re;
-            Bs_im[kk][c] = v.im;
-        }
-        __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 4) {
-            T are[2], aim[2], naim[2], bre[2], bim[2];
+            T are[WMB], aim[WMB], asum[WMB], bre[WNB], bim[WNB], bsum[WNB];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-                are[a] = As_re[ks + lk][wm + a * 16 + l16];
-                aim[a] = As_im[ks + lk][wm + a * 16 + l16];
-                naim[a] = -aim[a];
-                bre[a] = Bs_re[ks + lk][wn + a * 16 + l16];
-                bim[a] = Bs_im[ks + lk][wn + a * 16 + l16];
+            for (int a = 0; a < WMB; ++a) {
+                are[a] = As[cur][0][ks + lk][wm + a * 16 + l16];
+                aim[a] = As[cur][1][ks + lk][wm + a * 16 + l16];
+                if (M3) asum[a] = As[cur][NP - 1][ks + lk][wm + a * 16 + l16];
             }
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < WNB; ++b) {
+                bre[b] = Bs[cur][0][ks + lk][wn + b * 16 + l16];
+                bim[b] = Bs[cur][1][ks + lk][wn + b * 16 + l16];
+                if (M3) bsum[b] = Bs[cur][NP - 1][ks + lk][wn + b * 16 + l16];
+            }
+            // transposed product: MFMA-A <- op(B)^T rows (n), MFMA-B <- op(A)^T
+            // cols (m); the first products of every block before the second
+            // ones, so that no MFMA waits on the one just issued
+            if (M3) {
 #pragma unroll
-                for (int b = 0; b < 2; ++b) {
-                    // transposed product: MFMA-A <- op(B)^T rows (n), MFMA-B <- op(A)^T cols (m)
-                    cre[a][b] = mfma_t<T>::run(bre[b], are[a], cre[a][b]);
-                    cre[a][b] = mfma_t<T>::run(bim[b], naim[a], cre[a][b]);
-                    cim[a][b] = mfma_t<T>::run(bim[b], are[a], cim[a][b]);
-                    cim[a][b] = mfma_t<T>::run(bre[b], aim[a], cim[a][b]);
-                }
+                for (int a = 0; a < WMB; ++a)
+#pragma unroll
+                    for (int b = 0; b < WNB; ++b) {
+                        cre[a][b] = mfma_t<T>::run(bre[b], are[a], cre[a][b]);
+                        cim[a][b] = mfma_t<T>::run(bim[b], aim[a], cim[a][b]);
+                        c3[M3 ? a : 0][M3 ? b : 0] =
+                            mfma_t<T>::run(bsum[b], asum[a], c3[M3 ? a : 0][M3 ? b : 0]);
+                    }
+            } else {
+#pragma unroll
+                for (int a = 0; a < WMB; ++a)
+#pragma unroll
+                    for (int b = 0; b < WNB; ++b) {
+                        cre[a][b] = mfma_t<T>::run(bre[b], are[a], cre[a][b]);
+                        cim[a][b] = mfma_t<T>::run(bim[b], are[a], cim[a][b]);
+                    }
+#pragma unroll
+                for (int a = 0; a < WMB; ++a)
+#pragma unroll
+                    for (int b = 0; b < WNB; ++b) {
+                        cre[a][b] = mfma_t<T>::run(bim[b], -aim[a], cre[a][b]);
+                        cim[a][b] = mfma_t<T>::run(bre[b], aim[a], cim[a][b]);
+                    }
+            }
+        }
+        if (more) {
+            LA::template store<NP>(As[cur ^ 1], t, ra);
+            LB::template store<NP>(Bs[cur ^ 1], t, rb);
         }
         __syncthreads();
     }
     // ---- epilogue: lane holds D'[n = row(lk, i)][m = l16]
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < WMB; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < WNB; ++b)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                int gm = m0 + wm + a * 16 + l16;
-                int gn = n0 + wn + b * 16 + mfma_t<T>::row(lk, i);
+                const int gm = m0 + wm + a * 16 + l16;
+                const int gn = n0 + wn + b * 16 + mfma_t<T>::row(lk, i);
                 if (gm < M && gn < N) {
                     T xr = cre[a][b][i], xi = cim[a][b][i];
+                    if (M3) {
+                        const T s1 = xr, s2 = xi;
+                        xr = s1 - s2;
+                        xi = c3[M3 ? a : 0][M3 ? b : 0][i] - s1 - s2;
+                    }
                     cx<T> r{alpha.re * xr - alpha.im * xi, alpha.re * xi + alpha.im * xr};
-                    size_t o = (size_t)gn * ldc + gm;
+                    const size_t o = (size_t)gn * ldc + gm;
                     if (!BETA0) {
-                        cx<T> c0 = C[o];
+                        const cx<T> c0 = C[o];
                         r.re += beta.re * c0.re - beta.im * c0.im;
                         r.im += beta.re * c0.im + beta.im * c0.re;
                     }
@@ -174,85 +258,41 @@ __global__ __launch_bounds__(256) void k_zgemm(int M, int N, int K, cx<T> alpha,
             }
 }
 
-template <typename T, char OPA, char OPB>
-void dispatch_beta(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, size_t sA,
-                   const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc, size_t sC,
-                   int batch, hipStream_t s) {
-    dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+template <typename T, char OPA, char OPB, int WMB, int WNB, int BK, int OCC, bool M3>
+void launch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, size_t sA,
+                  const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc, size_t sC,
+                  int batch, hipStream_t s) {
+    const int tm = (M + 32 * WMB - 1) / (32 * WMB), tn = (N + 32 * WNB - 1) / (32 * WNB);
+    const dim3 grid((unsigned)((size_t)tm * tn * batch));
     if (beta.re == 0 && beta.im == 0)
-        hipLaunchKernelGGL((k_zgemm<T, OPA, OPB, true>), grid, dim3(256), 0, s, M, N, K, alpha, A,
-                           lda, sA, B, ldb, sB, beta, C, ldc, sC);
+        hipLaunchKernelGGL((k_zgemm<T, OPA, OPB, WMB, WNB, BK, OCC, M3, true>), grid, dim3(256), 0, s, M, N, K,
+                           alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, tm, tn, batch);
     else
-        hipLaunchKernelGGL((k_zgemm<T, OPA, OPB, false>), grid, dim3(256), 0, s, M, N, K, alpha, A,
-                           lda, sA, B, ldb, sB, beta, C, ldc, sC);
+        hipLaunchKernelGGL((k_zgemm<T, OPA, OPB, WMB, WNB, BK, OCC, M3, false>), grid, dim3(256), 0, s, M, N,
+                           K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, tm, tn, batch);
 }
 
-}  // namespace
-
-namespace {
-
-// rocBLAS handle of the current device and stream for this host thread
-// (created once per pair; streams that run concurrently never share a
-// handle's workspace)
-rocblas_handle blas_handle(hipStream_t s) {
-    struct Slot {
-        int dev;
-        hipStream_t s;
-        rocblas_handle h;
-    };
-    static thread_local Slot slots[16];
-    static thread_local int used = 0;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) {
-        (void)hipGetLastError();
-        return nullptr;
-    }
-    for (int i = 0; i < used; ++i)
-        if (slots[i].dev == dev && slots[i].s == s) return slots[i].h;
-    if (used == 16) return nullptr;  // the caller falls back to the own kernel
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
-    if (rocblas_set_stream(h, s) != rocblas_status_success) {
-        (void)rocblas_destroy_handle(h);
-        return nullptr;
-    }
-    slots[used++] = Slot{dev, s, h};
-    return h;
-}
-
-rocblas_operation blas_op(char op) {
-    return op == 'C' ? rocblas_operation_conjugate_transpose : rocblas_operation_none;
-}
-
-// Plain batched products go to rocBLAS (Tensile kernels tuned for gfx950
-// reach 70-95 % of the matrix-core peak on these shapes, against ~35 % for
-// the single-buffered 64 x 64 tile above, which remains for tiny products
-// and when IA_GEMM_OWN is set).  Returns false when not handled.
-template <typename T>
-bool blas_gemm(char opA, char opB, int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda,
-               size_t sA, const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc,
-               size_t sC, int batch, hipStream_t s) {
-    if ((double)M * N * K < 32768.0 || getenv("IA_GEMM_OWN")) return false;
-    rocblas_handle h = blas_handle(s);
-    if (!h) return false;
-    rocblas_status st;
-    if (sizeof(T) == sizeof(double))
-        st = rocblas_zgemm_strided_batched(
-            h, blas_op(opA), blas_op(opB), M, N, K,
-            reinterpret_cast<const rocblas_double_complex *>(&alpha),
-            reinterpret_cast<const rocblas_double_complex *>(A), lda, (rocblas_stride)sA,
-            reinterpret_cast<const rocblas_double_complex *>(B), ldb, (rocblas_stride)sB,
-            reinterpret_cast<const rocblas_double_complex *>(&beta),
-            reinterpret_cast<rocblas_double_complex *>(C), ldc, (rocblas_stride)sC, batch);
+// Tile choice (measured on the sweep's products, batch 256): 64 x 64 tiles
+// with the 3M product; 32-row (column) tiles when op(A) has at most 32 rows
+// (C at most 32 columns).  Four 64 x 64 workgroups share a CU: their
+// barriers, epilogues and global-load waits overlap each other's MFMAs,
+// which larger tiles at lower occupancy measured slower at.
+template <typename T, char OPA, char OPB>
+void dispatch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, size_t sA,
+                    const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc, size_t sC,
+                    int batch, hipStream_t s) {
+    constexpr bool f32 = sizeof(T) == sizeof(float);
+    constexpr int BK = 8, OCC = f32 ? 4 : 2;
+#define IA_TILES(WMB, WNB)                                                                        \
+    launch_tiles<T, OPA, OPB, WMB, WNB, BK, OCC, true>(M, N, K, alpha, A, lda, sA, B, ldb, sB, \
+                                                           beta, C, ldc, sC, batch, s)
+    if (M <= 32)
+        IA_TILES(1, 2);
+    else if (N <= 32)
+        IA_TILES(2, 1);
     else
-        st = rocblas_cgemm_strided_batched(
-            h, blas_op(opA), blas_op(opB), M, N, K,
-            reinterpret_cast<const rocblas_float_complex *>(&alpha),
-            reinterpret_cast<const rocblas_float_complex *>(A), lda, (rocblas_stride)sA,
-            reinterpret_cast<const rocblas_float_complex *>(B), ldb, (rocblas_stride)sB,
-            reinterpret_cast<const rocblas_float_complex *>(&beta),
-            reinterpret_cast<rocblas_float_complex *>(C), ldc, (rocblas_stride)sC, batch);
-    return st == rocblas_status_success;
+        IA_TILES(2, 2);
+#undef IA_TILES
 }
 
 }  // namespace
@@ -262,21 +302,18 @@ void launch_gemm(char opA, char opB, int M, int N, int K, cx<T> alpha, const cx<
                  size_t strideA, const cx<T> *B, int ldb, size_t strideB, cx<T> beta, cx<T> *C,
                  int ldc, size_t strideC, int batch, hipStream_t s) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
-    if (blas_gemm<T>(opA, opB, M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
-                     strideC, batch, s))
-        return;
     if (opA == 'N' && opB == 'N')
-        dispatch_beta<T, 'N', 'N'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
-                                   strideC, batch, s);
+        dispatch_tiles<T, 'N', 'N'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C,
+                                    ldc, strideC, batch, s);
     else if (opA == 'C' && opB == 'N')
-        dispatch_beta<T, 'C', 'N'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
-                                   strideC, batch, s);
+        dispatch_tiles<T, 'C', 'N'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C,
+                                    ldc, strideC, batch, s);
     else if (opA == 'N' && opB == 'C')
-        dispatch_beta<T, 'N', 'C'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
-                                   strideC, batch, s);
+        dispatch_tiles<T, 'N', 'C'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C,
+                                    ldc, strideC, batch, s);
     else
-        dispatch_beta<T, 'C', 'C'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C, ldc,
-                                   strideC, batch, s);
+        dispatch_tiles<T, 'C', 'C'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C,
+                                    ldc, strideC, batch, s);
 }
 
 template void launch_gemm<double>(char, char, int, int, int, cx<double>, const cx<double> *, int,

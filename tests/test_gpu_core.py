@@ -16,21 +16,16 @@ def _cr(shape, seed):
     return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
 
 
-@pytest.fixture(params=["library", "own"])
-def gemm_path(request, monkeypatch):
-    """Large products run on rocBLAS, tiny ones (and all with IA_GEMM_OWN set)
-    on the hand-written MFMA tile kernel; both are checked."""
-    if request.param == "own":
-        monkeypatch.setenv("IA_GEMM_OWN", "1")
-    else:
-        monkeypatch.delenv("IA_GEMM_OWN", raising=False)
-    return request.param
+# shapes that reach every tile configuration of the MFMA kernel (32-row,
+# 32-column and 64 x 64 tiles), partial tiles and K not a multiple of the
+# LDS stage depth
+GEMM_SHAPES = [(1, 1, 1), (17, 5, 3), (64, 64, 16), (130, 77, 45), (5, 200, 1), (300, 257, 96),
+               (32, 300, 70), (300, 20, 33), (40, 100, 9), (129, 65, 200)]
 
 
 @pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("C", "C")])
-@pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 5, 3), (64, 64, 16), (130, 77, 45), (5, 200, 1),
-                                 (300, 257, 96)])
-def test_zgemm(ia, opa, opb, mnk, gemm_path):
+@pytest.mark.parametrize("mnk", GEMM_SHAPES)
+def test_zgemm(ia, opa, opb, mnk):
     m, n, k = mnk
     A = _cr((m, k) if opa == "N" else (k, m), 1)
     B = _cr((k, n) if opb == "N" else (n, k), 2)
@@ -40,11 +35,15 @@ def test_zgemm(ia, opa, opb, mnk, gemm_path):
     ref = (0.5 - 2j) * opA @ opB + (1.5 + 0.25j) * C0
     got = ia.zgemm(A, B, opa, opb, alpha=0.5 - 2j, beta=1.5 + 0.25j, Cm=C0)
     assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()) * np.sqrt(k)
+    # beta = 0 reads nothing of C
+    got0 = ia.zgemm(A, B, opa, opb, alpha=1.0, beta=0.0, Cm=np.full((m, n), np.nan + 0j))
+    assert np.abs(got0 - opA @ opB).max() <= 1e-13 * max(1.0, np.abs(opA @ opB).max()) * np.sqrt(k)
 
 
 @pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("C", "C")])
-@pytest.mark.parametrize("mnk", [(17, 5, 3), (130, 77, 45), (300, 257, 96)])
-def test_cgemm_fp32(ia, opa, opb, mnk, gemm_path):
+@pytest.mark.parametrize("mnk", [(17, 5, 3), (130, 77, 45), (300, 257, 96), (32, 300, 70),
+                                 (300, 20, 33)])
+def test_cgemm_fp32(ia, opa, opb, mnk):
     m, n, k = mnk
     A = _cr((m, k) if opa == "N" else (k, m), 1).astype(np.complex64)
     B = _cr((k, n) if opb == "N" else (n, k), 2).astype(np.complex64)
