@@ -235,9 +235,9 @@ static ia_status enqueue_chunk(SweepCache<T> &W, const ia_sweep_desc &D, const R
     SvdWork<T> &sw = W.sw;
     // complex64: D^H is assembled and factored (P D^H = L U), so that
     // H_e^H = D^-H M^H takes its two triangular solves with the factors as
-    // stored (untransposed GEMM operands, ~8 % faster in rocBLAS complex64)
-    // and q = D^-1 x solves with the conjugate-transposed factors; complex128
-    // factors D, whose transposed-operand products are the faster ones there
+    // stored (untransposed GEMM operands, and M^H assembled in pivoted row
+    // order) and q = D^-1 x solves with the conjugate-transposed factors;
+    // complex128 factors D and solves with conjugate-transposed operands
     const bool herm_D = sizeof(T) == sizeof(float);
     const double *kf = k + f0;
     cx<T> *Dm = bD.as<cx<T>>();

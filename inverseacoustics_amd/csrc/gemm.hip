@@ -274,9 +274,11 @@ void launch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, siz
 
 // Tile choice (measured on the sweep's products, batch 256): 64 x 64 tiles
 // with the 3M product; 32-row (column) tiles when op(A) has at most 32 rows
-// (C at most 32 columns).  Four 64 x 64 workgroups share a CU: their
-// barriers, epilogues and global-load waits overlap each other's MFMAs,
-// which larger tiles at lower occupancy measured slower at.
+// (C at most 32 columns), and 32 x 32 tiles when 64 x 64 ones would leave
+// fewer than four workgroups per CU (small batches).  Four 64 x 64
+// workgroups share a CU: their barriers, epilogues and global-load waits
+// overlap each other's MFMAs, which larger tiles at lower occupancy
+// measured slower at.
 template <typename T, char OPA, char OPB>
 void dispatch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, size_t sA,
                     const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc, size_t sC,
@@ -286,10 +288,15 @@ void dispatch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, s
 #define IA_TILES(WMB, WNB)                                                                        \
     launch_tiles<T, OPA, OPB, WMB, WNB, BK, OCC, true>(M, N, K, alpha, A, lda, sA, B, ldb, sB, \
                                                            beta, C, ldc, sC, batch, s)
-    if (M <= 32)
+    const double wg64 = (double)((M + 63) / 64) * ((N + 63) / 64) * batch;
+    if (M <= 32 && N <= 32)
+        IA_TILES(1, 1);
+    else if (M <= 32)
         IA_TILES(1, 2);
     else if (N <= 32)
         IA_TILES(2, 1);
+    else if (wg64 < 4.0 * 256)
+        IA_TILES(1, 1);
     else
         IA_TILES(2, 2);
 #undef IA_TILES

@@ -17,10 +17,10 @@ def _cr(shape, seed):
 
 
 # shapes that reach every tile configuration of the MFMA kernel (32-row,
-# 32-column and 64 x 64 tiles), partial tiles and K not a multiple of the
-# LDS stage depth
+# 32-column, 32 x 32 and -- from 1024 tiles of 64 x 64 on -- 64 x 64 tiles),
+# partial tiles and K not a multiple of the LDS stage depth
 GEMM_SHAPES = [(1, 1, 1), (17, 5, 3), (64, 64, 16), (130, 77, 45), (5, 200, 1), (300, 257, 96),
-               (32, 300, 70), (300, 20, 33), (40, 100, 9), (129, 65, 200)]
+               (32, 300, 70), (300, 20, 33), (40, 100, 9), (129, 65, 200), (2048, 2047, 40)]
 
 
 @pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("C", "C")])
@@ -42,7 +42,7 @@ def test_zgemm(ia, opa, opb, mnk):
 
 @pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("C", "C")])
 @pytest.mark.parametrize("mnk", [(17, 5, 3), (130, 77, 45), (300, 257, 96), (32, 300, 70),
-                                 (300, 20, 33)])
+                                 (300, 20, 33), (2048, 2047, 40)])
 def test_cgemm_fp32(ia, opa, opb, mnk):
     m, n, k = mnk
     A = _cr((m, k) if opa == "N" else (k, m), 1).astype(np.complex64)
