@@ -24,6 +24,10 @@
 // lanes hit 16 consecutive words; two LDS buffers, the next stage loaded
 // from global memory into registers while the current one is multiplied.
 
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+
 #include "ia_internal.hpp"
 
 namespace ia {
@@ -304,11 +308,33 @@ void dispatch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, s
 
 }  // namespace
 
+namespace {
+
+// IA_GEMM_LOG=<path>: every product is appended to <path> as
+// "GEMMLOG <sizeof real> <opA> <opB> M N K batch lda ldb ldc <beta == 0>"
+// (the input of the rate benchmark csrc/bench/gemm_rates.hip)
+void log_call(int real_bytes, char opA, char opB, int M, int N, int K, int batch, int lda,
+              int ldb, int ldc, bool beta0) {
+    static const char *path = std::getenv("IA_GEMM_LOG");
+    if (!path || !*path) return;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    if (FILE *f = std::fopen(path, "a")) {
+        std::fprintf(f, "GEMMLOG %d %c %c %d %d %d %d %d %d %d %d\n", real_bytes, opA, opB, M, N, K,
+                     batch, lda, ldb, ldc, beta0 ? 1 : 0);
+        std::fclose(f);
+    }
+}
+
+}  // namespace
+
 template <typename T>
 void launch_gemm(char opA, char opB, int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda,
                  size_t strideA, const cx<T> *B, int ldb, size_t strideB, cx<T> beta, cx<T> *C,
                  int ldc, size_t strideC, int batch, hipStream_t s) {
     if (M <= 0 || N <= 0 || batch <= 0) return;
+    log_call((int)sizeof(T), opA, opB, M, N, K, batch, lda, ldb, ldc,
+             beta.re == 0 && beta.im == 0);
     if (opA == 'N' && opB == 'N')
         dispatch_tiles<T, 'N', 'N'>(M, N, K, alpha, A, lda, strideA, B, ldb, strideB, beta, C,
                                     ldc, strideC, batch, s);
