@@ -93,10 +93,9 @@ struct SweepCache {
     SvdWork<T> sw;
     void release() {
         for (DevBuf *b : {&bD, &bY, &bipiv, &bperm, &binfo, &bc, &bbeta, &bphi, &bx, &bq, &bps, &bpsd,
-                          &bn2, &bcn2, &sw.d, &sw.e, &sw.sig, &sw.tauL, &sw.tauR, &sw.Vr, &sw.QBt,
-                          &sw.PBt, &sw.work, &sw.gw, &sw.Rb, &sw.tauQ, &sw.qw, &sw.Bs, &sw.uR2,
-                          &sw.uL2, &sw.tR2, &sw.tL2, &sw.Pm, &sw.tauP})
+                          &bn2, &bcn2})
             b->reset();
+        sw.release();
     }
 };
 template <typename T>
@@ -442,6 +441,7 @@ void ia_release(void) {
     sweep_cache<float>().release();
     release_grad_cache();
     release_rad_cache();
+    release_solve_cache();
 }
 
 ia_status ia_last_stats(ia_stats *st) {

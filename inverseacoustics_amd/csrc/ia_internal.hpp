@@ -222,6 +222,8 @@ void launch_bulge_apply(bool fwd, int C, const cx<T> *u, const cx<T> *tau, cx<do
 // keep per host thread
 void release_grad_cache();
 void release_rad_cache();
+// ... and the regularised-solve / HELS workspaces (solve.hip)
+void release_solve_cache();
 
 // ADMM (admm.hip): A (rows x cols, batch stride rows*cols), b, c on the device
 ia_status admm_batched(int rows, int cols, int batch, const cx<double> *A, const cx<double> *b,
@@ -384,6 +386,11 @@ struct SvdWork {
     DevBuf Bs, uR2, uL2, tR2, tL2, Pm, tauP;   // two-stage form
     int R = 0, C = 0, batch = 0, sub = 1;
     bool qr = false, ts = false;
+    void release() {
+        for (DevBuf *b : {&d, &e, &sig, &tauL, &tauR, &Vr, &QBt, &PBt, &work, &gw, &Rb, &tauQ, &qw,
+                          &Bs, &uR2, &uL2, &tR2, &tL2, &Pm, &tauP})
+            b->reset();
+    }
     ia_status alloc(int R_, int C_, int batch_) {
         R = R_;
         C = C_;

@@ -7,6 +7,8 @@
 // reference (tsvd.cuh:10-172, MATLAB/regtools) without forming U or V.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -21,24 +23,57 @@ void launch_hels(int npts, int N, int nf, const double *k, double rho_c, const d
                  cx<T> *M, cx<T> *Q, size_t stride, hipStream_t s);
 int hels_nmax();
 
-// A is complex128 on the device; T selects the SVD precision.
+// Device workspace of one batched regularised solve.  Kept per host thread,
+// precision and slot between calls (buffers only grow; ia_release() frees
+// them); slot 0 serves the blocking callers, slots 1.. the HELS order
+// groups that run side by side on their own streams.
 template <typename T>
-ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, size_t sA,
-                           const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
-                           size_t sx, double *lam, int *idx, int *kts, double *smax,
-                           hipStream_t s, double *gap = nullptr) {
+struct RegsolveWork {
+    DevBuf bY, bYd, bc, bbeta, bphi, bn2, bcn2, bv;
+    SvdWork<T> sw;
+    ia_status reserve(int rows, int cols, int batch) {
+        const bool tall = rows >= cols;
+        const int R = tall ? rows : cols, C = tall ? cols : rows;
+        IA_TRY(bY.alloc(sizeof(cx<T>) * (size_t)R * C * batch));
+        if (sizeof(T) != sizeof(double))
+            IA_TRY(bYd.alloc(sizeof(cx<double>) * (size_t)R * C * batch));
+        IA_TRY(sw.alloc(R, C, batch));
+        IA_TRY(bc.alloc(sizeof(cx<double>) * (size_t)R * batch));
+        IA_TRY(bbeta.alloc(sizeof(cx<double>) * (size_t)C * batch));
+        IA_TRY(bphi.alloc(sizeof(cx<double>) * (size_t)C * batch));
+        IA_TRY(bv.alloc(sizeof(cx<double>) * (size_t)R * batch));
+        IA_TRY(bn2.alloc(sizeof(double) * batch));
+        return bcn2.alloc(sizeof(double) * batch);
+    }
+    void release() {
+        for (DevBuf *b : {&bY, &bYd, &bc, &bbeta, &bphi, &bn2, &bcn2, &bv}) b->reset();
+        sw.release();
+    }
+};
+
+constexpr int HELS_SLOTS = 8;  // order groups in flight at once
+
+template <typename T>
+RegsolveWork<T> &regsolve_work(int slot) {
+    static thread_local RegsolveWork<T> *w[HELS_SLOTS + 1] = {};
+    if (!w[slot]) w[slot] = new RegsolveWork<T>();
+    return *w[slot];
+}
+
+// Enqueue the solve on s with workspace W (nothing waits; W and the outputs
+// stay in use until s reaches this point).  A is complex128 on the device;
+// T selects the SVD precision.
+template <typename T>
+ia_status regsolve_enqueue(RegsolveWork<T> &W, int rows, int cols, int batch,
+                           const cx<double> *A, size_t sA, const cx<double> *b, size_t sb,
+                           const RegDev &prm, cx<double> *x, size_t sx, double *lam, int *idx,
+                           int *kts, double *smax, hipStream_t s, double *gap) {
     const bool tall = rows >= cols;
     const int R = tall ? rows : cols, C = tall ? cols : rows;
-    DevBuf bY, bYd, bc, bbeta, bphi, bn2, bcn2, bv;
-    IA_TRY(bY.alloc(sizeof(cx<T>) * (size_t)R * C * batch));
-    cx<T> *Y = bY.as<cx<T>>();
-    cx<double> *Yd = nullptr;
-    if (sizeof(T) == sizeof(double)) {
-        Yd = reinterpret_cast<cx<double> *>(Y);
-    } else {
-        IA_TRY(bYd.alloc(sizeof(cx<double>) * (size_t)R * C * batch));
-        Yd = bYd.as<cx<double>>();
-    }
+    IA_TRY(W.reserve(rows, cols, batch));
+    cx<T> *Y = W.bY.template as<cx<T>>();
+    cx<double> *Yd = sizeof(T) == sizeof(double) ? reinterpret_cast<cx<double> *>(Y)
+                                                 : W.bYd.template as<cx<double>>();
     if (tall) {
         IA_HIP(hipMemcpy2DAsync(Yd, sizeof(cx<double>) * (size_t)R * C, A, sizeof(cx<double>) * sA,
                                 sizeof(cx<double>) * (size_t)R * C, batch, hipMemcpyDeviceToDevice,
@@ -48,52 +83,45 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
     }
     if (sizeof(T) != sizeof(double))
         launch_convert<float, double>((size_t)R * C * batch, Yd, reinterpret_cast<cx<float> *>(Y), s);
-    SvdWork<T> sw;
-    IA_TRY(sw.alloc(R, C, batch));
+    SvdWork<T> &sw = W.sw;
     IA_TRY(svd_factor<T>(sw, Y, s));
-    IA_TRY(bc.alloc(sizeof(cx<double>) * (size_t)R * batch));
-    IA_TRY(bbeta.alloc(sizeof(cx<double>) * (size_t)C * batch));
-    IA_TRY(bphi.alloc(sizeof(cx<double>) * (size_t)C * batch));
-    IA_TRY(bv.alloc(sizeof(cx<double>) * (size_t)R * batch));
-    IA_TRY(bn2.alloc(sizeof(double) * batch));
-    IA_TRY(bcn2.alloc(sizeof(double) * batch));
-    cx<double> *c = bc.as<cx<double>>();
-    launch_norm2<double>(rows, b, sb, bn2.as<double>(), batch, s);
+    cx<double> *c = W.bc.template as<cx<double>>();
+    launch_norm2<double>(rows, b, sb, W.bn2.template as<double>(), batch, s);
     if (tall) {
         // beta = P_B^T (U_b^H b)[:C]
         IA_HIP(hipMemcpy2DAsync(c, sizeof(cx<double>) * R, b, sizeof(cx<double>) * sb,
                                 sizeof(cx<double>) * rows, batch, hipMemcpyDeviceToDevice, s));
         svd_apply_uh<T>(sw, Y, c, R, 1, s);
-        launch_norm2<double>(C, c, R, bcn2.as<double>(), batch, s);
-        launch_real_matvec(C, sw.PBt.template as<double>(), 0, c, R, bbeta.as<cx<double>>(), C,
+        launch_norm2<double>(C, c, R, W.bcn2.template as<double>(), batch, s);
+        launch_real_matvec(C, sw.PBt.template as<double>(), 0, c, R, W.bbeta.template as<cx<double>>(), C,
                            batch, s);
     } else {
         // beta = Q_B^T (V_b^H b)
         IA_HIP(hipMemcpy2DAsync(c, sizeof(cx<double>) * C, b, sizeof(cx<double>) * sb,
                                 sizeof(cx<double>) * rows, batch, hipMemcpyDeviceToDevice, s));
         svd_apply_v<T>(sw, true, c, C, 1, s);
-        launch_norm2<double>(C, c, C, bcn2.as<double>(), batch, s);
-        launch_real_matvec(C, sw.QBt.template as<double>(), 0, c, C, bbeta.as<cx<double>>(), C,
+        launch_norm2<double>(C, c, C, W.bcn2.template as<double>(), batch, s);
+        launch_real_matvec(C, sw.QBt.template as<double>(), 0, c, C, W.bbeta.template as<cx<double>>(), C,
                            batch, s);
     }
     RegDev prm2 = prm;
     if (prm2.select == IA_SELECT_RANK) prm2.k_fixed = std::max(rows, cols);  // rank cutoff dimension
-    launch_select(C, rows, sw.sig.template as<double>(), bbeta.as<cx<double>>(), bn2.as<double>(),
-                  prm2, lam, idx, kts, bphi.as<cx<double>>(), batch, s, gap, bcn2.as<double>(),
-                  svd_floor_rel<T>());
-    cx<double> *v = bv.as<cx<double>>();
+    launch_select(C, rows, sw.sig.template as<double>(), W.bbeta.template as<cx<double>>(),
+                  W.bn2.template as<double>(), prm2, lam, idx, kts, W.bphi.template as<cx<double>>(), batch, s, gap,
+                  W.bcn2.template as<double>(), svd_floor_rel<T>());
+    cx<double> *v = W.bv.template as<cx<double>>();
     if (tall) {
         // x = V_b (Q_B phi)
-        launch_real_matvec(C, sw.QBt.template as<double>(), 1, bphi.as<cx<double>>(), C, v, C,
+        launch_real_matvec(C, sw.QBt.template as<double>(), 1, W.bphi.template as<cx<double>>(), C, v, C,
                            batch, s);
         svd_apply_v<T>(sw, false, v, C, 1, s);
         IA_HIP(hipMemcpy2DAsync(x, sizeof(cx<double>) * sx, v, sizeof(cx<double>) * C,
                                 sizeof(cx<double>) * C, batch, hipMemcpyDeviceToDevice, s));
     } else {
         // x = U_b [P_B phi; 0]
-        launch_real_matvec(C, sw.PBt.template as<double>(), 1, bphi.as<cx<double>>(), C,
-                           bbeta.as<cx<double>>(), C, batch, s);
-        launch_pad_copy(C, R, bbeta.as<cx<double>>(), C, v, R, batch, s);
+        launch_real_matvec(C, sw.PBt.template as<double>(), 1, W.bphi.template as<cx<double>>(), C,
+                           W.bbeta.template as<cx<double>>(), C, batch, s);
+        launch_pad_copy(C, R, W.bbeta.template as<cx<double>>(), C, v, R, batch, s);
         svd_apply_u<T>(sw, Y, v, R, 1, s);
         IA_HIP(hipMemcpy2DAsync(x, sizeof(cx<double>) * sx, v, sizeof(cx<double>) * R,
                                 sizeof(cx<double>) * R, batch, hipMemcpyDeviceToDevice, s));
@@ -102,6 +130,17 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
         IA_HIP(hipMemcpy2DAsync(smax, sizeof(double), sw.sig.p, sizeof(double) * C, sizeof(double),
                                 batch, hipMemcpyDeviceToDevice, s));
     IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
+// Blocking form (slot-0 workspace): returns when s has finished the solve.
+template <typename T>
+ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, size_t sA,
+                           const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
+                           size_t sx, double *lam, int *idx, int *kts, double *smax,
+                           hipStream_t s, double *gap = nullptr) {
+    IA_TRY(regsolve_enqueue<T>(regsolve_work<T>(0), rows, cols, batch, A, sA, b, sb, prm, x, sx,
+                               lam, idx, kts, smax, s, gap));
     IA_HIP(hipStreamSynchronize(s));
     return IA_OK;
 }
@@ -118,6 +157,58 @@ template ia_status regsolve_batched<float>(int, int, int, const cx<double> *, si
 RegDev to_regdev(const ia_reg_params &p) {
     return RegDev{(int)p.filter, (int)p.select, p.n_lambda > 0 ? p.n_lambda : 50, p.k,
                   p.decades > 0 ? p.decades : 8.0, p.lambda};
+}
+
+// HELS transfer matrices and coefficients of one order group (per slot)
+struct HelsSlot {
+    DevBuf bH, bM, bQ, bx;
+    hipStream_t s = nullptr;
+};
+// inputs in group order and outputs of one HELS sweep
+struct HelsIo {
+    DevBuf ds, dh, dk, db, lam, idx, kt, sm, W, v, p;
+};
+// plain regularised solve (ia_regsolve): device copies of its arguments
+struct RegsolveIo {
+    DevBuf A, b, x, lam, idx, kt, sm;
+};
+
+HelsSlot &hels_slot(int i) {
+    static thread_local HelsSlot *h[HELS_SLOTS] = {};
+    if (!h[i]) h[i] = new HelsSlot();
+    return *h[i];
+}
+HelsIo &hels_io() {
+    static thread_local HelsIo *h = new HelsIo();
+    return *h;
+}
+RegsolveIo &regsolve_io() {
+    static thread_local RegsolveIo *r = new RegsolveIo();
+    return *r;
+}
+
+// the slot's stream: created once per host thread, default priority,
+// non-blocking (ordered against the legacy stream by explicit waits)
+ia_status hels_stream(int i, hipStream_t *out) {
+    HelsSlot &h = hels_slot(i);
+    if (!h.s) IA_HIP(hipStreamCreateWithFlags(&h.s, hipStreamNonBlocking));
+    *out = h.s;
+    return IA_OK;
+}
+
+void release_solve_cache() {
+    for (int i = 0; i <= HELS_SLOTS; ++i) {
+        regsolve_work<double>(i).release();
+        regsolve_work<float>(i).release();
+    }
+    for (int i = 0; i < HELS_SLOTS; ++i)
+        for (DevBuf *b : {&hels_slot(i).bH, &hels_slot(i).bM, &hels_slot(i).bQ, &hels_slot(i).bx})
+            b->reset();
+    HelsIo &h = hels_io();
+    for (DevBuf *b : {&h.ds, &h.dh, &h.dk, &h.db, &h.lam, &h.idx, &h.kt, &h.sm, &h.W, &h.v, &h.p})
+        b->reset();
+    RegsolveIo &r = regsolve_io();
+    for (DevBuf *b : {&r.A, &r.b, &r.x, &r.lam, &r.idx, &r.kt, &r.sm}) b->reset();
 }
 
 ia_status check_reg(const ia_reg_params &p, int pmax) {
@@ -201,85 +292,123 @@ ia_status ia_hels_sweep(const ia_hels_desc *desc, const int *order, const double
     IA_TRY(check_reg(desc->reg, pmax));
     const RegDev prm = to_regdev(desc->reg);
     const double rho_c = air().rho * air().c;
-    hipStream_t s = nullptr;
-    DevBuf ds, dh;
-    IA_TRY(to_device(ds, azelr_s, sizeof(double) * 3 * m, s));
-    IA_TRY(to_device(dh, azelr_h, sizeof(double) * 3 * l, s));
+    const bool mixed = desc->precision == IA_PREC_MIXED;
     ia_sweep_out o{};
     if (out) o = *out;
-    // group frequencies of equal order: one batched launch per system size
+
+    // Frequencies of equal order form one batched system size.  The groups
+    // run side by side, each on its own stream and workspace (slot), with a
+    // single wait at the end; frequencies are laid out in group order.
     std::map<int, std::vector<int>> groups;
     for (int f = 0; f < F; ++f) groups[order[f]].push_back(f);
+    std::vector<int> gN, goff, gnb, perm;
     for (auto &g : groups) {
-        const int N = g.first, nb = (int)g.second.size(), n_t = (N + 1) * (N + 1);
-        std::vector<double> kg(nb);
-        std::vector<ia_z> bg((size_t)nb * l);
-        for (int i = 0; i < nb; ++i) {
-            kg[i] = k[g.second[i]];
-            memcpy(&bg[(size_t)i * l], b + (size_t)g.second[i] * l, sizeof(ia_z) * l);
-        }
-        DevBuf dk, db, bH, bM, bQ, bx, bv, bp, blam, bidx, bkt, bsm, bW;
-        IA_TRY(to_device(dk, kg.data(), sizeof(double) * nb, s));
-        IA_TRY(to_device(db, bg.data(), sizeof(ia_z) * (size_t)nb * l, s));
-        IA_TRY(bH.alloc(sizeof(cx<double>) * (size_t)l * n_t * nb));
-        IA_TRY(bM.alloc(sizeof(cx<double>) * (size_t)m * n_t * nb));
-        IA_TRY(bQ.alloc(sizeof(cx<double>) * (size_t)m * n_t * nb));
-        IA_TRY(bx.alloc(sizeof(cx<double>) * (size_t)n_t * nb));
-        IA_TRY(bv.alloc(sizeof(cx<double>) * (size_t)m * nb));
-        IA_TRY(bp.alloc(sizeof(cx<double>) * (size_t)m * nb));
-        IA_TRY(blam.alloc(sizeof(double) * nb));
-        IA_TRY(bidx.alloc(sizeof(int) * nb));
-        IA_TRY(bkt.alloc(sizeof(int) * nb));
-        IA_TRY(bsm.alloc(sizeof(double) * nb));
-        IA_TRY(bW.alloc(sizeof(double) * nb));
-        launch_hels<double>(l, N, nb, dk.as<double>(), rho_c, dh.as<double>(), bH.as<cx<double>>(),
-                            nullptr, (size_t)l * n_t, s);
-        launch_hels<double>(m, N, nb, dk.as<double>(), rho_c, ds.as<double>(), bM.as<cx<double>>(),
-                            bQ.as<cx<double>>(), (size_t)m * n_t, s);
-        IA_HIP(hipGetLastError());
-        ia_status st;
-        if (desc->precision == IA_PREC_MIXED)
-            st = regsolve_batched<float>(l, n_t, nb, bH.as<cx<double>>(), (size_t)l * n_t,
-                                         db.as<cx<double>>(), l, prm, bx.as<cx<double>>(), n_t,
-                                         blam.as<double>(), bidx.as<int>(), bkt.as<int>(),
-                                         bsm.as<double>(), s);
+        gN.push_back(g.first);
+        goff.push_back((int)perm.size());
+        gnb.push_back((int)g.second.size());
+        perm.insert(perm.end(), g.second.begin(), g.second.end());
+    }
+    const int G = (int)gN.size(), NS = std::min(G, HELS_SLOTS);
+    std::vector<double> kp(F);
+    std::vector<ia_z> bp((size_t)F * l);
+    for (int i = 0; i < F; ++i) {
+        kp[i] = k[perm[i]];
+        memcpy(&bp[(size_t)i * l], b + (size_t)perm[i] * l, sizeof(ia_z) * l);
+    }
+    HelsIo &io = hels_io();
+    IA_TRY(to_device(io.ds, azelr_s, sizeof(double) * 3 * m, nullptr));
+    IA_TRY(to_device(io.dh, azelr_h, sizeof(double) * 3 * l, nullptr));
+    IA_TRY(to_device(io.dk, kp.data(), sizeof(double) * F, nullptr));
+    IA_TRY(to_device(io.db, bp.data(), sizeof(ia_z) * (size_t)F * l, nullptr));
+    IA_TRY(io.lam.alloc(sizeof(double) * F));
+    IA_TRY(io.idx.alloc(sizeof(int) * F));
+    IA_TRY(io.kt.alloc(sizeof(int) * F));
+    IA_TRY(io.sm.alloc(sizeof(double) * F));
+    IA_TRY(io.W.alloc(sizeof(double) * F));
+    IA_TRY(io.v.alloc(sizeof(cx<double>) * (size_t)F * m));
+    IA_TRY(io.p.alloc(sizeof(cx<double>) * (size_t)F * m));
+    // every buffer sized before the first group is enqueued, so that no
+    // slot reallocates under work in flight (groups beyond HELS_SLOTS share
+    // a slot and follow each other on its stream)
+    for (int g = 0; g < G; ++g) {
+        HelsSlot &S = hels_slot(g % NS);
+        const int n_t = (gN[g] + 1) * (gN[g] + 1), nb = gnb[g];
+        IA_TRY(S.bH.alloc(sizeof(cx<double>) * (size_t)l * n_t * nb));
+        IA_TRY(S.bM.alloc(sizeof(cx<double>) * (size_t)m * n_t * nb));
+        IA_TRY(S.bQ.alloc(sizeof(cx<double>) * (size_t)m * n_t * nb));
+        IA_TRY(S.bx.alloc(sizeof(cx<double>) * (size_t)n_t * nb));
+        if (mixed)
+            IA_TRY(regsolve_work<float>(1 + g % NS).reserve(l, n_t, nb));
         else
-            st = regsolve_batched<double>(l, n_t, nb, bH.as<cx<double>>(), (size_t)l * n_t,
-                                          db.as<cx<double>>(), l, prm, bx.as<cx<double>>(), n_t,
-                                          blam.as<double>(), bidx.as<int>(), bkt.as<int>(),
-                                          bsm.as<double>(), s);
-        IA_TRY(st);
+            IA_TRY(regsolve_work<double>(1 + g % NS).reserve(l, n_t, nb));
+    }
+    std::vector<hipStream_t> st(NS);
+    for (int i = 0; i < NS; ++i) IA_TRY(hels_stream(i, &st[i]));
+    // the inputs were copied on the legacy stream
+    IA_HIP(hipStreamSynchronize(nullptr));
+    ia_status rc = IA_OK;
+    for (int g = 0; g < G && rc == IA_OK; ++g) {
+        const int sl = g % NS, N = gN[g], nb = gnb[g], off = goff[g], n_t = (N + 1) * (N + 1);
+        HelsSlot &S = hels_slot(sl);
+        hipStream_t sg = st[sl];
+        const double *kg = io.dk.as<double>() + off;
+        const cx<double> *bg = io.db.as<cx<double>>() + (size_t)off * l;
+        cx<double> *H = S.bH.as<cx<double>>(), *Me = S.bM.as<cx<double>>(),
+                   *Qe = S.bQ.as<cx<double>>(), *x = S.bx.as<cx<double>>();
+        launch_hels<double>(l, N, nb, kg, rho_c, io.dh.as<double>(), H, nullptr, (size_t)l * n_t,
+                            sg);
+        launch_hels<double>(m, N, nb, kg, rho_c, io.ds.as<double>(), Me, Qe, (size_t)m * n_t, sg);
+        if (mixed)
+            rc = regsolve_enqueue<float>(regsolve_work<float>(1 + sl), l, n_t, nb, H,
+                                         (size_t)l * n_t, bg, l, prm, x, n_t,
+                                         io.lam.as<double>() + off, io.idx.as<int>() + off,
+                                         io.kt.as<int>() + off, io.sm.as<double>() + off, sg,
+                                         nullptr);
+        else
+            rc = regsolve_enqueue<double>(regsolve_work<double>(1 + sl), l, n_t, nb, H,
+                                          (size_t)l * n_t, bg, l, prm, x, n_t,
+                                          io.lam.as<double>() + off, io.idx.as<int>() + off,
+                                          io.kt.as<int>() + off, io.sm.as<double>() + off, sg,
+                                          nullptr);
+        if (rc != IA_OK) break;
         // v = Q_e c, p = M_e c (to_velocity / to_pressure), W = 1/2 Re(v^H p) S/m
         const cx<double> one{1, 0}, zero{0, 0};
-        launch_gemm<double>('N', 'N', m, 1, n_t, one, bQ.as<cx<double>>(), m, (size_t)m * n_t,
-                            bx.as<cx<double>>(), n_t, n_t, zero, bv.as<cx<double>>(), m, m, nb, s);
-        launch_gemm<double>('N', 'N', m, 1, n_t, one, bM.as<cx<double>>(), m, (size_t)m * n_t,
-                            bx.as<cx<double>>(), n_t, n_t, zero, bp.as<cx<double>>(), m, m, nb, s);
-        launch_power<double>(m, bv.as<cx<double>>(), m, bp.as<cx<double>>(), m, desc->area / m,
-                             bW.as<double>(), nb, s);
-        IA_HIP(hipGetLastError());
-        IA_HIP(hipStreamSynchronize(s));
-        std::vector<double> hl(nb), hs(nb), hW(nb);
-        std::vector<int> hi(nb), hk(nb);
-        std::vector<ia_z> hv((size_t)m * nb), hp((size_t)m * nb);
-        IA_HIP(hipMemcpy(hl.data(), blam.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
-        IA_HIP(hipMemcpy(hs.data(), bsm.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
-        IA_HIP(hipMemcpy(hW.data(), bW.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
-        IA_HIP(hipMemcpy(hi.data(), bidx.p, sizeof(int) * nb, hipMemcpyDeviceToHost));
-        IA_HIP(hipMemcpy(hk.data(), bkt.p, sizeof(int) * nb, hipMemcpyDeviceToHost));
-        IA_HIP(hipMemcpy(hv.data(), bv.p, sizeof(ia_z) * m * nb, hipMemcpyDeviceToHost));
-        IA_HIP(hipMemcpy(hp.data(), bp.p, sizeof(ia_z) * m * nb, hipMemcpyDeviceToHost));
-        for (int i = 0; i < nb; ++i) {
-            const int f = g.second[i];
-            if (o.lambda) o.lambda[f] = hl[i];
-            if (o.lambda_index) o.lambda_index[f] = hi[i];
-            if (o.k_tsvd) o.k_tsvd[f] = hk[i];
-            if (o.power) o.power[f] = hW[i];
-            if (o.sigma_max) o.sigma_max[f] = hs[i];
-            if (o.velocity) memcpy(o.velocity + (size_t)f * m, &hv[(size_t)i * m], sizeof(ia_z) * m);
-            if (o.pressure) memcpy(o.pressure + (size_t)f * m, &hp[(size_t)i * m], sizeof(ia_z) * m);
-            if (o.status) o.status[f] = IA_OK;
-        }
+        cx<double> *v = io.v.as<cx<double>>() + (size_t)off * m;
+        cx<double> *pr = io.p.as<cx<double>>() + (size_t)off * m;
+        launch_gemm<double>('N', 'N', m, 1, n_t, one, Qe, m, (size_t)m * n_t, x, n_t, n_t, zero,
+                            v, m, m, nb, sg);
+        launch_gemm<double>('N', 'N', m, 1, n_t, one, Me, m, (size_t)m * n_t, x, n_t, n_t, zero,
+                            pr, m, m, nb, sg);
+        launch_power<double>(m, v, m, pr, m, desc->area / m, io.W.as<double>() + off, nb, sg);
+        if (hipGetLastError() != hipSuccess)
+            rc = fail(IA_ERR_DEVICE, "ia_hels_sweep: launch failed");
+    }
+    // wait for every group, also after a failed enqueue (the workspaces are
+    // reused by the next call)
+    for (int i = 0; i < NS; ++i)
+        if (hipStreamSynchronize(st[i]) != hipSuccess && rc == IA_OK)
+            rc = fail(IA_ERR_DEVICE, "ia_hels_sweep: %s", hipGetErrorString(hipGetLastError()));
+    IA_TRY(rc);
+    std::vector<double> hl(F), hs(F), hW(F);
+    std::vector<int> hi(F), hk(F);
+    std::vector<ia_z> hv((size_t)m * F), hp((size_t)m * F);
+    IA_HIP(hipMemcpy(hl.data(), io.lam.p, sizeof(double) * F, hipMemcpyDeviceToHost));
+    IA_HIP(hipMemcpy(hs.data(), io.sm.p, sizeof(double) * F, hipMemcpyDeviceToHost));
+    IA_HIP(hipMemcpy(hW.data(), io.W.p, sizeof(double) * F, hipMemcpyDeviceToHost));
+    IA_HIP(hipMemcpy(hi.data(), io.idx.p, sizeof(int) * F, hipMemcpyDeviceToHost));
+    IA_HIP(hipMemcpy(hk.data(), io.kt.p, sizeof(int) * F, hipMemcpyDeviceToHost));
+    if (o.velocity) IA_HIP(hipMemcpy(hv.data(), io.v.p, sizeof(ia_z) * (size_t)m * F, hipMemcpyDeviceToHost));
+    if (o.pressure) IA_HIP(hipMemcpy(hp.data(), io.p.p, sizeof(ia_z) * (size_t)m * F, hipMemcpyDeviceToHost));
+    for (int i = 0; i < F; ++i) {
+        const int f = perm[i];
+        if (o.lambda) o.lambda[f] = hl[i];
+        if (o.lambda_index) o.lambda_index[f] = hi[i];
+        if (o.k_tsvd) o.k_tsvd[f] = hk[i];
+        if (o.power) o.power[f] = hW[i];
+        if (o.sigma_max) o.sigma_max[f] = hs[i];
+        if (o.velocity) memcpy(o.velocity + (size_t)f * m, &hv[(size_t)i * m], sizeof(ia_z) * m);
+        if (o.pressure) memcpy(o.pressure + (size_t)f * m, &hp[(size_t)i * m], sizeof(ia_z) * m);
+        if (o.status) o.status[f] = IA_OK;
     }
     return IA_OK;
 }
@@ -292,7 +421,9 @@ ia_status ia_regsolve(int rows, int cols, int batch, const ia_z *a, const ia_z *
         return fail(IA_ERR_ARG, "ia_regsolve: bad arguments");
     IA_TRY(check_reg(*prm, std::min(rows, cols)));
     hipStream_t s = nullptr;
-    DevBuf dA, db, dx, dl, di, dk, dsm;
+    RegsolveIo &io = regsolve_io();
+    DevBuf &dA = io.A, &db = io.b, &dx = io.x, &dl = io.lam, &di = io.idx, &dk = io.kt,
+           &dsm = io.sm;
     IA_TRY(to_device(dA, a, sizeof(ia_z) * (size_t)rows * cols * batch, s));
     IA_TRY(to_device(db, b, sizeof(ia_z) * (size_t)rows * batch, s));
     IA_TRY(dx.alloc(sizeof(ia_z) * (size_t)cols * batch));
