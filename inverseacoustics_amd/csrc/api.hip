@@ -442,6 +442,7 @@ void ia_release(void) {
     release_grad_cache();
     release_rad_cache();
     release_solve_cache();
+    release_ibem_cache();
 }
 
 ia_status ia_last_stats(ia_stats *st) {

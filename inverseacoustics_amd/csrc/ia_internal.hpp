@@ -222,8 +222,10 @@ void launch_bulge_apply(bool fwd, int C, const cx<T> *u, const cx<T> *tau, cx<do
 // keep per host thread
 void release_grad_cache();
 void release_rad_cache();
-// ... and the regularised-solve / HELS workspaces (solve.hip)
+// ... the regularised-solve / HELS workspaces (solve.hip) and the IBEM
+// sweep's (ibem_api.hip)
 void release_solve_cache();
+void release_ibem_cache();
 
 // ADMM (admm.hip): A (rows x cols, batch stride rows*cols), b, c on the device
 ia_status admm_batched(int rows, int cols, int batch, const cx<double> *A, const cx<double> *b,

@@ -171,9 +171,17 @@ ia_status build_mesh(int m, int n, const int *e_n, Mesh &M) {
     return IA_OK;
 }
 
-// Device-side IBEM system for nf wavenumbers: Z (m x m) and H_b (l x m), FP64.
+// Device-side IBEM system for nf wavenumbers: Z (m x m) and H_b (l x m), FP64,
+// with the assembly and factorisation scratch (buffers only grow, so a kept
+// IbemDev allocates nothing on a repeated call of the same size).
 struct IbemDev {
     DevBuf X, E, adj_ptr, adj, rule, pages, cp, rlh, k, Z, H;
+    DevBuf A, B, Ah, Bh, Ad, Bd, Ahd, Bhd, ipiv, info, tau;
+    void release() {
+        for (DevBuf *b : {&X, &E, &adj_ptr, &adj, &rule, &pages, &cp, &rlh, &k, &Z, &H, &A, &B,
+                          &Ah, &Bh, &Ad, &Bd, &Ahd, &Bhd, &ipiv, &info, &tau})
+            b->reset();
+    }
 };
 
 template <typename T>
@@ -194,7 +202,8 @@ ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, co
     if (n_ch > 0) IA_TRY(to_device(D.cp, cp, sizeof(double) * 3 * n_ch, s));
     IA_TRY(to_device(D.k, k, sizeof(double) * nf, s));
     const double rho_c = air().rho * air().c;
-    DevBuf bA, bB, bAh, bBh, bAd, bBd, bAhd, bBhd;
+    DevBuf &bA = D.A, &bB = D.B, &bAh = D.Ah, &bBh = D.Bh, &bAd = D.Ad, &bBd = D.Bd,
+           &bAhd = D.Ahd, &bBhd = D.Bhd;
     const size_t sS = (size_t)mt * m, sH = (size_t)std::max(l, 1) * m;
     IA_TRY(bA.alloc(sizeof(cx<T>) * sS * nf));
     IA_TRY(bB.alloc(sizeof(cx<T>) * sS * nf));
@@ -242,7 +251,7 @@ ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, co
     IA_TRY(D.Z.alloc(sizeof(cx<double>) * (size_t)m * m * nf));
     cx<double> *Z = D.Z.as<cx<double>>();
     if (n_ch == 0) {
-        DevBuf ipiv, info;
+        DevBuf &ipiv = D.ipiv, &info = D.info;
         IA_TRY(ipiv.alloc(sizeof(int) * (size_t)m * nf));
         IA_TRY(info.alloc(sizeof(int) * nf));
         IA_HIP(hipMemsetAsync(info.p, 0, sizeof(int) * nf, s));
@@ -255,7 +264,7 @@ ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, co
         IA_HIP(hipMemcpyAsync(Z, B, sizeof(cx<double>) * sS * nf, hipMemcpyDeviceToDevice, s));
     } else {
         // least squares through Householder QR (the reference's ZZgels)
-        DevBuf tau;
+        DevBuf &tau = D.tau;
         IA_TRY(tau.alloc(sizeof(cx<double>) * (size_t)m * nf));
         IA_TRY(launch_geqr2<double>(mt, m, A, mt, sS, tau.as<cx<double>>(), nf, s));
         launch_apply_left<double>(true, mt, m, A, mt, sS, tau.as<cx<double>>(), B, mt, m, nf, s);
@@ -284,7 +293,25 @@ ia_status ibem_args(int m, int l, int n, const double *r_ms, const int *e_n, con
     return IA_OK;
 }
 
+// The IBEM sweep's device workspace, kept per host thread between calls;
+// ia_release() frees it.
+struct IbemSweepCache {
+    IbemDev D;
+    DevBuf b, x, p, lam, idx, kt, sm, W;
+};
+IbemSweepCache &ibem_cache() {
+    static thread_local IbemSweepCache *c = new IbemSweepCache();
+    return *c;
+}
+
 }  // namespace
+
+void release_ibem_cache() {
+    IbemSweepCache &c = ibem_cache();
+    c.D.release();
+    for (DevBuf *b : {&c.b, &c.x, &c.p, &c.lam, &c.idx, &c.kt, &c.sm, &c.W}) b->reset();
+}
+
 }  // namespace ia
 
 using namespace ia;
@@ -406,14 +433,16 @@ ia_status ia_ibem_sweep(const ia_ibem_desc *desc, const double *k, const double 
     ia_sweep_out o{};
     if (out) o = *out;
     const int chunk = desc->chunk > 0 ? std::min(desc->chunk, F) : std::min(F, 64);
+    IbemSweepCache &C = ibem_cache();
+    IbemDev &D = C.D;
     for (int f0 = 0; f0 < F; f0 += chunk) {
         const int nf = std::min(chunk, F - f0);
-        IbemDev D;
         if (desc->precision == IA_PREC_MIXED)
             IA_TRY(ibem_build<float>(m, l, n, r_ms, e_n, r_lh, n_ch, cp, k + f0, nf, D, s));
         else
             IA_TRY(ibem_build<double>(m, l, n, r_ms, e_n, r_lh, n_ch, cp, k + f0, nf, D, s));
-        DevBuf db, bx, bp, bl, bi, bk, bs, bW;
+        DevBuf &db = C.b, &bx = C.x, &bp = C.p, &bl = C.lam, &bi = C.idx, &bk = C.kt,
+               &bs = C.sm, &bW = C.W;
         IA_TRY(to_device(db, b + (size_t)f0 * l, sizeof(ia_z) * (size_t)l * nf, s));
         IA_TRY(bx.alloc(sizeof(cx<double>) * (size_t)m * nf));
         IA_TRY(bp.alloc(sizeof(cx<double>) * (size_t)m * nf));

@@ -47,6 +47,26 @@ def test_hels_sweep_matches_oracle(ia):
     assert np.max(np.abs(post.power_db(rm.power) - post.power_db(r.power))) < 0.1
 
 
+def test_hels_sweep_workspace_reuse_and_shared_slots(ia):
+    """More order groups than concurrent slots (orders 5..31 over 24
+    frequencies, tall and wide systems), and kept workspaces: a smaller sweep
+    between two identical ones leaves the second bit for bit equal to the
+    first, and the subset's choices equal the full sweep's."""
+    S = sc.hels_scenario(n_freq=24, fmin=50.0, fmax=12000.0, l=96, m=64)
+    assert len(set(S.order.tolist())) > 8
+    ia.set_air(S.c, S.rho)
+    first = ia.hels_sweep(S.k, S.order, S.azelr_s, S.azelr_h, S.b, S.area)
+    sub = slice(3, 11)
+    part = ia.hels_sweep(S.k[sub], S.order[sub], S.azelr_s, S.azelr_h, S.b[sub], S.area)
+    again = ia.hels_sweep(S.k, S.order, S.azelr_s, S.azelr_h, S.b, S.area)
+    np.testing.assert_array_equal(again.velocity, first.velocity)
+    np.testing.assert_array_equal(again.lam_index, first.lam_index)
+    np.testing.assert_array_equal(part.lam_index, first.lam_index[sub])
+    for f in (0, 12, 23):
+        o = pipeline.hels_solve(S.k[f], S.order[f], S.azelr_s, S.azelr_h, S.b[f], S.area, S.rho, S.c)
+        assert first.lam_index[f] == o["idx"], f
+
+
 @pytest.mark.parametrize("shape", [(48, 20), (20, 48), (33, 33)])
 def test_regsolve_batched_matches_oracle(ia, shape):
     rng = np.random.default_rng(5)

@@ -118,6 +118,18 @@ def test_ibem_sweep_chunked_and_chief(ia, builtin_rules):
     assert np.abs(a.velocity - b.velocity).max() <= 1e-12 * np.abs(a.velocity).max()
 
 
+def test_ibem_sweep_workspace_reuse(ia, builtin_rules):
+    """The sweep keeps its workspace between calls: a smaller sweep in
+    between leaves a repeated sweep bit for bit equal to the first."""
+    S = sc.ibem_scenario(n_freq=5, subdiv=1, l=24)
+    ia.set_air(S.c, S.rho)
+    a = ia.ibem_sweep(S.k, S.X, S.E, S.r_h, S.b, S.area)
+    ia.ibem_sweep(S.k[:2], S.X, S.E, S.r_h, S.b[:2], S.area, precision="mixed")
+    c = ia.ibem_sweep(S.k, S.X, S.E, S.r_h, S.b, S.area)
+    np.testing.assert_array_equal(c.velocity, a.velocity)
+    np.testing.assert_array_equal(c.power, a.power)
+
+
 def test_ibem_bad_inputs(ia):
     X, E = sc.icosphere(1, 0.1)
     bad = E.copy()
