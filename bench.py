@@ -233,6 +233,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64+f64",
+        "products": "complex64 products of depth >= 512 as six bf16 partial products of "
+                    "three-way split FP32 operands with FP32 accumulation (FP32-level); the "
+                    "others on FP32 / FP64 matrix cores (3M); selection and eigen-solve in FP64",
         "data": "synthetic (plate, Rayleigh-integral pressures, seeded 30 dB SNR noise)",
         "config": {"workload": "ESM sweep 256 frequencies 200-1500 Hz, 1024 mics x 2048 sources "
                                "(0.6x0.5 m steel plate, d=0.02 m), Tikhonov + GCV over 50 lambdas, "

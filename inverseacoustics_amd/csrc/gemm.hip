@@ -262,6 +262,184 @@ __global__ __launch_bounds__(256, OCC) void k_zgemm(int M, int N, int K, cx<T> a
             }
 }
 
+// ---------------------------------------------------------------- complex64
+// on the bf16 matrix cores.  Every FP32 operand v is split exactly into three
+// bf16 parts by truncation, v = h + m + l (8 + 8 + 8 significant bits), and
+// the product keeps the six terms hh, hm, mh, hl, lh, mm: the dropped ones
+// are below 2^-23 |a b|, so the result carries FP32-level products with
+// FP32 accumulation (v_mfma_f32_16x16x32_bf16, k = 32 per instruction).
+// Complex products as S1 = sum re re, S2 = sum im im, S3 = sum (re im + im re):
+// real part S1 - S2, imaginary part S3.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4 mfma_bf16(u32x4 a, u32x4 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// the three bf16 parts of v as the high halves of three words
+__device__ __forceinline__ void split3(float v, unsigned &h, unsigned &m, unsigned &l) {
+    h = __float_as_uint(v) & 0xffff0000u;
+    const float r1 = v - __uint_as_float(h);
+    m = __float_as_uint(r1) & 0xffff0000u;
+    const float r2 = r1 - __uint_as_float(m);
+    l = __float_as_uint(r2) & 0xffff0000u;
+}
+
+constexpr int XB = 64, XK = 32, XKP = XK / 8 + 1;  // tile, stage depth, padded row (16 B units)
+
+// Operand tile (64 rows x 32 k) of one thread: one row, 8 consecutive k.
+// KC: element (r, k) is X[r ldx + k] (contiguous along k; four threads per
+// row), else X[k ldx + r] (contiguous along r; 64 threads per k).
+template <bool KC, bool CONJ>
+struct SplitLoader {
+    int r, kg;
+    __device__ __forceinline__ explicit SplitLoader(int t) {
+        if (KC) {
+            r = t >> 2;
+            kg = t & 3;
+        } else {
+            r = t & 63;
+            kg = t >> 6;
+        }
+    }
+    __device__ __forceinline__ void load(const cx<float> *__restrict__ X, int ldx, int r0, int nr,
+                                         int k0, int K, cx<float> (&v)[8]) const {
+        const int gr = r0 + r;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int gk = k0 + kg * 8 + j;
+            v[j] = cx<float>{0, 0};
+            if (gr < nr && gk < K) v[j] = KC ? X[(size_t)gr * ldx + gk] : X[(size_t)gk * ldx + gr];
+        }
+    }
+    // planes 0..2: re (h, m, l), 3..5: im (h, m, l)
+    __device__ __forceinline__ void store(u32x4 (*S)[XB][XKP], const cx<float> (&v)[8]) const {
+        unsigned p[6][8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            split3(v[j].re, p[0][j], p[1][j], p[2][j]);
+            split3(CONJ ? -v[j].im : v[j].im, p[3][j], p[4][j], p[5][j]);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            u32x4 w;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) w[j] = (p[q][2 * j] >> 16) | p[q][2 * j + 1];
+            S[q][r][kg] = w;
+        }
+    }
+};
+
+template <char OPA, char OPB, bool BETA0>
+__global__ __launch_bounds__(256, 2) void k_cgemm_x6(int M, int N, int K, cx<float> alpha,
+                                                    const cx<float> *__restrict__ A, int lda,
+                                                    size_t sA, const cx<float> *__restrict__ B,
+                                                    int ldb, size_t sB, cx<float> beta,
+                                                    cx<float> *__restrict__ C, int ldc, size_t sC,
+                                                    int tiles_m, int tiles_n, int batch) {
+    __shared__ u32x4 As[6][XB][XKP];
+    __shared__ u32x4 Bs[6][XB][XKP];
+    int bz, tile;
+    tile_of(blockIdx.x, tiles_m * tiles_n, batch, bz, tile);
+    A += (size_t)bz * sA;
+    B += (size_t)bz * sB;
+    C += (size_t)bz * sC;
+    const int m0 = (tile % tiles_m) * XB, n0 = (tile / tiles_m) * XB;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    const int l16 = lane & 15, g = lane >> 4;
+    // op(A) (64 x 32): A[k lda + m] or conj(A[m lda + k]); op(B)^T (64 x 32):
+    // B[n ldb + k] or conj(B[k ldb + n])
+    const SplitLoader<OPA == 'C', OPA == 'C'> la(t);
+    const SplitLoader<OPB == 'N', OPB == 'C'> lb(t);
+    f4 s1[2][2], s2[2][2], s3[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) s1[a][b] = s2[a][b] = s3[a][b] = f4{0, 0, 0, 0};
+    cx<float> va[8], vb[8];
+    const int nst = (K + XK - 1) / XK;
+    if (nst > 0) {
+        la.load(A, lda, m0, M, 0, K, va);
+        lb.load(B, ldb, n0, N, 0, K, vb);
+        la.store(As, va);
+        lb.store(Bs, vb);
+    }
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const bool more = st + 1 < nst;
+        if (more) {
+            la.load(A, lda, m0, M, (st + 1) * XK, K, va);
+            lb.load(B, ldb, n0, N, (st + 1) * XK, K, vb);
+        }
+        u32x4 fa[2][6], fb[2][6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a) fa[a][q] = As[q][wm + 16 * a + l16][g];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) fb[b][q] = Bs[q][wn + 16 * b + l16][g];
+        }
+        // (x, y) split pairs kept: hh, hm, mh, hl, lh, mm; MFMA-A <- op(B)^T
+        // rows (n), MFMA-B <- op(A)^T columns (m)
+        constexpr int PX[6] = {0, 0, 1, 0, 2, 1}, PY[6] = {0, 1, 0, 2, 0, 1};
+#pragma unroll
+        for (int pr = 0; pr < 6; ++pr)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) {
+                    const int x = PX[pr], y = PY[pr];
+                    s1[a][b] = mfma_bf16(fb[b][y], fa[a][x], s1[a][b]);
+                    s2[a][b] = mfma_bf16(fb[b][3 + y], fa[a][3 + x], s2[a][b]);
+                    s3[a][b] = mfma_bf16(fb[b][3 + y], fa[a][x], s3[a][b]);
+                    s3[a][b] = mfma_bf16(fb[b][y], fa[a][3 + x], s3[a][b]);
+                }
+        __syncthreads();  // every wave has read the stage
+        if (more) {
+            la.store(As, va);
+            lb.store(Bs, vb);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int gm = m0 + wm + a * 16 + l16;
+                const int gn = n0 + wn + b * 16 + 4 * g + i;
+                if (gm < M && gn < N) {
+                    const float xr = s1[a][b][i] - s2[a][b][i], xi = s3[a][b][i];
+                    cx<float> r{alpha.re * xr - alpha.im * xi, alpha.re * xi + alpha.im * xr};
+                    const size_t o = (size_t)gn * ldc + gm;
+                    if (!BETA0) {
+                        const cx<float> c0 = C[o];
+                        r.re += beta.re * c0.re - beta.im * c0.im;
+                        r.im += beta.re * c0.im + beta.im * c0.re;
+                    }
+                    C[o] = r;
+                }
+            }
+}
+
+template <char OPA, char OPB>
+void launch_x6(int M, int N, int K, cx<float> alpha, const cx<float> *A, int lda, size_t sA,
+               const cx<float> *B, int ldb, size_t sB, cx<float> beta, cx<float> *C, int ldc,
+               size_t sC, int batch, hipStream_t s) {
+    const int tm = (M + XB - 1) / XB, tn = (N + XB - 1) / XB;
+    const dim3 grid((unsigned)((size_t)tm * tn * batch));
+    if (beta.re == 0 && beta.im == 0)
+        hipLaunchKernelGGL((k_cgemm_x6<OPA, OPB, true>), grid, dim3(256), 0, s, M, N, K, alpha, A,
+                           lda, sA, B, ldb, sB, beta, C, ldc, sC, tm, tn, batch);
+    else
+        hipLaunchKernelGGL((k_cgemm_x6<OPA, OPB, false>), grid, dim3(256), 0, s, M, N, K, alpha, A,
+                           lda, sA, B, ldb, sB, beta, C, ldc, sC, tm, tn, batch);
+}
+
 template <typename T, char OPA, char OPB, int WMB, int WNB, int BK, int OCC, bool M3>
 void launch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, size_t sA,
                   const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc, size_t sC,
@@ -288,6 +466,15 @@ void dispatch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, s
                     const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc, size_t sC,
                     int batch, hipStream_t s) {
     constexpr bool f32 = sizeof(T) == sizeof(float);
+    if constexpr (f32) {
+        // deep complex64 products on the bf16 matrix cores (1.25x the FP32
+        // 3M kernel at K >= 512; at K <= 128 its two barriers per stage and
+        // the splitting make it slower)
+        if (K >= 512 && M > 32 && N > 32) {
+            launch_x6<OPA, OPB>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
+            return;
+        }
+    }
     constexpr int BK = 8, OCC = f32 ? 4 : 2;
 #define IA_TILES(WMB, WNB)                                                                        \
     launch_tiles<T, OPA, OPB, WMB, WNB, BK, OCC, true>(M, N, K, alpha, A, lda, sA, B, ldb, sB, \
