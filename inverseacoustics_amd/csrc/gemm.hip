@@ -287,7 +287,11 @@ __device__ __forceinline__ void split3(float v, unsigned &h, unsigned &m, unsign
     l = __float_as_uint(r2) & 0xffff0000u;
 }
 
-constexpr int XB = 64, XK = 32, XKP = XK / 8 + 1;  // tile, stage depth, padded row (16 B units)
+constexpr int XB = 64, XK = 32, XKP = XK / 8;  // tile, stage depth, row (16-byte chunks)
+// chunk c of row r is stored at chunk c ^ (r & 3): the 64 rows a wave writes
+// at one k (64-byte row stride) then fall on different banks, and a wave's
+// fragment reads (16 rows x 4 chunks) still cover 1 KB without conflict
+__device__ __forceinline__ int xsw(int r, int c) { return c ^ (r & 3); }
 
 // Operand tile (64 rows x 32 k) of one thread: one row, 8 consecutive k.
 // KC: element (r, k) is X[r ldx + k] (contiguous along k; four threads per
@@ -327,13 +331,14 @@ struct SplitLoader {
             u32x4 w;
 #pragma unroll
             for (int j = 0; j < 4; ++j) w[j] = (p[q][2 * j] >> 16) | p[q][2 * j + 1];
-            S[q][r][kg] = w;
+            S[q][r][xsw(r, kg)] = w;
         }
     }
 };
 
 template <char OPA, char OPB, bool BETA0>
-__global__ __launch_bounds__(256, 2) void k_cgemm_x6(int M, int N, int K, cx<float> alpha,
+// 48 KB of LDS and at most 168 VGPRs: three workgroups per CU
+__global__ __launch_bounds__(256, 3) void k_cgemm_x6(int M, int N, int K, cx<float> alpha,
                                                     const cx<float> *__restrict__ A, int lda,
                                                     size_t sA, const cx<float> *__restrict__ B,
                                                     int ldb, size_t sB, cx<float> beta,
@@ -374,29 +379,36 @@ __global__ __launch_bounds__(256, 2) void k_cgemm_x6(int M, int N, int K, cx<flo
             la.load(A, lda, m0, M, (st + 1) * XK, K, va);
             lb.load(B, ldb, n0, N, (st + 1) * XK, K, vb);
         }
-        u32x4 fa[2][6], fb[2][6];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a) fa[a][q] = As[q][wm + 16 * a + l16][g];
-#pragma unroll
-            for (int b = 0; b < 2; ++b) fb[b][q] = Bs[q][wn + 16 * b + l16][g];
-        }
-        // (x, y) split pairs kept: hh, hm, mh, hl, lh, mm; MFMA-A <- op(B)^T
-        // rows (n), MFMA-B <- op(A)^T columns (m)
+        // the B fragments of the stage stay in registers, the A fragments
+        // are read per row block
         constexpr int PX[6] = {0, 0, 1, 0, 2, 1}, PY[6] = {0, 1, 0, 2, 0, 1};
+        u32x4 fb[2][6];
 #pragma unroll
-        for (int pr = 0; pr < 6; ++pr)
+        for (int q = 0; q < 6; ++q)
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const int r = wn + 16 * b + l16;
+                fb[b][q] = Bs[q][r][xsw(r, g)];
+            }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            u32x4 fa[6];
+            const int ra = wm + 16 * a + l16;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) fa[q] = As[q][ra][xsw(ra, g)];
+            // (x, y) split pairs kept: hh, hm, mh, hl, lh, mm; MFMA-A <- op(B)^T
+            // rows (n), MFMA-B <- op(A)^T columns (m)
+#pragma unroll
+            for (int pr = 0; pr < 6; ++pr)
 #pragma unroll
                 for (int b = 0; b < 2; ++b) {
                     const int x = PX[pr], y = PY[pr];
-                    s1[a][b] = mfma_bf16(fb[b][y], fa[a][x], s1[a][b]);
-                    s2[a][b] = mfma_bf16(fb[b][3 + y], fa[a][3 + x], s2[a][b]);
-                    s3[a][b] = mfma_bf16(fb[b][3 + y], fa[a][x], s3[a][b]);
-                    s3[a][b] = mfma_bf16(fb[b][y], fa[a][3 + x], s3[a][b]);
+                    s1[a][b] = mfma_bf16(fb[b][y], fa[x], s1[a][b]);
+                    s2[a][b] = mfma_bf16(fb[b][3 + y], fa[3 + x], s2[a][b]);
+                    s3[a][b] = mfma_bf16(fb[b][3 + y], fa[x], s3[a][b]);
+                    s3[a][b] = mfma_bf16(fb[b][y], fa[3 + x], s3[a][b]);
                 }
+        }
         __syncthreads();  // every wave has read the stage
         if (more) {
             la.store(As, va);
@@ -467,10 +479,11 @@ void dispatch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, s
                     int batch, hipStream_t s) {
     constexpr bool f32 = sizeof(T) == sizeof(float);
     if constexpr (f32) {
-        // deep complex64 products on the bf16 matrix cores (1.25x the FP32
-        // 3M kernel at K >= 512; at K <= 128 its two barriers per stage and
-        // the splitting make it slower)
-        if (K >= 512 && M > 32 && N > 32) {
+        // deep complex64 products on the bf16 matrix cores (1.33x the FP32
+        // 3M kernel at K >= 512, on a par at K = 128; at K <= 64 the
+        // splitting and two barriers per stage of its few stages make it
+        // slower)
+        if (K >= 128 && M > 32 && N > 32) {
             launch_x6<OPA, OPB>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
             return;
         }

@@ -42,9 +42,10 @@ def test_zgemm(ia, opa, opb, mnk):
 
 @pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("C", "C")])
 @pytest.mark.parametrize("mnk", [(17, 5, 3), (130, 77, 45), (300, 257, 96), (32, 300, 70),
-                                 (300, 20, 33), (2048, 2047, 40), (200, 150, 700), (65, 97, 1030)])
+                                 (300, 20, 33), (2048, 2047, 40), (200, 150, 700), (65, 97, 1030),
+                                 (130, 200, 129)])
 def test_cgemm_fp32(ia, opa, opb, mnk):
-    """complex64 products; from K = 512 on they run on the bf16 matrix cores
+    """complex64 products; from K = 128 on they run on the bf16 matrix cores
     with three-way split operands, to the same FP32-level bound."""
     m, n, k = mnk
     A = _cr((m, k) if opa == "N" else (k, m), 1).astype(np.complex64)
