@@ -52,10 +52,16 @@ __device__ __forceinline__ int sturm_count(int N, const double *t2, double x, do
     return cnt;
 }
 
-// One workgroup per matrix (1024 threads, one value each for n <= 1024).
+// Workgroups of 1024 threads; K consecutive lanes share one value (1024 / K
+// values per workgroup, blockIdx.x picks the range): each step the K lanes
+// take Sturm counts at the K interior points that cut the interval into
+// K + 1 equal parts, so the interval shrinks (K + 1)-fold per step instead
+// of 2-fold (K = 1: plain bisection).  Small batches use K > 1 to fill the
+// device with a shorter chain per value.
 constexpr int GL = 10, GPTS = 1 << GL;
 constexpr int GGEO = GPTS / 2, GLIN = GPTS - GGEO;  // geometric / linear grid points
 // Stopping width max(rtol hi, atol bound).
+template <int K>
 __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict__ dall,
                                                 const double *__restrict__ eall,
                                                 double *__restrict__ sall, double rtol,
@@ -110,7 +116,9 @@ __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict
     for (int j = t + 1; j < GPTS; j += blockDim.x)
         cgrid[j] = sturm_count(N, t2, gpt(j), pivmin) - n;
     __syncthreads();
-    for (int i = t; i < n; i += blockDim.x) {
+    static_assert(K == 1 || K == 2 || K == 4 || K == 8, "lanes per value");
+    const int jk = t % K;  // this lane's point among the value's K
+    for (int i = blockIdx.x * (1024 / K) + t / K; i < n; i += gridDim.x * (1024 / K)) {
         const int target = n - 1 - i;  // sigma_i (descending) has target values below it
         // first grid point with more than target values below it (GPTS if none)
         int jl = 0, jh = GPTS;  // cgrid(jl) <= target < cgrid(jh), cgrid(0) = 0, cgrid(GPTS) = n
@@ -124,14 +132,31 @@ __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict
         double lo = gpt(jl), hi = gpt(jh);
         for (int it = 0; it < 128; ++it) {
             if (hi - lo <= fmax(rtol * hi, atol * bound)) break;
-            double mid = 0.5 * (lo + hi);
-            int c = sturm_count(N, t2, mid, pivmin) - n;
-            if (c > target)
-                hi = mid;
-            else
-                lo = mid;
+            if (K == 1) {
+                const double mid = 0.5 * (lo + hi);
+                const int c = sturm_count(N, t2, mid, pivmin) - n;
+                if (c > target)
+                    hi = mid;
+                else
+                    lo = mid;
+            } else {
+                // the K lanes of a value are consecutive and share lo, hi and
+                // the loop exit, so they reach the ballot together
+                const double w = (hi - lo) / (K + 1);
+                const double pj = lo + w * (jk + 1);
+                const bool above = sturm_count(N, t2, pj, pivmin) - n > target;
+                const unsigned long long bal = __ballot(above);
+                const int base = (threadIdx.x & 63) - jk;
+                const unsigned grp = (unsigned)(bal >> base) & ((1u << K) - 1);
+                // first point with more than target values below it
+                const int f = grp ? __builtin_ctz(grp) : K;
+                const double nlo = f == 0 ? lo : lo + w * f;
+                const double nhi = f == K ? hi : lo + w * (f + 1);
+                lo = nlo;
+                hi = nhi;
+            }
         }
-        sall[(size_t)b * n + i] = 0.5 * (lo + hi);
+        if (jk == 0) sall[(size_t)b * n + i] = 0.5 * (lo + hi);
     }
 }
 
@@ -688,8 +713,6 @@ __global__ __launch_bounds__(BT) void k_geqr2(int R, int C, cx<T> *__restrict__ 
 void launch_bisect(int n, const double *d, const double *e, double *sigma, int batch,
                    hipStream_t s, bool f32) {
     size_t lds = sizeof(double) * (2 * (size_t)n);
-    (void)hipFuncSetAttribute((const void *)k_bisect, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds);
     // FP64 form: full relative precision (down to 1e-3 eps of the bound).
     // Complex64 form: 1e-12 relative or 1e-13 of the bound, a quarter fewer
     // steps; the selection and the inverse-iteration shifts still see the
@@ -697,7 +720,26 @@ void launch_bisect(int n, const double *d, const double *e, double *sigma, int b
     // rounding (1e-9 relative measurably moved nearly tied GCV choices)
     const double eps = 2.220446049250313e-16;
     const double rtol = f32 ? 1e-12 : 2.0 * eps, atol = f32 ? 1e-13 : 1e-3 * eps;
-    hipLaunchKernelGGL(k_bisect, dim3(1, batch), dim3(1024), lds, s, n, d, e, sigma, rtol, atol);
+    // lanes per value: the most that keeps n * batch * K threads within
+    // about four waves per SIMD of the device (a full batch stays at plain
+    // bisection, which already fills it; the recheck's few frequencies get
+    // eight lanes and a third of the steps)
+    const double vals = (double)n * batch;
+    const int K = vals * 8 <= 262144.0 ? 8 : vals * 4 <= 262144.0 ? 4 : vals * 2 <= 262144.0 ? 2 : 1;
+    const unsigned gx = (unsigned)((n + 1024 / K - 1) / (1024 / K));
+    auto go = [&](auto kern) {
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        hipLaunchKernelGGL(kern, dim3(gx, batch), dim3(1024), lds, s, n, d, e, sigma, rtol, atol);
+    };
+    if (K == 8)
+        go(k_bisect<8>);
+    else if (K == 4)
+        go(k_bisect<4>);
+    else if (K == 2)
+        go(k_bisect<2>);
+    else
+        go(k_bisect<1>);
 }
 
 // scratch columns: 64 per wave, (n + 63) / 64 waves per matrix
