@@ -23,6 +23,11 @@
 // planes in k-major order so that the MFMA operand reads of 16 consecutive
 // lanes hit 16 consecutive words; two LDS buffers, the next stage loaded
 // from global memory into registers while the current one is multiplied.
+//
+// Deep complex64 products (depth >= 128, both dimensions > 32) take a second
+// kernel, k_cgemm_x6, on the bf16 matrix cores: the FP32 operands are split
+// exactly into three bf16 parts and six partial products are accumulated in
+// FP32, which keeps FP32-level products (see k_cgemm_x6 below).
 
 #include <cstdio>
 #include <cstdlib>
