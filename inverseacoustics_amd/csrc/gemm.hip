@@ -511,7 +511,151 @@ void dispatch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, s
 #undef IA_TILES
 }
 
+// ------------------------------------------------------------------ FP64 real
+// C = op(A) B over a batch, op in {'N', 'T'}, real double: the products of a
+// real symmetric matrix with a block of vectors (the radiation-mode subspace
+// iteration).  The scheme of k_zgemm with one LDS plane per operand: 4 waves
+// in a 2 x 2 grid, each computing the transposed product on
+// v_mfma_f64_16x16x4f64 so that result rows land on consecutive lanes along
+// the column-major C rows.
+template <char OPA, int WMB, int WNB, int BK, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_dgemm(int M, int N, int K,
+                                                    const double *__restrict__ A, int lda,
+                                                    size_t sA, const double *__restrict__ B,
+                                                    int ldb, size_t sB, double *__restrict__ C,
+                                                    int ldc, size_t sC, int tiles_m, int tiles_n,
+                                                    int batch) {
+    constexpr int BM = 32 * WMB, BN = 32 * WNB;
+    constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+    static_assert(NA * 256 == BM * BK && NB * 256 == BN * BK, "tiles split over 256 threads");
+    __shared__ double As[2][BK][BM + PAD];
+    __shared__ double Bs[2][BK][BN + PAD];
+    int bz, tile;
+    tile_of(blockIdx.x, tiles_m * tiles_n, batch, bz, tile);
+    A += (size_t)bz * sA;
+    B += (size_t)bz * sB;
+    C += (size_t)bz * sC;
+    const int m0 = (tile % tiles_m) * BM, n0 = (tile / tiles_m) * BN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = (w & 1) * 16 * WMB, wn = (w >> 1) * 16 * WNB;
+    const int l16 = lane & 15, lk = lane >> 4;
+    // element i of this thread's share of a stage: op(A) row r, depth k
+    // ('N': A[k lda + m], consecutive threads along m; 'T': A[m lda + k],
+    // consecutive threads along k); op(B)^T row r = column n of B, B[n ldb + k]
+    auto coord_a = [&](int i, int &r, int &k) {
+        if (OPA == 'N') {
+            r = t % BM;
+            k = t / BM + i * (256 / BM);
+        } else {
+            k = t % BK;
+            r = t / BK + i * (256 / BK);
+        }
+    };
+    auto coord_b = [&](int i, int &r, int &k) {
+        k = t % BK;
+        r = t / BK + i * (256 / BK);
+    };
+    double ra[NA], rb[NB];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            int r, k;
+            coord_a(i, r, k);
+            const int gm = m0 + r, gk = k0 + k;
+            ra[i] = (gm < M && gk < K)
+                        ? (OPA == 'N' ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk])
+                        : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            int r, k;
+            coord_b(i, r, k);
+            const int gn = n0 + r, gk = k0 + k;
+            rb[i] = (gn < N && gk < K) ? B[(size_t)gn * ldb + gk] : 0.0;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+            int r, k;
+            coord_a(i, r, k);
+            As[buf][k][r] = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+            int r, k;
+            coord_b(i, r, k);
+            Bs[buf][k][r] = rb[i];
+        }
+    };
+    d4 acc[WMB][WNB];
+#pragma unroll
+    for (int a = 0; a < WMB; ++a)
+#pragma unroll
+        for (int b = 0; b < WNB; ++b) acc[a][b] = d4{0, 0, 0, 0};
+    const int nst = (K + BK - 1) / BK;
+    if (nst > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int st = 0; st < nst; ++st) {
+        const int cur = st & 1;
+        const bool more = st + 1 < nst;
+        if (more) load((st + 1) * BK);
+#pragma unroll
+        for (int ks = 0; ks < BK; ks += 4) {
+            double av[WMB], bv[WNB];
+#pragma unroll
+            for (int a = 0; a < WMB; ++a) av[a] = As[cur][ks + lk][wm + a * 16 + l16];
+#pragma unroll
+            for (int b = 0; b < WNB; ++b) bv[b] = Bs[cur][ks + lk][wn + b * 16 + l16];
+#pragma unroll
+            for (int a = 0; a < WMB; ++a)
+#pragma unroll
+                for (int b = 0; b < WNB; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a], acc[a][b], 0,
+                                                                     0, 0);
+        }
+        if (more) store(cur ^ 1);
+        __syncthreads();
+    }
+    // lane holds C[m = l16][n = lk + 4 i] of its 16 x 16 block
+#pragma unroll
+    for (int a = 0; a < WMB; ++a)
+#pragma unroll
+        for (int b = 0; b < WNB; ++b)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int gm = m0 + wm + a * 16 + l16;
+                const int gn = n0 + wn + b * 16 + mfma_t<double>::row(lk, i);
+                if (gm < M && gn < N) C[(size_t)gn * ldc + gm] = acc[a][b][i];
+            }
+}
+
+template <char OPA, int WMB, int WNB>
+void launch_dtiles(int M, int N, int K, const double *A, int lda, size_t sA, const double *B,
+                   int ldb, size_t sB, double *C, int ldc, size_t sC, int batch, hipStream_t s) {
+    constexpr int BK = 16, OCC = 2;
+    const int tm = (M + 32 * WMB - 1) / (32 * WMB), tn = (N + 32 * WNB - 1) / (32 * WNB);
+    const dim3 grid((unsigned)((size_t)tm * tn * batch));
+    hipLaunchKernelGGL((k_dgemm<OPA, WMB, WNB, BK, OCC>), grid, dim3(256), 0, s, M, N, K, A, lda,
+                       sA, B, ldb, sB, C, ldc, sC, tm, tn, batch);
+}
+
 }  // namespace
+
+void launch_dgemm(char opA, int M, int N, int K, const double *A, int lda, size_t strideA,
+                  const double *B, int ldb, size_t strideB, double *C, int ldc, size_t strideC,
+                  int batch, hipStream_t s) {
+    if (M <= 0 || N <= 0 || batch <= 0) return;
+    if (opA == 'N')
+        launch_dtiles<'N', 2, 2>(M, N, K, A, lda, strideA, B, ldb, strideB, C, ldc, strideC,
+                                 batch, s);
+    else
+        launch_dtiles<'T', 2, 2>(M, N, K, A, lda, strideA, B, ldb, strideB, C, ldc, strideC,
+                                 batch, s);
+}
 
 namespace {
 

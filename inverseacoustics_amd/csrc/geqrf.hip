@@ -706,6 +706,76 @@ void launch_copy_upper(int C, const cx<T> *Y, int ldy, size_t sY, cx<T> *Rb, int
     hipLaunchKernelGGL(k_copy_upper<T>, g, dim3(256), 0, s, C, Y, ldy, sY, Rb);
 }
 
+namespace {
+
+// Q = [I; 0] (R x C per matrix, ld ldq)
+template <typename T>
+__global__ void k_set_eye(int R, int C, cx<T> *__restrict__ Qall, int ldq, size_t sQ) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)R * C) return;
+    const int r = (int)(q % R), c = (int)(q / R);
+    Qall[(size_t)blockIdx.y * sQ + (size_t)c * ldq + r] = cx<T>{r == c ? (T)1 : (T)0, (T)0};
+}
+
+// The explicit reflectors of columns j0..j0+nbp-1 of Y (unit diagonal,
+// zeros above, Y below) in the layout k_qr_panel writes: column c of Vw
+// (R x P per matrix, ld R) holds reflector j0 + c at its absolute rows.
+template <typename T>
+__global__ void k_refl_explicit(int R, int j0, int nbp, const cx<T> *__restrict__ Yall, int ldy,
+                                size_t sY, cx<T> *__restrict__ Vall, int P) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (size_t)R * nbp) return;
+    const int r = (int)(q % R), c = (int)(q / R), j = j0 + c;
+    const cx<T> *Y = Yall + (size_t)blockIdx.y * sY;
+    cx<T> v{0, 0};
+    if (r == j) v = cx<T>{1, 0};
+    else if (r > j) v = Y[(size_t)j * ldy + r];
+    Vall[(size_t)blockIdx.y * R * P + (size_t)c * R + r] = v;
+}
+
+}  // namespace
+
+// Q (R x C) = H_0 ... H_{C-1} [I; 0] from the reflectors launch_geqrf left
+// in Y and tau (LAPACK orgqr, blocked): the panels taken last to first, each
+// applied as X2 <- (I - V T V^H) X2 = X2 - V (T (V^H X2)) to rows and
+// columns j0.. of Q (the columns before j0 are still e_c then, and the
+// rows before j0 of the later ones zero), three GEMMs per panel; work as
+// for launch_geqrf.
+template <typename T>
+ia_status launch_orgqr(int R, int C, const cx<T> *Y, int ldy, size_t sY, const cx<T> *tau,
+                       cx<T> *Q, int ldq, size_t sQ, cx<T> *work, int batch, hipStream_t s) {
+    if (R < C) return fail(IA_ERR_ARG, "orgqr needs R >= C (%d x %d)", R, C);
+    constexpr int P = NBQ;
+    cx<T> *Vw = work;
+    cx<T> *G = Vw + (size_t)batch * R * P;
+    cx<T> *Tm = G + (size_t)batch * P * P;
+    cx<T> *W = Tm + (size_t)batch * P * P;
+    cx<T> *W2 = W + (size_t)batch * P * C;
+    const cx<T> one{1, 0}, zero{0, 0}, mone{-1, 0};
+    const size_t sV = (size_t)R * P, sT = (size_t)P * P, sW = (size_t)P * C;
+    hipLaunchKernelGGL(k_set_eye<T>, dim3((unsigned)(((size_t)R * C + 255) / 256), batch),
+                       dim3(256), 0, s, R, C, Q, ldq, sQ);
+    for (int j0 = ((C - 1) / P) * P; j0 >= 0; j0 -= P) {
+        const int nbp = std::min(P, C - j0), M = R - j0, N = C - j0;
+        hipLaunchKernelGGL(k_refl_explicit<T>,
+                           dim3((unsigned)(((size_t)R * nbp + 255) / 256), batch), dim3(256), 0,
+                           s, R, j0, nbp, Y, ldy, sY, Vw, P);
+        const cx<T> *V0 = Vw + j0;
+        launch_gemm<T>('C', 'N', nbp, nbp, M, one, V0, R, sV, V0, R, sV, zero, G, P, sT, batch, s);
+        hipLaunchKernelGGL((k_qr_tmat<T, P>), dim3(batch), dim3(256), 0, s, C, j0, nbp, G, tau, Tm);
+        cx<T> *X2 = Q + j0 + (size_t)j0 * ldq;
+        launch_gemm<T>('C', 'N', nbp, N, M, one, V0, R, sV, X2, ldq, sQ, zero, W, P, sW, batch, s);
+        launch_gemm<T>('N', 'N', nbp, N, nbp, one, Tm, P, sT, W, P, sW, zero, W2, P, sW, batch, s);
+        launch_gemm<T>('N', 'N', M, N, nbp, mone, V0, R, sV, W2, P, sW, one, X2, ldq, sQ, batch, s);
+    }
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
+template ia_status launch_orgqr<double>(int, int, const cx<double> *, int, size_t,
+                                        const cx<double> *, cx<double> *, int, size_t,
+                                        cx<double> *, int, hipStream_t);
+
 template ia_status launch_geqrf<double>(int, int, cx<double> *, int, size_t, cx<double> *,
                                         cx<double> *, int, hipStream_t);
 template ia_status launch_geqrf<float>(int, int, cx<float> *, int, size_t, cx<float> *,

@@ -137,6 +137,10 @@ template <typename T>
 void launch_gemm(char opA, char opB, int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda,
                  size_t strideA, const cx<T> *B, int ldb, size_t strideB, cx<T> beta, cx<T> *C,
                  int ldc, size_t strideC, int batch, hipStream_t s);
+// real FP64 GEMM (gemm.hip): C = op(A) B, op in {'N','T'}, column-major
+void launch_dgemm(char opA, int M, int N, int K, const double *A, int lda, size_t strideA,
+                  const double *B, int ldb, size_t strideB, double *C, int ldc, size_t strideC,
+                  int batch, hipStream_t s);
 
 // LU (lu.hip)
 template <typename T>
@@ -185,6 +189,11 @@ size_t geqrf_work_elems(int R, int C);
 template <typename T>
 ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, cx<T> *work,
                        int batch, hipStream_t s);
+// Q (R x C, ld ldq) = H_0 ... H_{C-1} [I; 0] from launch_geqrf's reflectors
+// in Y and tau (blocked, GEMMs); work as for launch_geqrf
+template <typename T>
+ia_status launch_orgqr(int R, int C, const cx<T> *Y, int ldy, size_t sY, const cx<T> *tau,
+                       cx<T> *Q, int ldq, size_t sQ, cx<T> *work, int batch, hipStream_t s);
 template <typename T>
 void launch_copy_upper(int C, const cx<T> *Y, int ldy, size_t sY, cx<T> *Rb, int batch,
                        hipStream_t s);

@@ -75,6 +75,52 @@ def test_radiation_sweep_small_matches_oracle(ia):
     assert np.max(np.abs(post.power_db(rm["power"]) - post.power_db(r["power"]))) < 0.1
 
 
+def test_radiation_subspace_iteration(ia, monkeypatch):
+    """1024 patches, 48 modes: the subspace iteration forced (IA_RAD_MODES=
+    subspace, no fallback) against eigh and the oracle's solve; the default
+    solver takes the same path; the full SVD agrees to 1e-12 e_1.  At
+    4000 Hz the spectrum past mode 48 is flat (e_128 / e_48 ~ 0.6): forced,
+    the iteration reports IA_ERR_NOCONV; by default that frequency alone goes
+    to the full SVD.  A zero tolerance sends every frequency there."""
+    from inverseacoustics_amd._lib import IAError
+
+    S = sc.radiation_scenario(freqs=[300.0, 1200.0, 2500.0, 4000.0], patches=(32, 32),
+                              mic_n=(16, 16), mic_pitch=0.035)
+    assert len(S.r_p) == 1024
+    ia.set_air(S.c, S.rho)
+    n, lo = 48, slice(0, 3)
+    run = lambda sl, **kw: ia.radiation_sweep(S.k[sl], S.r_p, S.patch_area, S.r_h, S.b[sl], n, **kw)
+    monkeypatch.setenv("IA_RAD_MODES", "subspace")
+    rs = run(lo, want_modes=True)
+    assert np.all(rs["status"] == 0)
+    for f in range(3):
+        e0, Q0 = orad.modes(S.r_p, S.patch_area, S.k[f], n, S.rho, S.c)
+        _check_modes(rs["eigval"][f], rs["modes"][f], e0, Q0, f"f={S.freqs[f]}")
+        _check_solve(rs, f, S, n, rs["modes"][f])
+    with pytest.raises(IAError) as ei:
+        run(slice(0, 4))
+    assert ei.value.status == 7 and "Ritz" in str(ei.value)
+    monkeypatch.delenv("IA_RAD_MODES")
+    ra = run(lo)
+    np.testing.assert_array_equal(ra["eigval"], rs["eigval"])
+    np.testing.assert_array_equal(ra["power"], rs["power"])
+    r4 = run(slice(0, 4), want_modes=True)
+    e0, Q0 = orad.modes(S.r_p, S.patch_area, S.k[3], n, S.rho, S.c)
+    _check_modes(r4["eigval"][3], r4["modes"][3], e0, Q0, "f=4000 (full SVD)")
+    np.testing.assert_array_equal(r4["power"][:3], rs["power"])
+    monkeypatch.setenv("IA_RAD_MODES", "full")
+    rf = run(slice(0, 4))
+    e1 = rf["eigval"][:, :1]
+    assert np.all(np.abs(r4["eigval"] - rf["eigval"]) <= 1e-12 * e1)
+    np.testing.assert_allclose(r4["power"], rf["power"], rtol=1e-6)
+    assert np.array_equal(r4["lambda_index"], rf["lambda_index"])
+    monkeypatch.delenv("IA_RAD_MODES")
+    monkeypatch.setenv("IA_RAD_TOL", "0")
+    r0 = run(slice(0, 4))
+    np.testing.assert_array_equal(r0["eigval"], rf["eigval"])
+    np.testing.assert_array_equal(r0["power"], rf["power"])
+
+
 class _Sub:
     """One frequency of a scenario."""
     def __init__(self, S, f):
