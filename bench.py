@@ -247,6 +247,7 @@ def main():
         "velocity_rel_l2_max_vs_fp64": vel_c,
         "rechecked_in_fp64": int(st_cert["rechecked"]),
         "recheck_ms": round(st_cert["recheck_ms"], 2),
+        "recheck_full_fp64_sweep": int(st_cert["recheck_full"]),
         "target_speedup": 5.0,
         "fp64_form": {"value": round(F / t_fp64, 3), "ms_per_step": round(t_fp64 * 1e3, 2),
                       "phases_ms": {k: round(v, 2) for k, v in st_fp64.items()}},

@@ -472,8 +472,11 @@ typedef struct {
     double assemble, factor, solve_he, bidiag, bisect, vectors, select,
            post, total;
     double bidiag_kernel_ms, bidiag_kernel_bytes, bidiag_kernel_launches;
-    double rechecked;         /* frequencies solved again in FP64 (mixed)  */
+    double rechecked;         /* frequencies decided again in FP64 (mixed) */
     double recheck_ms;        /* time of those FP64 solves                 */
+    double recheck_full;      /* of the rechecked, those that took the full
+                                 FP64 sweep (the others were decided from the
+                                 Gram form of the GCV function)            */
 } ia_stats;
 ia_status ia_last_stats(ia_stats *st);
 

@@ -84,7 +84,7 @@ class ia_stats(C.Structure):
     _fields_ = [(n, C.c_double) for n in
                 ("assemble", "factor", "solve_he", "bidiag", "bisect", "vectors", "select",
                  "post", "total", "bidiag_kernel_ms", "bidiag_kernel_bytes",
-                 "bidiag_kernel_launches", "rechecked", "recheck_ms")]
+                 "bidiag_kernel_launches", "rechecked", "recheck_ms", "recheck_full")]
 
 
 _zp = C.POINTER(ia_z)

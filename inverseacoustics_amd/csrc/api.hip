@@ -224,7 +224,7 @@ static ia_status enqueue_chunk(SweepCache<T> &W, const ia_sweep_desc &D, const R
                                const cx<double> *b, double *lam, int *lidx, int *ktsvd,
                                double *power, double *smax, cx<double> *vel, cx<double> *pres,
                                int *status, double *gap, hipStream_t s, Timer &tm,
-                               KernelClock &clk) {
+                               KernelClock &clk, double *curve) {
     const int m = D.m, l = D.l;
     const int R = m, C = l;  // Y = H_e^H (m x l), A = Y^H
     const double rho_c = air().rho * air().c;
@@ -290,7 +290,8 @@ static ia_status enqueue_chunk(SweepCache<T> &W, const ia_sweep_desc &D, const R
     launch_select(C, l, sw.sig.template as<double>(), bbeta.as<cx<double>>(), bn2.as<double>(),
                   prm, lam ? lam + f0 : nullptr, lidx ? lidx + f0 : nullptr,
                   ktsvd ? ktsvd + f0 : nullptr, bphi.as<cx<double>>(), nf, s,
-                  gap ? gap + f0 : nullptr, bcn2.as<double>(), svd_floor_rel<T>());
+                  gap ? gap + f0 : nullptr, bcn2.as<double>(), svd_floor_rel<T>(),
+                  curve ? curve + (size_t)f0 * prm.n_lambda : nullptr);
     // x = U_b [P_B phi; 0]
     launch_real_matvec(C, sw.PBt.template as<double>(), 1, bphi.as<cx<double>>(), C,
                        bc.as<cx<double>>(), C, nf, s);
@@ -366,7 +367,8 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
                                 const double *src, const double *nrm, const double *r_h,
                                 const cx<double> *b, double *lam, int *lidx, int *ktsvd,
                                 double *power, double *smax, cx<double> *vel, cx<double> *pres,
-                                int *status, double *gap, hipStream_t s) {
+                                int *status, double *gap, hipStream_t s,
+                                double *curve = nullptr) {
     const int m = D.m, l = D.l, F = D.n_freq;
     if (l > m)
         return esm_sweep_tall<T>(D, k, r_s, src, nrm, r_h, b, lam, lidx, ktsvd, power, smax, vel,
@@ -383,7 +385,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
     for (int f0 = 0; f0 < F; f0 += chunk) {
         const int nf = std::min(chunk, F - f0);
         IA_TRY(enqueue_chunk<T>(W, D, prm, f0, nf, k, r_s, src, nrm, r_h, b, lam, lidx, ktsvd,
-                                power, smax, vel, pres, status, gap, s, tm, clk));
+                                power, smax, vel, pres, status, gap, s, tm, clk, curve));
         IA_HIP(hipStreamSynchronize(s));
         st.assemble += tm.ms(0, 1);
         st.factor += tm.ms(1, 2);
@@ -399,6 +401,63 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
     st.bidiag_kernel_bytes = clk.bytes;
     st.bidiag_kernel_launches = clk.launches;
     g_stats = st;
+    return IA_OK;
+}
+
+
+// The full FP64 sweep of the frequencies L (sweep indices) and their
+// outputs copied over the complex64 ones in o (host or device pointers,
+// each may be NULL).
+static ia_status full_recheck(const ia_sweep_desc &desc, const std::vector<int> &L,
+                              const double *k, const double *r_s, const double *src,
+                              const double *nrm, const double *r_h, const ia_z *b,
+                              const ia_sweep_out &o, hipStream_t s) {
+    const int nL = (int)L.size(), m = desc.m, l = desc.l;
+    DevBuf kk, bb, tl, ti, tk, tw, tsm, tv, tp, tst, tg;
+    IA_TRY(kk.alloc(sizeof(double) * nL));
+    IA_TRY(bb.alloc(sizeof(ia_z) * (size_t)nL * l));
+    for (int q = 0; q < nL; ++q) {
+        IA_HIP(hipMemcpyAsync(kk.as<double>() + q, k + L[q], sizeof(double), hipMemcpyDefault, s));
+        IA_HIP(hipMemcpyAsync(bb.as<ia_z>() + (size_t)q * l, b + (size_t)L[q] * l,
+                              sizeof(ia_z) * l, hipMemcpyDefault, s));
+    }
+    IA_TRY(tl.alloc(sizeof(double) * nL));
+    IA_TRY(ti.alloc(sizeof(int) * nL));
+    IA_TRY(tk.alloc(sizeof(int) * nL));
+    IA_TRY(tw.alloc(sizeof(double) * nL));
+    IA_TRY(tsm.alloc(sizeof(double) * nL));
+    IA_TRY(tv.alloc(sizeof(ia_z) * (size_t)nL * m));
+    IA_TRY(tp.alloc(sizeof(ia_z) * (size_t)nL * m));
+    IA_TRY(tst.alloc(sizeof(int) * nL));
+    IA_TRY(tg.alloc(sizeof(double) * nL));
+    ia_sweep_desc d2 = desc;
+    d2.n_freq = nL;
+    d2.precision = IA_PREC_FP64;
+    d2.chunk = 0;
+    IA_TRY(esm_sweep_impl<double>(d2, kk.as<double>(), r_s, src, nrm, r_h, bb.as<cx<double>>(),
+                                  tl.as<double>(), ti.as<int>(), tk.as<int>(), tw.as<double>(),
+                                  tsm.as<double>(), tv.as<cx<double>>(), tp.as<cx<double>>(),
+                                  tst.as<int>(), tg.as<double>(), s));
+    for (int q = 0; q < nL; ++q) {
+        const int f = L[q];
+        if (o.lambda) IA_HIP(hipMemcpyAsync(o.lambda + f, tl.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
+        if (o.lambda_index) IA_HIP(hipMemcpyAsync(o.lambda_index + f, ti.as<int>() + q, sizeof(int), hipMemcpyDefault, s));
+        if (o.k_tsvd) IA_HIP(hipMemcpyAsync(o.k_tsvd + f, tk.as<int>() + q, sizeof(int), hipMemcpyDefault, s));
+        if (o.power) IA_HIP(hipMemcpyAsync(o.power + f, tw.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
+        if (o.sigma_max) IA_HIP(hipMemcpyAsync(o.sigma_max + f, tsm.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
+        if (o.status) IA_HIP(hipMemcpyAsync(o.status + f, tst.as<int>() + q, sizeof(int), hipMemcpyDefault, s));
+        if (o.selection_gap)
+            IA_HIP(hipMemcpyAsync(o.selection_gap + f, tg.as<double>() + q, sizeof(double),
+                                  hipMemcpyDefault, s));
+        if (o.velocity)
+            IA_HIP(hipMemcpyAsync(o.velocity + (size_t)f * m, tv.as<ia_z>() + (size_t)q * m,
+                                  sizeof(ia_z) * m, hipMemcpyDefault, s));
+        if (o.pressure)
+            IA_HIP(hipMemcpyAsync(o.pressure + (size_t)f * m, tp.as<ia_z>() + (size_t)q * m,
+                                  sizeof(ia_z) * m, hipMemcpyDefault, s));
+    }
+    // the device buffers above are freed on return: the copies finish first
+    IA_HIP(hipStreamSynchronize(s));
     return IA_OK;
 }
 
@@ -443,6 +502,7 @@ void ia_release(void) {
     release_rad_cache();
     release_solve_cache();
     release_ibem_cache();
+    release_recheck_cache();
 }
 
 ia_status ia_last_stats(ia_stats *st) {
@@ -481,22 +541,29 @@ ia_status ia_esm_sweep_device(const ia_sweep_desc *desc, const double *k, const 
                                       o.lambda, o.lambda_index, o.k_tsvd, o.power, o.sigma_max,
                                       (cx<double> *)o.velocity, (cx<double> *)o.pressure,
                                       o.status, o.selection_gap, s);
-    const int F = desc->n_freq, m = desc->m, l = desc->l;
+    const int F = desc->n_freq;
     // default: the complex64 pipeline's GCV / L-curve values carry relative
     // errors up to ~1e-4 (on the 256-frequency 1024 x 2048 plate sweep a GCV
     // difference of 1.7e-5 in FP64 came out as -6.7e-5), so choices closer
     // than twice that are decided in FP64
     const double tau = desc->recheck_gap == 0.0 ? 2e-4 : desc->recheck_gap;
-    DevBuf dgap;
+    DevBuf dgap, dcurve;
     double *gap = o.selection_gap;
     if (!gap && tau > 0) {
         IA_TRY(dgap.alloc(sizeof(double) * F));
         gap = dgap.as<double>();
     }
+    // Tikhonov + GCV: nearly tied choices are decided from the GCV function
+    // in its Gram form (recheck.hip), the others by the full FP64 sweep;
+    // IA_RECHECK=full sends every one to the full sweep
+    const char *rc = std::getenv("IA_RECHECK");
+    const bool gram = tau > 0 && gcv_recheck_applies(*desc) && !(rc && std::strcmp(rc, "full") == 0);
+    const int nl = desc->reg.n_lambda > 0 ? desc->reg.n_lambda : 50;
+    if (gram) IA_TRY(dcurve.alloc(sizeof(double) * (size_t)F * nl));
     IA_TRY(esm_sweep_impl<float>(*desc, k, r_s, src, nrm, r_h, (const cx<double> *)b, o.lambda,
                                  o.lambda_index, o.k_tsvd, o.power, o.sigma_max,
                                  (cx<double> *)o.velocity, (cx<double> *)o.pressure, o.status,
-                                 gap, s));
+                                 gap, s, gram ? dcurve.as<double>() : nullptr));
     if (!(tau > 0)) return IA_OK;
     // ---- certified selection: nearly tied frequencies again in FP64
     ia_stats st = g_stats;
@@ -507,61 +574,30 @@ ia_status ia_esm_sweep_device(const ia_sweep_desc *desc, const double *k, const 
     for (int f = 0; f < F; ++f)
         if (!(hgap[f] >= tau)) L.push_back(f);  // NaN gaps are rechecked too
     if (L.empty()) return IA_OK;
-    const int nL = (int)L.size();
     hipEvent_t e0, e1;
     IA_HIP(hipEventCreate(&e0));
     IA_HIP(hipEventCreate(&e1));
     IA_HIP(hipEventRecord(e0, s));
-    DevBuf kk, bb, tl, ti, tk, tw, tsm, tv, tp, tst, tg;
-    IA_TRY(kk.alloc(sizeof(double) * nL));
-    IA_TRY(bb.alloc(sizeof(ia_z) * (size_t)nL * l));
-    for (int q = 0; q < nL; ++q) {
-        IA_HIP(hipMemcpyAsync(kk.as<double>() + q, k + L[q], sizeof(double), hipMemcpyDefault, s));
-        IA_HIP(hipMemcpyAsync(bb.as<ia_z>() + (size_t)q * l, b + (size_t)L[q] * l,
-                              sizeof(ia_z) * l, hipMemcpyDefault, s));
+    std::vector<int> full;
+    if (gram) {
+        std::vector<double> curve(L.size() * (size_t)nl);
+        for (size_t q = 0; q < L.size(); ++q)
+            IA_HIP(hipMemcpyAsync(curve.data() + q * nl, dcurve.as<double>() + (size_t)L[q] * nl,
+                                  sizeof(double) * nl, hipMemcpyDeviceToHost, s));
+        IA_HIP(hipStreamSynchronize(s));
+        IA_TRY(gcv_recheck(*desc, L, curve, tau, k, r_s, src, nrm, r_h, b, o, full, s));
+    } else {
+        full = L;
     }
-    IA_TRY(tl.alloc(sizeof(double) * nL));
-    IA_TRY(ti.alloc(sizeof(int) * nL));
-    IA_TRY(tk.alloc(sizeof(int) * nL));
-    IA_TRY(tw.alloc(sizeof(double) * nL));
-    IA_TRY(tsm.alloc(sizeof(double) * nL));
-    IA_TRY(tv.alloc(sizeof(ia_z) * (size_t)nL * m));
-    IA_TRY(tp.alloc(sizeof(ia_z) * (size_t)nL * m));
-    IA_TRY(tst.alloc(sizeof(int) * nL));
-    IA_TRY(tg.alloc(sizeof(double) * nL));
-    ia_sweep_desc d2 = *desc;
-    d2.n_freq = nL;
-    d2.precision = IA_PREC_FP64;
-    d2.chunk = 0;
-    IA_TRY(esm_sweep_impl<double>(d2, kk.as<double>(), r_s, src, nrm, r_h, bb.as<cx<double>>(),
-                                  tl.as<double>(), ti.as<int>(), tk.as<int>(), tw.as<double>(),
-                                  tsm.as<double>(), tv.as<cx<double>>(), tp.as<cx<double>>(),
-                                  tst.as<int>(), tg.as<double>(), s));
-    for (int q = 0; q < nL; ++q) {
-        const int f = L[q];
-        if (o.lambda) IA_HIP(hipMemcpyAsync(o.lambda + f, tl.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
-        if (o.lambda_index) IA_HIP(hipMemcpyAsync(o.lambda_index + f, ti.as<int>() + q, sizeof(int), hipMemcpyDefault, s));
-        if (o.k_tsvd) IA_HIP(hipMemcpyAsync(o.k_tsvd + f, tk.as<int>() + q, sizeof(int), hipMemcpyDefault, s));
-        if (o.power) IA_HIP(hipMemcpyAsync(o.power + f, tw.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
-        if (o.sigma_max) IA_HIP(hipMemcpyAsync(o.sigma_max + f, tsm.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
-        if (o.status) IA_HIP(hipMemcpyAsync(o.status + f, tst.as<int>() + q, sizeof(int), hipMemcpyDefault, s));
-        if (o.selection_gap)
-            IA_HIP(hipMemcpyAsync(o.selection_gap + f, tg.as<double>() + q, sizeof(double),
-                                  hipMemcpyDefault, s));
-        if (o.velocity)
-            IA_HIP(hipMemcpyAsync(o.velocity + (size_t)f * m, tv.as<ia_z>() + (size_t)q * m,
-                                  sizeof(ia_z) * m, hipMemcpyDefault, s));
-        if (o.pressure)
-            IA_HIP(hipMemcpyAsync(o.pressure + (size_t)f * m, tp.as<ia_z>() + (size_t)q * m,
-                                  sizeof(ia_z) * m, hipMemcpyDefault, s));
-    }
+    if (!full.empty()) IA_TRY(full_recheck(*desc, full, k, r_s, src, nrm, r_h, b, o, s));
     IA_HIP(hipEventRecord(e1, s));
     IA_HIP(hipStreamSynchronize(s));
     float rms = 0;
     (void)hipEventElapsedTime(&rms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    st.rechecked = nL;
+    st.rechecked = (double)L.size();
+    st.recheck_full = (double)full.size();
     st.recheck_ms = rms;
     st.total += rms;
     g_stats = st;

@@ -235,6 +235,20 @@ void release_rad_cache();
 // sweep's (ibem_api.hip)
 void release_solve_cache();
 void release_ibem_cache();
+// ... and the certified recheck's (recheck.hip)
+void release_recheck_cache();
+
+// Certified selection of the complex64 ESM sweep (recheck.hip): for
+// Tikhonov + grid GCV with l <= min(m, 4096), the nearly tied frequencies L
+// (sweep indices; curve: their complex64 GCV values, n_lambda each, host)
+// decided in FP64 from the Gram form of the GCV function, their outputs
+// written into o; the frequencies it cannot decide are appended to full.
+bool gcv_recheck_applies(const ia_sweep_desc &D);
+ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
+                      const std::vector<double> &curve, double tau, const double *k,
+                      const double *r_s, const double *src, const double *nrm, const double *r_h,
+                      const ia_z *b, const ia_sweep_out &o, std::vector<int> &full,
+                      hipStream_t s);
 
 // ADMM (admm.hip): A (rows x cols, batch stride rows*cols), b, c on the device
 ia_status admm_batched(int rows, int cols, int batch, const cx<double> *A, const cx<double> *b,
@@ -310,10 +324,13 @@ struct RegDev {
 // replaced by one component of energy |c|^2 - sum(resolved |beta|^2) at s = 0
 // (cnorm2 = |c|^2 of the vector with beta = Q^T c; NULL: sum |beta|^2).
 // floor_rel: 1e-12 for an FP64 SVD, 1e-6 for a complex64 one.
+// curve_out (optional, grid GCV / L-curve only): the n_lambda criterion
+// values of each batch item.
 void launch_select(int p, int rows, const double *sigma, const cx<double> *beta,
                    const double *bnorm2, RegDev prm, double *lam_out, int *idx_out, int *k_out,
                    cx<double> *phi, int batch, hipStream_t s, double *gap_out = nullptr,
-                   const double *cnorm2 = nullptr, double floor_rel = 1e-12);
+                   const double *cnorm2 = nullptr, double floor_rel = 1e-12,
+                   double *curve_out = nullptr);
 template <typename T>
 constexpr double svd_floor_rel() { return sizeof(T) == sizeof(double) ? 1e-12 : 1e-6; }
 

@@ -1,0 +1,602 @@
+// Certified selection of the complex64 ESM sweep: nearly tied Tikhonov + GCV
+// choices are decided in FP64 without a second SVD.
+//
+// For H = H_e (l x m, l <= m), b and lambda > 0, with the Hermitian positive
+// definite A = H H^H + lambda^2 I (l x l):
+//   b - H x_lambda = lambda^2 A^-1 b,
+//   trace(I - H H^#_lambda) = sum_i lambda^2 / (s_i^2 + lambda^2) = lambda^2 tr(A^-1),
+// so the Tikhonov GCV function (gcvG.m:38-40, tikhonovG.m:29-30; the grid
+// form of the sweep, every singular value included) is
+//   G(lambda) = |A^-1 b|^2 / tr(A^-1)^2,
+// and the Tikhonov solution is x_lambda = H^H A^-1 b.  Its FP64 error is
+// about eps cond(A) = eps (s_1^2 + lambda^2) / lambda^2 -- small wherever a
+// GCV minimum of a noisy measurement sits (lambda >= 1e-4 s_1 gives 2e-8),
+// which the full SVD route only reaches by the bidiagonal chase, the
+// latency-bound part of an FP64 sweep of a few frequencies.
+//
+// Per rechecked frequency (the complex64 sweep's GCV value of the runner-up
+// within the recheck gap of its minimum):
+//   1. Y = P H_e^H in FP64 by the sweep's own kernels (assembly, LU of D,
+//      the two triangular solves);
+//   2. G = H H^H = Y^H Y (one batched GEMM);
+//   3. s_1^2, the largest eigenvalue of G, by Lanczos (three-term
+//      recurrence on the device, the largest Ritz value of the tridiagonal by
+//      Sturm bisection on the host, taken once it has settled);
+//   4. every candidate grid index j (complex64 GCV value within the recheck
+//      gap of the minimum: under the error that gap allows, the FP64 argmin
+//      is one of them): the LU of A(lambda_j) and one solve with [P I | P b]
+//      give tr(A^-1) and A^-1 b, hence G(lambda_j);
+//   5. the FP64 choice stands when its gap to the runner-up exceeds twice the
+//      two values' error bound (256 eps cond(A) + 1e-11 each); its x, p_s, W,
+//      lambda, index, s_1 and gap then replace the complex64 outputs.
+// A frequency that is not decided so (more than six candidates, no
+// settled Ritz value, a gap inside the bound, a singular factor) is listed
+// for the full FP64 sweep.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "ia_internal.hpp"
+
+namespace ia {
+
+namespace {
+
+constexpr int LZ_T = 1024;                 // threads of a Lanczos workgroup
+constexpr int LZ_NMAX = 4096;              // rows it serves (q in LDS)
+constexpr int LZ_RPT = LZ_NMAX / LZ_T;     // rows per thread at most
+constexpr int LZ_KMAX = 192;               // Lanczos steps at most
+constexpr int LZ_ROUND = 32;               // steps per launch
+constexpr int MAX_CAND = 6;                // grid candidates per frequency
+constexpr int RC_CHUNK = 32;               // frequencies per device batch
+
+__device__ __forceinline__ double wsum64(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the workgroup; every thread returns the same value (the wave
+// partials are added in one fixed order).
+__device__ __forceinline__ double block_sum(double v, double *part) {
+    v = wsum64(v);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    double t = 0;
+#pragma unroll
+    for (int i = 0; i < LZ_T / 64; ++i) t += part[i];
+    return t;
+}
+
+// fixed pseudo-random start entry in (-0.5, 0.5) (integer hash of i and c)
+__device__ __forceinline__ double hash_unit(unsigned i, unsigned c) {
+    unsigned h = i * 0x9E3779B1u + c * 0x85EBCA77u + 0x165667B1u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return (double)h * (1.0 / 4294967296.0) - 0.5;
+}
+
+// Lanczos on the Hermitian G (n x n column-major, n <= LZ_NMAX), `steps`
+// more steps from each active matrix's saved state (q_j, q_{j-1},
+// beta_{j-1}, j).  Step j: w = G q_j - beta_{j-1} q_{j-1}, alpha_j =
+// Re(q_j^H w), w -= alpha_j q_j, beta_j = |w|, q_{j+1} = w / beta_j (no
+// reorthogonalisation: lost orthogonality only repeats converged Ritz
+// values, the largest still converges to s_1^2).  One workgroup per matrix;
+// thread t owns rows t, t + 1024, ...: (G q)_i = sum_r G[i + r n] q_r reads,
+// per r, 64 consecutive entries of column r per wave, q_r broadcast from
+// LDS.  A beta_j that is not above 1e-13 (|alpha_j| + beta_{j-1}) ends the
+// recurrence (invariant subspace): done[b] = j + 1.
+__global__ __launch_bounds__(LZ_T) void k_lanczos(int n, const cx<double> *__restrict__ Gall,
+                                                 size_t sG, cx<double> *__restrict__ qall,
+                                                 cx<double> *__restrict__ qpall,
+                                                 double *__restrict__ bprev, int *__restrict__ nstep,
+                                                 int *__restrict__ done,
+                                                 const int *__restrict__ active, int steps,
+                                                 double *__restrict__ alpha,
+                                                 double *__restrict__ beta) {
+    extern __shared__ double lz_sh[];
+    double *part = lz_sh;                                       // 16
+    cx<double> *qs = reinterpret_cast<cx<double> *>(lz_sh + 16);  // n
+    const int b = blockIdx.x, t = threadIdx.x;
+    if (!active[b] || done[b]) return;
+    const cx<double> *G = Gall + (size_t)b * sG;
+    cx<double> *qg = qall + (size_t)b * n, *qpg = qpall + (size_t)b * n;
+    int j = nstep[b];
+    double bp = bprev[b];
+    cx<double> q[LZ_RPT], qp[LZ_RPT], w[LZ_RPT];
+    if (j == 0) {
+        double ss = 0;
+#pragma unroll
+        for (int u = 0; u < LZ_RPT; ++u) {
+            const int i = t + u * LZ_T;
+            q[u] = i < n ? cx<double>{hash_unit(i, 1), hash_unit(i, 2)} : cx<double>{0, 0};
+            qp[u] = cx<double>{0, 0};
+            ss += abs2(q[u]);
+        }
+        const double sc = 1.0 / sqrt(block_sum(ss, part));
+#pragma unroll
+        for (int u = 0; u < LZ_RPT; ++u) q[u] = sc * q[u];
+        bp = 0;
+    } else {
+#pragma unroll
+        for (int u = 0; u < LZ_RPT; ++u) {
+            const int i = t + u * LZ_T;
+            q[u] = i < n ? qg[i] : cx<double>{0, 0};
+            qp[u] = i < n ? qpg[i] : cx<double>{0, 0};
+        }
+    }
+    const int jend = min(j + steps, LZ_KMAX);
+    for (; j < jend; ++j) {
+#pragma unroll
+        for (int u = 0; u < LZ_RPT; ++u) {
+            const int i = t + u * LZ_T;
+            if (i < n) qs[i] = q[u];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < LZ_RPT; ++u) {
+            w[u] = cx<double>{0, 0};
+            const int i = t + u * LZ_T;
+            if (i >= n) continue;
+            const cx<double> *gi = G + i;
+            cx<double> a0{0, 0}, a1{0, 0}, a2{0, 0}, a3{0, 0};
+            int r = 0;
+            for (; r + 4 <= n; r += 4) {
+                const cx<double> g0 = gi[(size_t)r * n], g1 = gi[(size_t)(r + 1) * n],
+                                 g2 = gi[(size_t)(r + 2) * n], g3 = gi[(size_t)(r + 3) * n];
+                a0 = a0 + g0 * qs[r];
+                a1 = a1 + g1 * qs[r + 1];
+                a2 = a2 + g2 * qs[r + 2];
+                a3 = a3 + g3 * qs[r + 3];
+            }
+            for (; r < n; ++r) a0 = a0 + gi[(size_t)r * n] * qs[r];
+            w[u] = (a0 + a1) + (a2 + a3);
+        }
+        double a = 0;
+#pragma unroll
+        for (int u = 0; u < LZ_RPT; ++u) a += q[u].re * w[u].re + q[u].im * w[u].im;
+        const double al = block_sum(a, part);
+        double ss = 0;
+#pragma unroll
+        for (int u = 0; u < LZ_RPT; ++u) {
+            w[u] = w[u] - al * q[u] - bp * qp[u];
+            ss += abs2(w[u]);
+        }
+        const double be = sqrt(block_sum(ss, part));
+        if (t == 0) {
+            alpha[(size_t)b * LZ_KMAX + j] = al;
+            beta[(size_t)b * LZ_KMAX + j] = be;
+        }
+        if (!(be > 1e-13 * (fabs(al) + bp))) {
+            if (t == 0) done[b] = j + 1;
+            ++j;
+            break;
+        }
+        const double sc = 1.0 / be;
+#pragma unroll
+        for (int u = 0; u < LZ_RPT; ++u) {
+            qp[u] = q[u];
+            q[u] = sc * w[u];
+        }
+        bp = be;
+        __syncthreads();  // every read of qs done before the next step writes it
+    }
+#pragma unroll
+    for (int u = 0; u < LZ_RPT; ++u) {
+        const int i = t + u * LZ_T;
+        if (i < n) {
+            qg[i] = q[u];
+            qpg[i] = qp[u];
+        }
+    }
+    if (t == 0) {
+        nstep[b] = j;
+        bprev[b] = bp;
+    }
+}
+
+// A_c = G_{g[c]} + lam2[c] I (n x n per candidate)
+__global__ void k_shift(int n, const cx<double> *__restrict__ G, size_t sG, const int *__restrict__ g,
+                        const double *__restrict__ lam2, cx<double> *__restrict__ A) {
+    const int c = blockIdx.y;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nn = (size_t)n * n;
+    if (e >= nn) return;
+    cx<double> v = G[(size_t)g[c] * sG + e];
+    if (e % (n + 1) == 0) v.re += lam2[c];
+    A[(size_t)c * nn + e] = v;
+}
+
+// X_c (n x (n+1)) = P_c [I | b_{g[c]}] with (P B)[r] = B[idx[r]]
+__global__ void k_rhs_perm(int n, const int *__restrict__ idx, const int *__restrict__ g,
+                           const cx<double> *__restrict__ bq, cx<double> *__restrict__ X) {
+    const int c = blockIdx.y;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, ne = (size_t)n * (n + 1);
+    if (e >= ne) return;
+    const int r = (int)(e % n), col = (int)(e / n);
+    const int src = idx[(size_t)c * n + r];
+    cx<double> v{0, 0};
+    if (col < n)
+        v.re = src == col ? 1.0 : 0.0;
+    else
+        v = bq[(size_t)g[c] * n + src];
+    X[(size_t)c * ne + e] = v;
+}
+
+// per candidate: tr(A^-1) = sum_i Re X[i, i], |A^-1 b|^2 = |X[:, n]|^2
+__global__ __launch_bounds__(256) void k_gcv_terms(int n, const cx<double> *__restrict__ X,
+                                                  double *__restrict__ out) {
+    __shared__ double sa[4], sb[4];
+    const int c = blockIdx.x, t = threadIdx.x;
+    const cx<double> *Xc = X + (size_t)c * n * (n + 1);
+    double tr = 0, ny = 0;
+    for (int i = t; i < n; i += 256) {
+        tr += Xc[(size_t)i * n + i].re;
+        ny += abs2(Xc[(size_t)n * n + i]);
+    }
+    tr = wsum64(tr);
+    ny = wsum64(ny);
+    if ((t & 63) == 0) {
+        sa[t >> 6] = tr;
+        sb[t >> 6] = ny;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const double T = (sa[0] + sa[1]) + (sa[2] + sa[3]);
+        const double N = (sb[0] + sb[1]) + (sb[2] + sb[3]);
+        out[2 * c] = T;
+        out[2 * c + 1] = N;
+    }
+}
+
+// y_q = X_{c[q]}[:, n] (zero when c[q] < 0)
+__global__ void k_pick_y(int n, const int *__restrict__ cq, const cx<double> *__restrict__ X,
+                         cx<double> *__restrict__ y) {
+    const int q = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = cq[q];
+    y[(size_t)q * n + i] = c >= 0 ? X[(size_t)c * n * (n + 1) + (size_t)n * n + i] : cx<double>{0, 0};
+}
+
+// Largest eigenvalue of the symmetric tridiagonal (a[0..k), b[0..k-1)) by
+// Sturm-count bisection to a relative width of 4 eps.
+double tri_top(const double *a, const double *b, int k) {
+    double lo = INFINITY, hi = -INFINITY, bmax = 0;
+    for (int i = 0; i < k; ++i) {
+        const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(b[i]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+        if (i + 1 < k) bmax = std::max(bmax, std::fabs(b[i]));
+    }
+    const double scale = std::max(std::fabs(lo), std::fabs(hi));
+    if (!(scale > 0) || !std::isfinite(scale)) return scale == 0 ? 0.0 : NAN;
+    const double pivmin = 1e-290 * std::max(1.0, bmax * bmax);
+    lo -= 1e-14 * scale;
+    hi += 1e-14 * scale;
+    // eigenvalues below x
+    auto below = [&](double x) {
+        int c = 0;
+        double d = 1;
+        for (int i = 0; i < k; ++i) {
+            d = a[i] - x - (i > 0 ? b[i - 1] * b[i - 1] / d : 0.0);
+            if (std::fabs(d) < pivmin) d = -pivmin;
+            if (d < 0) ++c;
+        }
+        return c;
+    };
+    for (int it = 0; it < 200 && hi - lo > 4.4e-16 * std::max(std::fabs(lo), std::fabs(hi)); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (below(mid) == k)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+// Device workspace of the recheck, kept per host thread (buffers only grow).
+struct RecheckCache {
+    DevBuf kk, bq, Dm, Y, ipiv, pidx, info, G, lq, lqp, lbp, lns, ldone, lact, lal, lbe, A, X,
+        cpiv, cidx, cinfo, cg, clam2, cterm, cq, yv, xY, x, qv, ps, pw;
+    void release() {
+        for (DevBuf *p : {&kk, &bq, &Dm, &Y, &ipiv, &pidx, &info, &G, &lq, &lqp, &lbp, &lns, &ldone,
+                          &lact, &lal, &lbe, &A, &X, &cpiv, &cidx, &cinfo, &cg, &clam2, &cterm, &cq,
+                          &yv, &xY, &x, &qv, &ps, &pw})
+            p->reset();
+    }
+};
+RecheckCache &recheck_cache() {
+    static thread_local RecheckCache *c = new RecheckCache();
+    return *c;
+}
+
+template <typename V>
+ia_status put1(V *dst, size_t at, const V &v, hipStream_t s) {
+    if (!dst) return IA_OK;
+    IA_HIP(hipMemcpyAsync(dst + at, &v, sizeof(V), hipMemcpyDefault, s));
+    return IA_OK;
+}
+
+}  // namespace
+
+void release_recheck_cache() { recheck_cache().release(); }
+
+bool gcv_recheck_applies(const ia_sweep_desc &D) {
+    return D.reg.filter == IA_FILTER_TIKHONOV && D.reg.select == IA_SELECT_GCV && D.l <= D.m &&
+           D.l <= LZ_NMAX;
+}
+
+// L: the sweep indices of the frequencies to decide; curve: their complex64
+// GCV values (n_lambda per frequency, in L's order, host); tau: the recheck
+// gap.  Decided frequencies get their outputs in o (host or device
+// pointers, each may be NULL; k_tsvd is left as it is); the others are
+// appended to `full`.
+ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
+                      const std::vector<double> &curve, double tau, const double *k,
+                      const double *r_s, const double *src, const double *nrm, const double *r_h,
+                      const ia_z *b, const ia_sweep_out &o, std::vector<int> &full,
+                      hipStream_t s) {
+    const int m = D.m, l = D.l, nl = D.reg.n_lambda > 0 ? D.reg.n_lambda : 50;
+    const double dec = D.reg.decades > 0 ? D.reg.decades : 8.0;
+    const double rho_c = air().rho * air().c;
+    RecheckCache &W = recheck_cache();
+    const size_t mm = (size_t)m * m, ml = (size_t)m * l, ll = (size_t)l * l,
+                 lx = (size_t)l * (l + 1);
+    const size_t lz_lds = sizeof(double) * 16 + sizeof(cx<double>) * (size_t)l;
+    (void)hipFuncSetAttribute((const void *)k_lanczos, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lz_lds);
+    for (size_t c0 = 0; c0 < L.size(); c0 += RC_CHUNK) {
+        const int nq = (int)std::min<size_t>(RC_CHUNK, L.size() - c0);
+        // ---- candidates of each frequency from its complex64 GCV curve
+        std::vector<std::vector<int>> cand(nq);
+        std::vector<char> ok(nq, 1);
+        for (int q = 0; q < nq; ++q) {
+            const double *cv = curve.data() + (c0 + q) * (size_t)nl;
+            double cmin = INFINITY;
+            bool fin = true;
+            for (int j = 0; j < nl; ++j) {
+                fin = fin && std::isfinite(cv[j]);
+                cmin = std::min(cmin, cv[j]);
+            }
+            if (fin && cmin > 0)
+                for (int j = 0; j < nl; ++j)
+                    if ((cv[j] - cmin) / cmin < tau) cand[q].push_back(j);
+            if (!fin || !(cmin > 0) || cand[q].size() < 2 || cand[q].size() > MAX_CAND) ok[q] = 0;
+        }
+        // ---- 1. Y = P H_e^H in FP64, 2. G = Y^H Y
+        IA_TRY(W.kk.alloc(sizeof(double) * nq));
+        IA_TRY(W.bq.alloc(sizeof(cx<double>) * nq * (size_t)l));
+        IA_TRY(W.Dm.alloc(sizeof(cx<double>) * nq * mm));
+        IA_TRY(W.Y.alloc(sizeof(cx<double>) * nq * ml));
+        IA_TRY(W.ipiv.alloc(sizeof(int) * nq * (size_t)m));
+        IA_TRY(W.pidx.alloc(2 * sizeof(int) * nq * (size_t)m));
+        IA_TRY(W.info.alloc(sizeof(int) * nq));
+        IA_TRY(W.G.alloc(sizeof(cx<double>) * nq * ll));
+        double *kk = W.kk.as<double>();
+        cx<double> *bq = W.bq.as<cx<double>>(), *Dm = W.Dm.as<cx<double>>(),
+                   *Y = W.Y.as<cx<double>>(), *G = W.G.as<cx<double>>();
+        int *ipiv = W.ipiv.as<int>(), *pidx = W.pidx.as<int>(), *info = W.info.as<int>();
+        for (int q = 0; q < nq; ++q) {
+            const int f = L[c0 + q];
+            IA_HIP(hipMemcpyAsync(kk + q, k + f, sizeof(double), hipMemcpyDefault, s));
+            IA_HIP(hipMemcpyAsync(bq + (size_t)q * l, b + (size_t)f * l, sizeof(cx<double>) * l,
+                                  hipMemcpyDefault, s));
+        }
+        launch_esm_D<double>(m, nq, kk, r_s, src, nrm, Dm, mm, s, false);
+        launch_esm_M<double>(l, m, nq, kk, rho_c, r_h, src, Y, ml, true, s);
+        IA_HIP(hipGetLastError());
+        IA_HIP(hipMemsetAsync(info, 0, sizeof(int) * nq, s));
+        IA_TRY(getrf_batched<double>(m, Dm, m, mm, ipiv, info, nq, s));
+        IA_TRY(getrs_c_permuted<double>(m, l, Dm, m, mm, Y, m, ml, nq, s));
+        IA_TRY(lu_perm_index(m, ipiv, pidx, pidx + (size_t)m * nq, nq, s));
+        launch_gemm<double>('C', 'N', l, l, m, cx<double>{1, 0}, Y, m, ml, Y, m, ml,
+                            cx<double>{0, 0}, G, l, ll, nq, s);
+        IA_HIP(hipGetLastError());
+        // ---- 3. s_1^2 by Lanczos, LZ_ROUND steps per launch until settled
+        IA_TRY(W.lq.alloc(sizeof(cx<double>) * nq * (size_t)l));
+        IA_TRY(W.lqp.alloc(sizeof(cx<double>) * nq * (size_t)l));
+        IA_TRY(W.lbp.alloc(sizeof(double) * nq));
+        IA_TRY(W.lns.alloc(sizeof(int) * nq));
+        IA_TRY(W.ldone.alloc(sizeof(int) * nq));
+        IA_TRY(W.lact.alloc(sizeof(int) * nq));
+        IA_TRY(W.lal.alloc(sizeof(double) * nq * LZ_KMAX));
+        IA_TRY(W.lbe.alloc(sizeof(double) * nq * LZ_KMAX));
+        IA_HIP(hipMemsetAsync(W.lns.p, 0, sizeof(int) * nq, s));
+        IA_HIP(hipMemsetAsync(W.ldone.p, 0, sizeof(int) * nq, s));
+        std::vector<int> act(nq, 1), hns(nq), hdone(nq);
+        std::vector<double> theta(nq, NAN), hal((size_t)nq * LZ_KMAX), hbe((size_t)nq * LZ_KMAX);
+        for (int round = 0; round * LZ_ROUND < LZ_KMAX; ++round) {
+            IA_HIP(hipMemcpyAsync(W.lact.p, act.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_lanczos, dim3(nq), dim3(LZ_T), lz_lds, s, l, G, ll,
+                               W.lq.as<cx<double>>(), W.lqp.as<cx<double>>(), W.lbp.as<double>(),
+                               W.lns.as<int>(), W.ldone.as<int>(), W.lact.as<int>(), LZ_ROUND,
+                               W.lal.as<double>(), W.lbe.as<double>());
+            IA_HIP(hipGetLastError());
+            IA_HIP(hipMemcpyAsync(hns.data(), W.lns.p, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
+            IA_HIP(hipMemcpyAsync(hdone.data(), W.ldone.p, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
+            IA_HIP(hipMemcpyAsync(hal.data(), W.lal.p, sizeof(double) * nq * LZ_KMAX,
+                                  hipMemcpyDeviceToHost, s));
+            IA_HIP(hipMemcpyAsync(hbe.data(), W.lbe.p, sizeof(double) * nq * LZ_KMAX,
+                                  hipMemcpyDeviceToHost, s));
+            IA_HIP(hipStreamSynchronize(s));
+            bool more = false;
+            for (int q = 0; q < nq; ++q) {
+                if (!act[q]) continue;
+                const double *a = hal.data() + (size_t)q * LZ_KMAX, *be = hbe.data() + (size_t)q * LZ_KMAX;
+                const int kq = hdone[q] ? hdone[q] : hns[q];
+                const double th = tri_top(a, be, kq);
+                const double th0 = kq > 8 ? tri_top(a, be, kq - 8) : NAN;
+                if (hdone[q] || std::fabs(th - th0) <= 1e-13 * std::fabs(th)) {
+                    theta[q] = th;
+                    act[q] = 0;
+                } else if (hns[q] >= LZ_KMAX) {
+                    act[q] = 0;  // not settled: theta stays NaN
+                } else {
+                    more = true;
+                }
+            }
+            if (!more) break;
+        }
+        for (int q = 0; q < nq; ++q)
+            if (!(theta[q] > 0) || !std::isfinite(theta[q])) ok[q] = 0;
+        // ---- 4. the candidates' GCV values in FP64
+        std::vector<int> cg;
+        std::vector<double> clam2, clam;
+        std::vector<int> cfirst(nq, -1);
+        for (int q = 0; q < nq; ++q) {
+            if (!ok[q]) continue;
+            cfirst[q] = (int)cg.size();
+            const double s1 = std::sqrt(theta[q]);
+            for (int j : cand[q]) {
+                const double la = s1 * std::pow(10.0, -dec + dec * j / (double)(nl - 1));
+                cg.push_back(q);
+                clam.push_back(la);
+                clam2.push_back(la * la);
+            }
+        }
+        const int nc = (int)cg.size();
+        std::vector<double> hterm(2 * (size_t)nc);
+        std::vector<int> hcinfo(nc, 0), hinfo(nq, 0);
+        if (nc > 0) {
+            IA_TRY(W.A.alloc(sizeof(cx<double>) * nc * ll));
+            IA_TRY(W.X.alloc(sizeof(cx<double>) * nc * lx));
+            IA_TRY(W.cpiv.alloc(sizeof(int) * nc * (size_t)l));
+            IA_TRY(W.cidx.alloc(2 * sizeof(int) * nc * (size_t)l));
+            IA_TRY(W.cinfo.alloc(sizeof(int) * nc));
+            IA_TRY(W.cg.alloc(sizeof(int) * nc));
+            IA_TRY(W.clam2.alloc(sizeof(double) * nc));
+            IA_TRY(W.cterm.alloc(2 * sizeof(double) * nc));
+            IA_HIP(hipMemcpyAsync(W.cg.p, cg.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
+            IA_HIP(hipMemcpyAsync(W.clam2.p, clam2.data(), sizeof(double) * nc,
+                                  hipMemcpyHostToDevice, s));
+            cx<double> *A = W.A.as<cx<double>>(), *X = W.X.as<cx<double>>();
+            hipLaunchKernelGGL(k_shift, dim3((unsigned)((ll + 255) / 256), nc), dim3(256), 0, s, l,
+                               G, ll, W.cg.as<int>(), W.clam2.as<double>(), A);
+            IA_HIP(hipGetLastError());
+            IA_HIP(hipMemsetAsync(W.cinfo.p, 0, sizeof(int) * nc, s));
+            IA_TRY(getrf_batched<double>(l, A, l, ll, W.cpiv.as<int>(), W.cinfo.as<int>(), nc, s));
+            IA_TRY(lu_perm_index(l, W.cpiv.as<int>(), W.cidx.as<int>(),
+                                 W.cidx.as<int>() + (size_t)l * nc, nc, s));
+            hipLaunchKernelGGL(k_rhs_perm, dim3((unsigned)((lx + 255) / 256), nc), dim3(256), 0, s,
+                               l, W.cidx.as<int>(), W.cg.as<int>(), bq, X);
+            IA_HIP(hipGetLastError());
+            IA_TRY(getrs_prepermuted<double>(l, l + 1, A, l, ll, X, l, lx, nc, s));
+            hipLaunchKernelGGL(k_gcv_terms, dim3(nc), dim3(256), 0, s, l, X, W.cterm.as<double>());
+            IA_HIP(hipGetLastError());
+            IA_HIP(hipMemcpyAsync(hterm.data(), W.cterm.p, 2 * sizeof(double) * nc,
+                                  hipMemcpyDeviceToHost, s));
+            IA_HIP(hipMemcpyAsync(hcinfo.data(), W.cinfo.p, sizeof(int) * nc,
+                                  hipMemcpyDeviceToHost, s));
+        }
+        IA_HIP(hipMemcpyAsync(hinfo.data(), info, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
+        IA_HIP(hipStreamSynchronize(s));
+        // ---- 5. decide (IA_RECHECK_LOG=1: one line per frequency on stderr)
+        const char *lg = std::getenv("IA_RECHECK_LOG");
+        const bool log = lg && *lg && *lg != '0';
+        const double eps = 2.220446049250313e-16;
+        std::vector<int> cq(nq, -1), jsel(nq, -1);
+        std::vector<double> gsel(nq, 0), lsel(nq, 0);
+        for (int q = 0; q < nq; ++q) {
+            if (!ok[q] || hinfo[q] != 0) {
+                if (log)
+                    std::fprintf(stderr,
+                                 "RECHECK f=%d full: %zu candidates, s1^2 %.17g (%d steps), info %d\n",
+                                 L[c0 + q], cand[q].size(), theta[q], hdone[q] ? hdone[q] : hns[q],
+                                 hinfo[q]);
+                full.push_back(L[c0 + q]);
+                continue;
+            }
+            const int c0q = cfirst[q], nk = (int)cand[q].size();
+            int best = -1, second = -1;
+            bool good = true;
+            std::vector<double> gv(nk), ev(nk);
+            for (int u = 0; u < nk; ++u) {
+                const int c = c0q + u;
+                const double T = hterm[2 * c], N = hterm[2 * c + 1];
+                gv[u] = N / (T * T);
+                ev[u] = 256 * eps * (theta[q] + clam2[c]) / clam2[c] + 1e-11;
+                good = good && hcinfo[c] == 0 && std::isfinite(gv[u]) && T > 0;
+                if (best < 0 || gv[u] < gv[best]) best = u;  // ties: the lowest index
+            }
+            for (int u = 0; u < nk; ++u)
+                if (u != best && (second < 0 || gv[u] < gv[second])) second = u;
+            const bool decided = good && best >= 0 && second >= 0 &&
+                                 gv[second] - gv[best] > 2 * (ev[best] + ev[second]) * gv[best];
+            if (log) {
+                std::fprintf(stderr, "RECHECK f=%d %s: s1^2 %.17g (%d steps)", L[c0 + q],
+                             decided ? "decided" : "full", theta[q], hdone[q] ? hdone[q] : hns[q]);
+                for (int u = 0; u < nk; ++u)
+                    std::fprintf(stderr, " j%d G %.17g err %.1e", cand[q][u], gv[u], ev[u]);
+                std::fprintf(stderr, "\n");
+            }
+            if (!decided) {
+                full.push_back(L[c0 + q]);
+                continue;
+            }
+            cq[q] = c0q + best;
+            jsel[q] = cand[q][best];
+            lsel[q] = clam[c0q + best];
+            gsel[q] = (gv[second] - gv[best]) / gv[best];
+        }
+        bool any = false;
+        for (int q = 0; q < nq; ++q) any = any || cq[q] >= 0;
+        if (!any) continue;
+        // ---- outputs: x = P^T Y y, q = D^-1 x, p_s = M_sq q, W
+        IA_TRY(W.cq.alloc(sizeof(int) * nq));
+        IA_TRY(W.yv.alloc(sizeof(cx<double>) * nq * (size_t)l));
+        IA_TRY(W.xY.alloc(sizeof(cx<double>) * nq * (size_t)m));
+        IA_TRY(W.x.alloc(sizeof(cx<double>) * nq * (size_t)m));
+        IA_TRY(W.qv.alloc(sizeof(cx<double>) * nq * (size_t)m));
+        IA_TRY(W.ps.alloc(sizeof(cx<double>) * nq * (size_t)m));
+        IA_TRY(W.pw.alloc(sizeof(double) * nq));
+        cx<double> *yv = W.yv.as<cx<double>>(), *xY = W.xY.as<cx<double>>(),
+                   *xv = W.x.as<cx<double>>(), *qv = W.qv.as<cx<double>>(),
+                   *ps = W.ps.as<cx<double>>();
+        IA_HIP(hipMemcpyAsync(W.cq.p, cq.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_pick_y, dim3((l + 255) / 256, nq), dim3(256), 0, s, l, W.cq.as<int>(),
+                           W.X.as<cx<double>>(), yv);
+        launch_gemm<double>('N', 'N', m, 1, l, cx<double>{1, 0}, Y, m, ml, yv, l, l,
+                            cx<double>{0, 0}, xY, m, m, nq, s);
+        launch_row_gather<double>(m, 1, pidx + (size_t)m * nq, xY, m, m, xv, m, m, nq, s);
+        IA_HIP(hipMemcpyAsync(qv, xv, sizeof(cx<double>) * nq * (size_t)m, hipMemcpyDeviceToDevice, s));
+        IA_TRY(getrs_batched<double>('N', m, 1, Dm, m, mm, ipiv, qv, m, m, nq, s));
+        launch_esm_msq_apply<double>(m, nq, kk, rho_c, r_s, src, qv, m, ps, m, s);
+        launch_power<double>(m, xv, m, ps, m, D.area / m, W.pw.as<double>(), nq, s);
+        IA_HIP(hipGetLastError());
+        std::vector<double> s1(nq);
+        const std::vector<int> zero(nq, 0);
+        for (int q = 0; q < nq; ++q) s1[q] = std::sqrt(theta[q]);
+        for (int q = 0; q < nq; ++q) {
+            if (cq[q] < 0) continue;
+            const int f = L[c0 + q];
+            IA_TRY(put1(o.lambda, f, lsel[q], s));
+            IA_TRY(put1(o.lambda_index, f, jsel[q], s));
+            IA_TRY(put1(o.sigma_max, f, s1[q], s));
+            IA_TRY(put1(o.selection_gap, f, gsel[q], s));
+            IA_TRY(put1(o.status, f, zero[q], s));
+            if (o.power)
+                IA_HIP(hipMemcpyAsync(o.power + f, W.pw.as<double>() + q, sizeof(double),
+                                      hipMemcpyDefault, s));
+            if (o.velocity)
+                IA_HIP(hipMemcpyAsync(o.velocity + (size_t)f * m, xv + (size_t)q * m,
+                                      sizeof(ia_z) * m, hipMemcpyDefault, s));
+            if (o.pressure)
+                IA_HIP(hipMemcpyAsync(o.pressure + (size_t)f * m, ps + (size_t)q * m,
+                                      sizeof(ia_z) * m, hipMemcpyDefault, s));
+        }
+        // the host values the copies read (lsel, jsel, s1, gsel, zero) live until here
+        IA_HIP(hipStreamSynchronize(s));
+    }
+    return IA_OK;
+}
+
+}  // namespace ia

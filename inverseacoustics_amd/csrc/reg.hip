@@ -131,7 +131,8 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                                                const double *__restrict__ cn2all, double floor_rel,
                                                RegDev prm, double *lam_out, int *idx_out,
                                                int *k_out, double *gap_out,
-                                               cx<double> *__restrict__ phiall) {
+                                               cx<double> *__restrict__ phiall,
+                                               double *__restrict__ curve_out) {
     extern __shared__ double sh[];
     double *s = sh;            // p
     double *ab2 = sh + p;      // |beta|^2, p
@@ -286,6 +287,7 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
                 if (lane == 0) {
                     Gv[j] = val;
                     Gi[j] = j;
+                    if (curve_out) curve_out[(size_t)b * nl + j] = val;
                 }
             }
             __syncthreads();
@@ -433,12 +435,12 @@ __global__ __launch_bounds__(256) void k_select(int p, int rows, const double *_
 void launch_select(int p, int rows, const double *sigma, const cx<double> *beta,
                    const double *bnorm2, RegDev prm, double *lam_out, int *idx_out, int *k_out,
                    cx<double> *phi, int batch, hipStream_t s, double *gap_out,
-                   const double *cnorm2, double floor_rel) {
+                   const double *cnorm2, double floor_rel, double *curve_out) {
     size_t lds = sizeof(double) * (4 * (size_t)p + 1);
     (void)hipFuncSetAttribute((const void *)k_select, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lds);
     hipLaunchKernelGGL(k_select, dim3(batch), dim3(256), lds, s, p, rows, sigma, beta, bnorm2,
-                       cnorm2, floor_rel, prm, lam_out, idx_out, k_out, gap_out, phi);
+                       cnorm2, floor_rel, prm, lam_out, idx_out, k_out, gap_out, phi, curve_out);
 }
 
 }  // namespace ia

@@ -150,3 +150,45 @@ def test_more_microphones_than_sources(ia, filter, select):
     assert np.max(np.abs(post.power_db(rm.power) - post.power_db(r.power))) < 0.1
     if filter != "tsvd":
         np.testing.assert_array_equal(rm.lam_index, r.lam_index)
+
+
+def test_gram_recheck_decides_near_ties_in_fp64(ia, monkeypatch):
+    """Certified mixed sweep, Tikhonov + GCV at 1024 x 2048: the nearly tied
+    frequencies (complex64 GCV gap below 2e-3 here, to catch many) are
+    decided from the Gram form of the GCV function in FP64 (LU of D, G =
+    H_e H_e^H, Lanczos for s_1, one LU per candidate lambda).  Each of them
+    takes the FP64 sweep's lambda index, lambda and s_1 within 1e-11, and
+    velocity and W within 1e-7 (the Gram form's error, eps cond(H H^H +
+    lambda^2 I), is ~1e-9 at the lambdas GCV picks here); most are decided
+    without the full FP64 sweep.  IA_RECHECK=full sends all of them to the
+    full sweep."""
+    P = sc.plate_scenario(freqs=np.linspace(200.0, 1500.0, 24))
+    r = _run(ia, P)
+    raw = _run(ia, P, precision="mixed", recheck_gap=-1.0)
+    tau = 2e-3
+    near = np.where(~(raw.selection_gap >= tau))[0]
+    print("rechecked:", near.tolist(), "complex64 gaps:", np.round(raw.selection_gap[near], 5).tolist())
+    assert len(near) >= 4
+    for mode in ("gram", "full"):
+        if mode == "full":
+            monkeypatch.setenv("IA_RECHECK", "full")
+        g = _run(ia, P, precision="mixed", recheck_gap=tau)
+        print(mode, {k: round(v, 2) for k, v in g.stats.items() if "recheck" in k})
+        assert g.stats["rechecked"] == len(near)
+        if mode == "gram":
+            assert g.stats["recheck_full"] <= len(near) // 4
+        else:
+            assert g.stats["recheck_full"] == len(near)
+        np.testing.assert_array_equal(g.lam_index, r.lam_index)
+        for f in near:
+            ev = np.linalg.norm(g.velocity[f] - r.velocity[f]) / np.linalg.norm(r.velocity[f])
+            print(f"  f={P.freqs[f]:.1f} Hz idx {g.lam_index[f]} gap {g.selection_gap[f]:.2e} "
+                  f"velocity {ev:.1e} W {abs(g.power[f] / r.power[f] - 1):.1e}")
+            assert g.lam[f] == pytest.approx(r.lam[f], rel=1e-11)
+            assert g.sigma_max[f] == pytest.approx(r.sigma_max[f], rel=1e-11)
+            assert ev <= 1e-7
+            assert g.power[f] == pytest.approx(r.power[f], rel=1e-7)
+            assert g.status[f] == 0
+        rest = np.setdiff1d(np.arange(len(P.k)), near)
+        np.testing.assert_array_equal(g.velocity[rest], raw.velocity[rest])
+        assert np.max(np.abs(post.power_db(g.power) - post.power_db(r.power))) < 0.1

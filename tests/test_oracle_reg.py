@@ -229,3 +229,37 @@ def test_esm_pipeline_dsvd_branch():
     U, s, Vh = sl.svd(H, full_matrices=False, lapack_driver="gesvd")
     x = Vh.conj().T @ ((U.conj().T @ S.b[0]) / (s + o["lam"]))
     assert np.linalg.norm(o["x"] - x) <= 1e-9 * np.linalg.norm(x)
+
+
+def test_gcv_gram_form_equals_svd_gcv():
+    """The Gram form the certified sweep decides near ties with: for H (l x m,
+    l < m), A = H H^H + lam^2 I, the Tikhonov GCV function equals
+    |A^-1 b|^2 / tr(A^-1)^2 and the Tikhonov solution H^H A^-1 b, at every
+    grid point (against the SVD form of gcv_tikhonov_grid and tikhonov, from
+    LAPACK's SVD), within the Gram form's error model 64 eps cond(A),
+    cond(A) = (s_1^2 + lam^2) / lam^2 (a dropped lam^2 factor or a missing
+    trace term fails by O(1))."""
+    rng = np.random.default_rng(7)
+    l, m = 40, 70
+    U0 = np.linalg.qr(rng.standard_normal((l, l)) + 1j * rng.standard_normal((l, l)))[0]
+    V0 = np.linalg.qr(rng.standard_normal((m, l)) + 1j * rng.standard_normal((m, l)))[0]
+    H = (U0 * np.logspace(0, -6, l)) @ V0.conj().T
+    b = rng.standard_normal(l) + 1j * rng.standard_normal(l)
+    U, s, V = rt.csvd(H)
+    beta = U.conj().T @ b
+    j, lam, G = rt.gcv_tikhonov_grid(s, beta, l)
+    Gg = np.empty_like(G)
+    checked = 0
+    for i, la in enumerate(lam):
+        A = H @ H.conj().T + la * la * np.eye(l)
+        X = np.linalg.solve(A, np.column_stack([np.eye(l), b]))
+        Gg[i] = np.sum(np.abs(X[:, l]) ** 2) / np.trace(X[:, :l]).real ** 2
+        tol = 64 * np.finfo(float).eps * (s[0] ** 2 + la * la) / (la * la)
+        if tol < 1e-3:
+            checked += 1
+            assert abs(Gg[i] / G[i] - 1) <= tol, (i, Gg[i], G[i], tol)
+            x = H.conj().T @ X[:, l]
+            xt = rt.tikhonov(U, s, V, b, la)
+            assert np.linalg.norm(x - xt) <= tol * np.linalg.norm(xt), i
+    assert checked >= 25
+    assert int(np.argmin(Gg)) == j
