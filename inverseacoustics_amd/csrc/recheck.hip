@@ -46,11 +46,12 @@ namespace ia {
 
 namespace {
 
-constexpr int LZ_T = 1024;                 // threads of a Lanczos workgroup
-constexpr int LZ_NMAX = 4096;              // rows it serves (q in LDS)
+constexpr int LZ_T = 1024;                 // threads of the Lanczos update
+constexpr int LZ_NMAX = 4096;              // rows served (q in LDS)
 constexpr int LZ_RPT = LZ_NMAX / LZ_T;     // rows per thread at most
 constexpr int LZ_KMAX = 192;               // Lanczos steps at most
-constexpr int LZ_ROUND = 32;               // steps per launch
+constexpr int LZ_FIRST = 24;               // steps before the first check
+constexpr int LZ_MORE = 8;                 // steps per later check
 constexpr int MAX_CAND = 6;                // grid candidates per frequency
 constexpr int RC_CHUNK = 32;               // frequencies per device batch
 
@@ -85,123 +86,127 @@ __device__ __forceinline__ double hash_unit(unsigned i, unsigned c) {
     return (double)h * (1.0 / 4294967296.0) - 0.5;
 }
 
-// Lanczos on the Hermitian G (n x n column-major, n <= LZ_NMAX), `steps`
-// more steps from each active matrix's saved state (q_j, q_{j-1},
-// beta_{j-1}, j).  Step j: w = G q_j - beta_{j-1} q_{j-1}, alpha_j =
-// Re(q_j^H w), w -= alpha_j q_j, beta_j = |w|, q_{j+1} = w / beta_j (no
-// reorthogonalisation: lost orthogonality only repeats converged Ritz
-// values, the largest still converges to s_1^2).  One workgroup per matrix;
-// thread t owns rows t, t + 1024, ...: (G q)_i = sum_r G[i + r n] q_r reads,
-// per r, 64 consecutive entries of column r per wave, q_r broadcast from
-// LDS.  A beta_j that is not above 1e-13 (|alpha_j| + beta_{j-1}) ends the
-// recurrence (invariant subspace): done[b] = j + 1.
-__global__ __launch_bounds__(LZ_T) void k_lanczos(int n, const cx<double> *__restrict__ Gall,
-                                                 size_t sG, cx<double> *__restrict__ qall,
-                                                 cx<double> *__restrict__ qpall,
-                                                 double *__restrict__ bprev, int *__restrict__ nstep,
-                                                 int *__restrict__ done,
-                                                 const int *__restrict__ active, int steps,
-                                                 double *__restrict__ alpha,
-                                                 double *__restrict__ beta) {
-    extern __shared__ double lz_sh[];
-    double *part = lz_sh;                                       // 16
-    cx<double> *qs = reinterpret_cast<cx<double> *>(lz_sh + 16);  // n
+// Lanczos on the Hermitian G (n x n column-major), one step per pair of
+// launches (k_lz_gemv, k_lz_update) for every active matrix: step j takes
+// w = G q_j - beta_{j-1} q_{j-1}, alpha_j = Re(q_j^H w), w -= alpha_j q_j,
+// beta_j = |w|, q_{j+1} = w / beta_j (no reorthogonalisation: lost
+// orthogonality only repeats converged Ritz values, the largest still
+// converges to s_1^2).  State per matrix: q, q_prev, w (n each), beta_prev,
+// steps done; a beta_j that is not above 1e-13 (|alpha_j| + beta_{j-1})
+// ends the recurrence (invariant subspace): done[b] = j + 1.
+
+// q_0: fixed pseudo-random entries, normalised; q_prev = 0
+__global__ __launch_bounds__(LZ_T) void k_lz_start(int n, cx<double> *__restrict__ qall,
+                                                  cx<double> *__restrict__ qpall,
+                                                  double *__restrict__ bprev,
+                                                  int *__restrict__ nstep, int *__restrict__ done) {
+    __shared__ double part[LZ_T / 64];
     const int b = blockIdx.x, t = threadIdx.x;
+    cx<double> *q = qall + (size_t)b * n, *qp = qpall + (size_t)b * n;
+    double ss = 0;
+    for (int i = t; i < n; i += LZ_T) {
+        const cx<double> v{hash_unit(i, 1), hash_unit(i, 2)};
+        ss += abs2(v);
+    }
+    const double sc = 1.0 / sqrt(block_sum(ss, part));
+    for (int i = t; i < n; i += LZ_T) {
+        q[i] = cx<double>{sc * hash_unit(i, 1), sc * hash_unit(i, 2)};
+        qp[i] = cx<double>{0, 0};
+    }
+    if (t == 0) {
+        bprev[b] = 0;
+        nstep[b] = 0;
+        done[b] = 0;
+    }
+}
+
+// w = G q for 64 rows per workgroup: wave v takes the columns r = v mod 4,
+// lane = row (64 consecutive entries of a column per load), q in LDS; the
+// four partial sums are added in LDS in a fixed order.
+__global__ __launch_bounds__(256) void k_lz_gemv(int n, const cx<double> *__restrict__ Gall,
+                                                size_t sG, const cx<double> *__restrict__ qall,
+                                                cx<double> *__restrict__ wall,
+                                                const int *__restrict__ active,
+                                                const int *__restrict__ done) {
+    extern __shared__ double lz_sh[];
+    cx<double> *qs = reinterpret_cast<cx<double> *>(lz_sh);  // n
+    cx<double> *ps = qs + n;                                  // 4 x 64
+    const int b = blockIdx.y, t = threadIdx.x, v = t >> 6, lane = t & 63;
     if (!active[b] || done[b]) return;
-    const cx<double> *G = Gall + (size_t)b * sG;
-    cx<double> *qg = qall + (size_t)b * n, *qpg = qpall + (size_t)b * n;
-    int j = nstep[b];
-    double bp = bprev[b];
-    cx<double> q[LZ_RPT], qp[LZ_RPT], w[LZ_RPT];
-    if (j == 0) {
-        double ss = 0;
-#pragma unroll
-        for (int u = 0; u < LZ_RPT; ++u) {
-            const int i = t + u * LZ_T;
-            q[u] = i < n ? cx<double>{hash_unit(i, 1), hash_unit(i, 2)} : cx<double>{0, 0};
-            qp[u] = cx<double>{0, 0};
-            ss += abs2(q[u]);
+    const cx<double> *q = qall + (size_t)b * n;
+    for (int i = t; i < n; i += 256) qs[i] = q[i];
+    __syncthreads();
+    const int i = blockIdx.x * 64 + lane;
+    cx<double> a0{0, 0}, a1{0, 0};
+    if (i < n) {
+        const cx<double> *gi = Gall + (size_t)b * sG + i;
+        int r = v;
+        for (; r + 4 < n; r += 8) {
+            const cx<double> g0 = gi[(size_t)r * n], g1 = gi[(size_t)(r + 4) * n];
+            a0 = a0 + g0 * qs[r];
+            a1 = a1 + g1 * qs[r + 4];
         }
-        const double sc = 1.0 / sqrt(block_sum(ss, part));
-#pragma unroll
-        for (int u = 0; u < LZ_RPT; ++u) q[u] = sc * q[u];
-        bp = 0;
-    } else {
-#pragma unroll
-        for (int u = 0; u < LZ_RPT; ++u) {
-            const int i = t + u * LZ_T;
-            q[u] = i < n ? qg[i] : cx<double>{0, 0};
-            qp[u] = i < n ? qpg[i] : cx<double>{0, 0};
-        }
+        for (; r < n; r += 4) a0 = a0 + gi[(size_t)r * n] * qs[r];
     }
-    const int jend = min(j + steps, LZ_KMAX);
-    for (; j < jend; ++j) {
+    ps[v * 64 + lane] = a0 + a1;
+    __syncthreads();
+    if (v == 0 && i < n)
+        wall[(size_t)b * n + i] = (ps[lane] + ps[64 + lane]) + (ps[128 + lane] + ps[192 + lane]);
+}
+
+__global__ __launch_bounds__(LZ_T) void k_lz_update(int n, cx<double> *__restrict__ qall,
+                                                   cx<double> *__restrict__ qpall,
+                                                   const cx<double> *__restrict__ wall,
+                                                   double *__restrict__ bprev,
+                                                   int *__restrict__ nstep, int *__restrict__ done,
+                                                   const int *__restrict__ active,
+                                                   double *__restrict__ alpha,
+                                                   double *__restrict__ beta) {
+    __shared__ double part[LZ_T / 64];
+    const int b = blockIdx.x, t = threadIdx.x;
+    if (!active[b] || done[b] || nstep[b] >= LZ_KMAX) return;
+    cx<double> *q = qall + (size_t)b * n, *qp = qpall + (size_t)b * n;
+    const cx<double> *w = wall + (size_t)b * n;
+    const int j = nstep[b];
+    const double bp = bprev[b];
+    cx<double> qv[LZ_RPT], wv[LZ_RPT], pv[LZ_RPT];
+    double a = 0;
 #pragma unroll
-        for (int u = 0; u < LZ_RPT; ++u) {
-            const int i = t + u * LZ_T;
-            if (i < n) qs[i] = q[u];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < LZ_RPT; ++u) {
-            w[u] = cx<double>{0, 0};
-            const int i = t + u * LZ_T;
-            if (i >= n) continue;
-            const cx<double> *gi = G + i;
-            cx<double> a0{0, 0}, a1{0, 0}, a2{0, 0}, a3{0, 0};
-            int r = 0;
-            for (; r + 4 <= n; r += 4) {
-                const cx<double> g0 = gi[(size_t)r * n], g1 = gi[(size_t)(r + 1) * n],
-                                 g2 = gi[(size_t)(r + 2) * n], g3 = gi[(size_t)(r + 3) * n];
-                a0 = a0 + g0 * qs[r];
-                a1 = a1 + g1 * qs[r + 1];
-                a2 = a2 + g2 * qs[r + 2];
-                a3 = a3 + g3 * qs[r + 3];
-            }
-            for (; r < n; ++r) a0 = a0 + gi[(size_t)r * n] * qs[r];
-            w[u] = (a0 + a1) + (a2 + a3);
-        }
-        double a = 0;
-#pragma unroll
-        for (int u = 0; u < LZ_RPT; ++u) a += q[u].re * w[u].re + q[u].im * w[u].im;
-        const double al = block_sum(a, part);
-        double ss = 0;
-#pragma unroll
-        for (int u = 0; u < LZ_RPT; ++u) {
-            w[u] = w[u] - al * q[u] - bp * qp[u];
-            ss += abs2(w[u]);
-        }
-        const double be = sqrt(block_sum(ss, part));
-        if (t == 0) {
-            alpha[(size_t)b * LZ_KMAX + j] = al;
-            beta[(size_t)b * LZ_KMAX + j] = be;
-        }
-        if (!(be > 1e-13 * (fabs(al) + bp))) {
-            if (t == 0) done[b] = j + 1;
-            ++j;
-            break;
-        }
-        const double sc = 1.0 / be;
-#pragma unroll
-        for (int u = 0; u < LZ_RPT; ++u) {
-            qp[u] = q[u];
-            q[u] = sc * w[u];
-        }
-        bp = be;
-        __syncthreads();  // every read of qs done before the next step writes it
+    for (int u = 0; u < LZ_RPT; ++u) {
+        const int i = t + u * LZ_T;
+        qv[u] = i < n ? q[i] : cx<double>{0, 0};
+        wv[u] = i < n ? w[i] : cx<double>{0, 0};
+        pv[u] = i < n ? qp[i] : cx<double>{0, 0};
+        a += qv[u].re * wv[u].re + qv[u].im * wv[u].im;
     }
+    const double al = block_sum(a, part);
+    double ss = 0;
+#pragma unroll
+    for (int u = 0; u < LZ_RPT; ++u) {
+        wv[u] = wv[u] - al * qv[u] - bp * pv[u];
+        ss += abs2(wv[u]);
+    }
+    const double be = sqrt(block_sum(ss, part));
+    __syncthreads();  // every thread has read nstep / done / bprev before thread 0 writes them
+    if (t == 0) {
+        alpha[(size_t)b * LZ_KMAX + j] = al;
+        beta[(size_t)b * LZ_KMAX + j] = be;
+        nstep[b] = j + 1;
+    }
+    if (!(be > 1e-13 * (fabs(al) + bp))) {
+        if (t == 0) done[b] = j + 1;
+        return;
+    }
+    const double sc = 1.0 / be;
 #pragma unroll
     for (int u = 0; u < LZ_RPT; ++u) {
         const int i = t + u * LZ_T;
         if (i < n) {
-            qg[i] = q[u];
-            qpg[i] = qp[u];
+            qp[i] = qv[u];
+            q[i] = sc * wv[u];
         }
     }
-    if (t == 0) {
-        nstep[b] = j;
-        bprev[b] = bp;
-    }
+    if (t == 0) bprev[b] = be;
 }
 
 // A_c = G_{g[c]} + lam2[c] I (n x n per candidate)
@@ -305,10 +310,10 @@ double tri_top(const double *a, const double *b, int k) {
 
 // Device workspace of the recheck, kept per host thread (buffers only grow).
 struct RecheckCache {
-    DevBuf kk, bq, Dm, Y, ipiv, pidx, info, G, lq, lqp, lbp, lns, ldone, lact, lal, lbe, A, X,
+    DevBuf kk, bq, Dm, Y, ipiv, pidx, info, G, lq, lqp, lw, lbp, lns, ldone, lact, lal, lbe, A, X,
         cpiv, cidx, cinfo, cg, clam2, cterm, cq, yv, xY, x, qv, ps, pw;
     void release() {
-        for (DevBuf *p : {&kk, &bq, &Dm, &Y, &ipiv, &pidx, &info, &G, &lq, &lqp, &lbp, &lns, &ldone,
+        for (DevBuf *p : {&kk, &bq, &Dm, &Y, &ipiv, &pidx, &info, &G, &lq, &lqp, &lw, &lbp, &lns, &ldone,
                           &lact, &lal, &lbe, &A, &X, &cpiv, &cidx, &cinfo, &cg, &clam2, &cterm, &cq,
                           &yv, &xY, &x, &qv, &ps, &pw})
             p->reset();
@@ -351,27 +356,50 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
     RecheckCache &W = recheck_cache();
     const size_t mm = (size_t)m * m, ml = (size_t)m * l, ll = (size_t)l * l,
                  lx = (size_t)l * (l + 1);
-    const size_t lz_lds = sizeof(double) * 16 + sizeof(cx<double>) * (size_t)l;
-    (void)hipFuncSetAttribute((const void *)k_lanczos, hipFuncAttributeMaxDynamicSharedMemorySize,
+    const size_t lz_lds = sizeof(cx<double>) * ((size_t)l + 256);
+    (void)hipFuncSetAttribute((const void *)k_lz_gemv, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)lz_lds);
-    for (size_t c0 = 0; c0 < L.size(); c0 += RC_CHUNK) {
-        const int nq = (int)std::min<size_t>(RC_CHUNK, L.size() - c0);
-        // ---- candidates of each frequency from its complex64 GCV curve
-        std::vector<std::vector<int>> cand(nq);
-        std::vector<char> ok(nq, 1);
-        for (int q = 0; q < nq; ++q) {
-            const double *cv = curve.data() + (c0 + q) * (size_t)nl;
-            double cmin = INFINITY;
-            bool fin = true;
-            for (int j = 0; j < nl; ++j) {
-                fin = fin && std::isfinite(cv[j]);
-                cmin = std::min(cmin, cv[j]);
-            }
-            if (fin && cmin > 0)
-                for (int j = 0; j < nl; ++j)
-                    if ((cv[j] - cmin) / cmin < tau) cand[q].push_back(j);
-            if (!fin || !(cmin > 0) || cand[q].size() < 2 || cand[q].size() > MAX_CAND) ok[q] = 0;
+    const char *lg = std::getenv("IA_RECHECK_LOG");
+    const bool log = lg && *lg && *lg != '0';
+    // ---- candidates of each frequency from its complex64 GCV curve; the
+    // frequencies with too many (a flat GCV minimum) go to the full sweep
+    std::vector<int> L2;
+    std::vector<std::vector<int>> cand2;
+    for (size_t q = 0; q < L.size(); ++q) {
+        const double *cv = curve.data() + q * (size_t)nl;
+        double cmin = INFINITY;
+        bool fin = true;
+        for (int j = 0; j < nl; ++j) {
+            fin = fin && std::isfinite(cv[j]);
+            cmin = std::min(cmin, cv[j]);
         }
+        std::vector<int> c;
+        if (fin && cmin > 0)
+            for (int j = 0; j < nl; ++j)
+                if ((cv[j] - cmin) / cmin < tau) c.push_back(j);
+        if (!fin || !(cmin > 0) || c.size() < 2 || c.size() > MAX_CAND) {
+            if (log)
+                std::fprintf(stderr, "RECHECK f=%d full: %zu candidates\n", L[q], c.size());
+            full.push_back(L[q]);
+        } else {
+            L2.push_back(L[q]);
+            cand2.push_back(std::move(c));
+        }
+    }
+    hipEvent_t ev[7];
+    for (auto &e : ev) IA_HIP(hipEventCreate(&e));
+    struct EvFree {
+        hipEvent_t *e;
+        ~EvFree() {
+            for (int i = 0; i < 7; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } ev_free{ev};
+    for (size_t c0 = 0; c0 < L2.size(); c0 += RC_CHUNK) {
+        const int nq = (int)std::min<size_t>(RC_CHUNK, L2.size() - c0);
+        const int *Lc = L2.data() + c0;
+        const std::vector<int> *cand = cand2.data() + c0;
+        std::vector<char> ok(nq, 1);
+        IA_HIP(hipEventRecord(ev[0], s));
         // ---- 1. Y = P H_e^H in FP64, 2. G = Y^H Y
         IA_TRY(W.kk.alloc(sizeof(double) * nq));
         IA_TRY(W.bq.alloc(sizeof(cx<double>) * nq * (size_t)l));
@@ -386,7 +414,7 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
                    *Y = W.Y.as<cx<double>>(), *G = W.G.as<cx<double>>();
         int *ipiv = W.ipiv.as<int>(), *pidx = W.pidx.as<int>(), *info = W.info.as<int>();
         for (int q = 0; q < nq; ++q) {
-            const int f = L[c0 + q];
+            const int f = Lc[q];
             IA_HIP(hipMemcpyAsync(kk + q, k + f, sizeof(double), hipMemcpyDefault, s));
             IA_HIP(hipMemcpyAsync(bq + (size_t)q * l, b + (size_t)f * l, sizeof(cx<double>) * l,
                                   hipMemcpyDefault, s));
@@ -398,28 +426,38 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
         IA_TRY(getrf_batched<double>(m, Dm, m, mm, ipiv, info, nq, s));
         IA_TRY(getrs_c_permuted<double>(m, l, Dm, m, mm, Y, m, ml, nq, s));
         IA_TRY(lu_perm_index(m, ipiv, pidx, pidx + (size_t)m * nq, nq, s));
+        IA_HIP(hipEventRecord(ev[1], s));
         launch_gemm<double>('C', 'N', l, l, m, cx<double>{1, 0}, Y, m, ml, Y, m, ml,
                             cx<double>{0, 0}, G, l, ll, nq, s);
         IA_HIP(hipGetLastError());
-        // ---- 3. s_1^2 by Lanczos, LZ_ROUND steps per launch until settled
+        IA_HIP(hipEventRecord(ev[2], s));
+        // ---- 3. s_1^2 by Lanczos: LZ_FIRST steps, then LZ_MORE at a time
+        // for the matrices whose largest Ritz value has not settled
         IA_TRY(W.lq.alloc(sizeof(cx<double>) * nq * (size_t)l));
         IA_TRY(W.lqp.alloc(sizeof(cx<double>) * nq * (size_t)l));
+        IA_TRY(W.lw.alloc(sizeof(cx<double>) * nq * (size_t)l));
         IA_TRY(W.lbp.alloc(sizeof(double) * nq));
         IA_TRY(W.lns.alloc(sizeof(int) * nq));
         IA_TRY(W.ldone.alloc(sizeof(int) * nq));
         IA_TRY(W.lact.alloc(sizeof(int) * nq));
         IA_TRY(W.lal.alloc(sizeof(double) * nq * LZ_KMAX));
         IA_TRY(W.lbe.alloc(sizeof(double) * nq * LZ_KMAX));
-        IA_HIP(hipMemsetAsync(W.lns.p, 0, sizeof(int) * nq, s));
-        IA_HIP(hipMemsetAsync(W.ldone.p, 0, sizeof(int) * nq, s));
+        cx<double> *lq = W.lq.as<cx<double>>(), *lqp = W.lqp.as<cx<double>>(),
+                   *lw = W.lw.as<cx<double>>();
+        hipLaunchKernelGGL(k_lz_start, dim3(nq), dim3(LZ_T), 0, s, l, lq, lqp, W.lbp.as<double>(),
+                           W.lns.as<int>(), W.ldone.as<int>());
+        IA_HIP(hipGetLastError());
         std::vector<int> act(nq, 1), hns(nq), hdone(nq);
         std::vector<double> theta(nq, NAN), hal((size_t)nq * LZ_KMAX), hbe((size_t)nq * LZ_KMAX);
-        for (int round = 0; round * LZ_ROUND < LZ_KMAX; ++round) {
+        for (int steps = LZ_FIRST;;) {
             IA_HIP(hipMemcpyAsync(W.lact.p, act.data(), sizeof(int) * nq, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_lanczos, dim3(nq), dim3(LZ_T), lz_lds, s, l, G, ll,
-                               W.lq.as<cx<double>>(), W.lqp.as<cx<double>>(), W.lbp.as<double>(),
-                               W.lns.as<int>(), W.ldone.as<int>(), W.lact.as<int>(), LZ_ROUND,
-                               W.lal.as<double>(), W.lbe.as<double>());
+            for (int st = 0; st < steps; ++st) {
+                hipLaunchKernelGGL(k_lz_gemv, dim3((l + 63) / 64, nq), dim3(256), lz_lds, s, l, G, ll,
+                                   lq, lw, W.lact.as<int>(), W.ldone.as<int>());
+                hipLaunchKernelGGL(k_lz_update, dim3(nq), dim3(LZ_T), 0, s, l, lq, lqp, lw,
+                                   W.lbp.as<double>(), W.lns.as<int>(), W.ldone.as<int>(),
+                                   W.lact.as<int>(), W.lal.as<double>(), W.lbe.as<double>());
+            }
             IA_HIP(hipGetLastError());
             IA_HIP(hipMemcpyAsync(hns.data(), W.lns.p, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
             IA_HIP(hipMemcpyAsync(hdone.data(), W.ldone.p, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
@@ -445,9 +483,11 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
                 }
             }
             if (!more) break;
+            steps = LZ_MORE;
         }
         for (int q = 0; q < nq; ++q)
             if (!(theta[q] > 0) || !std::isfinite(theta[q])) ok[q] = 0;
+        IA_HIP(hipEventRecord(ev[3], s));
         // ---- 4. the candidates' GCV values in FP64
         std::vector<int> cg;
         std::vector<double> clam2, clam;
@@ -484,12 +524,14 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
             IA_HIP(hipGetLastError());
             IA_HIP(hipMemsetAsync(W.cinfo.p, 0, sizeof(int) * nc, s));
             IA_TRY(getrf_batched<double>(l, A, l, ll, W.cpiv.as<int>(), W.cinfo.as<int>(), nc, s));
+            IA_HIP(hipEventRecord(ev[4], s));
             IA_TRY(lu_perm_index(l, W.cpiv.as<int>(), W.cidx.as<int>(),
                                  W.cidx.as<int>() + (size_t)l * nc, nc, s));
             hipLaunchKernelGGL(k_rhs_perm, dim3((unsigned)((lx + 255) / 256), nc), dim3(256), 0, s,
                                l, W.cidx.as<int>(), W.cg.as<int>(), bq, X);
             IA_HIP(hipGetLastError());
             IA_TRY(getrs_prepermuted<double>(l, l + 1, A, l, ll, X, l, lx, nc, s));
+            IA_HIP(hipEventRecord(ev[5], s));
             hipLaunchKernelGGL(k_gcv_terms, dim3(nc), dim3(256), 0, s, l, X, W.cterm.as<double>());
             IA_HIP(hipGetLastError());
             IA_HIP(hipMemcpyAsync(hterm.data(), W.cterm.p, 2 * sizeof(double) * nc,
@@ -499,9 +541,16 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
         }
         IA_HIP(hipMemcpyAsync(hinfo.data(), info, sizeof(int) * nq, hipMemcpyDeviceToHost, s));
         IA_HIP(hipStreamSynchronize(s));
-        // ---- 5. decide (IA_RECHECK_LOG=1: one line per frequency on stderr)
-        const char *lg = std::getenv("IA_RECHECK_LOG");
-        const bool log = lg && *lg && *lg != '0';
+        // ---- 5. decide (IA_RECHECK_LOG=1: one line per frequency and the
+        // phase times on stderr)
+        if (log && nc > 0) {
+            float t[5];
+            for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&t[i], ev[i], ev[i + 1]);
+            std::fprintf(stderr,
+                         "RECHECK %d frequencies, %d candidates: H_e %.2f ms, G %.2f, Lanczos "
+                         "%.2f, candidate LU %.2f, solves %.2f\n",
+                         nq, nc, t[0], t[1], t[2], t[3], t[4]);
+        }
         const double eps = 2.220446049250313e-16;
         std::vector<int> cq(nq, -1), jsel(nq, -1);
         std::vector<double> gsel(nq, 0), lsel(nq, 0);
@@ -510,36 +559,36 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
                 if (log)
                     std::fprintf(stderr,
                                  "RECHECK f=%d full: %zu candidates, s1^2 %.17g (%d steps), info %d\n",
-                                 L[c0 + q], cand[q].size(), theta[q], hdone[q] ? hdone[q] : hns[q],
+                                 Lc[q], cand[q].size(), theta[q], hdone[q] ? hdone[q] : hns[q],
                                  hinfo[q]);
-                full.push_back(L[c0 + q]);
+                full.push_back(Lc[q]);
                 continue;
             }
             const int c0q = cfirst[q], nk = (int)cand[q].size();
             int best = -1, second = -1;
             bool good = true;
-            std::vector<double> gv(nk), ev(nk);
+            std::vector<double> gv(nk), er(nk);
             for (int u = 0; u < nk; ++u) {
                 const int c = c0q + u;
                 const double T = hterm[2 * c], N = hterm[2 * c + 1];
                 gv[u] = N / (T * T);
-                ev[u] = 256 * eps * (theta[q] + clam2[c]) / clam2[c] + 1e-11;
+                er[u] = 256 * eps * (theta[q] + clam2[c]) / clam2[c] + 1e-11;
                 good = good && hcinfo[c] == 0 && std::isfinite(gv[u]) && T > 0;
                 if (best < 0 || gv[u] < gv[best]) best = u;  // ties: the lowest index
             }
             for (int u = 0; u < nk; ++u)
                 if (u != best && (second < 0 || gv[u] < gv[second])) second = u;
             const bool decided = good && best >= 0 && second >= 0 &&
-                                 gv[second] - gv[best] > 2 * (ev[best] + ev[second]) * gv[best];
+                                 gv[second] - gv[best] > 2 * (er[best] + er[second]) * gv[best];
             if (log) {
-                std::fprintf(stderr, "RECHECK f=%d %s: s1^2 %.17g (%d steps)", L[c0 + q],
+                std::fprintf(stderr, "RECHECK f=%d %s: s1^2 %.17g (%d steps)", Lc[q],
                              decided ? "decided" : "full", theta[q], hdone[q] ? hdone[q] : hns[q]);
                 for (int u = 0; u < nk; ++u)
-                    std::fprintf(stderr, " j%d G %.17g err %.1e", cand[q][u], gv[u], ev[u]);
+                    std::fprintf(stderr, " j%d G %.17g err %.1e", cand[q][u], gv[u], er[u]);
                 std::fprintf(stderr, "\n");
             }
             if (!decided) {
-                full.push_back(L[c0 + q]);
+                full.push_back(Lc[q]);
                 continue;
             }
             cq[q] = c0q + best;
@@ -577,7 +626,7 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
         for (int q = 0; q < nq; ++q) s1[q] = std::sqrt(theta[q]);
         for (int q = 0; q < nq; ++q) {
             if (cq[q] < 0) continue;
-            const int f = L[c0 + q];
+            const int f = Lc[q];
             IA_TRY(put1(o.lambda, f, lsel[q], s));
             IA_TRY(put1(o.lambda_index, f, jsel[q], s));
             IA_TRY(put1(o.sigma_max, f, s1[q], s));
