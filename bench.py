@@ -39,6 +39,8 @@ def _parse():
                     help="skip the IBEM / HELS / radiation-mode / gradient throughputs")
     ap.add_argument("--only", choices=("both", "fp64", "mixed", "certified"), default="both",
                     help="profile runs: time one form only")
+    ap.add_argument("--workload", choices=("ibem", "hels", "radiation", "gradient"), default=None,
+                    help="profile runs: time this one of the other workloads only")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
@@ -82,6 +84,9 @@ def main():
     import inverseacoustics_amd as ia
     from inverseacoustics_amd import _lib, scenarios as sc
 
+    if a.workload:
+        print(json.dumps({"profile_run": a.workload, **other_workloads(ia, sc, (a.workload,))}))
+        return
     dev = torch.device("cuda:0")
     L = _lib.lib()
     t0 = time.time()
@@ -337,7 +342,7 @@ def cpu_baseline(P, nf, o_fp64):
                                   "velocity_rel_l2_max": max(v_err)}}
 
 
-def other_workloads(ia, sc):
+def other_workloads(ia, sc, which=("ibem", "hels", "radiation", "gradient")):
     """Throughput of the other solves at their full sizes (host inputs and
     outputs, one warm-up call then one timed call each)."""
     import torch
@@ -352,31 +357,36 @@ def other_workloads(ia, sc):
         torch.cuda.synchronize()
         return time.perf_counter() - t
 
-    S = sc.ibem_scenario()
-    ia.set_air(S.c, S.rho)
-    ia.set_quadrature()
-    for prec in ("fp64", "mixed"):
+    if "ibem" in which:
+        S = sc.ibem_scenario()
+        ia.set_air(S.c, S.rho)
+        ia.set_quadrature()
+    for prec in ("fp64", "mixed") if "ibem" in which else ():
         dt = clock(lambda: ia.ibem_sweep(S.k, S.X, S.E, S.r_h, S.b, S.area, precision=prec))
         out[f"ibem_{prec}"] = {"workload": "IBEM icosphere(3) 642 nodes, 128 mics, 64 frequencies "
                                            "100-2000 Hz, 7-point rule",
                                "frequencies_per_s": round(len(S.k) / dt, 2), "s": round(dt, 4)}
-    H = sc.hels_scenario()
-    ia.set_air(H.c, H.rho)
-    for prec in ("fp64", "mixed"):
+    if "hels" in which:
+        H = sc.hels_scenario()
+        ia.set_air(H.c, H.rho)
+    for prec in ("fp64", "mixed") if "hels" in which else ():
         dt = clock(lambda: ia.hels_sweep(H.k, H.order, H.azelr_s, H.azelr_h, H.b, H.area,
                                          precision=prec))
         out[f"hels_{prec}"] = {"workload": "HELS 256 mics, 256 surface points, 128 frequencies "
                                            "50-3000 Hz, N = ceil(ka) + 4 (5..11)",
                                "frequencies_per_s": round(len(H.k) / dt, 2), "s": round(dt, 4)}
-    R = sc.radiation_scenario(device="cuda")
-    ia.set_air(R.c, R.rho)
-    for prec in ("fp64", "mixed"):
+    if "radiation" in which:
+        R = sc.radiation_scenario(device="cuda")
+        ia.set_air(R.c, R.rho)
+    for prec in ("fp64", "mixed") if "radiation" in which else ():
         dt = clock(lambda: ia.radiation_sweep(R.k, R.r_p, R.patch_area, R.r_h, R.b, 128,
                                               precision=prec))
         out[f"radiation_{prec}"] = {
             "workload": "radiation modes, 64 x 64 patches (4096), 1024 mics, 256 frequencies "
                         "50-2500 Hz, 128 modes, FP64 eigendecomposition",
             "frequencies_per_s": round(len(R.k) / dt, 2), "s": round(dt, 4)}
+    if "gradient" not in which:
+        return out
     G = sc.plate_scenario(n_freq=32, device="cuda")
     ia.set_air(G.c, G.rho)
     dt = clock(lambda: ia.esm_gradient(G.k, G.r_s, G.normals, 0.02, G.r_h, G.b, G.area))
