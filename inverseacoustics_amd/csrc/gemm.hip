@@ -29,9 +29,14 @@
 // exactly into three bf16 parts and six partial products are accumulated in
 // FP32, which keeps FP32-level products (see k_cgemm_x6 below).
 
+#include <dlfcn.h>
+
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <mutex>
+#include <string>
+#include <unordered_map>
 
 #include "ia_internal.hpp"
 
@@ -471,44 +476,136 @@ void launch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, siz
                            K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, tm, tn, batch);
 }
 
-// Tile choice (measured on the sweep's products, batch 256): 64 x 64 tiles
-// with the 3M product; 32-row (column) tiles when op(A) has at most 32 rows
-// (C at most 32 columns), and 32 x 32 tiles when 64 x 64 ones would leave
-// fewer than four workgroups per CU (small batches).  Four 64 x 64
-// workgroups share a CU: their barriers, epilogues and global-load waits
-// overlap each other's MFMAs, which larger tiles at lower occupancy
-// measured slower at.
+// Kernel candidates of one product: the bf16 split kernel (complex64 only)
+// and the 3M kernel with 32 x 32, 32 x 64, 64 x 32 and 64 x 64 tiles.
+enum GemmCand { GC_X6 = 0, GC_T11 = 1, GC_T12 = 2, GC_T21 = 3, GC_T22 = 4, GC_COUNT = 5 };
+
+template <typename T, char OPA, char OPB>
+void launch_cand(int cand, int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, size_t sA,
+                 const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc, size_t sC,
+                 int batch, hipStream_t s) {
+    constexpr int BK = 8, OCC = sizeof(T) == sizeof(float) ? 4 : 2;
+#define IA_TILES(WMB, WNB)                                                                        \
+    launch_tiles<T, OPA, OPB, WMB, WNB, BK, OCC, true>(M, N, K, alpha, A, lda, sA, B, ldb, sB, \
+                                                           beta, C, ldc, sC, batch, s)
+    switch (cand) {
+    case GC_X6:
+        if constexpr (sizeof(T) == sizeof(float)) {
+            launch_x6<OPA, OPB>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
+            return;
+        }
+        IA_TILES(2, 2);
+        return;
+    case GC_T11: IA_TILES(1, 1); return;
+    case GC_T12: IA_TILES(1, 2); return;
+    case GC_T21: IA_TILES(2, 1); return;
+    default: IA_TILES(2, 2); return;
+    }
+#undef IA_TILES
+}
+
+// Built-in choice (measured on the sweep's products, batch 256): complex64
+// products of depth >= 128 with both dimensions > 32 on the bf16 matrix
+// cores (1.33x the FP32 3M kernel at K >= 512, on a par at K = 128; at
+// K <= 64 the splitting and two barriers per stage of its few stages make
+// it slower); otherwise the 3M kernel with 64 x 64 tiles, 32-row (column)
+// tiles when op(A) has at most 32 rows (C at most 32 columns), and 32 x 32
+// tiles when 64 x 64 ones would leave fewer than four workgroups per CU
+// (small batches).  Four 64 x 64 workgroups share a CU: their barriers,
+// epilogues and global-load waits overlap each other's MFMAs, which larger
+// tiles at lower occupancy measured slower at.
+int default_cand(bool f32, int M, int N, int K, int batch) {
+    if (f32 && K >= 128 && M > 32 && N > 32) return GC_X6;
+    const double wg64 = (double)((M + 63) / 64) * ((N + 63) / 64) * batch;
+    if (M <= 32 && N <= 32) return GC_T11;
+    if (M <= 32) return GC_T12;
+    if (N <= 32) return GC_T21;
+    if (wg64 < 4.0 * 256) return GC_T11;
+    return GC_T22;
+}
+
+// ---- tuning table: the autotuner (csrc/bench/gemm_tune.hip) times every
+// candidate on each product a run issued and writes the fastest per exact
+// shape; the library reads the table once, from IA_GEMM_TUNE (a path; "off"
+// disables it) or else gemm_tune.txt beside libia.so, and falls back to the
+// built-in choice for shapes it does not list.  Line format:
+//   GEMMTUNE <sizeof real> <opA> <opB> M N K batch <beta == 0> <candidate> ...
+struct TuneKey {
+    int bytes, opa, opb, M, N, K, batch, beta0;
+    bool operator==(const TuneKey &o) const {
+        return bytes == o.bytes && opa == o.opa && opb == o.opb && M == o.M && N == o.N &&
+               K == o.K && batch == o.batch && beta0 == o.beta0;
+    }
+};
+struct TuneHash {
+    size_t operator()(const TuneKey &k) const {
+        size_t h = 1469598103934665603ull;
+        for (int v : {k.bytes, k.opa, k.opb, k.M, k.N, k.K, k.batch, k.beta0})
+            h = (h ^ (size_t)(unsigned)v) * 1099511628211ull;
+        return h;
+    }
+};
+using TuneTable = std::unordered_map<TuneKey, int, TuneHash>;
+
+std::string default_tune_path() {
+    Dl_info info;
+    if (dladdr((const void *)&default_cand, &info) && info.dli_fname) {
+        std::string so(info.dli_fname);
+        const size_t slash = so.find_last_of('/');
+        return (slash == std::string::npos ? std::string(".") : so.substr(0, slash)) +
+               "/gemm_tune.txt";
+    }
+    return "gemm_tune.txt";
+}
+
+const TuneTable &tune_table() {
+    static TuneTable table;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char *e = std::getenv("IA_GEMM_TUNE");
+        if (e && (std::strcmp(e, "off") == 0 || std::strcmp(e, "0") == 0)) return;
+        const std::string path = (e && *e) ? std::string(e) : default_tune_path();
+        FILE *f = std::fopen(path.c_str(), "r");
+        if (!f) return;
+        char line[512];
+        while (std::fgets(line, sizeof line, f)) {
+            TuneKey k{};
+            char oa = 0, ob = 0;
+            int cand = -1;
+            if (std::sscanf(line, "GEMMTUNE %d %c %c %d %d %d %d %d %d", &k.bytes, &oa, &ob, &k.M,
+                            &k.N, &k.K, &k.batch, &k.beta0, &cand) == 9 &&
+                cand >= 0 && cand < GC_COUNT) {
+                k.opa = oa;
+                k.opb = ob;
+                table[k] = cand;
+            }
+        }
+        std::fclose(f);
+    });
+    return table;
+}
+
 template <typename T, char OPA, char OPB>
 void dispatch_tiles(int M, int N, int K, cx<T> alpha, const cx<T> *A, int lda, size_t sA,
                     const cx<T> *B, int ldb, size_t sB, cx<T> beta, cx<T> *C, int ldc, size_t sC,
                     int batch, hipStream_t s) {
     constexpr bool f32 = sizeof(T) == sizeof(float);
-    if constexpr (f32) {
-        // deep complex64 products on the bf16 matrix cores (1.33x the FP32
-        // 3M kernel at K >= 512, on a par at K = 128; at K <= 64 the
-        // splitting and two barriers per stage of its few stages make it
-        // slower)
-        if (K >= 128 && M > 32 && N > 32) {
-            launch_x6<OPA, OPB>(M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, batch, s);
-            return;
+    const bool beta0 = beta.re == 0 && beta.im == 0;
+    int cand = -1;
+    // IA_GEMM_FORCE=<candidate>: every product on that kernel (tests)
+    if (const char *fe = std::getenv("IA_GEMM_FORCE"); fe && *fe) cand = std::atoi(fe);
+    if (cand < 0 || cand >= GC_COUNT) {
+        const TuneTable &tt = tune_table();
+        cand = -1;
+        if (!tt.empty()) {
+            const auto it = tt.find(TuneKey{(int)sizeof(T), OPA, OPB, M, N, K, batch, beta0 ? 1 : 0});
+            if (it != tt.end()) cand = it->second;
         }
+        if (cand < 0) cand = default_cand(f32, M, N, K, batch);
     }
-    constexpr int BK = 8, OCC = f32 ? 4 : 2;
-#define IA_TILES(WMB, WNB)                                                                        \
-    launch_tiles<T, OPA, OPB, WMB, WNB, BK, OCC, true>(M, N, K, alpha, A, lda, sA, B, ldb, sB, \
-                                                           beta, C, ldc, sC, batch, s)
-    const double wg64 = (double)((M + 63) / 64) * ((N + 63) / 64) * batch;
-    if (M <= 32 && N <= 32)
-        IA_TILES(1, 1);
-    else if (M <= 32)
-        IA_TILES(1, 2);
-    else if (N <= 32)
-        IA_TILES(2, 1);
-    else if (wg64 < 4.0 * 256)
-        IA_TILES(1, 1);
-    else
-        IA_TILES(2, 2);
-#undef IA_TILES
+    if (!f32 && cand == GC_X6) cand = default_cand(false, M, N, K, batch);
+    launch_cand<T, OPA, OPB>(cand, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC,
+                             batch, s);
 }
 
 // ------------------------------------------------------------------ FP64 real

@@ -298,3 +298,28 @@ def test_one_and_two_stage_solutions_agree(ia, shape, precision, monkeypatch):
         x0 = rt.tikhonov(U, s, V, b[i], 0.05)
         assert np.linalg.norm(x1[i] - x0) <= tol * np.linalg.norm(x0)
         assert np.linalg.norm(x2[i] - x0) <= tol * np.linalg.norm(x0)
+
+
+@pytest.mark.parametrize("cand", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("C", "C")])
+def test_gemm_every_candidate(ia, monkeypatch, cand, opa, opb):
+    """Every kernel the GEMM autotuner may pick (IA_GEMM_FORCE: 0 = bf16
+    split, complex64 only, 1..4 = 3M with 32x32 / 32x64 / 64x32 / 64x64
+    tiles) on shapes with ragged edges, shallow and deep, to the FP32 /
+    FP64 bounds of the built-in choice's tests."""
+    monkeypatch.setenv("IA_GEMM_FORCE", str(cand))
+    for m, n, k in [(17, 5, 3), (130, 77, 45), (65, 97, 300), (33, 200, 129)]:
+        A = _cr((m, k) if opa == "N" else (k, m), 1)
+        B = _cr((k, n) if opb == "N" else (n, k), 2)
+        C0 = _cr((m, n), 3)
+        opA = A if opa == "N" else A.conj().T
+        opB = B if opb == "N" else B.conj().T
+        ref = (0.5 - 2j) * opA @ opB + (1.5 + 0.25j) * C0
+        got = ia.zgemm(A, B, opa, opb, alpha=0.5 - 2j, beta=1.5 + 0.25j, Cm=C0)
+        assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()) * np.sqrt(k)
+        A4, B4, C4 = A.astype(np.complex64), B.astype(np.complex64), C0.astype(np.complex64)
+        opA4 = A4.astype(complex) if opa == "N" else A4.astype(complex).conj().T
+        opB4 = B4.astype(complex) if opb == "N" else B4.astype(complex).conj().T
+        ref4 = (0.5 - 2j) * opA4 @ opB4 + (1.5 + 0.25j) * C4
+        got4 = ia.cgemm(A4, B4, opa, opb, alpha=0.5 - 2j, beta=1.5 + 0.25j, Cm=C4)
+        assert np.abs(got4 - ref4).max() <= 2e-6 * max(1.0, np.abs(ref4).max()) * np.sqrt(k)
