@@ -11,7 +11,8 @@
 //   inverseacoustics_amd/_build/gemm_tune inverseacoustics_amd/gemm_tune.txt calls.txt
 //
 // Candidates: the bf16 split kernel (complex64 only) and the 3M kernel with
-// 32 x 32, 32 x 64, 64 x 32 and 64 x 64 tiles.  Each is run once to warm
+// 32 x 32, 32 x 64, 64 x 32 and 64 x 64 tiles (IA_TUNE_VERBOSE=1 prints every
+// candidate's time per shape).  Each is run once to warm
 // up, then timed over three launches (HIP events, the minimum kept), on
 // pseudo-random operands with the logged leading dimensions and batch.  A
 // candidate replaces the built-in choice only when it is at least 3 % faster
@@ -171,6 +172,12 @@ void tune(const std::vector<Call> &all, FILE *out, double &t_def, double &t_best
                 best = std::min(best, ms);
             }
             tc[cand] = best;
+        }
+        if (std::getenv("IA_TUNE_VERBOSE")) {
+            std::printf("  %d %c%c M%d N%d K%d x%d:", (int)sizeof(T), c.oa, c.ob, c.M, c.N, c.K,
+                        c.batch);
+            for (int cand = 0; cand < GC_COUNT; ++cand) std::printf(" %.4f", tc[cand]);
+            std::printf("\n");
         }
         int pick = def;
         for (int cand = 0; cand < GC_COUNT; ++cand)
