@@ -13,8 +13,11 @@
 // Gather layout, no atomics: one thread owns matrix entry (row o, column c)
 // and walks the elements adjacent to node c (CSR adjacency); a wave holds
 // 64 consecutive rows of one column, so the element loop is uniform and
-// the column-major stores are coalesced.  The diagonal term C(o) is a
-// separate block reduction over all elements.
+// the column-major stores are coalesced.  The singular elements of a surface
+// row (those with the row's own node, integrated on the page rule of that
+// vertex) are summed by a second kernel with the page points across a
+// wave's lanes.  The diagonal term C(o) is a separate block reduction over
+// all elements.
 #include "ia_internal.hpp"
 
 namespace ia {
@@ -81,7 +84,6 @@ __global__ __launch_bounds__(256) void k_ibem_rows(int rows, const double *__res
                                                   const int *__restrict__ adj_ptr,
                                                   const int *__restrict__ adj,
                                                   const double *__restrict__ rule, int nq,
-                                                  const double *__restrict__ pages, int nqs,
                                                   const double *__restrict__ kv, double rho_c,
                                                   cx<T> *__restrict__ Aout, cx<T> *__restrict__ Bout,
                                                   int ldo, size_t stride) {
@@ -98,21 +100,86 @@ __global__ __launch_bounds__(256) void k_ibem_rows(int rows, const double *__res
         int nd[3];
         load_elem(X, m, E, ne, e, el, nd);
         const int a = (nd[0] == c) ? 0 : ((nd[1] == c) ? 1 : 2);
-        int ikn = -1;
-        if (surface) {
-            for (int t = 0; t < 3; ++t)
-                if (nd[t] == o) ikn = t;
-        }
-        if (ikn >= 0)
-            integrate(el, pt, k, pages + (size_t)ikn * 3 * nqs, nqs, a, Ar, Ai, Br, Bi, Cs);
-        else
-            integrate(el, pt, k, rule, nq, a, Ar, Ai, Br, Bi, Cs);
+        // an element with the row's own node is singular: its page integral
+        // is added by k_ibem_sing (one wave per entry, the page points over
+        // the lanes) instead of here, where the lanes of a wave would wait
+        // for the few that carry it
+        if (surface && (nd[0] == o || nd[1] == o || nd[2] == o)) continue;
+        integrate(el, pt, k, rule, nq, a, Ar, Ai, Br, Bi, Cs);
     }
     // B scaled by i rho c k: (Br + i Bi) i s = -s Bi + i s Br
     const double s = rho_c * k;
     const size_t off = (size_t)f * stride + (size_t)c * ldo + o;
     Aout[off] = cx<T>{(T)(INV4PI * Ar), (T)(INV4PI * Ai)};
     Bout[off] = cx<T>{(T)(-s * INV4PI * Bi), (T)(s * INV4PI * Br)};
+}
+
+// Singular entries, one wave per (entry, frequency): entry s has row
+// o = sing_oc[2 s], column c = sing_oc[2 s + 1] and the contributions
+// sing_list[3 j .. 3 j + 2] = (element e, o's vertex in e, c's vertex in e),
+// j in [sing_ptr[s], sing_ptr[s + 1]); the lanes take the page points in
+// turn, the four sums are reduced across the wave in a fixed order and lane
+// 0 adds them to A[o, c] and B[o, c] (each entry has one wave: no atomics).
+template <typename T>
+__global__ __launch_bounds__(256) void k_ibem_sing(int nse, const int *__restrict__ sing_ptr,
+                                                  const int *__restrict__ sing_oc,
+                                                  const int *__restrict__ sing_list, int m,
+                                                  const double *__restrict__ X,
+                                                  const int *__restrict__ E, int ne,
+                                                  const double *__restrict__ pages, int nqs,
+                                                  const double *__restrict__ kv, double rho_c,
+                                                  cx<T> *__restrict__ A, cx<T> *__restrict__ B,
+                                                  int ldo, size_t stride) {
+    const int sidx = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int f = blockIdx.y;
+    if (sidx >= nse) return;
+    const int o = sing_oc[2 * sidx], c = sing_oc[2 * sidx + 1];
+    const double k = kv[f];
+    const double pt[3] = {X[o], X[o + m], X[o + 2 * m]};
+    double Ar = 0, Ai = 0, Br = 0, Bi = 0, Cs = 0;
+    for (int j = sing_ptr[sidx]; j < sing_ptr[sidx + 1]; ++j) {
+        const int e = sing_list[3 * j], ikn = sing_list[3 * j + 1], a = sing_list[3 * j + 2];
+        Elem el;
+        int nd[3];
+        load_elem(X, m, E, ne, e, el, nd);
+        const double *pg = pages + (size_t)ikn * 3 * nqs;
+        for (int q0 = 0; q0 < nqs; q0 += 64) {
+            const int q = q0 + lane;
+            if (q >= nqs) break;
+            const double x1 = pg[q], x2 = pg[q + nqs], w = pg[q + 2 * nqs];
+            const double N[3] = {x1, x2, 1.0 - x1 - x2};
+            double d[3];
+            for (int t = 0; t < 3; ++t)
+                d[t] = N[0] * el.p[0][t] + N[1] * el.p[1][t] + N[2] * el.p[2][t] - pt[t];
+            const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double R = sqrt(R2);
+            const double F = (el.n[0] * d[0] + el.n[1] * d[1] + el.n[2] * d[2]) / R2;
+            double sn, cs;
+            sincos(k * R, &sn, &cs);
+            const double gr = cs / R, gi = sn / R;
+            const double wn = w * N[a];
+            Br += wn * el.an * gr;
+            Bi += wn * el.an * gi;
+            const double kR = k * R;
+            Ar += wn * F * (-gr - kR * gi);
+            Ai += wn * F * (kR * gr - gi);
+        }
+    }
+    (void)Cs;
+#pragma unroll
+    for (int w = 32; w > 0; w >>= 1) {
+        Ar += __shfl_xor(Ar, w, 64);
+        Ai += __shfl_xor(Ai, w, 64);
+        Br += __shfl_xor(Br, w, 64);
+        Bi += __shfl_xor(Bi, w, 64);
+    }
+    if (lane == 0) {
+        const double s = rho_c * k;
+        const size_t off = (size_t)f * stride + (size_t)c * ldo + o;
+        const cx<T> a0 = A[off], b0 = B[off];
+        A[off] = cx<T>{(T)((double)a0.re + INV4PI * Ar), (T)((double)a0.im + INV4PI * Ai)};
+        B[off] = cx<T>{(T)((double)b0.re - s * INV4PI * Bi), (T)((double)b0.im + s * INV4PI * Br)};
+    }
 }
 
 // A[o,o] -= 1 + C(o), C(o) = (1/4pi) sum_e sum_q w (-F/R); one block per node
@@ -197,13 +264,25 @@ __global__ void k_gram(int m, const double *__restrict__ X, const int *__restric
 template <typename T>
 void launch_ibem_rows(int rows, const double *pts, int npts_ld, bool surface, int m,
                       const double *X, const int *E, int ne, const int *adj_ptr, const int *adj,
-                      const double *rule, int nq, const double *pages, int nqs, const double *k,
-                      int nf, double rho_c, cx<T> *A, cx<T> *B, int ldo, size_t stride,
-                      hipStream_t s) {
+                      const double *rule, int nq, const double *k, int nf, double rho_c, cx<T> *A,
+                      cx<T> *B, int ldo, size_t stride, hipStream_t s) {
     dim3 grid((rows + 63) / 64, (m + 3) / 4, nf);
     hipLaunchKernelGGL(k_ibem_rows<T>, grid, dim3(256), 0, s, rows, pts, npts_ld, surface ? 1 : 0,
-                       m, X, E, ne, adj_ptr, adj, rule, nq, pages, nqs, k, rho_c, A, B, ldo,
-                       stride);
+                       m, X, E, ne, adj_ptr, adj, rule, nq, k, rho_c, A, B, ldo, stride);
+}
+
+// The singular entries of the surface rows (row o, column c, o and c nodes of
+// one element): sum over the elements e with both nodes of the page integral
+// of o's vertex with c's shape function, added to A and B.
+template <typename T>
+void launch_ibem_sing(int nse, const int *sing_ptr, const int *sing_oc, const int *sing_list,
+                      int m, const double *X, const int *E, int ne, const double *pages, int nqs,
+                      const double *k, int nf, double rho_c, cx<T> *A, cx<T> *B, int ldo,
+                      size_t stride, hipStream_t s) {
+    if (nse <= 0) return;
+    dim3 grid((nse + 3) / 4, nf);
+    hipLaunchKernelGGL(k_ibem_sing<T>, grid, dim3(256), 0, s, nse, sing_ptr, sing_oc, sing_list,
+                       m, X, E, ne, pages, nqs, k, rho_c, A, B, ldo, stride);
 }
 
 template <typename T>
@@ -222,12 +301,20 @@ void launch_gram(int m, const double *X, const int *E, int ne, const int *adj_pt
 
 template void launch_ibem_rows<double>(int, const double *, int, bool, int, const double *,
                                        const int *, int, const int *, const int *, const double *,
-                                       int, const double *, int, const double *, int, double,
-                                       cx<double> *, cx<double> *, int, size_t, hipStream_t);
+                                       int, const double *, int, double, cx<double> *,
+                                       cx<double> *, int, size_t, hipStream_t);
 template void launch_ibem_rows<float>(int, const double *, int, bool, int, const double *,
                                       const int *, int, const int *, const int *, const double *,
-                                      int, const double *, int, const double *, int, double,
-                                      cx<float> *, cx<float> *, int, size_t, hipStream_t);
+                                      int, const double *, int, double, cx<float> *, cx<float> *,
+                                      int, size_t, hipStream_t);
+template void launch_ibem_sing<double>(int, const int *, const int *, const int *, int,
+                                       const double *, const int *, int, const double *, int,
+                                       const double *, int, double, cx<double> *, cx<double> *,
+                                       int, size_t, hipStream_t);
+template void launch_ibem_sing<float>(int, const int *, const int *, const int *, int,
+                                      const double *, const int *, int, const double *, int,
+                                      const double *, int, double, cx<float> *, cx<float> *, int,
+                                      size_t, hipStream_t);
 template void launch_ibem_cdiag<double>(int, const double *, const int *, int, const double *, int,
                                         const double *, int, cx<double> *, int, size_t, int,
                                         double *, hipStream_t);

@@ -4,6 +4,7 @@
 // CHIEF rows), H_b = A_hs Z - B_hs, the IBEM sweep, and radiation modes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,9 +18,13 @@ namespace ia {
 template <typename T>
 void launch_ibem_rows(int rows, const double *pts, int npts_ld, bool surface, int m,
                       const double *X, const int *E, int ne, const int *adj_ptr, const int *adj,
-                      const double *rule, int nq, const double *pages, int nqs, const double *k,
-                      int nf, double rho_c, cx<T> *A, cx<T> *B, int ldo, size_t stride,
-                      hipStream_t s);
+                      const double *rule, int nq, const double *k, int nf, double rho_c, cx<T> *A,
+                      cx<T> *B, int ldo, size_t stride, hipStream_t s);
+template <typename T>
+void launch_ibem_sing(int nse, const int *sing_ptr, const int *sing_oc, const int *sing_list,
+                      int m, const double *X, const int *E, int ne, const double *pages, int nqs,
+                      const double *k, int nf, double rho_c, cx<T> *A, cx<T> *B, int ldo,
+                      size_t stride, hipStream_t s);
 template <typename T>
 void launch_ibem_cdiag(int m, const double *X, const int *E, int ne, const double *rule, int nq,
                        const double *pages, int nqs, cx<T> *A, int lda, size_t stride, int nf,
@@ -145,6 +150,10 @@ std::vector<std::string> read_lines(const char *path, ia_status &st) {
 struct Mesh {
     int m = 0, ne = 0;
     std::vector<int> E, adj_ptr, adj;
+    // singular entries of the surface rows: (row o, column c) with o and c
+    // nodes of one element, and per entry the (element, o's vertex, c's
+    // vertex) triples of the elements holding both
+    std::vector<int> sing_ptr, sing_oc, sing_list;
 };
 
 ia_status build_mesh(int m, int n, const int *e_n, Mesh &M) {
@@ -168,6 +177,32 @@ ia_status build_mesh(int m, int n, const int *e_n, Mesh &M) {
                 return fail(IA_ERR_ARG, "element %d is degenerate", e);
             M.adj[fill[v]++] = e;
         }
+    M.sing_ptr.assign(1, 0);
+    M.sing_oc.clear();
+    M.sing_list.clear();
+    std::vector<int> cols;
+    for (int o = 0; o < m; ++o) {
+        cols.clear();
+        for (int q = M.adj_ptr[o]; q < M.adj_ptr[o + 1]; ++q)
+            for (int a = 0; a < 3; ++a) cols.push_back(M.E[M.adj[q] + a * n]);
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        for (int c : cols) {
+            for (int q = M.adj_ptr[o]; q < M.adj_ptr[o + 1]; ++q) {
+                const int e = M.adj[q];
+                int ikn = -1, a = -1;
+                for (int t = 0; t < 3; ++t) {
+                    if (M.E[e + t * n] == o) ikn = t;
+                    if (M.E[e + t * n] == c) a = t;
+                }
+                if (a < 0) continue;
+                M.sing_list.insert(M.sing_list.end(), {e, ikn, a});
+            }
+            M.sing_oc.push_back(o);
+            M.sing_oc.push_back(c);
+            M.sing_ptr.push_back((int)(M.sing_list.size() / 3));
+        }
+    }
     return IA_OK;
 }
 
@@ -175,11 +210,12 @@ ia_status build_mesh(int m, int n, const int *e_n, Mesh &M) {
 // with the assembly and factorisation scratch (buffers only grow, so a kept
 // IbemDev allocates nothing on a repeated call of the same size).
 struct IbemDev {
-    DevBuf X, E, adj_ptr, adj, rule, pages, cp, rlh, k, Z, H;
+    DevBuf X, E, adj_ptr, adj, sing_ptr, sing_oc, sing_list, rule, pages, cp, rlh, k, Z, H;
     DevBuf A, B, Ah, Bh, Ad, Bd, Ahd, Bhd, ipiv, info, tau;
     void release() {
-        for (DevBuf *b : {&X, &E, &adj_ptr, &adj, &rule, &pages, &cp, &rlh, &k, &Z, &H, &A, &B,
-                          &Ah, &Bh, &Ad, &Bd, &Ahd, &Bhd, &ipiv, &info, &tau})
+        for (DevBuf *b : {&X, &E, &adj_ptr, &adj, &sing_ptr, &sing_oc, &sing_list, &rule, &pages,
+                          &cp, &rlh, &k, &Z, &H, &A, &B, &Ah, &Bh, &Ad, &Bd, &Ahd, &Bhd, &ipiv,
+                          &info, &tau})
             b->reset();
     }
 };
@@ -196,6 +232,10 @@ ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, co
     IA_TRY(to_device(D.E, M.E.data(), sizeof(int) * 3 * (size_t)n, s));
     IA_TRY(to_device(D.adj_ptr, M.adj_ptr.data(), sizeof(int) * (m + 1), s));
     IA_TRY(to_device(D.adj, M.adj.data(), sizeof(int) * M.adj.size(), s));
+    const int nse = (int)M.sing_oc.size() / 2;
+    IA_TRY(to_device(D.sing_ptr, M.sing_ptr.data(), sizeof(int) * M.sing_ptr.size(), s));
+    IA_TRY(to_device(D.sing_oc, M.sing_oc.data(), sizeof(int) * M.sing_oc.size(), s));
+    IA_TRY(to_device(D.sing_list, M.sing_list.data(), sizeof(int) * M.sing_list.size(), s));
     IA_TRY(to_device(D.rule, Q.rule.data(), sizeof(double) * Q.rule.size(), s));
     IA_TRY(to_device(D.pages, Q.pages.data(), sizeof(double) * Q.pages.size(), s));
     if (l > 0) IA_TRY(to_device(D.rlh, r_lh, sizeof(double) * 3 * l, s));
@@ -212,18 +252,20 @@ ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, co
     const int *dE = D.E.as<int>();
     launch_ibem_rows<T>(m, D.X.as<double>(), m, true, m, D.X.as<double>(), dE, n,
                         D.adj_ptr.as<int>(), D.adj.as<int>(), D.rule.as<double>(), Q.nq,
-                        D.pages.as<double>(), Q.nqs, D.k.as<double>(), nf, rho_c, bA.as<cx<T>>(),
-                        bB.as<cx<T>>(), mt, sS, s);
+                        D.k.as<double>(), nf, rho_c, bA.as<cx<T>>(), bB.as<cx<T>>(), mt, sS, s);
+    launch_ibem_sing<T>(nse, D.sing_ptr.as<int>(), D.sing_oc.as<int>(), D.sing_list.as<int>(), m,
+                        D.X.as<double>(), dE, n, D.pages.as<double>(), Q.nqs, D.k.as<double>(),
+                        nf, rho_c, bA.as<cx<T>>(), bB.as<cx<T>>(), mt, sS, s);
     if (n_ch > 0)
         launch_ibem_rows<T>(n_ch, D.cp.as<double>(), n_ch, false, m, D.X.as<double>(), dE, n,
                             D.adj_ptr.as<int>(), D.adj.as<int>(), D.rule.as<double>(), Q.nq,
-                            D.pages.as<double>(), Q.nqs, D.k.as<double>(), nf, rho_c,
-                            bA.as<cx<T>>() + m, bB.as<cx<T>>() + m, mt, sS, s);
+                            D.k.as<double>(), nf, rho_c, bA.as<cx<T>>() + m, bB.as<cx<T>>() + m,
+                            mt, sS, s);
     if (l > 0)
         launch_ibem_rows<T>(l, D.rlh.as<double>(), l, false, m, D.X.as<double>(), dE, n,
                             D.adj_ptr.as<int>(), D.adj.as<int>(), D.rule.as<double>(), Q.nq,
-                            D.pages.as<double>(), Q.nqs, D.k.as<double>(), nf, rho_c,
-                            bAh.as<cx<T>>(), bBh.as<cx<T>>(), l, sH, s);
+                            D.k.as<double>(), nf, rho_c, bAh.as<cx<T>>(), bBh.as<cx<T>>(), l, sH,
+                            s);
     launch_ibem_cdiag<T>(m, D.X.as<double>(), dE, n, D.rule.as<double>(), Q.nq,
                          D.pages.as<double>(), Q.nqs, bA.as<cx<T>>(), mt, sS, nf, nullptr, s);
     IA_HIP(hipGetLastError());
