@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -89,15 +90,53 @@ struct Timer {
 // runtime has shut down.
 template <typename T>
 struct SweepCache {
-    DevBuf bD, bY, bipiv, bperm, binfo, bc, bbeta, bphi, bx, bq, bps, bpsd, bn2, bcn2;
+    DevBuf bD, bY, bipiv, bperm, binfo, bc, bbeta, bphi, bx, bq, bps, bpsd, bn2, bcn2, bgw;
     SvdWork<T> sw;
+    // IA_GRAPH=1: the captured sweep, the key it was captured for (sizes,
+    // parameters, every input, output and workspace address) and the stream
+    // it is replayed on
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uintptr_t> gkey;
+    hipStream_t gs = nullptr;
+    hipEvent_t gev[2] = {nullptr, nullptr};
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        gkey.clear();
+    }
     void release() {
+        drop_graph();
         for (DevBuf *b : {&bD, &bY, &bipiv, &bperm, &binfo, &bc, &bbeta, &bphi, &bx, &bq, &bps, &bpsd,
-                          &bn2, &bcn2})
+                          &bn2, &bcn2, &bgw})
             b->reset();
         sw.release();
     }
+    // the address of every device buffer of the workspace (a reallocation
+    // invalidates a captured graph)
+    void addresses(std::vector<uintptr_t> &v) const {
+        for (const DevBuf *b : {&bD, &bY, &bipiv, &bperm, &binfo, &bc, &bbeta, &bphi, &bx, &bq, &bps,
+                                &bpsd, &bn2, &bcn2, &bgw, &sw.d, &sw.e, &sw.sig, &sw.tauL, &sw.tauR,
+                                &sw.Vr, &sw.QBt, &sw.PBt, &sw.work, &sw.gw, &sw.Rb, &sw.tauQ, &sw.qw,
+                                &sw.Bs, &sw.uR2, &sw.uL2, &sw.tR2, &sw.tL2, &sw.Pm, &sw.tauP})
+            v.push_back((uintptr_t)b->p);
+    }
 };
+
+__global__ void k_fill_int(int *p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+uintptr_t dbits(double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, sizeof u);
+    return (uintptr_t)u;
+}
+
+bool graph_mode() {
+    const char *e = std::getenv("IA_GRAPH");
+    return e && *e && *e != '0';
+}
 template <typename T>
 static SweepCache<T> &sweep_cache() {
     static thread_local SweepCache<T> *c = new SweepCache<T>();
@@ -224,7 +263,7 @@ static ia_status enqueue_chunk(SweepCache<T> &W, const ia_sweep_desc &D, const R
                                const cx<double> *b, double *lam, int *lidx, int *ktsvd,
                                double *power, double *smax, cx<double> *vel, cx<double> *pres,
                                int *status, double *gap, hipStream_t s, Timer &tm,
-                               KernelClock &clk, double *curve) {
+                               KernelClock &clk, double *curve, bool graph = false) {
     const int m = D.m, l = D.l;
     const int R = m, C = l;  // Y = H_e^H (m x l), A = Y^H
     const double rho_c = air().rho * air().c;
@@ -241,15 +280,21 @@ static ia_status enqueue_chunk(SweepCache<T> &W, const ia_sweep_desc &D, const R
     const double *kf = k + f0;
     cx<T> *Dm = bD.as<cx<T>>();
     cx<T> *Y = bY.as<cx<T>>();
+    // a captured sweep records no timing events, and its info counters are
+    // cleared by a kernel (no memset node in a graph)
+    auto mark = [&] {
+        if (!graph) tm.mark();
+    };
     tm.n = 0;
-    tm.mark();  // 0
+    mark();  // 0
     launch_esm_D<T>(m, nf, kf, r_s, src, nrm, Dm, (size_t)m * m, s, herm_D);
     if (!herm_D) launch_esm_M<T>(l, m, nf, kf, rho_c, r_h, src, Y, (size_t)m * l, true, s);
     IA_HIP(hipGetLastError());
-    tm.mark();  // 1
-    IA_HIP(hipMemsetAsync(binfo.p, 0, sizeof(int) * nf, s));
-    IA_TRY(getrf_batched<T>(m, Dm, m, (size_t)m * m, bipiv.as<int>(), binfo.as<int>(), nf, s));
-    tm.mark();  // 2
+    mark();  // 1
+    hipLaunchKernelGGL(k_fill_int, dim3((nf + 255) / 256), dim3(256), 0, s, binfo.as<int>(), nf, 0);
+    IA_TRY(getrf_batched<T>(m, Dm, m, (size_t)m * m, bipiv.as<int>(), binfo.as<int>(), nf, s,
+                            W.bgw.template as<int>()));
+    mark();  // 2
     if (herm_D) {
         // P D^H = L U: M^H is assembled with its rows already in the
         // pivoted order P M^H, so the solve is the two triangular
@@ -267,18 +312,18 @@ static ia_status enqueue_chunk(SweepCache<T> &W, const ia_sweep_desc &D, const R
         int *pidx = bperm.as<int>();
         IA_TRY(lu_perm_index(m, bipiv.as<int>(), pidx, pidx + (size_t)m * nf, nf, s));
     }
-    tm.mark();  // 3
+    mark();  // 3
     sw.batch = nf;
-    IA_TRY(svd_reduce<T>(sw, Y, s, &clk));
-    tm.mark();  // 4
+    IA_TRY(svd_reduce<T>(sw, Y, s, graph ? nullptr : &clk));
+    mark();  // 4
     launch_bisect(C, sw.d.template as<double>(), sw.e.template as<double>(),
                   sw.sig.template as<double>(), nf, s, sizeof(T) == sizeof(float));
-    tm.mark();  // 5
+    mark();  // 5
     launch_inviter(C, sw.d.template as<double>(), sw.e.template as<double>(),
                    sw.sig.template as<double>(), sw.QBt.template as<double>(),
                    sw.PBt.template as<double>(), sw.work.p, sw.sub, nf, s,
                    sizeof(T) == sizeof(float));
-    tm.mark();  // 6
+    mark();  // 6
     // beta = Q_B^T V_b^H b
     const cx<double> *bf = b + (size_t)f0 * l;
     IA_HIP(hipMemcpyAsync(bc.p, bf, sizeof(cx<double>) * (size_t)nf * l, hipMemcpyDefault, s));
@@ -298,7 +343,7 @@ static ia_status enqueue_chunk(SweepCache<T> &W, const ia_sweep_desc &D, const R
     launch_pad_copy(C, R, bc.as<cx<double>>(), C, bx.as<cx<double>>(), R, nf, s);
     svd_apply_u<T>(sw, Y, bx.as<cx<double>>(), R, 1, s);
     IA_HIP(hipGetLastError());
-    tm.mark();  // 7
+    mark();  // 7
     // post: q = D^-1 x, p_s = M_sq q, W = 1/2 Re(x^H p_s) S/m
     if (sizeof(T) == sizeof(double)) {
         // Y held P X: x = P^T x_Y, x[r] = x_Y[inv[r]]
@@ -332,7 +377,7 @@ static ia_status enqueue_chunk(SweepCache<T> &W, const ia_sweep_desc &D, const R
                                 sizeof(double), nf, hipMemcpyDefault, s));
     if (status) IA_HIP(hipMemcpyAsync(status + f0, binfo.p, sizeof(int) * nf, hipMemcpyDefault, s));
     IA_HIP(hipGetLastError());
-    tm.mark();  // 8
+    mark();  // 8
     return IA_OK;
 }
 
@@ -353,6 +398,7 @@ static ia_status alloc_chunk(SweepCache<T> &W, int m, int l, int chunk) {
     IA_TRY(W.bpsd.alloc(sizeof(cx<double>) * (size_t)chunk * m));
     IA_TRY(W.bn2.alloc(sizeof(double) * (size_t)chunk));
     IA_TRY(W.bcn2.alloc(sizeof(double) * (size_t)chunk));
+    IA_TRY(W.bgw.alloc(sizeof(int) * getrf_work_ints(m, chunk)));
     return W.sw.alloc(R, C, chunk);
 }
 
@@ -368,7 +414,7 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
                                 const cx<double> *b, double *lam, int *lidx, int *ktsvd,
                                 double *power, double *smax, cx<double> *vel, cx<double> *pres,
                                 int *status, double *gap, hipStream_t s,
-                                double *curve = nullptr) {
+                                double *curve = nullptr, bool allow_graph = true) {
     const int m = D.m, l = D.l, F = D.n_freq;
     if (l > m)
         return esm_sweep_tall<T>(D, k, r_s, src, nrm, r_h, b, lam, lidx, ktsvd, power, smax, vel,
@@ -379,6 +425,59 @@ static ia_status esm_sweep_impl(const ia_sweep_desc &D, const double *k, const d
     if (prm.n_lambda > 512) return fail(IA_ERR_ARG, "n_lambda must be <= 512");
     SweepCache<T> &W = sweep_cache<T>();
     IA_TRY(alloc_chunk<T>(W, m, l, chunk));
+    if (allow_graph && graph_mode()) {
+        // IA_GRAPH=1: the whole sweep (every chunk) is captured once into a
+        // graph on a stream of the library's own, for this key, and replayed:
+        // one launch per sweep; the caller's stream waits for it
+        std::vector<uintptr_t> key = {
+            (uintptr_t)F, (uintptr_t)m, (uintptr_t)l, (uintptr_t)chunk,
+            (uintptr_t)prm.filter, (uintptr_t)prm.select, (uintptr_t)prm.n_lambda,
+            (uintptr_t)prm.k_fixed, dbits(prm.decades), dbits(prm.lambda_fixed), dbits(D.area),
+            dbits(air().c), dbits(air().rho),
+            (uintptr_t)k, (uintptr_t)r_s, (uintptr_t)src, (uintptr_t)nrm, (uintptr_t)r_h,
+            (uintptr_t)b, (uintptr_t)lam, (uintptr_t)lidx, (uintptr_t)ktsvd, (uintptr_t)power,
+            (uintptr_t)smax, (uintptr_t)vel, (uintptr_t)pres, (uintptr_t)status, (uintptr_t)gap,
+            (uintptr_t)curve};
+        W.addresses(key);
+        if (!W.gs) {
+            IA_HIP(hipStreamCreateWithFlags(&W.gs, hipStreamNonBlocking));
+            IA_HIP(hipEventCreate(&W.gev[0]));
+            IA_HIP(hipEventCreate(&W.gev[1]));
+        }
+        if (!W.gexec || W.gkey != key) {
+            W.drop_graph();
+            Timer tg(W.gs);
+            KernelClock cg;
+            IA_HIP(hipStreamBeginCapture(W.gs, hipStreamCaptureModeThreadLocal));
+            ia_status st = IA_OK;
+            for (int f0 = 0; f0 < F && st == IA_OK; f0 += chunk)
+                st = enqueue_chunk<T>(W, D, prm, f0, std::min(chunk, F - f0), k, r_s, src, nrm,
+                                      r_h, b, lam, lidx, ktsvd, power, smax, vel, pres, status,
+                                      gap, W.gs, tg, cg, curve, true);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(W.gs, &g);
+            if (st == IA_OK && ce == hipSuccess && g) {
+                const hipError_t ie = hipGraphInstantiate(&W.gexec, g, nullptr, nullptr, 0);
+                if (ie != hipSuccess) W.gexec = nullptr;
+            }
+            if (g) (void)hipGraphDestroy(g);
+            IA_TRY(st);
+            if (!W.gexec) return fail(IA_ERR_DEVICE, "capture of the sweep into a graph failed");
+            W.gkey = key;
+        }
+        IA_HIP(hipEventRecord(W.gev[0], s));
+        IA_HIP(hipStreamWaitEvent(W.gs, W.gev[0], 0));
+        IA_HIP(hipGraphLaunch(W.gexec, W.gs));
+        IA_HIP(hipEventRecord(W.gev[1], W.gs));
+        IA_HIP(hipStreamWaitEvent(s, W.gev[1], 0));
+        IA_HIP(hipEventSynchronize(W.gev[1]));
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, W.gev[0], W.gev[1]);
+        ia_stats st{};
+        st.total = ms;
+        g_stats = st;
+        return IA_OK;
+    }
     Timer tm(s);
     KernelClock clk;
     ia_stats st{};
@@ -437,7 +536,7 @@ static ia_status full_recheck(const ia_sweep_desc &desc, const std::vector<int> 
     IA_TRY(esm_sweep_impl<double>(d2, kk.as<double>(), r_s, src, nrm, r_h, bb.as<cx<double>>(),
                                   tl.as<double>(), ti.as<int>(), tk.as<int>(), tw.as<double>(),
                                   tsm.as<double>(), tv.as<cx<double>>(), tp.as<cx<double>>(),
-                                  tst.as<int>(), tg.as<double>(), s));
+                                  tst.as<int>(), tg.as<double>(), s, nullptr, false));
     for (int q = 0; q < nL; ++q) {
         const int f = L[q];
         if (o.lambda) IA_HIP(hipMemcpyAsync(o.lambda + f, tl.as<double>() + q, sizeof(double), hipMemcpyDefault, s));
@@ -547,7 +646,9 @@ ia_status ia_esm_sweep_device(const ia_sweep_desc *desc, const double *k, const 
     // difference of 1.7e-5 in FP64 came out as -6.7e-5), so choices closer
     // than twice that are decided in FP64
     const double tau = desc->recheck_gap == 0.0 ? 2e-4 : desc->recheck_gap;
-    DevBuf dgap, dcurve;
+    // kept between calls (a captured sweep is replayed only for the same
+    // addresses)
+    static thread_local DevBuf &dgap = *new DevBuf(), &dcurve = *new DevBuf();
     double *gap = o.selection_gap;
     if (!gap && tau > 0) {
         IA_TRY(dgap.alloc(sizeof(double) * F));

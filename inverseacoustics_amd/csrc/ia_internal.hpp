@@ -143,9 +143,12 @@ void launch_dgemm(char opA, int M, int N, int K, const double *A, int lda, size_
                   int batch, hipStream_t s);
 
 // LU (lu.hip)
+// work (optional): getrf_work_ints(n, batch) ints of device scratch; without
+// it the scratch is taken with hipMallocAsync on s
+size_t getrf_work_ints(int n, int batch);
 template <typename T>
 ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int *info, int batch,
-                        hipStream_t s);
+                        hipStream_t s, int *work = nullptr);
 // Solve op(A) X = B with A = P^T L U from getrf; op 'N' or 'C'.  B n x nrhs.
 template <typename T>
 ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_t strideA,

@@ -1072,14 +1072,20 @@ size_t getrs1_lds(int n) {
 
 }  // namespace
 
+size_t getrf_work_ints(int n, int batch) {
+    return ((size_t)(1 + 2 * PERM_MAX) + (size_t)(n / NBO) * n) * (size_t)batch;
+}
+
 template <typename T>
 ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int *info, int batch,
-                        hipStream_t s) {
+                        hipStream_t s, int *work) {
     if (n <= 0 || batch <= 0) return IA_OK;
     const cx<T> mone{-1, 0}, one{1, 0};
     int *perm = nullptr;  // per matrix: the current panel's row permutation
-    if (hipMallocAsync((void **)&perm, sizeof(int) * (1 + 2 * PERM_MAX) * (size_t)batch, s) !=
-        hipSuccess) {
+    if (work) {
+        perm = work;
+    } else if (hipMallocAsync((void **)&perm, sizeof(int) * (1 + 2 * PERM_MAX) * (size_t)batch,
+                              s) != hipSuccess) {
         (void)hipGetLastError();
         perm = nullptr;
     }
@@ -1145,8 +1151,9 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
     if (defer) {
         // columns 0 .. (last outer panel start) - 1
         const int ncols = ((n - 1) / NBO) * NBO;
-        int *idx = nullptr;
-        IA_HIP(hipMallocAsync((void **)&idx, sizeof(int) * (size_t)(n / NBO) * n * batch, s));
+        int *idx = work ? work + (size_t)(1 + 2 * PERM_MAX) * batch : nullptr;
+        if (!work)
+            IA_HIP(hipMallocAsync((void **)&idx, sizeof(int) * (size_t)(n / NBO) * n * batch, s));
         const size_t lds = sizeof(int) * (size_t)n;
         (void)hipFuncSetAttribute((const void *)k_perm_suffix,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1154,9 +1161,9 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
                            idx);
         hipLaunchKernelGGL(k_laswp_deferred<T>, dim3((ncols + 7) / 8, batch), dim3(256), 0, s, n,
                            ncols, (const int *)idx, A, lda, strideA);
-        IA_HIP(hipFreeAsync(idx, s));
+        if (!work) IA_HIP(hipFreeAsync(idx, s));
     }
-    if (perm) IA_HIP(hipFreeAsync(perm, s));
+    if (perm && !work) IA_HIP(hipFreeAsync(perm, s));
     IA_HIP(hipGetLastError());
     return IA_OK;
 }
@@ -1284,9 +1291,9 @@ template ia_status getrs_prepermuted<double>(int, int, const cx<double> *, int, 
 template ia_status getrs_prepermuted<float>(int, int, const cx<float> *, int, size_t, cx<float> *,
                                             int, size_t, int, hipStream_t);
 template ia_status getrf_batched<double>(int, cx<double> *, int, size_t, int *, int *, int,
-                                         hipStream_t);
+                                         hipStream_t, int *);
 template ia_status getrf_batched<float>(int, cx<float> *, int, size_t, int *, int *, int,
-                                        hipStream_t);
+                                        hipStream_t, int *);
 template ia_status getrs_batched<double>(char, int, int, const cx<double> *, int, size_t,
                                          const int *, cx<double> *, int, size_t, int, hipStream_t);
 template ia_status getrs_batched<float>(char, int, int, const cx<float> *, int, size_t,

@@ -14,6 +14,15 @@ def pytest_configure(config):
 
 
 def _have_gpu():
+    # torch's HIP runtime first (tests that hand device tensors to the C ABI
+    # need it; initialised after libia.so it sees no device), then the library
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
     try:
         import inverseacoustics_amd as ia
         return ia.device_count() > 0
