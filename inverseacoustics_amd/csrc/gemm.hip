@@ -321,6 +321,21 @@ struct SplitLoader {
     __device__ __forceinline__ void load(const cx<float> *__restrict__ X, int ldx, int r0, int nr,
                                          int k0, int K, cx<float> (&v)[8]) const {
         const int gr = r0 + r;
+        if (KC) {
+            // the thread's 8 consecutive k (64 bytes) as four 16-byte loads
+            // when they are all in range and aligned
+            const cx<float> *p = X + (size_t)gr * ldx + (k0 + kg * 8);
+            if (gr < nr && k0 + kg * 8 + 8 <= K && ((uintptr_t)p & 15) == 0) {
+                const float4 *q = reinterpret_cast<const float4 *>(p);
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    const float4 f = q[h];
+                    v[2 * h] = cx<float>{f.x, f.y};
+                    v[2 * h + 1] = cx<float>{f.z, f.w};
+                }
+                return;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int gk = k0 + kg * 8 + j;
