@@ -192,3 +192,23 @@ def test_gram_recheck_decides_near_ties_in_fp64(ia, monkeypatch):
         rest = np.setdiff1d(np.arange(len(P.k)), near)
         np.testing.assert_array_equal(g.velocity[rest], raw.velocity[rest])
         assert np.max(np.abs(post.power_db(g.power) - post.power_db(r.power))) < 0.1
+
+
+def test_gram_recheck_small_sphere(ia):
+    """The Gram-form decision on a small system (64 microphones, 120 sources,
+    7 frequencies, recheck gap 0.05): every frequency it decides matches the
+    FP64 sweep's lambda index, lambda and s_1 to 1e-11 and its velocity to
+    1e-7; frequencies with more than six candidates take the full sweep."""
+    S = sc.sphere_scenario(freqs=np.linspace(300, 2000, 7), m=120, l=64)
+    r = _run(ia, S)
+    raw = _run(ia, S, precision="mixed", recheck_gap=-1.0)
+    tau = 0.05
+    near = np.where(~(raw.selection_gap >= tau))[0]
+    g = _run(ia, S, precision="mixed", recheck_gap=tau)
+    print("rechecked", near.tolist(), {k: v for k, v in g.stats.items() if "recheck" in k})
+    assert g.stats["rechecked"] == len(near)
+    np.testing.assert_array_equal(g.lam_index, r.lam_index)
+    for f in near:
+        assert g.lam[f] == pytest.approx(r.lam[f], rel=1e-11)
+        assert g.sigma_max[f] == pytest.approx(r.sigma_max[f], rel=1e-11)
+        assert np.linalg.norm(g.velocity[f] - r.velocity[f]) <= 1e-7 * np.linalg.norm(r.velocity[f])
