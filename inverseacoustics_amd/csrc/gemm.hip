@@ -96,8 +96,14 @@ template <typename T, bool KC, bool CONJ, int R, int BK>
 struct TileLoader {
     static constexpr int NE = R * BK / 256;
     static_assert(NE * 256 == R * BK, "tile must split evenly over 256 threads");
+    // KC with two elements per thread: the pair (r, 2j), (r, 2j + 1) is
+    // adjacent in memory and loads as one vector
+    static constexpr bool PAIR = KC && NE == 2 && BK % 2 == 0;
     static __device__ __forceinline__ void coords(int t, int i, int &r, int &k) {
-        if (KC) {
+        if (PAIR) {
+            k = (t % (BK / 2)) * 2 + i;
+            r = t / (BK / 2);
+        } else if (KC) {
             k = t % BK;
             r = t / BK + i * (256 / BK);
         } else {
@@ -107,6 +113,25 @@ struct TileLoader {
     }
     static __device__ __forceinline__ void load(const cx<T> *__restrict__ X, int ldx, int r0,
                                                 int nr, int k0, int K, int t, cx<T> (&v)[NE]) {
+        if constexpr (PAIR) {
+            int r, k;
+            coords(t, 0, r, k);
+            const int gr = r0 + r, gk = k0 + k;
+            const cx<T> *p = X + (size_t)gr * ldx + gk;
+            if (gr < nr && gk + 1 < K && ((uintptr_t)p & (2 * sizeof(cx<T>) - 1)) == 0) {
+                if constexpr (sizeof(T) == sizeof(float)) {
+                    const float4 f = *reinterpret_cast<const float4 *>(p);
+                    v[0] = cx<T>{f.x, f.y};
+                    v[1] = cx<T>{f.z, f.w};
+                } else {
+                    const double2 *q = reinterpret_cast<const double2 *>(p);
+                    const double2 a = q[0], b = q[1];
+                    v[0] = cx<T>{a.x, a.y};
+                    v[1] = cx<T>{b.x, b.y};
+                }
+                return;
+            }
+        }
 #pragma unroll
         for (int i = 0; i < NE; ++i) {
             int r, k;
