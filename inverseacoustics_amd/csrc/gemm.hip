@@ -423,14 +423,9 @@ __global__ __launch_bounds__(256, 3) void k_cgemm_x6(int M, int N, int K, cx<flo
         lb.store(Bs, vb);
     }
     __syncthreads();
-    for (int st = 0; st < nst; ++st) {
-        const bool more = st + 1 < nst;
-        if (more) {
-            la.load(A, lda, m0, M, (st + 1) * XK, K, va);
-            lb.load(B, ldb, n0, N, (st + 1) * XK, K, vb);
-        }
-        // the B fragments of the stage stay in registers, the A fragments
-        // are read per row block
+    // one stage's products: the B fragments stay in registers, the A
+    // fragments are read per row block
+    auto stage = [&]() {
         constexpr int PX[6] = {0, 0, 1, 0, 2, 1}, PY[6] = {0, 1, 0, 2, 0, 1};
         u32x4 fb[2][6];
 #pragma unroll
@@ -459,13 +454,33 @@ __global__ __launch_bounds__(256, 3) void k_cgemm_x6(int M, int N, int K, cx<flo
                     s3[a][b] = mfma_bf16(fb[b][y], fa[3 + x], s3[a][b]);
                 }
         }
+    };
+    for (int st = 0; st + 1 < nst; ++st) {
+        la.load(A, lda, m0, M, (st + 1) * XK, K, va);
+        lb.load(B, ldb, n0, N, (st + 1) * XK, K, vb);
+        stage();
         __syncthreads();  // every wave has read the stage
-        if (more) {
-            la.store(As, va);
-            lb.store(Bs, vb);
-        }
+        la.store(As, va);
+        lb.store(Bs, vb);
         __syncthreads();
     }
+    // last stage: with beta != 0 the reads of C are issued before its
+    // products (into the registers the next-stage loads no longer need), so
+    // their latency overlaps the MFMAs instead of following them
+    cx<float> c0v[2][2][4];
+    if (!BETA0) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int gm = m0 + wm + a * 16 + l16;
+                    const int gn = n0 + wn + b * 16 + 4 * g + i;
+                    c0v[a][b][i] = (gm < M && gn < N) ? C[(size_t)gn * ldc + gm] : cx<float>{0, 0};
+                }
+    }
+    if (nst > 0) stage();
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -477,13 +492,12 @@ __global__ __launch_bounds__(256, 3) void k_cgemm_x6(int M, int N, int K, cx<flo
                 if (gm < M && gn < N) {
                     const float xr = s1[a][b][i] - s2[a][b][i], xi = s3[a][b][i];
                     cx<float> r{alpha.re * xr - alpha.im * xi, alpha.re * xi + alpha.im * xr};
-                    const size_t o = (size_t)gn * ldc + gm;
                     if (!BETA0) {
-                        const cx<float> c0 = C[o];
+                        const cx<float> c0 = c0v[a][b][i];
                         r.re += beta.re * c0.re - beta.im * c0.im;
                         r.im += beta.re * c0.im + beta.im * c0.re;
                     }
-                    C[o] = r;
+                    C[(size_t)gn * ldc + gm] = r;
                 }
             }
 }
