@@ -11,7 +11,8 @@
 //      T = tridiag(d_0, e_0, d_1, e_1, ...) whose eigenvalues are +-sigma
 //      (Sturm counts; one thread per singular value);
 //   3. singular vectors of B by inverse iteration on T - sigma I
-//      (pivoted tridiagonal LU, three solves, one thread per vector); the
+//      (pivoted tridiagonal LU, three solves or two in the complex64 form,
+//      one thread per vector); the
 //      eigenvector interleaves (q_0, p_0, q_1, p_1, ...) with B q = sigma p;
 //   4. back-transformation: only the vectors a solve needs (U^H b, V phi)
 //      are pushed through the reflectors.
@@ -162,8 +163,9 @@ __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict
 
 // ----------------------------------------------------- inverse iteration
 // One thread per singular value: pivoted LU of T - sigma I (N = 2n), three
-// solves from a deterministic pseudo-random start, split into q (even) and
-// p (odd).  Scratch is one slab per wave with the 64 lanes interleaved
+// solves (two with float storage) from a deterministic pseudo-random start,
+// split into q (even) and p (odd).  Stored per row: U's diagonal, the
+// multiplier, the pivot flag and the iterate.  Scratch is one slab per wave with the 64 lanes interleaved
 // (element k of lane l at (wave N + k) 64 + l): every step is one 512-byte
 // coalesced access and each wave streams through consecutive memory.
 // G is the scratch column count (64 per wave).  Outputs QBt, PBt hold the
@@ -206,12 +208,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const double *e = eall + (size_t)b * n;
     const double sigma = sall[(size_t)b * n + i];
     S *work = reinterpret_cast<S *>(workv);
-    S *DD = work;                       // U diagonal
-    S *DU = work + (size_t)N * G;       // U first superdiagonal
-    S *DU2 = work + (size_t)2 * N * G;  // U second superdiagonal (pivoted rows only)
-    S *DL = work + (size_t)3 * N * G;   // multipliers
-    S *X = work + (size_t)4 * N * G;    // iterate
-    unsigned char *PV = reinterpret_cast<unsigned char *>(work + (size_t)5 * N * G);
+    // U's superdiagonals are not stored: the backward solve rebuilds them
+    // from the pivot flags, the multipliers and T (see backward below)
+    S *DD = work;                      // U diagonal
+    S *DL = work + (size_t)N * G;      // multipliers
+    S *X = work + (size_t)2 * N * G;   // iterate
+    unsigned char *PV = reinterpret_cast<unsigned char *>(work + (size_t)3 * N * G);
 #define AT(arr, k) arr[(wid * N + (k)) * 64 + lane]
     auto toff = [&](int k) { return (k & 1) ? e[k >> 1] : d[k >> 1]; };
     double tnorm = 0;
@@ -244,7 +246,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     AT(PV, k) = 0;
                     AT(DL, k) = fact;
                     AT(DD, k) = clampd(dd);
-                    AT(DU, k) = du;
                     AT(X, k) = x;
                     x = xn - fact * x;
                     dd = ddn - fact * du;
@@ -254,8 +255,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                     AT(PV, k) = 1;
                     AT(DL, k) = fact;
                     AT(DD, k) = clampd(dlk);
-                    AT(DU, k) = ddn;
-                    if (k < N - 2) AT(DU2, k) = dun;
                     AT(X, k) = xn;
                     x = x - fact * xn;
                     dd = du - fact * ddn;
@@ -266,37 +265,51 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
         AT(DD, N - 1) = clampd(dd);
         AT(X, N - 1) = x;
     }
-    // backward solve with U; returns 1 / max |x|
+    // backward solve with U; returns 1 / max |x|.  Row k of U holds
+    // (DD_k, u_k, w_k): pivoted, u_k = -sigma and w_k = t_{k+1} (the rows
+    // swapped in); unpivoted, w_k = 0 and u_k = t_k, or -l_{k-1} t_k when
+    // row k - 1 was pivoted (t_k: T's off-diagonal k, l: the multipliers).
+    // Pivot flags are fetched two blocks ahead, so that the flag of the row
+    // below each block is known before the block's conditional loads.
     auto backward = [&]() {
         double scale = 0, x2 = 0, x1 = 0;  // x[k+2], x[k+1]
-        int pvc[IU];
+        auto pvat = [&](int k) { return (k >= 0 && k < N - 1) ? (int)AT(PV, k) : 0; };
+        int pvc[IU], pvn[IU];
 #pragma unroll
         for (int u = 0; u < IU; ++u) {
-            const int k = Np - IU + u;
-            pvc[u] = (k < N - 2) ? AT(PV, k) : 0;
+            pvc[u] = pvat(Np - IU + u);
+            pvn[u] = pvat(Np - 2 * IU + u);
         }
         for (int kb = Np - IU; kb >= 0; kb -= IU) {
-            double xv[IU], du[IU], du2[IU], dd[IU];
-            int pvn[IU];
+            double xv[IU], dd[IU], dlm[IU], tk[IU + 1];
+            int pnn[IU];
+#pragma unroll
+            for (int u = 0; u <= IU; ++u) tk[u] = (kb + u < N - 1) ? toff(kb + u) : 0.0;
 #pragma unroll
             for (int u = 0; u < IU; ++u) {
                 const int k = kb + u;
+                const int pm = u > 0 ? pvc[u - 1] : pvn[IU - 1];  // row k - 1 pivoted
+                xv[u] = 0.0;
+                dd[u] = 1.0;
+                dlm[u] = 0.0;
                 if (k < N) {
                     xv[u] = AT(X, k);
-                    du[u] = (k < N - 1) ? AT(DU, k) : 0.0;
                     dd[u] = AT(DD, k);
-                    du2[u] = pvc[u] ? AT(DU2, k) : 0.0;
+                    if (pm && k >= 1) dlm[u] = AT(DL, k - 1);
                 }
-                const int kn = k - IU;  // next (lower) block
-                pvn[u] = (kn >= 0 && kn < N - 2) ? AT(PV, kn) : 0;
+                pnn[u] = pvat(kb - 2 * IU + u);
             }
 #pragma unroll
             for (int u = IU - 1; u >= 0; --u) {
                 const int k = kb + u;
                 if (k >= N) continue;
+                const int pm = u > 0 ? pvc[u - 1] : pvn[IU - 1];
                 double sv = xv[u];
-                if (k < N - 1) sv -= du[u] * x1;
-                if (pvc[u]) sv -= du2[u] * x2;
+                if (k < N - 1) {
+                    const double uk = pvc[u] ? -sigma : pm ? -dlm[u] * tk[u] : tk[u];
+                    sv -= uk * x1;
+                    if (pvc[u]) sv -= tk[u + 1] * x2;
+                }
                 sv /= dd[u];
                 AT(X, k) = sv;
                 x2 = x1;
@@ -304,7 +317,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 scale = fmax(scale, fabs(sv));
             }
 #pragma unroll
-            for (int u = 0; u < IU; ++u) pvc[u] = pvn[u];
+            for (int u = 0; u < IU; ++u) {
+                pvc[u] = pvn[u];
+                pvn[u] = pnn[u];
+            }
         }
         return scale > 0 ? 1.0 / scale : 1.0;
     };
@@ -753,7 +769,7 @@ static size_t inviter_cols(int n, int sub) { return (size_t)(n + 63) / 64 * 64 *
 
 size_t inviter_work_bytes(int n, int sub, bool f32) {
     size_t G = inviter_cols(n, sub), N = 2 * (size_t)n;
-    return 5 * N * G * (f32 ? sizeof(float) : sizeof(double)) + N * G + 256;
+    return 3 * N * G * (f32 ? sizeof(float) : sizeof(double)) + N * G + 256;
 }
 
 void launch_inviter(int n, const double *d, const double *e, const double *sigma, double *QBt,
