@@ -10,6 +10,9 @@
 //   k_qr_panel   one workgroup per matrix: unblocked Householder QR of the
 //                panel (H_i = I - tau_i v_i v_i^H, LAPACK zlarfg), explicit
 //                copy of the reflectors V (unit diagonal, zeros above);
+//                a tall panel whose LDS sub-blocks hold fewer than 8
+//                columns is factored as two 32-column halves, the first
+//                half's reflectors applied to the second by three GEMMs;
 //   G = V^H V    GEMM;  k_qr_tmat builds T with H_0 ... H_{nb-1} = I - V T V^H;
 //   trailing     A2 <- (I - V T^H V^H) A2 = A2 - V (T^H (V^H A2)), three GEMMs.
 // The reflectors stay below the diagonal of Y with tau per column, the form
@@ -465,6 +468,15 @@ __global__ __launch_bounds__(256) void k_qr_tmat(int C, int j0, int nbp, const c
     }
 }
 
+// zero rows [0, nr) of columns [0, nc) of every matrix's slab (ld ld,
+// stride sV)
+template <typename T>
+__global__ void k_zero_block(int nr, int nc, cx<T> *__restrict__ Vall, int ld, size_t sV) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nr * nc) return;
+    Vall[(size_t)blockIdx.y * sV + (size_t)(e / nr) * ld + e % nr] = cx<T>{0, 0};
+}
+
 template <typename T>
 __global__ void k_copy_upper(int C, const cx<T> *__restrict__ Yall, int ldy, size_t sY,
                              cx<T> *__restrict__ Rball) {
@@ -566,12 +578,38 @@ ia_status launch_geqrf(int R, int C, cx<T> *Y, int ldy, size_t sY, cx<T> *tau, c
     cx<T> *W2 = W + (size_t)batch * P * C;
     const cx<T> one{1, 0}, zero{0, 0}, mone{-1, 0};
     const size_t sV = (size_t)R * P, sT = (size_t)P * P, sW = (size_t)P * C;
+    constexpr int H = P / 2;
+    const size_t sTh = (size_t)H * H;
     for (int j0 = 0; j0 < C; j0 += P) {
-        const int nbp = std::min(P, C - j0);
-        launch_qr_panel<T>(R, C, j0, nbp, Y, ldy, sY, tau, Vw, P, batch, s);
-        const int N = C - j0 - nbp, M = R - j0;
-        if (N <= 0) continue;
+        const int nbp = std::min(P, C - j0), M = R - j0;
         const cx<T> *V0 = Vw + j0;  // rows j0.. of the explicit panel reflectors
+        if (nbp > H && (size_t)M * sizeof(cx<T>) * 8 > 128 * 1024) {
+            // the panel as two halves when its LDS sub-blocks are narrower
+            // than 8 columns: the panel kernel streams the columns right of
+            // each sub-block once per sub-block, so the first half's
+            // reflectors reach the second half as one blocked update (three
+            // GEMMs) instead of through H / SB sub-block passes (with 8-column
+            // sub-blocks the extra launches cost what the passes save)
+            launch_qr_panel<T>(R, C, j0, H, Y, ldy, sY, tau, Vw, P, batch, s);
+            const int nr = nbp - H;
+            cx<T> *Ar = Y + j0 + (size_t)(j0 + H) * ldy;
+            launch_gemm<T>('C', 'N', H, H, M, one, V0, R, sV, V0, R, sV, zero, G, H, sTh, batch, s);
+            hipLaunchKernelGGL((k_qr_tmat<T, H>), dim3(batch), dim3(256), 0, s, C, j0, H, G, tau,
+                               Tm);
+            launch_gemm<T>('C', 'N', H, nr, M, one, V0, R, sV, Ar, ldy, sY, zero, W, H, sW, batch, s);
+            launch_gemm<T>('C', 'N', H, nr, H, one, Tm, H, sTh, W, H, sW, zero, W2, H, sW, batch, s);
+            launch_gemm<T>('N', 'N', M, nr, H, mone, V0, R, sV, W2, H, sW, one, Ar, ldy, sY, batch,
+                           s);
+            // second half: its reflectors start at row j0 + H; rows j0 .. j0 + H - 1
+            // of their explicit columns are zero
+            launch_qr_panel<T>(R, C, j0 + H, nr, Y, ldy, sY, tau, Vw + (size_t)H * R, P, batch, s);
+            hipLaunchKernelGGL(k_zero_block<T>, dim3((H * nr + 255) / 256, batch), dim3(256), 0, s,
+                               H, nr, Vw + (size_t)H * R + j0, R, sV);
+        } else {
+            launch_qr_panel<T>(R, C, j0, nbp, Y, ldy, sY, tau, Vw, P, batch, s);
+        }
+        const int N = C - j0 - nbp;
+        if (N <= 0) continue;
         // G = V^H V
         launch_gemm<T>('C', 'N', nbp, nbp, M, one, V0, R, sV, V0, R, sV, zero, G, P, sT, batch, s);
         hipLaunchKernelGGL((k_qr_tmat<T, P>), dim3(batch), dim3(256), 0, s, C, j0, nbp, G, tau, Tm);
