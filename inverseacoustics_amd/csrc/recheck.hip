@@ -24,13 +24,14 @@
 //      Sturm bisection on the host, taken once it has settled);
 //   4. every candidate grid index j (complex64 GCV value within the recheck
 //      gap of the minimum: under the error that gap allows, the FP64 argmin
-//      is one of them): the LU of A(lambda_j) and one solve with [P I | P b]
-//      give tr(A^-1) and A^-1 b, hence G(lambda_j);
+//      is one of them): the Cholesky factor A(lambda_j) = L L^H and the
+//      explicit X = L^-1 give tr(A^-1) = |X|_F^2 and A^-1 b = X^H X b, hence
+//      G(lambda_j);
 //   5. the FP64 choice stands when its gap to the runner-up exceeds twice the
 //      two values' error bound (256 eps cond(A) + 1e-11 each); its x, p_s, W,
 //      lambda, index, s_1 and gap then replace the complex64 outputs.
 // A frequency that is not decided so (more than six candidates, no
-// settled Ritz value, a gap inside the bound, a singular factor) is listed
+// settled Ritz value, a gap inside the bound, a non-positive pivot) is listed
 // for the full FP64 sweep.
 #include <hip/hip_runtime.h>
 
@@ -220,33 +221,106 @@ __global__ void k_shift(int n, const cx<double> *__restrict__ G, size_t sG, cons
     A[(size_t)c * nn + e] = v;
 }
 
-// X_c (n x (n+1)) = P_c [I | b_{g[c]}] with (P B)[r] = B[idx[r]]
-__global__ void k_rhs_perm(int n, const int *__restrict__ idx, const int *__restrict__ g,
-                           const cx<double> *__restrict__ bq, cx<double> *__restrict__ X) {
-    const int c = blockIdx.y;
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x, ne = (size_t)n * (n + 1);
-    if (e >= ne) return;
-    const int r = (int)(e % n), col = (int)(e / n);
-    const int src = idx[(size_t)c * n + r];
-    cx<double> v{0, 0};
-    if (col < n)
-        v.re = src == col ? 1.0 : 0.0;
-    else
-        v = bq[(size_t)g[c] * n + src];
-    X[(size_t)c * ne + e] = v;
+// ---- candidates: A = L L^H (blocked right-looking Cholesky, lower) and
+// the explicit X = L^-1, so that tr(A^-1) = |X|_F^2 and A^-1 b = X^H (X b).
+// No pivots: A is Hermitian positive definite (lambda^2 I added to a Gram
+// matrix), and a non-positive pivot marks the candidate failed.
+constexpr int CB = 64;  // Cholesky block
+
+// Diagonal block k0 of every candidate (one workgroup each): factor the
+// lower triangle of A[k0:k0+jb, k0:k0+jb] in LDS (column j: the trailing
+// triangle loses a_ij conj(a_kj) / a_jj, one barrier per column), write L_kk
+// to Lb and its inverse (row-by-row forward substitution, four lanes per
+// column) to X; both blocks get zeros above the diagonal.
+__global__ __launch_bounds__(256) void k_potrf_diag(int n, int k0, const cx<double> *__restrict__ Aall,
+                                                   size_t sA, cx<double> *__restrict__ Lball,
+                                                   cx<double> *__restrict__ Xall, size_t sX,
+                                                   int *__restrict__ info) {
+    __shared__ cx<double> S[CB][CB + 1];
+    __shared__ cx<double> Z[CB][CB + 1];
+    __shared__ double dg[CB];
+    const int c = blockIdx.x, t = threadIdx.x;
+    const int jb = min(CB, n - k0);
+    const cx<double> *A = Aall + (size_t)c * sA;
+    for (int e = t; e < CB * CB; e += 256) {
+        const int i = e % CB, j = e / CB;
+        S[i][j] = (i < jb && j < jb && i >= j) ? A[(size_t)(k0 + j) * n + k0 + i] : cx<double>{0, 0};
+        Z[i][j] = cx<double>{0, 0};
+    }
+    __syncthreads();
+    for (int j = 0; j < jb; ++j) {
+        double p = S[j][j].re;
+        if (!(p > 0)) {
+            if (t == 0 && info[c] == 0) info[c] = k0 + j + 1;
+            p = 1.0;
+        }
+        const double ip = 1.0 / p;
+        const int m = jb - j - 1;
+        for (int e = t; e < m * m; e += 256) {
+            const int ii = e % m, kk = e / m;
+            if (ii >= kk) {
+                const int i = j + 1 + ii, k = j + 1 + kk;
+                S[i][k] = S[i][k] - ip * (S[i][j] * conj(S[k][j]));
+            }
+        }
+        if (t == 0) dg[j] = p;
+        __syncthreads();
+    }
+    // L: l_jj = sqrt(p_j), l_ij = a_ij / sqrt(p_j) (column j no longer changes)
+    for (int e = t; e < CB * CB; e += 256) {
+        const int i = e % CB, j = e / CB;
+        if (i < jb && j < jb && i >= j) {
+            const double r = sqrt(dg[j]);
+            S[i][j] = i == j ? cx<double>{r, 0} : (1.0 / r) * S[i][j];
+        }
+    }
+    __syncthreads();
+    cx<double> *Lb = Lball + (size_t)c * sA;
+    for (int e = t; e < jb * jb; e += 256) {
+        const int i = e % jb, j = e / jb;
+        Lb[(size_t)(k0 + j) * n + k0 + i] = S[i][j];
+    }
+    // Z = L_kk^-1: z_ic = (delta_ic - sum_{k=c}^{i-1} l_ik z_kc) / l_ii
+    const int col = t >> 2, part = t & 3;
+    for (int i = 0; i < jb; ++i) {
+        cx<double> acc{0, 0};
+        if (col <= i)
+            for (int k = col + part; k < i; k += 4) acc = acc + S[i][k] * Z[k][col];
+        acc.re += __shfl_xor(acc.re, 1, 64);
+        acc.im += __shfl_xor(acc.im, 1, 64);
+        acc.re += __shfl_xor(acc.re, 2, 64);
+        acc.im += __shfl_xor(acc.im, 2, 64);
+        if (part == 0 && col <= i) {
+            const cx<double> rhs{(col == i ? 1.0 : 0.0) - acc.re, -acc.im};
+            Z[i][col] = (1.0 / S[i][i].re) * rhs;
+        }
+        __syncthreads();
+    }
+    cx<double> *X = Xall + (size_t)c * sX;
+    for (int e = t; e < jb * jb; e += 256) {
+        const int i = e % jb, j = e / jb;
+        X[(size_t)(k0 + j) * n + k0 + i] = Z[i][j];
+    }
 }
 
-// per candidate: tr(A^-1) = sum_i Re X[i, i], |A^-1 b|^2 = |X[:, n]|^2
+// B_c = b_{g[c]} (n per candidate)
+__global__ void k_cand_rhs(int n, const int *__restrict__ g, const cx<double> *__restrict__ bq,
+                           cx<double> *__restrict__ B) {
+    const int c = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) B[(size_t)c * n + i] = bq[(size_t)g[c] * n + i];
+}
+
+// per candidate (X_c: n x (n+1), L^-1 then A^-1 b): tr(A^-1) = |L^-1|_F^2,
+// |A^-1 b|^2
 __global__ __launch_bounds__(256) void k_gcv_terms(int n, const cx<double> *__restrict__ X,
                                                   double *__restrict__ out) {
     __shared__ double sa[4], sb[4];
     const int c = blockIdx.x, t = threadIdx.x;
     const cx<double> *Xc = X + (size_t)c * n * (n + 1);
     double tr = 0, ny = 0;
-    for (int i = t; i < n; i += 256) {
-        tr += Xc[(size_t)i * n + i].re;
-        ny += abs2(Xc[(size_t)n * n + i]);
-    }
+    for (size_t e = t; e < (size_t)n * n; e += 256) tr += abs2(Xc[e]);
+    for (int i = t; i < n; i += 256) ny += abs2(Xc[(size_t)n * n + i]);
     tr = wsum64(tr);
     ny = wsum64(ny);
     if ((t & 63) == 0) {
@@ -311,11 +385,11 @@ double tri_top(const double *a, const double *b, int k) {
 // Device workspace of the recheck, kept per host thread (buffers only grow).
 struct RecheckCache {
     DevBuf kk, bq, Dm, Y, ipiv, pidx, info, G, lq, lqp, lw, lbp, lns, ldone, lact, lal, lbe, A, X,
-        cpiv, cidx, cinfo, cg, clam2, cterm, cq, yv, xY, x, qv, ps, pw;
+        Lb, Tb, cb, cy, cinfo, cg, clam2, cterm, cq, yv, xY, x, qv, ps, pw;
     void release() {
         for (DevBuf *p : {&kk, &bq, &Dm, &Y, &ipiv, &pidx, &info, &G, &lq, &lqp, &lw, &lbp, &lns, &ldone,
-                          &lact, &lal, &lbe, &A, &X, &cpiv, &cidx, &cinfo, &cg, &clam2, &cterm, &cq,
-                          &yv, &xY, &x, &qv, &ps, &pw})
+                          &lact, &lal, &lbe, &A, &X, &Lb, &Tb, &cb, &cy, &cinfo, &cg, &clam2, &cterm,
+                          &cq, &yv, &xY, &x, &qv, &ps, &pw})
             p->reset();
     }
 };
@@ -509,8 +583,10 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
         if (nc > 0) {
             IA_TRY(W.A.alloc(sizeof(cx<double>) * nc * ll));
             IA_TRY(W.X.alloc(sizeof(cx<double>) * nc * lx));
-            IA_TRY(W.cpiv.alloc(sizeof(int) * nc * (size_t)l));
-            IA_TRY(W.cidx.alloc(2 * sizeof(int) * nc * (size_t)l));
+            IA_TRY(W.Lb.alloc(sizeof(cx<double>) * nc * ll));
+            IA_TRY(W.Tb.alloc(sizeof(cx<double>) * nc * (size_t)CB * l));
+            IA_TRY(W.cb.alloc(sizeof(cx<double>) * nc * (size_t)l));
+            IA_TRY(W.cy.alloc(sizeof(cx<double>) * nc * (size_t)l));
             IA_TRY(W.cinfo.alloc(sizeof(int) * nc));
             IA_TRY(W.cg.alloc(sizeof(int) * nc));
             IA_TRY(W.clam2.alloc(sizeof(double) * nc));
@@ -523,14 +599,43 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
                                G, ll, W.cg.as<int>(), W.clam2.as<double>(), A);
             IA_HIP(hipGetLastError());
             IA_HIP(hipMemsetAsync(W.cinfo.p, 0, sizeof(int) * nc, s));
-            IA_TRY(getrf_batched<double>(l, A, l, ll, W.cpiv.as<int>(), W.cinfo.as<int>(), nc, s));
+            // X starts at zero: the blocks above the diagonal stay so
+            IA_HIP(hipMemsetAsync(X, 0, sizeof(cx<double>) * nc * lx, s));
+            cx<double> *Lb = W.Lb.as<cx<double>>(), *Tb = W.Tb.as<cx<double>>();
+            const cx<double> one{1, 0}, zero{0, 0}, mone{-1, 0};
+            for (int k0 = 0; k0 < l; k0 += CB) {
+                const int jb = std::min(CB, l - k0), k1 = k0 + jb, rest = l - k1;
+                hipLaunchKernelGGL(k_potrf_diag, dim3(nc), dim3(256), 0, s, l, k0, A, ll, Lb, X, lx,
+                                   W.cinfo.as<int>());
+                IA_HIP(hipGetLastError());
+                if (rest > 0) {
+                    // L21 = A21 L11^-H, A22 -= L21 L21^H
+                    launch_gemm<double>('N', 'C', rest, jb, jb, one, A + (size_t)k0 * l + k1, l, ll,
+                                        X + (size_t)k0 * l + k0, l, lx, zero,
+                                        Lb + (size_t)k0 * l + k1, l, ll, nc, s);
+                    launch_gemm<double>('N', 'C', rest, rest, jb, mone, Lb + (size_t)k0 * l + k1, l,
+                                        ll, Lb + (size_t)k0 * l + k1, l, ll, one,
+                                        A + (size_t)k1 * l + k1, l, ll, nc, s);
+                }
+            }
             IA_HIP(hipEventRecord(ev[4], s));
-            IA_TRY(lu_perm_index(l, W.cpiv.as<int>(), W.cidx.as<int>(),
-                                 W.cidx.as<int>() + (size_t)l * nc, nc, s));
-            hipLaunchKernelGGL(k_rhs_perm, dim3((unsigned)((lx + 255) / 256), nc), dim3(256), 0, s,
-                               l, W.cidx.as<int>(), W.cg.as<int>(), bq, X);
+            // X = L^-1 below the diagonal blocks, one block row at a time:
+            // X[i, 0:i] = -X_ii (L[i, 0:i] X[0:i, 0:i])
+            for (int i0 = CB; i0 < l; i0 += CB) {
+                const int jb = std::min(CB, l - i0);
+                launch_gemm<double>('N', 'N', jb, i0, i0, one, Lb + i0, l, ll, X, l, lx, zero, Tb,
+                                    CB, (size_t)CB * l, nc, s);
+                launch_gemm<double>('N', 'N', jb, i0, jb, mone, X + (size_t)i0 * l + i0, l, lx, Tb,
+                                    CB, (size_t)CB * l, zero, X + i0, l, lx, nc, s);
+            }
+            // A^-1 b = X^H (X b) into column l of X
+            hipLaunchKernelGGL(k_cand_rhs, dim3((unsigned)((l + 255) / 256), nc), dim3(256), 0, s, l,
+                               W.cg.as<int>(), bq, W.cb.as<cx<double>>());
             IA_HIP(hipGetLastError());
-            IA_TRY(getrs_prepermuted<double>(l, l + 1, A, l, ll, X, l, lx, nc, s));
+            launch_gemm<double>('N', 'N', l, 1, l, one, X, l, lx, W.cb.as<cx<double>>(), l,
+                                (size_t)l, zero, W.cy.as<cx<double>>(), l, (size_t)l, nc, s);
+            launch_gemm<double>('C', 'N', l, 1, l, one, X, l, lx, W.cy.as<cx<double>>(), l,
+                                (size_t)l, zero, X + ll, l, lx, nc, s);
             IA_HIP(hipEventRecord(ev[5], s));
             hipLaunchKernelGGL(k_gcv_terms, dim3(nc), dim3(256), 0, s, l, X, W.cterm.as<double>());
             IA_HIP(hipGetLastError());
@@ -548,7 +653,7 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
             for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&t[i], ev[i], ev[i + 1]);
             std::fprintf(stderr,
                          "RECHECK %d frequencies, %d candidates: H_e %.2f ms, G %.2f, Lanczos "
-                         "%.2f, candidate LU %.2f, solves %.2f\n",
+                         "%.2f, candidate Cholesky %.2f, inverses and solves %.2f\n",
                          nq, nc, t[0], t[1], t[2], t[3], t[4]);
         }
         const double eps = 2.220446049250313e-16;
