@@ -361,6 +361,14 @@ struct SplitLoader {
                 return;
             }
         }
+        if (!KC && gr < nr && k0 + kg * 8 + 8 <= K) {
+            // the thread's 8 k (one row, eight columns) all in range: eight
+            // plain loads, each a 512-byte access of 64 consecutive rows
+            const cx<float> *p = X + (size_t)(k0 + kg * 8) * ldx + gr;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] = p[(size_t)j * ldx];
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int gk = k0 + kg * 8 + j;
