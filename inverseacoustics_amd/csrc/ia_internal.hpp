@@ -150,6 +150,13 @@ template <typename T>
 ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int *info, int batch,
                         hipStream_t s, int *work = nullptr);
 // Solve op(A) X = B with A = P^T L U from getrf; op 'N' or 'C'.  B n x nrhs.
+// x <- A^-1 b for one right-hand side per matrix (A = P^T L U from
+// getrf_batched, op 'N'), for a few matrices: many workgroups per matrix,
+// one launch per 64-row block; S: n entries per matrix of workspace (stride
+// strideB).
+template <typename T>
+ia_status getrs1_spread(int n, const cx<T> *A, int lda, size_t strideA, const int *ipiv,
+                        cx<T> *B, size_t strideB, cx<T> *S, int batch, hipStream_t s);
 template <typename T>
 ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_t strideA,
                         const int *ipiv, cx<T> *B, int ldb, size_t strideB, int batch,

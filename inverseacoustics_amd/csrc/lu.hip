@@ -1238,6 +1238,119 @@ ia_status lu_perm_index(int n, const int *ipiv, int *idx, int *inv, int batch, h
     return IA_OK;
 }
 
+// ---- one right-hand side for a few matrices (A = P^T L U, op 'N'): the
+// single-workgroup solve above leaves most of the device idle at small
+// batches, so here each 64-row block of each triangle is one launch whose
+// workgroups each solve the block's diagonal triangle (redundantly, one
+// wave) and update their own GM_ROWS rows of the rest.  The solved blocks
+// go to a second vector (the block's unsolved entries stay readable by
+// every workgroup of the launch): forward from B into S, backward from S
+// into B.
+constexpr int GM_ROWS = 256;
+
+// B <- P B: the interchanges in order, in LDS (one workgroup per matrix)
+template <typename T>
+__global__ __launch_bounds__(256) void k_perm_rhs(int n, const int *__restrict__ ipivall,
+                                                 cx<T> *__restrict__ Ball, size_t sB) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *xs = reinterpret_cast<cx<T> *>(smem_raw);
+    int *ps = reinterpret_cast<int *>(xs + n);
+    const int b = blockIdx.x, t = threadIdx.x;
+    cx<T> *B = Ball + (size_t)b * sB;
+    const int *ip = ipivall + (size_t)b * n;
+    for (int r = t; r < n; r += 256) {
+        xs[r] = B[r];
+        ps[r] = ip[r];
+    }
+    __syncthreads();
+    if (t == 0)
+        for (int c = 0; c < n; ++c) {
+            const int p = ps[c];
+            if (p != c) {
+                const cx<T> tmp = xs[c];
+                xs[c] = xs[p];
+                xs[p] = tmp;
+            }
+        }
+    __syncthreads();
+    for (int r = t; r < n; r += 256) B[r] = xs[r];
+}
+
+// LOWER: x_blk = L_blk^-1 (unit) src_blk into dst, rows below lose
+// L[r, blk] x_blk in src.  Upper: x_blk = U_blk^-1 src_blk into dst, rows
+// above lose U[r, blk] x_blk in src.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_block(int n, int i0, const cx<T> *__restrict__ Aall,
+                                                   int lda, size_t sA, cx<T> *__restrict__ srcall,
+                                                   cx<T> *__restrict__ dstall, size_t sB) {
+    __shared__ cx<T> Ds[G1][G1 + 1];  // Ds[j][l] = A[i0 + l, i0 + j]
+    __shared__ cx<T> xb[G1];
+    const int b = blockIdx.y, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const cx<T> *A = Aall + (size_t)b * sA;
+    cx<T> *src = srcall + (size_t)b * sB;
+    cx<T> *dst = dstall + (size_t)b * sB;
+    const int jb = min(G1, n - i0);
+    for (int e = t; e < jb * jb; e += 256) {
+        const int l = e % jb, j = e / jb;
+        Ds[j][l] = A[(size_t)(i0 + j) * lda + i0 + l];
+    }
+    __syncthreads();
+    if (wv == 0) {
+        cx<T> x = lane < jb ? src[i0 + lane] : cx<T>{0, 0};
+        if (LOWER) {
+            for (int j = 0; j < jb; ++j) {
+                const cx<T> xj = bcast_cx(x, j);
+                if (lane > j && lane < jb) x = x - Ds[j][lane] * xj;
+            }
+        } else {
+            const cx<T> dl = lane < jb ? cdiv(cx<T>{1, 0}, Ds[lane][lane]) : cx<T>{0, 0};
+            for (int j = jb - 1; j >= 0; --j) {
+                if (lane == j) x = x * dl;
+                const cx<T> xj = bcast_cx(x, j);
+                if (lane < j) x = x - Ds[j][lane] * xj;
+            }
+        }
+        if (lane < jb) {
+            xb[lane] = x;
+            if (blockIdx.x == 0) dst[i0 + lane] = x;
+        }
+    }
+    __syncthreads();
+    const int r0 = (LOWER ? i0 + jb : 0) + blockIdx.x * GM_ROWS;
+    const int r1 = min(r0 + GM_ROWS, LOWER ? n : i0);
+    for (int r = r0 + t; r < r1; r += 256) {
+        cx<T> acc{0, 0};
+#pragma unroll 16
+        for (int j = 0; j < jb; ++j) acc = acc + A[(size_t)(i0 + j) * lda + r] * xb[j];
+        src[r] = src[r] - acc;
+    }
+}
+
+template <typename T>
+ia_status getrs1_spread(int n, const cx<T> *A, int lda, size_t strideA, const int *ipiv,
+                        cx<T> *B, size_t strideB, cx<T> *S, int batch, hipStream_t s) {
+    if (n <= 0 || batch <= 0) return IA_OK;
+    const size_t lds = (sizeof(cx<T>) + sizeof(int)) * (size_t)n;
+    if (lds > 64 * 1024) return getrs_batched<T>('N', n, 1, A, lda, strideA, ipiv, B, n, strideB, batch, s);
+    (void)hipFuncSetAttribute((const void *)k_perm_rhs<T>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_perm_rhs<T>, dim3(batch), dim3(256), lds, s, n, ipiv, B, strideB);
+    const int nblk = (n + G1 - 1) / G1;
+    auto groups = [](int rows) { return (unsigned)std::max(1, (rows + GM_ROWS - 1) / GM_ROWS); };
+    for (int bk = 0; bk < nblk; ++bk) {
+        const int i0 = bk * G1, jb = std::min(G1, n - i0);
+        hipLaunchKernelGGL((k_trsv_block<T, true>), dim3(groups(n - i0 - jb), batch), dim3(256), 0,
+                           s, n, i0, A, lda, strideA, B, S, strideB);
+    }
+    for (int bk = nblk - 1; bk >= 0; --bk) {
+        const int i0 = bk * G1;
+        hipLaunchKernelGGL((k_trsv_block<T, false>), dim3(groups(i0), batch), dim3(256), 0, s, n,
+                           i0, A, lda, strideA, S, B, strideB);
+    }
+    IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
 template <typename T>
 ia_status getrs_prepermuted(int n, int nrhs, const cx<T> *A, int lda, size_t strideA, cx<T> *B,
                             int ldb, size_t strideB, int batch, hipStream_t s) {
@@ -1294,6 +1407,8 @@ template ia_status getrf_batched<double>(int, cx<double> *, int, size_t, int *, 
                                          hipStream_t, int *);
 template ia_status getrf_batched<float>(int, cx<float> *, int, size_t, int *, int *, int,
                                         hipStream_t, int *);
+template ia_status getrs1_spread<double>(int, const cx<double> *, int, size_t, const int *,
+                                         cx<double> *, size_t, cx<double> *, int, hipStream_t);
 template ia_status getrs_batched<double>(char, int, int, const cx<double> *, int, size_t,
                                          const int *, cx<double> *, int, size_t, int, hipStream_t);
 template ia_status getrs_batched<float>(char, int, int, const cx<float> *, int, size_t,

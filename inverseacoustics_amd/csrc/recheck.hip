@@ -311,16 +311,21 @@ __global__ void k_cand_rhs(int n, const int *__restrict__ g, const cx<double> *_
     if (i < n) B[(size_t)c * n + i] = bq[(size_t)g[c] * n + i];
 }
 
-// per candidate (X_c: n x (n+1), L^-1 then A^-1 b): tr(A^-1) = |L^-1|_F^2,
-// |A^-1 b|^2
-__global__ __launch_bounds__(256) void k_gcv_terms(int n, const cx<double> *__restrict__ X,
-                                                  double *__restrict__ out) {
+// per candidate (X_c: n x (n+1), L^-1 then A^-1 b): tr(A^-1) = |L^-1|_F^2
+// and |A^-1 b|^2, in GT_PARTS slices per candidate (so that the reads of
+// L^-1 spread over the device), the slices' sums added in a fixed order
+constexpr int GT_PARTS = 64;
+__global__ __launch_bounds__(256) void k_gcv_parts(int n, const cx<double> *__restrict__ X,
+                                                  double *__restrict__ part) {
     __shared__ double sa[4], sb[4];
-    const int c = blockIdx.x, t = threadIdx.x;
+    const int p = blockIdx.x, c = blockIdx.y, t = threadIdx.x;
     const cx<double> *Xc = X + (size_t)c * n * (n + 1);
+    const size_t nn = (size_t)n * n, per = (nn + GT_PARTS - 1) / GT_PARTS;
+    const size_t e0 = (size_t)p * per, e1 = e0 + per < nn ? e0 + per : nn;
     double tr = 0, ny = 0;
-    for (size_t e = t; e < (size_t)n * n; e += 256) tr += abs2(Xc[e]);
-    for (int i = t; i < n; i += 256) ny += abs2(Xc[(size_t)n * n + i]);
+    for (size_t e = e0 + t; e < e1; e += 256) tr += abs2(Xc[e]);
+    if (p == 0)
+        for (int i = t; i < n; i += 256) ny += abs2(Xc[nn + i]);
     tr = wsum64(tr);
     ny = wsum64(ny);
     if ((t & 63) == 0) {
@@ -329,11 +334,21 @@ __global__ __launch_bounds__(256) void k_gcv_terms(int n, const cx<double> *__re
     }
     __syncthreads();
     if (t == 0) {
-        const double T = (sa[0] + sa[1]) + (sa[2] + sa[3]);
-        const double N = (sb[0] + sb[1]) + (sb[2] + sb[3]);
-        out[2 * c] = T;
-        out[2 * c + 1] = N;
+        part[((size_t)c * GT_PARTS + p) * 2] = (sa[0] + sa[1]) + (sa[2] + sa[3]);
+        part[((size_t)c * GT_PARTS + p) * 2 + 1] = (sb[0] + sb[1]) + (sb[2] + sb[3]);
     }
+}
+
+__global__ void k_gcv_terms(int nc, const double *__restrict__ part, double *__restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double T = 0, N = 0;
+    for (int p = 0; p < GT_PARTS; ++p) {
+        T += part[((size_t)c * GT_PARTS + p) * 2];
+        N += part[((size_t)c * GT_PARTS + p) * 2 + 1];
+    }
+    out[2 * c] = T;
+    out[2 * c + 1] = N;
 }
 
 // y_q = X_{c[q]}[:, n] (zero when c[q] < 0)
@@ -385,11 +400,11 @@ double tri_top(const double *a, const double *b, int k) {
 // Device workspace of the recheck, kept per host thread (buffers only grow).
 struct RecheckCache {
     DevBuf kk, bq, Dm, Y, ipiv, pidx, info, G, lq, lqp, lw, lbp, lns, ldone, lact, lal, lbe, A, X,
-        Lb, Tb, cb, cy, cinfo, cg, clam2, cterm, cq, yv, xY, x, qv, ps, pw;
+        Lb, Tb, cb, cy, cinfo, cg, clam2, cterm, cpart, qs, cq, yv, xY, x, qv, ps, pw;
     void release() {
-        for (DevBuf *p : {&kk, &bq, &Dm, &Y, &ipiv, &pidx, &info, &G, &lq, &lqp, &lw, &lbp, &lns, &ldone,
-                          &lact, &lal, &lbe, &A, &X, &Lb, &Tb, &cb, &cy, &cinfo, &cg, &clam2, &cterm,
-                          &cq, &yv, &xY, &x, &qv, &ps, &pw})
+        for (DevBuf *p : {&kk, &bq, &Dm, &Y, &ipiv, &pidx, &info, &G, &lq, &lqp, &lw, &lbp, &lns,
+                          &ldone, &lact, &lal, &lbe, &A, &X, &Lb, &Tb, &cb, &cy, &cinfo, &cg, &clam2,
+                          &cterm, &cpart, &qs, &cq, &yv, &xY, &x, &qv, &ps, &pw})
             p->reset();
     }
 };
@@ -591,6 +606,7 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
             IA_TRY(W.cg.alloc(sizeof(int) * nc));
             IA_TRY(W.clam2.alloc(sizeof(double) * nc));
             IA_TRY(W.cterm.alloc(2 * sizeof(double) * nc));
+            IA_TRY(W.cpart.alloc(2 * sizeof(double) * nc * (size_t)GT_PARTS));
             IA_HIP(hipMemcpyAsync(W.cg.p, cg.data(), sizeof(int) * nc, hipMemcpyHostToDevice, s));
             IA_HIP(hipMemcpyAsync(W.clam2.p, clam2.data(), sizeof(double) * nc,
                                   hipMemcpyHostToDevice, s));
@@ -637,7 +653,10 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
             launch_gemm<double>('C', 'N', l, 1, l, one, X, l, lx, W.cy.as<cx<double>>(), l,
                                 (size_t)l, zero, X + ll, l, lx, nc, s);
             IA_HIP(hipEventRecord(ev[5], s));
-            hipLaunchKernelGGL(k_gcv_terms, dim3(nc), dim3(256), 0, s, l, X, W.cterm.as<double>());
+            hipLaunchKernelGGL(k_gcv_parts, dim3(GT_PARTS, nc), dim3(256), 0, s, l, X,
+                               W.cpart.as<double>());
+            hipLaunchKernelGGL(k_gcv_terms, dim3((nc + 63) / 64), dim3(64), 0, s, nc,
+                               W.cpart.as<double>(), W.cterm.as<double>());
             IA_HIP(hipGetLastError());
             IA_HIP(hipMemcpyAsync(hterm.data(), W.cterm.p, 2 * sizeof(double) * nc,
                                   hipMemcpyDeviceToHost, s));
@@ -722,7 +741,8 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
                             cx<double>{0, 0}, xY, m, m, nq, s);
         launch_row_gather<double>(m, 1, pidx + (size_t)m * nq, xY, m, m, xv, m, m, nq, s);
         IA_HIP(hipMemcpyAsync(qv, xv, sizeof(cx<double>) * nq * (size_t)m, hipMemcpyDeviceToDevice, s));
-        IA_TRY(getrs_batched<double>('N', m, 1, Dm, m, mm, ipiv, qv, m, m, nq, s));
+        IA_TRY(W.qs.alloc(sizeof(cx<double>) * nq * (size_t)m));
+        IA_TRY(getrs1_spread<double>(m, Dm, m, mm, ipiv, qv, m, W.qs.as<cx<double>>(), nq, s));
         launch_esm_msq_apply<double>(m, nq, kk, rho_c, r_s, src, qv, m, ps, m, s);
         launch_power<double>(m, xv, m, ps, m, D.area / m, W.pw.as<double>(), nq, s);
         IA_HIP(hipGetLastError());
