@@ -243,7 +243,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64+f64",
-        "products": "complex64 products of depth >= 512 as six bf16 partial products of "
+        "products": "complex64 products of depth >= 64 as six bf16 partial products of "
                     "three-way split FP32 operands with FP32 accumulation (FP32-level); the "
                     "others on FP32 / FP64 matrix cores (3M); selection and eigen-solve in FP64",
         "data": "synthetic (plate, Rayleigh-integral pressures, seeded 30 dB SNR noise)",

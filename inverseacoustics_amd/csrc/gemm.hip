@@ -24,7 +24,7 @@
 // lanes hit 16 consecutive words; two LDS buffers, the next stage loaded
 // from global memory into registers while the current one is multiplied.
 //
-// Deep complex64 products (depth >= 128, both dimensions > 32) take a second
+// Deep complex64 products (depth >= 64, both dimensions > 32) take a second
 // kernel, k_cgemm_x6, on the bf16 matrix cores: the FP32 operands are split
 // exactly into three bf16 parts and six partial products are accumulated in
 // FP32, which keeps FP32-level products (see k_cgemm_x6 below).
@@ -567,17 +567,17 @@ void launch_cand(int cand, int M, int N, int K, cx<T> alpha, const cx<T> *A, int
 }
 
 // Built-in choice (measured on the sweep's products, batch 256): complex64
-// products of depth >= 128 with both dimensions > 32 on the bf16 matrix
-// cores (1.33x the FP32 3M kernel at K >= 512, on a par at K = 128; at
-// K <= 64 the splitting and two barriers per stage of its few stages make
-// it slower); otherwise the 3M kernel with 64 x 64 tiles, 32-row (column)
+// products of depth >= 64 with both dimensions > 32 on the bf16 matrix
+// cores (1.33x the FP32 3M kernel at K >= 512, ahead by 10-20 % at K = 64
+// to 128; at K = 32 its one stage, split and barriers make it slower);
+// otherwise the 3M kernel with 64 x 64 tiles, 32-row (column)
 // tiles when op(A) has at most 32 rows (C at most 32 columns), and 32 x 32
 // tiles when 64 x 64 ones would leave fewer than four workgroups per CU
 // (small batches).  Four 64 x 64 workgroups share a CU: their barriers,
 // epilogues and global-load waits overlap each other's MFMAs, which larger
 // tiles at lower occupancy measured slower at.
 int default_cand(bool f32, int M, int N, int K, int batch) {
-    if (f32 && K >= 128 && M > 32 && N > 32) return GC_X6;
+    if (f32 && K >= 64 && M > 32 && N > 32) return GC_X6;
     const double wg64 = (double)((M + 63) / 64) * ((N + 63) / 64) * batch;
     if (M <= 32 && N <= 32) return GC_T11;
     if (M <= 32) return GC_T12;
