@@ -210,6 +210,18 @@ __global__ __launch_bounds__(LZ_T) void k_lz_update(int n, cx<double> *__restric
     if (t == 0) bprev[b] = be;
 }
 
+// G[i][j] = conj(G[j][i]) above the diagonal blocks the block-row products
+// left unwritten (j >= (i / GB + 1) GB, column-major n x n per matrix)
+constexpr int GB = 128;
+__global__ void k_mirror_upper(int n, cx<double> *__restrict__ Gall, size_t sG) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const int i = (int)(e % n), j = (int)(e / n);
+    if (j < (i / GB + 1) * GB) return;
+    cx<double> *G = Gall + (size_t)blockIdx.y * sG;
+    G[e] = conj(G[(size_t)i * n + j]);
+}
+
 // A_c = G_{g[c]} + lam2[c] I (n x n per candidate)
 __global__ void k_shift(int n, const cx<double> *__restrict__ G, size_t sG, const int *__restrict__ g,
                         const double *__restrict__ lam2, cx<double> *__restrict__ A) {
@@ -516,8 +528,15 @@ ia_status gcv_recheck(const ia_sweep_desc &D, const std::vector<int> &L,
         IA_TRY(getrs_c_permuted<double>(m, l, Dm, m, mm, Y, m, ml, nq, s));
         IA_TRY(lu_perm_index(m, ipiv, pidx, pidx + (size_t)m * nq, nq, s));
         IA_HIP(hipEventRecord(ev[1], s));
-        launch_gemm<double>('C', 'N', l, l, m, cx<double>{1, 0}, Y, m, ml, Y, m, ml,
-                            cx<double>{0, 0}, G, l, ll, nq, s);
+        // G is Hermitian: its block rows up to the diagonal (GB rows each,
+        // 56 % of the products for l = 1024), then the rest mirrored
+        for (int r0 = 0; r0 < l; r0 += GB) {
+            const int rb = std::min(GB, l - r0);
+            launch_gemm<double>('C', 'N', rb, r0 + rb, m, cx<double>{1, 0}, Y + (size_t)r0 * m, m,
+                                ml, Y, m, ml, cx<double>{0, 0}, G + r0, l, ll, nq, s);
+        }
+        hipLaunchKernelGGL(k_mirror_upper, dim3((unsigned)((ll + 255) / 256), nq), dim3(256), 0, s,
+                           l, G, ll);
         IA_HIP(hipGetLastError());
         IA_HIP(hipEventRecord(ev[2], s));
         // ---- 3. s_1^2 by Lanczos: LZ_FIRST steps, then LZ_MORE at a time
