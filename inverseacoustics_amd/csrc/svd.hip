@@ -53,6 +53,27 @@ __device__ __forceinline__ int sturm_count(int N, const double *t2, double x, do
     return cnt;
 }
 
+// The count and S = d/dx log|det(T - x I)| = sum_i q_i' / q_i from the same
+// recurrence (q_{i+1}' = -1 + t2_i q_i' / q_i^2): a Newton step for the
+// root of det is 1 / S.
+__device__ __forceinline__ int sturm_count_d(int N, const double *t2, double x, double pivmin,
+                                             double &S) {
+    int cnt = 0;
+    double q = -x, dq = -1.0, sum = 0.0;
+    if (q < 0) cnt++;
+    for (int i = 0; i < N - 1; ++i) {
+        if (fabs(q) < pivmin) q = -pivmin;
+        const double r = recip(q);
+        sum = __builtin_fma(dq, r, sum);
+        dq = __builtin_fma(t2[i] * r * r, dq, -1.0);
+        q = __builtin_fma(-t2[i], r, -x);
+        if (q < 0) cnt++;
+    }
+    if (fabs(q) < pivmin) q = -pivmin;
+    S = __builtin_fma(dq, recip(q), sum);
+    return cnt;
+}
+
 // Workgroups of 1024 threads; K consecutive lanes share one value (1024 / K
 // values per workgroup, blockIdx.x picks the range): each step the K lanes
 // take Sturm counts at the K interior points that cut the interval into
@@ -61,8 +82,13 @@ __device__ __forceinline__ int sturm_count(int N, const double *t2, double x, do
 // device with a shorter chain per value.
 constexpr int GL = 10, GPTS = 1 << GL;
 constexpr int GGEO = GPTS / 2, GLIN = GPTS - GGEO;  // geometric / linear grid points
-// Stopping width max(rtol hi, atol bound).
-template <int K>
+// Stopping width max(rtol hi, atol bound).  NEWTON (one lane per value):
+// once the bracket holds only its value and is narrower than 1e-3 of it,
+// the Sturm pass also returns the Newton step for det(T - x I), and the
+// next point is the Newton iterate when it falls inside the bracket (the
+// midpoint otherwise); a step below rtol x / 4 ends the search at the
+// iterate (quadratic convergence leaves it far inside the tolerance).
+template <int K, bool NEWTON>
 __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict__ dall,
                                                 const double *__restrict__ eall,
                                                 double *__restrict__ sall, double rtol,
@@ -131,15 +157,30 @@ __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict
                 jl = jm;
         }
         double lo = gpt(jl), hi = gpt(jh);
+        int clo = jl > 0 ? cgrid[jl] : 0, chi = jh < GPTS ? cgrid[jh] : n;
+        double xn = -1.0, res = -1.0;  // Newton iterate; result once it has converged
         for (int it = 0; it < 128; ++it) {
             if (hi - lo <= fmax(rtol * hi, atol * bound)) break;
             if (K == 1) {
-                const double mid = 0.5 * (lo + hi);
-                const int c = sturm_count(N, t2, mid, pivmin) - n;
-                if (c > target)
-                    hi = mid;
-                else
-                    lo = mid;
+                const bool nt = NEWTON && clo == target && chi == target + 1 && hi - lo <= 1e-3 * hi;
+                const double p = (nt && xn > lo && xn < hi) ? xn : 0.5 * (lo + hi);
+                double S = 0.0;
+                const int c = (nt ? sturm_count_d(N, t2, p, pivmin, S) : sturm_count(N, t2, p, pivmin)) - n;
+                if (c > target) {
+                    hi = p;
+                    chi = c;
+                } else {
+                    lo = p;
+                    clo = c;
+                }
+                if (nt) {
+                    const double step = 1.0 / S;
+                    xn = p - step;
+                    if (fabs(step) <= 0.25 * rtol * p && xn >= lo && xn <= hi) {
+                        res = xn;
+                        break;
+                    }
+                }
             } else {
                 // the K lanes of a value are consecutive and share lo, hi and
                 // the loop exit, so they reach the ballot together
@@ -157,7 +198,7 @@ __global__ __launch_bounds__(1024) void k_bisect(int n, const double *__restrict
                 hi = nhi;
             }
         }
-        if (jk == 0) sall[(size_t)b * n + i] = 0.5 * (lo + hi);
+        if (jk == 0) sall[(size_t)b * n + i] = res >= 0.0 ? res : 0.5 * (lo + hi);
     }
 }
 
@@ -754,14 +795,17 @@ void launch_bisect(int n, const double *d, const double *e, double *sigma, int b
                                   (int)lds);
         hipLaunchKernelGGL(kern, dim3(gx, batch), dim3(1024), lds, s, n, d, e, sigma, rtol, atol);
     };
+    // complex64 form at full batches: Newton steps once a value is isolated
     if (K == 8)
-        go(k_bisect<8>);
+        go(k_bisect<8, false>);
     else if (K == 4)
-        go(k_bisect<4>);
+        go(k_bisect<4, false>);
     else if (K == 2)
-        go(k_bisect<2>);
+        go(k_bisect<2, false>);
+    else if (f32)
+        go(k_bisect<1, true>);
     else
-        go(k_bisect<1>);
+        go(k_bisect<1, false>);
 }
 
 // scratch columns: 64 per wave, (n + 63) / 64 waves per matrix
