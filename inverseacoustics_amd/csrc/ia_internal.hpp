@@ -362,6 +362,15 @@ namespace ia {
 // dst (R x C column-major, per batch) column i = row i of Pt (C x C, Pt[i + k C]),
 // rows C..R-1 zero.
 void launch_rows_to_cols(int R, int C, const double *Pt, cx<double> *dst, int batch, hipStream_t s);
+// radiation.hip: y = (x, 0); y = re(x); y.im = 0 (n entries); column j of U
+// (np x n per frequency) rotated by the conjugate phase of its largest entry
+// and made real; S_f = sin(k_f r_ij)/(k_f r_ij) for points p (np x 3,
+// column-major), as complex, np x np per frequency
+void launch_real_to_cx(size_t n, const double *x, cx<double> *y, hipStream_t s);
+void launch_cx_to_real(size_t n, const cx<double> *x, double *y, hipStream_t s);
+void launch_keep_real(size_t n, cx<double> *y, hipStream_t s);
+void launch_realign(int np, int n, int nf, cx<double> *U, hipStream_t s);
+void launch_rad_R(int np, const double *kv, const double *p, cx<double> *S, int nf, hipStream_t s);
 // dst (cols x rows) = src^H (rows x cols), batched.
 template <typename T>
 void launch_conj_transpose(int rows, int cols, const cx<T> *src, size_t ss, cx<T> *dst, size_t sd,

@@ -545,20 +545,13 @@ ia_status ia_invr(int m, int l, int n, const double *r_ms, const int *e_n, const
     launch_gram(m, D.X.as<double>(), D.E.as<int>(), n, D.adj_ptr.as<int>(), D.adj.as<int>(),
                 D.rule.as<double>(), Q.nq, bG.as<double>(), s);
     IA_TRY(bGz.alloc(sizeof(cx<double>) * (size_t)m * m));
-    std::vector<double> Gh((size_t)m * m);
-    IA_HIP(hipMemcpy(Gh.data(), bG.p, sizeof(double) * Gh.size(), hipMemcpyDeviceToHost));
-    std::vector<ia_z> Gc((size_t)m * m);
-    for (size_t i = 0; i < Gh.size(); ++i) Gc[i] = ia_z{Gh[i], 0.0};
-    IA_HIP(hipMemcpy(bGz.p, Gc.data(), sizeof(ia_z) * Gc.size(), hipMemcpyHostToDevice));
+    launch_real_to_cx((size_t)m * m, bG.as<double>(), bGz.as<cx<double>>(), s);
     IA_TRY(bR.alloc(sizeof(cx<double>) * (size_t)m * m));
     const cx<double> one{1, 0}, zero{0, 0};
     launch_gemm<double>('N', 'N', m, m, m, one, bGz.as<cx<double>>(), m, 0, D.Z.as<cx<double>>(),
                         m, 0, zero, bR.as<cx<double>>(), m, 0, 1, s);
-    std::vector<ia_z> Rh((size_t)m * m);
-    IA_HIP(hipMemcpy(Rh.data(), bR.p, sizeof(ia_z) * Rh.size(), hipMemcpyDeviceToHost));
-    for (auto &v : Rh) v.im = 0.0;
+    launch_keep_real((size_t)m * m, bR.as<cx<double>>(), s);
     // left singular vectors of R (invrS.m:161): explicit U = U_b [P_B; 0]
-    IA_HIP(hipMemcpy(bR.p, Rh.data(), sizeof(ia_z) * Rh.size(), hipMemcpyHostToDevice));
     SvdWork<double> sw;
     IA_TRY(sw.alloc(m, m, 1));
     IA_TRY(svd_factor<double>(sw, bR.as<cx<double>>(), s));
@@ -566,29 +559,9 @@ ia_status ia_invr(int m, int l, int n, const double *r_ms, const int *e_n, const
     IA_TRY(bU.alloc(sizeof(cx<double>) * (size_t)m * m));
     launch_rows_to_cols(m, m, sw.PBt.as<double>(), bU.as<cx<double>>(), 1, s);
     svd_apply_u<double>(sw, bR.as<cx<double>>(), bU.as<cx<double>>(), m, n_am, s);
-    // modes of a real symmetric matrix are real up to a phase: rotate each to real
-    std::vector<ia_z> Uh((size_t)m * n_am);
-    IA_HIP(hipMemcpy(Uh.data(), bU.p, sizeof(ia_z) * Uh.size(), hipMemcpyDeviceToHost));
-    for (int j = 0; j < n_am; ++j) {
-        ia_z *u = Uh.data() + (size_t)j * m;
-        int imax = 0;
-        double amax = -1;
-        for (int i = 0; i < m; ++i) {
-            double a = u[i].re * u[i].re + u[i].im * u[i].im;
-            if (a > amax) {
-                amax = a;
-                imax = i;
-            }
-        }
-        double nr = std::sqrt(amax);
-        double cr = u[imax].re / nr, ci = -u[imax].im / nr;  // conj(phase)
-        for (int i = 0; i < m; ++i) {
-            double re = u[i].re * cr - u[i].im * ci, im = u[i].re * ci + u[i].im * cr;
-            u[i].re = re;
-            u[i].im = im;
-        }
-    }
-    IA_HIP(hipMemcpy(bU.p, Uh.data(), sizeof(ia_z) * Uh.size(), hipMemcpyHostToDevice));
+    // modes of a real symmetric matrix are real up to a phase: rotate each to
+    // real with its largest entry positive
+    launch_realign(m, n_am, 1, bU.as<cx<double>>(), s);
     DevBuf bMr, bHr;
     IA_TRY(bMr.alloc(sizeof(cx<double>) * (size_t)m * n_am));
     launch_gemm<double>('N', 'N', m, n_am, m, one, D.Z.as<cx<double>>(), m, 0, bU.as<cx<double>>(),
@@ -603,11 +576,7 @@ ia_status ia_invr(int m, int l, int n, const double *r_ms, const int *e_n, const
     if (q) IA_TRY(host_out<ia_z>(q, bU.p, (size_t)m * n_am));
     if (m_r) IA_TRY(host_out<ia_z>(m_r, bMr.p, (size_t)m * n_am));
     if (h_r && l > 0) IA_TRY(host_out<ia_z>(h_r, bHr.p, (size_t)l * n_am));
-    if (g_mass) {
-        *g_mass = (ia_z *)malloc(sizeof(ia_z) * Gc.size());
-        if (!*g_mass) return fail(IA_ERR_NOMEM, "ia_invr: malloc");
-        memcpy(*g_mass, Gc.data(), sizeof(ia_z) * Gc.size());
-    }
+    if (g_mass) IA_TRY(host_out<ia_z>(g_mass, bGz.p, (size_t)m * m));
     return IA_OK;
 }
 
@@ -647,46 +616,30 @@ ia_status ia_radiation_modes(int np, const double *r_p, double a_e, double k, in
     n_modes = std::min(n_modes, np);
     const double c = air().c, rho = air().rho, w = k * c;
     const double scale = w * w * rho * a_e * a_e / (4 * M_PI * c);
-    std::vector<ia_z> R((size_t)np * np);
-    for (int j = 0; j < np; ++j)
-        for (int i = 0; i < np; ++i) {
-            double dx = r_p[i] - r_p[j], dy = r_p[i + np] - r_p[j + np], dz = r_p[i + 2 * np] - r_p[j + 2 * np];
-            double kr = k * std::sqrt(dx * dx + dy * dy + dz * dz);
-            R[(size_t)j * np + i] = ia_z{scale * (kr == 0 ? 1.0 : std::sin(kr) / kr), 0.0};
-        }
     hipStream_t s = nullptr;
-    DevBuf bR, bU;
-    IA_TRY(to_device(bR, R.data(), sizeof(ia_z) * R.size(), s));
+    // S = R / scale = sin(k r_ij)/(k r_ij) assembled on the device
+    DevBuf bp, bk, bR, bU, bM;
+    IA_TRY(to_device(bp, r_p, sizeof(double) * 3 * (size_t)np, s));
+    IA_TRY(to_device(bk, &k, sizeof(double), s));
+    IA_TRY(bR.alloc(sizeof(cx<double>) * (size_t)np * np));
+    launch_rad_R(np, bk.as<double>(), bp.as<double>(), bR.as<cx<double>>(), 1, s);
     SvdWork<double> sw;
     IA_TRY(sw.alloc(np, np, 1));
     IA_TRY(svd_factor<double>(sw, bR.as<cx<double>>(), s));
     IA_TRY(bU.alloc(sizeof(cx<double>) * (size_t)np * np));
     launch_rows_to_cols(np, np, sw.PBt.as<double>(), bU.as<cx<double>>(), 1, s);
     svd_apply_u<double>(sw, bR.as<cx<double>>(), bU.as<cx<double>>(), np, n_modes, s);
+    launch_realign(np, n_modes, 1, bU.as<cx<double>>(), s);
+    IA_TRY(bM.alloc(sizeof(double) * (size_t)np * n_modes));
+    launch_cx_to_real((size_t)np * n_modes, bU.as<cx<double>>(), bM.as<double>(), s);
     IA_HIP(hipGetLastError());
     IA_HIP(hipDeviceSynchronize());
-    std::vector<double> sig(np);
-    std::vector<ia_z> U((size_t)np * n_modes);
-    IA_HIP(hipMemcpy(sig.data(), sw.sig.p, sizeof(double) * np, hipMemcpyDeviceToHost));
-    IA_HIP(hipMemcpy(U.data(), bU.p, sizeof(ia_z) * U.size(), hipMemcpyDeviceToHost));
+    std::vector<double> sig(n_modes);
+    IA_HIP(hipMemcpy(sig.data(), sw.sig.p, sizeof(double) * n_modes, hipMemcpyDeviceToHost));
+    IA_TRY(host_out<double>(modes, bM.p, (size_t)np * n_modes));
     *eigval = (double *)malloc(sizeof(double) * n_modes);
-    *modes = (double *)malloc(sizeof(double) * (size_t)np * n_modes);
-    if (!*eigval || !*modes) return fail(IA_ERR_NOMEM, "ia_radiation_modes: malloc");
-    for (int j = 0; j < n_modes; ++j) {
-        (*eigval)[j] = sig[j];
-        const ia_z *u = U.data() + (size_t)j * np;
-        int imax = 0;
-        double amax = -1;
-        for (int i = 0; i < np; ++i) {
-            double a = u[i].re * u[i].re + u[i].im * u[i].im;
-            if (a > amax) {
-                amax = a;
-                imax = i;
-            }
-        }
-        double nr = std::sqrt(amax), cr = u[imax].re / nr, ci = -u[imax].im / nr;
-        for (int i = 0; i < np; ++i) (*modes)[(size_t)j * np + i] = u[i].re * cr - u[i].im * ci;
-    }
+    if (!*eigval) return fail(IA_ERR_NOMEM, "ia_radiation_modes: malloc");
+    for (int j = 0; j < n_modes; ++j) (*eigval)[j] = sig[j] * scale;
     return IA_OK;
 }
 

@@ -312,6 +312,12 @@ __global__ void k_cx_to_real(size_t n, const cx<double> *__restrict__ x, double 
     if (q < n) y[q] = x[q].re;
 }
 
+// y = re(y): a real matrix formed by a complex product keeps only its real part
+__global__ void k_keep_real(size_t n, cx<double> *__restrict__ y) {
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) y[q].im = 0.0;
+}
+
 // Bc = (B + B^T) / 2 as complex (kk x kk per frequency, blockIdx.y)
 __global__ void k_sym_cx(int kk, const double *__restrict__ B, cx<double> *__restrict__ Bc) {
     const size_t per = (size_t)kk * kk;
@@ -612,6 +618,26 @@ __global__ void k_scatter_modes(int np, int n, const int *__restrict__ idx,
 }
 
 }  // namespace
+
+// Single-frequency helpers of ia_invr and ia_radiation_modes (ibem_api.hip):
+// the conversions and the mode alignment stay on the device.
+void launch_real_to_cx(size_t n, const double *x, cx<double> *y, hipStream_t s) {
+    hipLaunchKernelGGL(k_real_to_cx, dim3(blocks256(n)), dim3(256), 0, s, n, x, y);
+}
+void launch_cx_to_real(size_t n, const cx<double> *x, double *y, hipStream_t s) {
+    hipLaunchKernelGGL(k_cx_to_real, dim3(blocks256(n)), dim3(256), 0, s, n, x, y);
+}
+void launch_keep_real(size_t n, cx<double> *y, hipStream_t s) {
+    hipLaunchKernelGGL(k_keep_real, dim3(blocks256(n)), dim3(256), 0, s, n, y);
+}
+void launch_realign(int np, int n, int nf, cx<double> *U, hipStream_t s) {
+    hipLaunchKernelGGL(k_realign, dim3(n, nf), dim3(256), 0, s, np, n, U);
+}
+void launch_rad_R(int np, const double *kv, const double *p, cx<double> *S, int nf,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(k_rad_R<cx<double>>, dim3((np + RT - 1) / RT, (np + RT - 1) / RT, nf),
+                       dim3(256), 0, s, np, kv, p, S, (size_t)np * np);
+}
 
 }  // namespace ia
 
