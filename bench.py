@@ -381,6 +381,16 @@ def other_workloads(ia, sc, which=("ibem", "hels", "radiation", "gradient")):
         out[f"hels_{prec}"] = {"workload": "HELS 256 mics, 256 surface points, 128 frequencies "
                                            "50-3000 Hz, N = ceil(ka) + 4 (5..11)",
                                "frequencies_per_s": round(len(H.k) / dt, 2), "s": round(dt, 4)}
+        # the same sweep captured into a HIP graph (IA_GRAPH=1): the warm-up
+        # call captures, the timed call replays
+        os.environ["IA_GRAPH"] = "1"
+        try:
+            dt = clock(lambda: ia.hels_sweep(H.k, H.order, H.azelr_s, H.azelr_h, H.b, H.area,
+                                             precision=prec))
+        finally:
+            del os.environ["IA_GRAPH"]
+        out[f"hels_{prec}"]["graph_replay"] = {"frequencies_per_s": round(len(H.k) / dt, 2),
+                                               "s": round(dt, 4)}
     if "radiation" in which:
         R = sc.radiation_scenario(device="cuda")
         ia.set_air(R.c, R.rho)

@@ -115,10 +115,9 @@ struct SweepCache {
     // invalidates a captured graph)
     void addresses(std::vector<uintptr_t> &v) const {
         for (const DevBuf *b : {&bD, &bY, &bipiv, &bperm, &binfo, &bc, &bbeta, &bphi, &bx, &bq, &bps,
-                                &bpsd, &bn2, &bcn2, &bgw, &sw.d, &sw.e, &sw.sig, &sw.tauL, &sw.tauR,
-                                &sw.Vr, &sw.QBt, &sw.PBt, &sw.work, &sw.gw, &sw.Rb, &sw.tauQ, &sw.qw,
-                                &sw.Bs, &sw.uR2, &sw.uL2, &sw.tR2, &sw.tL2, &sw.Pm, &sw.tauP})
+                                &bpsd, &bn2, &bcn2, &bgw})
             v.push_back((uintptr_t)b->p);
+        sw.addresses(v);
     }
 };
 

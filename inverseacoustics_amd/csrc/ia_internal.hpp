@@ -111,6 +111,10 @@ struct Air {
 Air &air();
 
 bool is_device_ptr(const void *p);
+// api.hip: IA_GRAPH set (and not "0"): the sweeps are captured into HIP
+// graphs and replayed; the bits of a double as a graph-key entry
+bool graph_mode();
+uintptr_t dbits(double v);
 // Copies n bytes of `src` (host or device) into a fresh device buffer.
 ia_status to_device(DevBuf &dst, const void *src, size_t n, hipStream_t s);
 
@@ -437,6 +441,12 @@ struct SvdWork {
         for (DevBuf *b : {&d, &e, &sig, &tauL, &tauR, &Vr, &QBt, &PBt, &work, &gw, &Rb, &tauQ, &qw,
                           &Bs, &uR2, &uL2, &tR2, &tL2, &Pm, &tauP})
             b->reset();
+    }
+    // the address of every buffer (a reallocation invalidates a captured graph)
+    void addresses(std::vector<uintptr_t> &v) const {
+        for (const DevBuf *b : {&d, &e, &sig, &tauL, &tauR, &Vr, &QBt, &PBt, &work, &gw, &Rb,
+                                &tauQ, &qw, &Bs, &uR2, &uL2, &tR2, &tL2, &Pm, &tauP})
+            v.push_back((uintptr_t)b->p);
     }
     ia_status alloc(int R_, int C_, int batch_) {
         R = R_;

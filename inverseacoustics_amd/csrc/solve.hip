@@ -49,6 +49,11 @@ struct RegsolveWork {
         for (DevBuf *b : {&bY, &bYd, &bc, &bbeta, &bphi, &bn2, &bcn2, &bv}) b->reset();
         sw.release();
     }
+    void addresses(std::vector<uintptr_t> &v) const {
+        for (const DevBuf *b : {&bY, &bYd, &bc, &bbeta, &bphi, &bn2, &bcn2, &bv})
+            v.push_back((uintptr_t)b->p);
+        sw.addresses(v);
+    }
 };
 
 constexpr int HELS_SLOTS = 8;  // order groups in flight at once
@@ -164,9 +169,21 @@ struct HelsSlot {
     DevBuf bH, bM, bQ, bx;
     hipStream_t s = nullptr;
 };
-// inputs in group order and outputs of one HELS sweep
+// inputs in group order and outputs of one HELS sweep; with IA_GRAPH the
+// captured sweep (every group on its slot stream, forked from and joined
+// back to the stream gs), the key it was captured for (sizes, parameters,
+// the order groups, every buffer address) and its fork / join events
 struct HelsIo {
     DevBuf ds, dh, dk, db, lam, idx, kt, sm, W, v, p;
+    hipStream_t gs = nullptr;
+    hipEvent_t ev[HELS_SLOTS + 1] = {};
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uintptr_t> gkey;
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        gkey.clear();
+    }
 };
 // plain regularised solve (ia_regsolve): device copies of its arguments
 struct RegsolveIo {
@@ -205,6 +222,7 @@ void release_solve_cache() {
         for (DevBuf *b : {&hels_slot(i).bH, &hels_slot(i).bM, &hels_slot(i).bQ, &hels_slot(i).bx})
             b->reset();
     HelsIo &h = hels_io();
+    h.drop_graph();
     for (DevBuf *b : {&h.ds, &h.dh, &h.dk, &h.db, &h.lam, &h.idx, &h.kt, &h.sm, &h.W, &h.v, &h.p})
         b->reset();
     RegsolveIo &r = regsolve_io();
@@ -346,48 +364,113 @@ ia_status ia_hels_sweep(const ia_hels_desc *desc, const int *order, const double
     for (int i = 0; i < NS; ++i) IA_TRY(hels_stream(i, &st[i]));
     // the inputs were copied on the legacy stream
     IA_HIP(hipStreamSynchronize(nullptr));
+    // every group's work, each on its slot's stream
+    auto enqueue = [&]() -> ia_status {
+        for (int g = 0; g < G; ++g) {
+            const int sl = g % NS, N = gN[g], nb = gnb[g], off = goff[g], n_t = (N + 1) * (N + 1);
+            HelsSlot &S = hels_slot(sl);
+            hipStream_t sg = st[sl];
+            const double *kg = io.dk.as<double>() + off;
+            const cx<double> *bg = io.db.as<cx<double>>() + (size_t)off * l;
+            cx<double> *H = S.bH.as<cx<double>>(), *Me = S.bM.as<cx<double>>(),
+                       *Qe = S.bQ.as<cx<double>>(), *x = S.bx.as<cx<double>>();
+            launch_hels<double>(l, N, nb, kg, rho_c, io.dh.as<double>(), H, nullptr,
+                                (size_t)l * n_t, sg);
+            launch_hels<double>(m, N, nb, kg, rho_c, io.ds.as<double>(), Me, Qe, (size_t)m * n_t,
+                                sg);
+            if (mixed)
+                IA_TRY(regsolve_enqueue<float>(regsolve_work<float>(1 + sl), l, n_t, nb, H,
+                                               (size_t)l * n_t, bg, l, prm, x, n_t,
+                                               io.lam.as<double>() + off, io.idx.as<int>() + off,
+                                               io.kt.as<int>() + off, io.sm.as<double>() + off, sg,
+                                               nullptr));
+            else
+                IA_TRY(regsolve_enqueue<double>(regsolve_work<double>(1 + sl), l, n_t, nb, H,
+                                                (size_t)l * n_t, bg, l, prm, x, n_t,
+                                                io.lam.as<double>() + off, io.idx.as<int>() + off,
+                                                io.kt.as<int>() + off, io.sm.as<double>() + off,
+                                                sg, nullptr));
+            // v = Q_e c, p = M_e c (to_velocity / to_pressure), W = 1/2 Re(v^H p) S/m
+            const cx<double> one{1, 0}, zero{0, 0};
+            cx<double> *v = io.v.as<cx<double>>() + (size_t)off * m;
+            cx<double> *pr = io.p.as<cx<double>>() + (size_t)off * m;
+            launch_gemm<double>('N', 'N', m, 1, n_t, one, Qe, m, (size_t)m * n_t, x, n_t, n_t,
+                                zero, v, m, m, nb, sg);
+            launch_gemm<double>('N', 'N', m, 1, n_t, one, Me, m, (size_t)m * n_t, x, n_t, n_t,
+                                zero, pr, m, m, nb, sg);
+            launch_power<double>(m, v, m, pr, m, desc->area / m, io.W.as<double>() + off, nb, sg);
+            if (hipGetLastError() != hipSuccess)
+                return fail(IA_ERR_DEVICE, "ia_hels_sweep: launch failed");
+        }
+        return IA_OK;
+    };
     ia_status rc = IA_OK;
-    for (int g = 0; g < G && rc == IA_OK; ++g) {
-        const int sl = g % NS, N = gN[g], nb = gnb[g], off = goff[g], n_t = (N + 1) * (N + 1);
-        HelsSlot &S = hels_slot(sl);
-        hipStream_t sg = st[sl];
-        const double *kg = io.dk.as<double>() + off;
-        const cx<double> *bg = io.db.as<cx<double>>() + (size_t)off * l;
-        cx<double> *H = S.bH.as<cx<double>>(), *Me = S.bM.as<cx<double>>(),
-                   *Qe = S.bQ.as<cx<double>>(), *x = S.bx.as<cx<double>>();
-        launch_hels<double>(l, N, nb, kg, rho_c, io.dh.as<double>(), H, nullptr, (size_t)l * n_t,
-                            sg);
-        launch_hels<double>(m, N, nb, kg, rho_c, io.ds.as<double>(), Me, Qe, (size_t)m * n_t, sg);
-        if (mixed)
-            rc = regsolve_enqueue<float>(regsolve_work<float>(1 + sl), l, n_t, nb, H,
-                                         (size_t)l * n_t, bg, l, prm, x, n_t,
-                                         io.lam.as<double>() + off, io.idx.as<int>() + off,
-                                         io.kt.as<int>() + off, io.sm.as<double>() + off, sg,
-                                         nullptr);
-        else
-            rc = regsolve_enqueue<double>(regsolve_work<double>(1 + sl), l, n_t, nb, H,
-                                          (size_t)l * n_t, bg, l, prm, x, n_t,
-                                          io.lam.as<double>() + off, io.idx.as<int>() + off,
-                                          io.kt.as<int>() + off, io.sm.as<double>() + off, sg,
-                                          nullptr);
-        if (rc != IA_OK) break;
-        // v = Q_e c, p = M_e c (to_velocity / to_pressure), W = 1/2 Re(v^H p) S/m
-        const cx<double> one{1, 0}, zero{0, 0};
-        cx<double> *v = io.v.as<cx<double>>() + (size_t)off * m;
-        cx<double> *pr = io.p.as<cx<double>>() + (size_t)off * m;
-        launch_gemm<double>('N', 'N', m, 1, n_t, one, Qe, m, (size_t)m * n_t, x, n_t, n_t, zero,
-                            v, m, m, nb, sg);
-        launch_gemm<double>('N', 'N', m, 1, n_t, one, Me, m, (size_t)m * n_t, x, n_t, n_t, zero,
-                            pr, m, m, nb, sg);
-        launch_power<double>(m, v, m, pr, m, desc->area / m, io.W.as<double>() + off, nb, sg);
-        if (hipGetLastError() != hipSuccess)
-            rc = fail(IA_ERR_DEVICE, "ia_hels_sweep: launch failed");
-    }
-    // wait for every group, also after a failed enqueue (the workspaces are
-    // reused by the next call)
-    for (int i = 0; i < NS; ++i)
-        if (hipStreamSynchronize(st[i]) != hipSuccess && rc == IA_OK)
+    if (graph_mode()) {
+        // IA_GRAPH=1: the sweep (the fork to the slot streams, every group,
+        // the join) is captured once per key into a graph and replayed with
+        // one launch; ~2,000 small launches per sweep become graph nodes
+        std::vector<uintptr_t> key = {
+            (uintptr_t)F, (uintptr_t)m, (uintptr_t)l, (uintptr_t)mixed, (uintptr_t)prm.filter,
+            (uintptr_t)prm.select, (uintptr_t)prm.n_lambda, (uintptr_t)prm.k_fixed,
+            dbits(prm.decades), dbits(prm.lambda_fixed), dbits(desc->area), dbits(rho_c),
+            (uintptr_t)G, (uintptr_t)NS};
+        for (int g = 0; g < G; ++g) {
+            key.push_back((uintptr_t)gN[g]);
+            key.push_back((uintptr_t)gnb[g]);
+        }
+        for (const DevBuf *q :
+             {&io.ds, &io.dh, &io.dk, &io.db, &io.lam, &io.idx, &io.kt, &io.sm, &io.W, &io.v, &io.p})
+            key.push_back((uintptr_t)q->p);
+        for (int i = 0; i < NS; ++i) {
+            const HelsSlot &S = hels_slot(i);
+            for (const DevBuf *q : {&S.bH, &S.bM, &S.bQ, &S.bx}) key.push_back((uintptr_t)q->p);
+            if (mixed)
+                regsolve_work<float>(1 + i).addresses(key);
+            else
+                regsolve_work<double>(1 + i).addresses(key);
+        }
+        if (!io.gs) {
+            IA_HIP(hipStreamCreateWithFlags(&io.gs, hipStreamNonBlocking));
+            for (hipEvent_t &e : io.ev) IA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+        if (!io.gexec || io.gkey != key) {
+            io.drop_graph();
+            IA_HIP(hipStreamBeginCapture(io.gs, hipStreamCaptureModeThreadLocal));
+            hipError_t e = hipEventRecord(io.ev[0], io.gs);
+            for (int i = 0; i < NS && e == hipSuccess; ++i) e = hipStreamWaitEvent(st[i], io.ev[0], 0);
+            rc = e == hipSuccess ? enqueue()
+                                 : fail(IA_ERR_DEVICE, "ia_hels_sweep: fork of the capture failed");
+            // every slot stream joins back (also after a failure), so that the
+            // capture ends
+            for (int i = 0; i < NS; ++i) {
+                (void)hipEventRecord(io.ev[1 + i], st[i]);
+                (void)hipStreamWaitEvent(io.gs, io.ev[1 + i], 0);
+            }
+            hipGraph_t gr = nullptr;
+            const hipError_t ce = hipStreamEndCapture(io.gs, &gr);
+            if (rc == IA_OK && ce == hipSuccess && gr &&
+                hipGraphInstantiate(&io.gexec, gr, nullptr, nullptr, 0) != hipSuccess)
+                io.gexec = nullptr;
+            if (gr) (void)hipGraphDestroy(gr);
+            if (rc == IA_OK && !io.gexec) {
+                (void)hipGetLastError();
+                rc = fail(IA_ERR_DEVICE, "ia_hels_sweep: capture into a graph failed");
+            }
+            if (rc == IA_OK) io.gkey = key;
+        }
+        if (rc == IA_OK && hipGraphLaunch(io.gexec, io.gs) != hipSuccess)
             rc = fail(IA_ERR_DEVICE, "ia_hels_sweep: %s", hipGetErrorString(hipGetLastError()));
+        if (hipStreamSynchronize(io.gs) != hipSuccess && rc == IA_OK)
+            rc = fail(IA_ERR_DEVICE, "ia_hels_sweep: %s", hipGetErrorString(hipGetLastError()));
+    } else {
+        rc = enqueue();
+        // wait for every group, also after a failed enqueue (the workspaces
+        // are reused by the next call)
+        for (int i = 0; i < NS; ++i)
+            if (hipStreamSynchronize(st[i]) != hipSuccess && rc == IA_OK)
+                rc = fail(IA_ERR_DEVICE, "ia_hels_sweep: %s",
+                          hipGetErrorString(hipGetLastError()));
+    }
     IA_TRY(rc);
     std::vector<double> hl(F), hs(F), hW(F);
     std::vector<int> hi(F), hk(F);

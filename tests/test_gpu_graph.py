@@ -79,3 +79,40 @@ def test_graph_replays_match_eager(ia, monkeypatch, prec, recheck):
     got = run()
     np.testing.assert_array_equal(got["v"], ref["v"])
     monkeypatch.delenv("IA_GRAPH")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("n_freq,fmax", [(8, 3000.0), (24, 12000.0)])
+def test_hels_graph_replays_match_eager(ia, monkeypatch, precision, n_freq, fmax):
+    """ia_hels_sweep with IA_GRAPH=1: the order groups' work on the slot
+    streams is captured once (fork from and join to one stream) and replayed;
+    three replays equal the eager sweep bit for bit with other eager sweeps
+    between them, a replay with doubled pressures gives the doubled velocity,
+    and 24 frequencies over orders 5..31 put more groups than slots (shared
+    slot streams) into the graph."""
+    S = sc.hels_scenario(n_freq=n_freq, fmin=50.0, fmax=fmax, l=96, m=64)
+    ia.set_air(S.c, S.rho)
+
+    def run(b):
+        r = ia.hels_sweep(S.k, S.order, S.azelr_s, S.azelr_h, b, S.area, precision=precision)
+        return r
+
+    monkeypatch.delenv("IA_GRAPH", raising=False)
+    ref = run(S.b)
+    monkeypatch.setenv("IA_GRAPH", "1")
+    for rep in range(3):
+        got = run(S.b)
+        for n in ("lam_index", "k_tsvd", "lam", "power", "velocity", "pressure"):
+            np.testing.assert_array_equal(getattr(got, n), getattr(ref, n),
+                                          err_msg=f"replay {rep}: {n}")
+        monkeypatch.delenv("IA_GRAPH")
+        e = ia.hels_sweep(S.k[:3], S.order[:3], S.azelr_s, S.azelr_h, S.b[:3], S.area)
+        assert np.all(np.isfinite(e.power))
+        monkeypatch.setenv("IA_GRAPH", "1")
+    got = run(2 * S.b)
+    np.testing.assert_array_equal(got.lam_index, ref.lam_index)
+    np.testing.assert_allclose(got.velocity, 2 * ref.velocity, rtol=0,
+                               atol=1e-9 * np.abs(ref.velocity).max())
+    got = run(S.b)
+    np.testing.assert_array_equal(got.velocity, ref.velocity)
+    monkeypatch.delenv("IA_GRAPH")
