@@ -323,3 +323,36 @@ def test_gemm_every_candidate(ia, monkeypatch, cand, opa, opb):
         ref4 = (0.5 - 2j) * opA4 @ opB4 + (1.5 + 0.25j) * C4
         got4 = ia.cgemm(A4, B4, opa, opb, alpha=0.5 - 2j, beta=1.5 + 0.25j, Cm=C4)
         assert np.abs(got4 - ref4).max() <= 2e-6 * max(1.0, np.abs(ref4).max()) * np.sqrt(k)
+
+
+@pytest.mark.parametrize("cand", [0, 4])
+@pytest.mark.parametrize("opa,opb", [("N", "N"), ("C", "N"), ("N", "C"), ("C", "C")])
+def test_gemm_complex64_scaled_rows(ia, monkeypatch, cand, opa, opb):
+    """The complex64 products (0 = bf16 three-part split, 4 = FP32 3M)
+    against the FP64 product of the same complex64 operands, entry by
+    entry: rows of op(A) and columns of op(B)
+    spread over 1e-9 .. 1e9, a zero row and column, rows whose magnitude
+    jumps by 1e10 between stages, and ragged edges (every product and sum
+    stays inside the FP32 normal range).  Bound per entry:
+    (3 2^-22 + K 2^-23) (|op(A)| |op(B)|)_ij, the split (or 3M) error plus
+    FP32 accumulation."""
+    monkeypatch.setenv("IA_GEMM_FORCE", str(cand))
+    rng = np.random.default_rng(7)
+    for m, n, k in [(70, 90, 100), (130, 65, 257)]:
+        opA = _cr((m, k), 11) * 10.0 ** rng.uniform(-9, 9, (m, 1))
+        opB = _cr((k, n), 12) * 10.0 ** rng.uniform(-9, 9, (1, n))
+        opA[3] = 0
+        opB[:, 5] = 0
+        opA[7, :32] *= 1e-10   # first stage tiny, later stages O(1)
+        opA[9, 32:] *= 1e-10   # first stage O(1), later stages tiny
+        opB[40:, 11] *= 1e10
+        A4 = (opA if opa == "N" else opA.conj().T).astype(np.complex64)
+        B4 = (opB if opb == "N" else opB.conj().T).astype(np.complex64)
+        a = A4.astype(complex) if opa == "N" else A4.astype(complex).conj().T
+        bm = B4.astype(complex) if opb == "N" else B4.astype(complex).conj().T
+        ref = a @ bm
+        bound = (3 * 2.0 ** -22 + k * 2.0 ** -23) * (np.abs(a) @ np.abs(bm))
+        got = ia.cgemm(A4, B4, opa, opb).astype(complex)
+        err = np.abs(got - ref)
+        assert np.all(err <= bound), (m, n, k, float(np.max(err / np.maximum(bound, 1e-300))))
+        assert np.all(got[3] == 0) and np.all(got[:, 5] == 0)
