@@ -285,8 +285,8 @@ def main():
 # algorithmic bytes: the rank-128 / rank-64 trailing updates re-read their
 # operand panels per output tile.
 SWEEP_HBM_BYTES = {
-    ("mixed", 256, 2048, 1024, 0): 846438802638,
-    ("fp64", 256, 2048, 1024, 0): 1908633402001,
+    ("mixed", 256, 2048, 1024, 0): 838809403715,
+    ("fp64", 256, 2048, 1024, 0): 1834993596342,
 }
 
 
