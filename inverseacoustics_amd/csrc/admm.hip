@@ -12,8 +12,11 @@
 // lambda = 5 lambda_0 * 0.75^k while lambda > lambda_0 / 10 (ADMMG.m:3-30),
 // L2 x = (rho c + u) / (2 lambda_0 + rho) (ADMMG.m:31-48), Lp generalised
 // soft thresholding with N fixed-point steps (ADMMG.m:49-88; Zuo et al.,
-// ICCV 2013).  The start x_0 is the minimum-norm least-squares solution
-// (TSVD at the numerical rank) where ADMMG.m uses MATLAB's A\b.
+// ICCV 2013).  The start x_0 = A\b as MATLAB's mldivide computes it
+// (ADMMG.m:7,35,55): for a rectangular A the basic solution of Householder
+// QR with column pivoting (k_qrcp_basic, LAPACK zlaqp2's pivot order and
+// norm downdating), for a square A the solution of the system (TSVD at the
+// numerical rank, which is A^-1 b for a nonsingular A).
 #include "householder.hpp"
 
 namespace ia {
@@ -132,6 +135,175 @@ __global__ __launch_bounds__(BT) void k_admm(int n, const cx<double> *__restrict
 }
 
 // M = 2 A^H A + rho I: add rho to the diagonal of the product
+// x = A \ b for a rectangular A (m x n, column-major copy W, b copy y; both
+// overwritten), one workgroup per system: Householder QR with column
+// pivoting as LAPACK zlaqp2 -- step j takes the column with the largest
+// remaining norm (the first on ties), reflects it (zlarfg), applies H^H to
+// the columns right of it and to y, and downdates their norms, recomputing
+// a norm when the downdate has lost more than half its digits
+// ((1 - (|a_jc| / vn1)^2) (vn1 / vn2)^2 <= sqrt(eps)); then the rank
+// r = #{|R_ii| > max(m, n) eps |R_00|} and the basic solution
+// x[jp[i]] = (R[:r, :r]^-1 (Q^H b)[:r])_i, zero elsewhere.  Scratch per
+// system: norms vn (2n doubles) and the column order jp (n ints).
+__global__ __launch_bounds__(BT) void k_qrcp_basic(int m, int n, cx<double> *__restrict__ Wall,
+                                                  cx<double> *__restrict__ yall,
+                                                  double *__restrict__ vall,
+                                                  int *__restrict__ jall,
+                                                  cx<double> *__restrict__ xall) {
+    __shared__ double red[NW];
+    __shared__ double rv[NW];
+    __shared__ int ri[NW];
+    __shared__ int s_int;
+    __shared__ cx<double> s_x;
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    cx<double> *W = Wall + (size_t)b * m * n;
+    cx<double> *y = yall + (size_t)b * m;
+    double *vn1 = vall + (size_t)b * 2 * n, *vn2 = vn1 + n;
+    int *jp = jall + (size_t)b * n;
+    cx<double> *x = xall + (size_t)b * n;
+    const double tol3z = 1.0536712127723509e-08;  // sqrt(2^-53), LAPACK's sqrt(dlamch('E'))
+    for (int c = wv; c < n; c += NW) {
+        const cx<double> *col = W + (size_t)c * m;
+        double q = 0;
+        for (int r = lane; r < m; r += 64) q += abs2(col[r]);
+        q = hh::wave_sum(q);
+        if (lane == 0) {
+            vn1[c] = vn2[c] = sqrt(q);
+            jp[c] = c;
+        }
+    }
+    __syncthreads();
+    const int k = min(m, n);
+    for (int j = 0; j < k; ++j) {
+        // ---- pivot: the first column of largest vn1 among j..n-1
+        double best = -1.0;
+        int bi = n;
+        for (int c = j + t; c < n; c += BT)
+            if (vn1[c] > best) {
+                best = vn1[c];
+                bi = c;
+            }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ob = __shfl_xor(best, o, 64);
+            const int oi = __shfl_xor(bi, o, 64);
+            if (ob > best || (ob == best && oi < bi)) {
+                best = ob;
+                bi = oi;
+            }
+        }
+        if (lane == 0) {
+            rv[wv] = best;
+            ri[wv] = bi;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double bb = rv[0];
+            int ii = ri[0];
+            for (int w = 1; w < NW; ++w)
+                if (rv[w] > bb || (rv[w] == bb && ri[w] < ii)) {
+                    bb = rv[w];
+                    ii = ri[w];
+                }
+            s_int = ii;
+        }
+        __syncthreads();
+        const int p = s_int < n ? s_int : j;  // every remaining norm NaN: keep column j
+        if (p != j) {
+            cx<double> *cj = W + (size_t)j * m, *cp = W + (size_t)p * m;
+            for (int r = t; r < m; r += BT) {
+                const cx<double> a = cj[r];
+                cj[r] = cp[r];
+                cp[r] = a;
+            }
+            if (t == 0) {
+                const int q = jp[p];
+                jp[p] = jp[j];
+                jp[j] = q;
+                vn1[p] = vn1[j];
+                vn2[p] = vn2[j];
+            }
+        }
+        __syncthreads();
+        // ---- reflector of column j (rows j..m-1)
+        cx<double> *cj = W + (size_t)j * m;
+        double q = 0;
+        for (int r = j + 1 + t; r < m; r += BT) q += abs2(cj[r]);
+        const double xn2 = block_sum(q, red);
+        const hh::Refl rf = hh::larfg(cj[j], xn2);
+        __syncthreads();  // cj[j] read by all before it is overwritten
+        for (int r = j + 1 + t; r < m; r += BT) cj[r] = cj[r] * rf.scale;
+        if (t == 0) cj[j] = cx<double>{rf.beta, 0};
+        __syncthreads();
+        // ---- H^H = I - conj(tau) v v^H on columns j+1..n-1 and on y (c == n),
+        // one wave per column; lane 0 holds row j
+        const cx<double> ctau = conj(rf.tau);
+        for (int c = j + 1 + wv; c <= n; c += NW) {
+            cx<double> *col = (c < n) ? W + (size_t)c * m : y;
+            double sr = 0, si = 0;
+            for (int r = j + lane; r < m; r += 64) {
+                const cx<double> v = (r == j) ? cx<double>{1, 0} : cj[r];
+                const cx<double> a = col[r];
+                sr += v.re * a.re + v.im * a.im;
+                si += v.re * a.im - v.im * a.re;
+            }
+            const cx<double> f = ctau * cx<double>{hh::wave_sum(sr), hh::wave_sum(si)};
+            double below = 0;
+            cx<double> ajc{0, 0};
+            for (int r = j + lane; r < m; r += 64) {
+                const cx<double> v = (r == j) ? cx<double>{1, 0} : cj[r];
+                const cx<double> a = col[r] - v * f;
+                col[r] = a;
+                if (r == j)
+                    ajc = a;
+                else
+                    below += abs2(a);
+            }
+            if (c < n) {
+                const double are = __shfl(ajc.re, 0, 64), aim = __shfl(ajc.im, 0, 64);
+                const double below2 = hh::wave_sum(below);
+                if (lane == 0 && vn1[c] != 0.0) {
+                    const double v1 = vn1[c], v2 = vn2[c];
+                    const double rat = sqrt(are * are + aim * aim) / v1;
+                    const double temp = fmax(1.0 - rat * rat, 0.0);
+                    const double temp2 = temp * (v1 / v2) * (v1 / v2);
+                    if (temp2 <= tol3z) {
+                        const double nv = (j < m - 1) ? sqrt(below2) : 0.0;
+                        vn1[c] = nv;
+                        vn2[c] = nv;
+                    } else {
+                        vn1[c] = v1 * sqrt(temp);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // ---- rank and the basic solution
+    if (t == 0) {
+        const double tol = (double)max(m, n) * 2.220446049250313e-16 * sqrt(abs2(W[0]));
+        int r = 0;
+        for (int i = 0; i < k; ++i)
+            if (sqrt(abs2(W[(size_t)i * m + i])) > tol) ++r;
+        s_int = r;
+    }
+    __syncthreads();
+    const int r = s_int;
+    for (int i = r - 1; i >= 0; --i) {
+        if (t == 0) {
+            s_x = cdiv(y[i], W[(size_t)i * m + i]);
+            y[i] = s_x;
+        }
+        __syncthreads();
+        const cx<double> xi = s_x;
+        const cx<double> *ci = W + (size_t)i * m;
+        for (int q2 = t; q2 < i; q2 += BT) y[q2] = y[q2] - ci[q2] * xi;
+        __syncthreads();
+    }
+    for (int c = t; c < n; c += BT) x[c] = cx<double>{0, 0};
+    __syncthreads();
+    for (int i = t; i < r; i += BT) x[jp[i]] = y[i];
+}
+
 __global__ void k_add_diag(int n, cx<double> *M, double rho) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
     if (i < n) M[(size_t)b * n * n + (size_t)i * n + i].re += rho;
@@ -177,10 +349,28 @@ ia_status admm_batched(int rows, int cols, int batch, const cx<double> *A, const
     // r0 = 2 A^H b
     launch_gemm<double>('C', 'N', n, 1, rows, two, A, rows, sA, b, rows, rows, zero,
                         br0.as<cx<double>>(), n, n, batch, s);
-    // x0: minimum-norm least squares (TSVD at the numerical rank)
-    RegDev rk{IA_FILTER_TSVD, IA_SELECT_RANK, 50, 0, 8.0, 0.0};
-    IA_TRY(regsolve_batched<double>(rows, cols, batch, A, sA, b, rows, rk, bx.as<cx<double>>(), n,
-                                    nullptr, nullptr, nullptr, nullptr, s));
+    // x0 = A \ b as MATLAB's mldivide: the QR-with-column-pivoting basic
+    // solution for a rectangular A, the solution of the system for a square
+    // one (TSVD at the numerical rank: A^-1 b when A is nonsingular)
+    DevBuf bW, by, bv, bj;
+    if (rows != cols) {
+        IA_TRY(bW.alloc(sizeof(cx<double>) * sA * batch));
+        IA_TRY(by.alloc(sizeof(cx<double>) * (size_t)rows * batch));
+        IA_TRY(bv.alloc(sizeof(double) * 2 * (size_t)cols * batch));
+        IA_TRY(bj.alloc(sizeof(int) * (size_t)cols * batch));
+        IA_HIP(hipMemcpyAsync(bW.p, A, sizeof(cx<double>) * sA * batch, hipMemcpyDeviceToDevice, s));
+        IA_HIP(hipMemcpyAsync(by.p, b, sizeof(cx<double>) * (size_t)rows * batch,
+                              hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_qrcp_basic, dim3(batch), dim3(BT), 0, s, rows, cols,
+                           bW.as<cx<double>>(), by.as<cx<double>>(), bv.as<double>(),
+                           bj.as<int>(), bx.as<cx<double>>());
+        IA_HIP(hipGetLastError());
+    } else {
+        RegDev rk{IA_FILTER_TSVD, IA_SELECT_RANK, 50, 0, 8.0, 0.0};
+        IA_TRY(regsolve_batched<double>(rows, cols, batch, A, sA, b, rows, rk,
+                                        bx.as<cx<double>>(), n, nullptr, nullptr, nullptr,
+                                        nullptr, s));
+    }
     AdmmDev d{P.lambda0, P.rho, P.gamma, P.epsilon * P.epsilon, P.p, P.max_iter, P.norm,
               P.n_gst};
     hipLaunchKernelGGL(k_admm, dim3(batch), dim3(BT), lds, s, n, bMi.as<cx<double>>(),

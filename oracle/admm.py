@@ -19,11 +19,15 @@ when |c - c_old|, |c - x| and |u - u_old| are all <= epsilon:
   s <- |y| - lp p s^(p-1), x = sign(y) s where |y| > tau, else 0; the dual
   starts at u = x_0 (ADMMG.m:56).
 
-Deviation: MATLAB's A\\b (ADMMG.m:7,35,55) returns, for an underdetermined
-A, a basic solution from column-pivoted QR; this restatement (and the GPU
-path) starts from the minimum-norm least-squares solution.  Parity of the
-starting point against MATLAB is unpinned; everything after it is
-deterministic arithmetic.
+Start x_0 = A\\b (ADMMG.m:7,35,55) as MATLAB's mldivide computes it:
+square A by LU with partial pivoting; rectangular A by Householder QR with
+column pivoting (LAPACK zgeqp3), rank r = #{|R_ii| > max(m, n) eps |R_11|},
+x[E[:r]] = R[:r, :r]^-1 (Q^H b)[:r] and zero elsewhere -- the basic solution
+with at most r nonzeros (for a full-rank overdetermined A the unique
+least-squares solution).  Pinned in tests/test_oracle_admm.py against an
+independent greedy column selection (largest residual norm against the
+span of the columns already chosen, recomputed from scratch at every step)
+followed by a least-squares solve on the chosen columns.
 
 Returns (c, iterations, lambda_last) -- ADMMG.m returns c; lambda_last is the
 continuation value the last iteration ran at (lambda_0 for L2).
@@ -31,10 +35,29 @@ continuation value the last iteration ran at (lambda_0 for L2).
 from __future__ import annotations
 
 import numpy as np
+import scipy.linalg
+
+
+def basic_solution(A, b):
+    """x = A \\ b as MATLAB's mldivide computes it (ADMMG.m:7,35,55)."""
+    A = np.asarray(A, dtype=complex)
+    b = np.asarray(b, dtype=complex)
+    m, n = A.shape
+    if m == n:
+        return np.linalg.solve(A, b)
+    Q, R, E = scipy.linalg.qr(A, mode="economic", pivoting=True)
+    d = np.abs(np.diag(R))
+    tol = max(m, n) * np.finfo(float).eps * d[0] if d.size else 0.0
+    r = int(np.sum(d > tol))
+    x = np.zeros(n, complex)
+    if r:
+        y = Q[:, :r].conj().T @ b
+        x[E[:r]] = scipy.linalg.solve_triangular(R[:r, :r], y)
+    return x
 
 
 def _start(A, b):
-    return np.linalg.lstsq(A, b, rcond=None)[0]
+    return basic_solution(A, b)
 
 
 def gst_threshold(lam_p, p):

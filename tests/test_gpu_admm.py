@@ -48,3 +48,22 @@ def test_admm_rejects_bad_parameters(ia):
         ia.admm(A, b, lambda0=1.0, norm=3, p=1.5)
     with pytest.raises(ia.IAError):
         ia.admm(A, b, lambda0=1.0, rho=0.0)
+
+
+@pytest.mark.parametrize("shape", [(24, 40), (40, 24), (10, 37)])
+def test_admm_start_is_matlab_basic_solution(ia, shape):
+    """One L2 iteration from x_0 gives c_1 = M^-1 (r0 + rho x_0), so the start
+    is recovered as x_0 = (M c_1 - r0) / rho: against the oracle's A\\b (the
+    basic solution of QR with column pivoting for a rectangular A, at most
+    min(m, n) nonzeros in the columns LAPACK's pivoting takes)."""
+    A, b = _batch(*shape, 2, 9)
+    rho = 2.0
+    c, it = ia.admm(A, b, lambda0=1.0, rho=rho, gamma=1.0, epsilon=0.0, max_iter=1, norm=2)
+    assert np.all(it == 1)
+    for i in range(2):
+        M = 2.0 * A[i].conj().T @ A[i] + rho * np.eye(shape[1])
+        x0 = (M @ c[i] - 2.0 * A[i].conj().T @ b[i]) / rho
+        ref = oadmm.basic_solution(A[i], b[i])
+        assert np.linalg.norm(x0 - ref) <= 1e-9 * np.linalg.norm(ref)
+        assert np.count_nonzero(ref) == min(shape)
+        assert np.abs(x0[ref == 0]).max(initial=0.0) <= 1e-9 * np.linalg.norm(ref)

@@ -2,7 +2,8 @@
 against its closed form, the L1 branch against an independent LASSO solver
 (proximal gradient) at the lambda the continuation ended on, and the
 generalised soft thresholding of the Lp branch against brute-force scalar
-minimisation.  CPU only."""
+minimisation, and the start A\\b (MATLAB's basic solution for rectangular A)
+against an independent greedy column selection.  CPU only."""
 import numpy as np
 import pytest
 
@@ -78,3 +79,54 @@ def test_gst_threshold_and_magnitude_minimise_the_scalar_problem(p, lam_p):
         else:
             s = admm.gst_magnitude(np.array([a]), lam_p, p, 60)[0].real
             assert abs(s - s_best) <= 2 * (s_grid[1] - s_grid[0]) + 1e-9 * a
+
+
+def _greedy_basic(A, b, r):
+    """Column selection of QR with column pivoting recomputed from scratch:
+    step j takes the column whose residual against the span of the columns
+    already taken has the largest norm (the first on ties), then x solves
+    least squares on the r columns taken and is zero elsewhere."""
+    n = A.shape[1]
+    taken = []
+    for _ in range(r):
+        if taken:
+            Qt, _ = np.linalg.qr(A[:, taken])
+            res = A - Qt @ (Qt.conj().T @ A)
+        else:
+            res = A
+        nrm = np.linalg.norm(res, axis=0)
+        nrm[taken] = -1.0
+        taken.append(int(np.argmax(nrm)))
+    x = np.zeros(n, complex)
+    x[taken] = np.linalg.lstsq(A[:, taken], b, rcond=None)[0]
+    return x, taken
+
+
+@pytest.mark.parametrize("shape", [(24, 40), (10, 37), (40, 24), (7, 7)])
+def test_start_is_matlab_basic_solution(shape):
+    m, n = shape
+    A, b = _problem(m, n, 21)
+    x = admm.basic_solution(A, b)
+    if m == n:
+        np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=0,
+                                   atol=1e-12 * np.linalg.norm(x))
+        return
+    x0, taken = _greedy_basic(A, b, min(m, n))
+    assert set(np.flatnonzero(x).tolist()) == set(taken)
+    np.testing.assert_allclose(x, x0, rtol=0, atol=1e-10 * np.linalg.norm(x0))
+    if m < n:  # consistent: the basic solution fits b exactly with m nonzeros
+        assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b)
+    else:      # full column rank: the unique least-squares solution
+        np.testing.assert_allclose(x, np.linalg.lstsq(A, b, rcond=None)[0], rtol=0,
+                                   atol=1e-10 * np.linalg.norm(x))
+
+
+def test_start_skips_zero_columns():
+    """Zero columns have zero norm, are never pivoted ahead of the others and
+    get zero entries; the rank is that of the nonzero columns."""
+    A, b = _problem(12, 30, 5)
+    A[:, [0, 7, 29]] = 0
+    x = admm.basic_solution(A, b)
+    assert np.all(x[[0, 7, 29]] == 0)
+    assert np.count_nonzero(x) == 12
+    assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b)
