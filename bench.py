@@ -415,6 +415,16 @@ def other_workloads(ia, sc, which=("ibem", "hels", "radiation", "gradient")):
             "workload": "radiation modes, 64 x 64 patches (4096), 1024 mics, 256 frequencies "
                         "50-2500 Hz, 128 modes, FP64 eigendecomposition",
             "frequencies_per_s": round(len(R.k) / dt, 2), "s": round(dt, 4)}
+        # captured into a HIP graph (IA_GRAPH=1): the warm-up call runs eagerly
+        # and captures, the timed call replays
+        os.environ["IA_GRAPH"] = "1"
+        try:
+            dt = clock(lambda: ia.radiation_sweep(R.k, R.r_p, R.patch_area, R.r_h, R.b, 128,
+                                                  precision=prec))
+        finally:
+            del os.environ["IA_GRAPH"]
+        out[f"radiation_{prec}"]["graph_replay"] = {
+            "frequencies_per_s": round(len(R.k) / dt, 2), "s": round(dt, 4)}
     if "gradient" not in which:
         return out
     G = sc.plate_scenario(n_freq=32, device="cuda")

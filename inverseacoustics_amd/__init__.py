@@ -455,7 +455,6 @@ def radiation_sweep(k, r_p, area, r_h, b, n_modes, precision="fp64", filter="tik
 
 
 # ------------------------------------------------------------ ESM sweep
-@dataclass# ------------------------------------------------------------ ESM sweep
 @dataclass
 class SweepResult:
     lam: np.ndarray

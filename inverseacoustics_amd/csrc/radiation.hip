@@ -42,6 +42,13 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
                            const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
                            size_t sx, double *lam, int *idx, int *kts, double *smax,
                            hipStream_t s, double *gap);
+template <typename T>
+ia_status regsolve_async(int rows, int cols, int batch, const cx<double> *A, size_t sA,
+                         const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
+                         size_t sx, double *lam, int *idx, int *kts, double *smax,
+                         hipStream_t s);
+template <typename T>
+void regsolve_addresses(std::vector<uintptr_t> &v);
 RegDev to_regdev(const ia_reg_params &p);
 ia_status check_reg(const ia_reg_params &p, int pmax);
 
@@ -366,14 +373,44 @@ inline unsigned blocks256(size_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
 
-// device workspace of the radiation sweep, kept per host thread
+// The decisions of one eigen-solve of a chunk: the window [a, b) of each
+// Rayleigh-Ritz round of the subspace iteration (round 0 covers the chunk),
+// the frequencies sent to the full SVD and how many of those one full-SVD
+// launch takes.  They depend on the geometry and the wavenumbers only (not on
+// the pressures), so a graph that replays them serves every later call with
+// the same ones.
+struct RadPlan {
+    std::vector<std::pair<int, int>> rounds;
+    std::vector<int> failed;
+    int fc = 0;
+};
+
+// device workspace of the radiation sweep, kept per host thread; with
+// IA_GRAPH the captured sweep, its key and plan, and the stream it replays on
 struct RadCache {
     DevBuf S, U, sg, G, H, y, v, lam, idx, kts, smax, W, eig, modes, k, p, rh, b;
     DevBuf fidx, kg, U2, sg2;  // frequencies the full SVD serves
     // subspace iteration
     DevBuf Rr, Om, Yr, Qr, Yc, Qc, tq, qw, Br, Bc, UBc, UBr, Mr, Zr, rr, th;
     SvdWork<double> sw, sw2;
+    hipStream_t gs = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uintptr_t> gkey;
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        gkey.clear();
+    }
+    void addresses(std::vector<uintptr_t> &out) const {
+        for (const DevBuf *q : {&S, &U, &sg, &G, &H, &y, &v, &lam, &idx, &kts, &smax, &W, &eig,
+                                &modes, &k, &p, &rh, &b, &Rr, &Om, &Yr, &Qr, &Yc, &Qc, &tq, &qw,
+                                &Br, &Bc, &UBc, &UBr, &Mr, &Zr, &rr, &th, &fidx, &kg, &U2, &sg2})
+            out.push_back((uintptr_t)q->p);
+        sw.addresses(out);
+        sw2.addresses(out);
+    }
     void release() {
+        drop_graph();
         for (DevBuf *q : {&S, &U, &sg, &G, &H, &y, &v, &lam, &idx, &kts, &smax, &W, &eig, &modes,
                           &k, &p, &rh, &b, &Rr, &Om, &Yr, &Qr, &Yc, &Qc, &tq, &qw, &Br, &Bc, &UBc,
                           &UBr, &Mr, &Zr, &rr, &th, &fidx, &kg, &U2, &sg2})
@@ -476,7 +513,7 @@ size_t subspace_bytes(int np, int n, int kk) {
 // whose U and sg are to be overwritten.
 ia_status subspace_modes(RadCache &W, int np, int n, int kk, const double *kf, int nf,
                          double tol, double floor_rel, cx<double> *U, double *sg, hipStream_t s,
-                         std::vector<int> &failed) {
+                         std::vector<int> &failed, RadPlan *rec, const RadPlan *play) {
     const size_t nn = (size_t)np * np, pk = (size_t)np * kk, pn = (size_t)np * n,
                  kn = (size_t)kk * n, k2 = (size_t)kk * kk;
     IA_TRY(W.Rr.alloc(sizeof(double) * nn * nf));
@@ -553,9 +590,19 @@ ia_status subspace_modes(RadCache &W, int np, int n, int kk, const double *kf, i
     // [a, b) of the frequencies still short of the tolerance that can reach
     // it (in a sweep sorted by frequency, the highest ones)
     IA_TRY(step(0, nf));
+    if (play) {
+        // the recorded rounds, no host reads (graph capture)
+        for (size_t round = 0; round < play->rounds.size(); ++round) {
+            const int a = play->rounds[round].first, b = play->rounds[round].second;
+            if (round > 0) IA_TRY(step(a, b - a));
+            IA_TRY(ritz(a, b - a));
+        }
+        failed = play->failed;
+    }
     std::vector<double> rr((size_t)n * nf), th((size_t)kk * nf), res(nf, INFINITY);
     int a = 0, b = nf;
-    for (int round = 0; round < RAD_ROUNDS && a < b; ++round) {
+    for (int round = 0; !play && round < RAD_ROUNDS && a < b; ++round) {
+        if (rec) rec->rounds.emplace_back(a, b);
         if (round > 0) IA_TRY(step(a, b - a));
         IA_TRY(ritz(a, b - a));
         IA_HIP(hipStreamSynchronize(s));
@@ -584,9 +631,12 @@ ia_status subspace_modes(RadCache &W, int np, int n, int kk, const double *kf, i
         a = na;
         b = nb;
     }
-    failed.clear();
-    for (int f = 0; f < nf; ++f)
-        if (!(res[f] <= tol)) failed.push_back(f);
+    if (!play) {
+        failed.clear();
+        for (int f = 0; f < nf; ++f)
+            if (!(res[f] <= tol)) failed.push_back(f);
+        if (rec) rec->failed = failed;
+    }
     // the modes: U_B's kept columns orthonormalised (then so are those of
     // M = Q U_B, to Q's orthogonality), the dropped ones zero
     hipLaunchKernelGGL(k_lead, dim3(blocks256((size_t)n * nf)), dim3(256), 0, s, n, kk, nf, TH,
@@ -615,6 +665,119 @@ __global__ void k_scatter_modes(int np, int n, const int *__restrict__ idx,
         U[(size_t)f * per + q] = U2[(size_t)i * per + q];
     if (blockIdx.x == 0)
         for (int j = threadIdx.x; j < n; j += blockDim.x) sg[(size_t)f * n + j] = sg2[(size_t)i * n + j];
+}
+
+// One chunk of the sweep from the eigen-solve to W and v, enqueued on s.
+// rec: record the eigen-solver's decisions; play: replay recorded ones (no
+// host reads, uploads or synchronisation: graph capture).
+ia_status rad_chunk(RadCache &W, const ia_radiation_desc &desc, int f0, int nf, RadSolver solver,
+                    int kk, double tol, double eig_floor, const RegDev &prm, bool want_modes,
+                    bool want_vel, hipStream_t s, RadPlan *rec, const RadPlan *play) {
+    const int np = desc.np, l = desc.l, n = desc.n_modes;
+    const double c = air().c, rho = air().rho, A = desc.area;
+    // w^2 rho A^2/(4 pi c) = k^2 (c rho A^2 / (4 pi))
+    const double scale_c = c * rho * A * A / (4 * M_PI);
+    const bool sub = solver != RAD_FULL;
+    const double *kf = W.k.as<double>() + f0;
+    IA_TRY(W.U.alloc(sizeof(cx<double>) * (size_t)np * n * nf));
+    IA_TRY(W.sg.alloc(sizeof(double) * (size_t)n * nf));
+    cx<double> *U = W.U.as<cx<double>>();
+    double *sg = W.sg.as<double>();
+    std::vector<int> failed;
+    if (sub) {
+        IA_TRY(subspace_modes(W, np, n, kk, kf, nf, tol, eig_floor, U, sg, s, failed, rec, play));
+        if (!failed.empty() && solver == RAD_SUBSPACE)
+            return fail(IA_ERR_NOCONV,
+                        "ia_radiation_sweep: subspace iteration missed the Ritz tolerance %g "
+                        "at %zu of frequencies %d..%d (first %d)",
+                        tol, failed.size(), f0, f0 + nf - 1, f0 + failed[0]);
+    } else {
+        for (int f = 0; f < nf; ++f) failed.push_back(f);
+        if (rec) rec->failed = failed;
+    }
+    if (!failed.empty()) {
+        // the full SVD for the listed frequencies, as many at a time as
+        // memory holds, into U2 / sg2 and moved into place
+        const int m = (int)failed.size();
+        const int fc = play ? play->fc
+                            : desc.chunk > 0
+                                  ? m
+                                  : (int)std::min<size_t>(
+                                        std::min<size_t>(m, 128),
+                                        std::max<size_t>(1, (free_bytes() / 5 * 2) / full_bytes(np)));
+        if (rec) rec->fc = fc;
+        IA_TRY(W.fidx.alloc(sizeof(int) * m));
+        IA_TRY(W.kg.alloc(sizeof(double) * m));
+        IA_TRY(W.U2.alloc(sizeof(cx<double>) * (size_t)np * n * fc));
+        IA_TRY(W.sg2.alloc(sizeof(double) * (size_t)n * fc));
+        if (!play)
+            IA_HIP(hipMemcpy(W.fidx.p, failed.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_gather_k, dim3(blocks256(m)), dim3(256), 0, s, m, W.fidx.as<int>(),
+                           kf, W.kg.as<double>());
+        for (int g0 = 0; g0 < m; g0 += fc) {
+            const int ng = std::min(fc, m - g0);
+            IA_TRY(full_modes(W, np, n, W.kg.as<double>() + g0, ng, eig_floor,
+                              W.U2.as<cx<double>>(), W.sg2.as<double>(), s));
+            hipLaunchKernelGGL(k_scatter_modes, dim3(256, ng), dim3(256), 0, s, np, n,
+                               W.fidx.as<int>() + g0, W.U2.as<cx<double>>(), W.sg2.as<double>(),
+                               U, sg);
+            IA_HIP(hipGetLastError());
+        }
+    }
+    double *md = nullptr;
+    if (want_modes) {
+        IA_TRY(W.modes.alloc(sizeof(double) * (size_t)np * n * nf));
+        md = W.modes.as<double>();
+    }
+    // each mode real with its largest entry positive; kept ones already
+    // orthonormal, dropped ones zero
+    hipLaunchKernelGGL(k_realign, dim3(n, nf), dim3(256), 0, s, np, n, U);
+    launch_orth_modes(np, n, nf, sg, eig_floor, false, U, md, s);
+    // H_r = G Q
+    IA_TRY(W.G.alloc(sizeof(cx<double>) * (size_t)l * np * nf));
+    IA_TRY(W.H.alloc(sizeof(cx<double>) * (size_t)l * n * nf));
+    cx<double> *G = W.G.as<cx<double>>(), *H = W.H.as<cx<double>>();
+    hipLaunchKernelGGL(k_rad_G, dim3((l + RT - 1) / RT, (np + RT - 1) / RT, nf), dim3(256), 0, s,
+                       l, np, kf, rho * c * A, W.rh.as<double>(), W.p.as<double>(), G,
+                       (size_t)l * np);
+    const cx<double> one{1, 0}, zero{0, 0};
+    launch_gemm<double>('N', 'N', l, n, np, one, G, l, (size_t)l * np, U, np, (size_t)np * n,
+                        zero, H, l, (size_t)l * n, nf, s);
+    IA_HIP(hipGetLastError());
+    // regularised solve in mode space
+    IA_TRY(W.y.alloc(sizeof(cx<double>) * (size_t)n * nf));
+    IA_TRY(W.lam.alloc(sizeof(double) * nf));
+    IA_TRY(W.idx.alloc(sizeof(int) * nf));
+    IA_TRY(W.kts.alloc(sizeof(int) * nf));
+    IA_TRY(W.smax.alloc(sizeof(double) * nf));
+    const cx<double> *bf = W.b.as<cx<double>>() + (size_t)f0 * l;
+    cx<double> *y = W.y.as<cx<double>>();
+    if (desc.precision == IA_PREC_MIXED)
+        IA_TRY(play ? regsolve_async<float>(l, n, nf, H, (size_t)l * n, bf, l, prm, y, n,
+                                            W.lam.as<double>(), W.idx.as<int>(), W.kts.as<int>(),
+                                            W.smax.as<double>(), s)
+                    : regsolve_batched<float>(l, n, nf, H, (size_t)l * n, bf, l, prm, y, n,
+                                              W.lam.as<double>(), W.idx.as<int>(),
+                                              W.kts.as<int>(), W.smax.as<double>(), s, nullptr));
+    else
+        IA_TRY(play ? regsolve_async<double>(l, n, nf, H, (size_t)l * n, bf, l, prm, y, n,
+                                             W.lam.as<double>(), W.idx.as<int>(), W.kts.as<int>(),
+                                             W.smax.as<double>(), s)
+                    : regsolve_batched<double>(l, n, nf, H, (size_t)l * n, bf, l, prm, y, n,
+                                               W.lam.as<double>(), W.idx.as<int>(),
+                                               W.kts.as<int>(), W.smax.as<double>(), s, nullptr));
+    // W = sum e |y|^2, v = Q y
+    IA_TRY(W.W.alloc(sizeof(double) * nf));
+    IA_TRY(W.eig.alloc(sizeof(double) * (size_t)n * nf));
+    hipLaunchKernelGGL(k_modal_power, dim3(nf), dim3(256), 0, s, np, n, kf, scale_c, sg, y,
+                       W.eig.as<double>(), W.W.as<double>());
+    if (want_vel) {
+        IA_TRY(W.v.alloc(sizeof(cx<double>) * (size_t)np * nf));
+        launch_gemm<double>('N', 'N', np, 1, n, one, U, np, (size_t)np * n, y, n, n, zero,
+                            W.v.as<cx<double>>(), np, np, nf, s);
+    }
+    IA_HIP(hipGetLastError());
+    return IA_OK;
 }
 
 }  // namespace
@@ -666,8 +829,6 @@ extern "C" ia_status ia_radiation_sweep(const ia_radiation_desc *desc, const dou
     hipStream_t s = nullptr;
     RadCache &W = rad_cache();
     const double c = air().c, rho = air().rho, A = desc->area;
-    // w^2 rho A^2/(4 pi c) = k^2 (c rho A^2 / (4 pi))
-    const double scale_c = c * rho * A * A / (4 * M_PI);
     const RegDev prm = to_regdev(desc->reg);
     const double eig_floor = desc->eig_floor == 0.0 ? 1e-8 : std::max(desc->eig_floor, 0.0);
     IA_TRY(to_device(W.k, k, sizeof(double) * F, s));
@@ -689,100 +850,13 @@ extern "C" ia_status ia_radiation_sweep(const ia_radiation_desc *desc, const dou
     chunk = std::min(std::min(chunk, F), sub ? 256 : 128);
     ia_radiation_out o{};
     if (out) o = *out;
-    for (int f0 = 0; f0 < F; f0 += chunk) {
-        const int nf = std::min(chunk, F - f0);
-        const double *kf = W.k.as<double>() + f0;
-        IA_TRY(W.U.alloc(sizeof(cx<double>) * (size_t)np * n * nf));
-        IA_TRY(W.sg.alloc(sizeof(double) * (size_t)n * nf));
-        cx<double> *U = W.U.as<cx<double>>();
-        double *sg = W.sg.as<double>();
-        std::vector<int> failed;
-        if (sub) {
-            IA_TRY(subspace_modes(W, np, n, kk, kf, nf, tol, eig_floor, U, sg, s, failed));
-            if (!failed.empty() && solver == RAD_SUBSPACE)
-                return fail(IA_ERR_NOCONV,
-                            "ia_radiation_sweep: subspace iteration missed the Ritz tolerance %g "
-                            "at %zu of frequencies %d..%d (first %d)",
-                            tol, failed.size(), f0, f0 + nf - 1, f0 + failed[0]);
-        } else {
-            for (int f = 0; f < nf; ++f) failed.push_back(f);
-        }
-        if (!failed.empty()) {
-            // the full SVD for the listed frequencies, as many at a time as
-            // memory holds, into U2 / sg2 and moved into place
-            const int m = (int)failed.size();
-            const int fc = desc->chunk > 0
-                               ? m
-                               : (int)std::min<size_t>(
-                                     std::min<size_t>(m, 128),
-                                     std::max<size_t>(1, (free_bytes() / 5 * 2) / full_bytes(np)));
-            IA_TRY(W.fidx.alloc(sizeof(int) * m));
-            IA_TRY(W.kg.alloc(sizeof(double) * m));
-            IA_TRY(W.U2.alloc(sizeof(cx<double>) * (size_t)np * n * fc));
-            IA_TRY(W.sg2.alloc(sizeof(double) * (size_t)n * fc));
-            IA_HIP(hipMemcpy(W.fidx.p, failed.data(), sizeof(int) * m, hipMemcpyHostToDevice));
-            hipLaunchKernelGGL(k_gather_k, dim3(blocks256(m)), dim3(256), 0, s, m,
-                               W.fidx.as<int>(), kf, W.kg.as<double>());
-            for (int g0 = 0; g0 < m; g0 += fc) {
-                const int ng = std::min(fc, m - g0);
-                IA_TRY(full_modes(W, np, n, W.kg.as<double>() + g0, ng, eig_floor,
-                                  W.U2.as<cx<double>>(), W.sg2.as<double>(), s));
-                hipLaunchKernelGGL(k_scatter_modes, dim3(256, ng), dim3(256), 0, s, np, n,
-                                   W.fidx.as<int>() + g0, W.U2.as<cx<double>>(),
-                                   W.sg2.as<double>(), U, sg);
-                IA_HIP(hipGetLastError());
-            }
-        }
-        double *md = nullptr;
-        if (o.modes) {
-            IA_TRY(W.modes.alloc(sizeof(double) * (size_t)np * n * nf));
-            md = W.modes.as<double>();
-        }
-        // each mode real with its largest entry positive; kept ones already
-        // orthonormal, dropped ones zero
-        hipLaunchKernelGGL(k_realign, dim3(n, nf), dim3(256), 0, s, np, n, U);
-        launch_orth_modes(np, n, nf, sg, eig_floor, false, U, md, s);
-        // H_r = G Q
-        IA_TRY(W.G.alloc(sizeof(cx<double>) * (size_t)l * np * nf));
-        IA_TRY(W.H.alloc(sizeof(cx<double>) * (size_t)l * n * nf));
-        cx<double> *G = W.G.as<cx<double>>(), *H = W.H.as<cx<double>>();
-        hipLaunchKernelGGL(k_rad_G, dim3((l + RT - 1) / RT, (np + RT - 1) / RT, nf), dim3(256), 0,
-                           s, l, np, kf, rho * c * A, W.rh.as<double>(), W.p.as<double>(), G,
-                           (size_t)l * np);
-        const cx<double> one{1, 0}, zero{0, 0};
-        launch_gemm<double>('N', 'N', l, n, np, one, G, l, (size_t)l * np, U, np, (size_t)np * n,
-                            zero, H, l, (size_t)l * n, nf, s);
-        IA_HIP(hipGetLastError());
-        // regularised solve in mode space
-        IA_TRY(W.y.alloc(sizeof(cx<double>) * (size_t)n * nf));
-        IA_TRY(W.lam.alloc(sizeof(double) * nf));
-        IA_TRY(W.idx.alloc(sizeof(int) * nf));
-        IA_TRY(W.kts.alloc(sizeof(int) * nf));
-        IA_TRY(W.smax.alloc(sizeof(double) * nf));
-        const cx<double> *bf = W.b.as<cx<double>>() + (size_t)f0 * l;
-        if (desc->precision == IA_PREC_MIXED)
-            IA_TRY(regsolve_batched<float>(l, n, nf, H, (size_t)l * n, bf, l, prm,
-                                           W.y.as<cx<double>>(), n, W.lam.as<double>(),
-                                           W.idx.as<int>(), W.kts.as<int>(), W.smax.as<double>(), s,
-                                           nullptr));
-        else
-            IA_TRY(regsolve_batched<double>(l, n, nf, H, (size_t)l * n, bf, l, prm,
-                                            W.y.as<cx<double>>(), n, W.lam.as<double>(),
-                                            W.idx.as<int>(), W.kts.as<int>(), W.smax.as<double>(),
-                                            s, nullptr));
-        // W = sum e |y|^2, v = Q y
-        IA_TRY(W.W.alloc(sizeof(double) * nf));
-        IA_TRY(W.eig.alloc(sizeof(double) * (size_t)n * nf));
-        hipLaunchKernelGGL(k_modal_power, dim3(nf), dim3(256), 0, s, np, n, kf, scale_c, sg,
-                           W.y.as<cx<double>>(), W.eig.as<double>(), W.W.as<double>());
-        if (o.velocity) {
-            IA_TRY(W.v.alloc(sizeof(cx<double>) * (size_t)np * nf));
-            launch_gemm<double>('N', 'N', np, 1, n, one, U, np, (size_t)np * n,
-                                W.y.as<cx<double>>(), n, n, zero, W.v.as<cx<double>>(), np, np, nf,
-                                s);
-        }
-        IA_HIP(hipGetLastError());
-        IA_HIP(hipStreamSynchronize(s));
+    const bool want_modes = o.modes != nullptr, want_vel = o.velocity != nullptr;
+    auto chunk_work = [&](int f0, int nf, hipStream_t st, RadPlan *rec,
+                          const RadPlan *play) -> ia_status {
+        return rad_chunk(W, *desc, f0, nf, solver, kk, tol, eig_floor, prm, want_modes, want_vel,
+                         st, rec, play);
+    };
+    auto copy_out = [&](int f0, int nf) -> ia_status {
         if (o.power) IA_HIP(hipMemcpy(o.power + f0, W.W.p, sizeof(double) * nf, hipMemcpyDefault));
         if (o.lambda) IA_HIP(hipMemcpy(o.lambda + f0, W.lam.p, sizeof(double) * nf, hipMemcpyDefault));
         if (o.lambda_index)
@@ -804,6 +878,75 @@ extern "C" ia_status ia_radiation_sweep(const ia_radiation_desc *desc, const dou
             std::vector<int> ok(nf, IA_OK);
             IA_HIP(hipMemcpy(o.status + f0, ok.data(), sizeof(int) * nf, hipMemcpyDefault));
         }
+        return IA_OK;
+    };
+    if (graph_mode() && chunk >= F) {
+        // IA_GRAPH=1 (one chunk): the sweep is captured into a graph that
+        // replays the eigen-solver's decisions of an eager run; the key holds
+        // the sizes, parameters, a hash of the wavenumbers and patch
+        // positions (the decisions depend on them) and every buffer address
+        auto make_key = [&](std::vector<uintptr_t> &key) -> ia_status {
+            std::vector<double> hv((size_t)F + 3 * (size_t)np);
+            IA_HIP(hipMemcpy(hv.data(), W.k.p, sizeof(double) * F, hipMemcpyDeviceToHost));
+            IA_HIP(hipMemcpy(hv.data() + F, W.p.p, sizeof(double) * 3 * np, hipMemcpyDeviceToHost));
+            uint64_t h = 1469598103934665603ull;
+            for (double v : hv) h = (h ^ (uint64_t)dbits(v)) * 1099511628211ull;
+            key = {(uintptr_t)F, (uintptr_t)np, (uintptr_t)l, (uintptr_t)n, (uintptr_t)kk,
+                   (uintptr_t)desc->precision, (uintptr_t)prm.filter, (uintptr_t)prm.select,
+                   (uintptr_t)prm.n_lambda, (uintptr_t)prm.k_fixed, dbits(prm.decades),
+                   dbits(prm.lambda_fixed), dbits(A), dbits(c), dbits(rho), dbits(eig_floor),
+                   dbits(tol), (uintptr_t)solver, (uintptr_t)want_modes, (uintptr_t)want_vel,
+                   (uintptr_t)h};
+            W.addresses(key);
+            if (desc->precision == IA_PREC_MIXED)
+                regsolve_addresses<float>(key);
+            else
+                regsolve_addresses<double>(key);
+            return IA_OK;
+        };
+        if (!W.gs) IA_HIP(hipStreamCreateWithFlags(&W.gs, hipStreamNonBlocking));
+        static thread_local RadPlan plan;
+        std::vector<uintptr_t> key;
+        IA_TRY(make_key(key));
+        if (W.gexec && key == W.gkey) {
+            // the plan's full-SVD list is the one piece of state the graph
+            // reads that another call may have overwritten
+            if (!plan.failed.empty())
+                IA_HIP(hipMemcpyAsync(W.fidx.p, plan.failed.data(), sizeof(int) * plan.failed.size(),
+                                      hipMemcpyHostToDevice, s));
+            IA_HIP(hipStreamSynchronize(s));  // inputs copied on the legacy stream
+            IA_HIP(hipGraphLaunch(W.gexec, W.gs));
+            IA_HIP(hipStreamSynchronize(W.gs));
+            return copy_out(0, F);
+        }
+        W.drop_graph();
+        RadPlan rec;
+        IA_TRY(chunk_work(0, F, s, &rec, nullptr));
+        IA_HIP(hipStreamSynchronize(s));
+        IA_TRY(copy_out(0, F));
+        IA_TRY(make_key(key));  // addresses after this run's allocations
+        IA_HIP(hipStreamBeginCapture(W.gs, hipStreamCaptureModeThreadLocal));
+        const ia_status st = chunk_work(0, F, W.gs, nullptr, &rec);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(W.gs, &g);
+        if (st == IA_OK && ce == hipSuccess && g &&
+            hipGraphInstantiate(&W.gexec, g, nullptr, nullptr, 0) != hipSuccess)
+            W.gexec = nullptr;
+        if (g) (void)hipGraphDestroy(g);
+        IA_TRY(st);
+        if (!W.gexec) {
+            (void)hipGetLastError();
+            return fail(IA_ERR_DEVICE, "ia_radiation_sweep: capture into a graph failed");
+        }
+        W.gkey = key;
+        plan = rec;
+        return IA_OK;
+    }
+    for (int f0 = 0; f0 < F; f0 += chunk) {
+        const int nf = std::min(chunk, F - f0);
+        IA_TRY(chunk_work(f0, nf, s, nullptr, nullptr));
+        IA_HIP(hipStreamSynchronize(s));
+        IA_TRY(copy_out(f0, nf));
     }
     return IA_OK;
 }

@@ -150,6 +150,29 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
     return IA_OK;
 }
 
+// Non-blocking form on the slot-0 workspace (callers that capture the solve
+// into a graph), and that workspace's buffer addresses (a graph key).
+template <typename T>
+ia_status regsolve_async(int rows, int cols, int batch, const cx<double> *A, size_t sA,
+                         const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
+                         size_t sx, double *lam, int *idx, int *kts, double *smax,
+                         hipStream_t s) {
+    return regsolve_enqueue<T>(regsolve_work<T>(0), rows, cols, batch, A, sA, b, sb, prm, x, sx,
+                               lam, idx, kts, smax, s, nullptr);
+}
+template <typename T>
+void regsolve_addresses(std::vector<uintptr_t> &v) {
+    regsolve_work<T>(0).addresses(v);
+}
+template ia_status regsolve_async<double>(int, int, int, const cx<double> *, size_t,
+                                          const cx<double> *, size_t, const RegDev &, cx<double> *,
+                                          size_t, double *, int *, int *, double *, hipStream_t);
+template ia_status regsolve_async<float>(int, int, int, const cx<double> *, size_t,
+                                         const cx<double> *, size_t, const RegDev &, cx<double> *,
+                                         size_t, double *, int *, int *, double *, hipStream_t);
+template void regsolve_addresses<double>(std::vector<uintptr_t> &);
+template void regsolve_addresses<float>(std::vector<uintptr_t> &);
+
 template ia_status regsolve_batched<double>(int, int, int, const cx<double> *, size_t,
                                             const cx<double> *, size_t, const RegDev &,
                                             cx<double> *, size_t, double *, int *, int *, double *,

@@ -116,3 +116,41 @@ def test_hels_graph_replays_match_eager(ia, monkeypatch, precision, n_freq, fmax
     got = run(S.b)
     np.testing.assert_array_equal(got.velocity, ref.velocity)
     monkeypatch.delenv("IA_GRAPH")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_radiation_graph_replays_match_eager(ia, monkeypatch, precision):
+    """ia_radiation_sweep with IA_GRAPH=1: the first call runs eagerly,
+    recording the eigen-solver's decisions (subspace-iteration rounds, the
+    4000 Hz frequency sent to the full SVD), and captures a graph that
+    replays them; later calls with the same wavenumbers and patches replay
+    it.  Replays equal the eager sweep bit for bit, also after an eager
+    sweep of other frequencies reused the workspace in between, and doubled
+    pressures give doubled amplitudes and four times the power."""
+    S = sc.radiation_scenario(freqs=[300.0, 1200.0, 2500.0, 4000.0], patches=(32, 32),
+                              mic_n=(16, 16), mic_pitch=0.035)
+    ia.set_air(S.c, S.rho)
+    n = 48
+
+    def run(b):
+        return ia.radiation_sweep(S.k, S.r_p, S.patch_area, S.r_h, b, n, precision=precision,
+                                  want_modes=True)
+
+    monkeypatch.delenv("IA_GRAPH", raising=False)
+    ref = run(S.b)
+    assert np.all(ref["status"] == 0)
+    for rep in range(3):
+        monkeypatch.setenv("IA_GRAPH", "1")
+        got = run(S.b)
+        for q in ("power", "lambda", "lambda_index", "eigval", "modes", "amplitude", "velocity"):
+            np.testing.assert_array_equal(got[q], ref[q], err_msg=f"call {rep}: {q}")
+        monkeypatch.delenv("IA_GRAPH")
+        other = ia.radiation_sweep(S.k[3:], S.r_p, S.patch_area, S.r_h, S.b[3:], n)
+        assert np.all(other["status"] == 0)
+    monkeypatch.setenv("IA_GRAPH", "1")
+    got = run(2.0 * S.b)
+    np.testing.assert_array_equal(got["lambda_index"], ref["lambda_index"])
+    np.testing.assert_allclose(got["power"], 4.0 * ref["power"], rtol=1e-9)
+    np.testing.assert_allclose(got["amplitude"], 2.0 * ref["amplitude"], rtol=0,
+                               atol=1e-9 * np.abs(ref["amplitude"]).max())
+    monkeypatch.delenv("IA_GRAPH")
