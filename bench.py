@@ -386,6 +386,14 @@ def other_workloads(ia, sc, which=("ibem", "hels", "radiation", "gradient")):
         out[f"ibem_{prec}"] = {"workload": "IBEM icosphere(3) 642 nodes, 128 mics, 64 frequencies "
                                            "100-2000 Hz, 7-point rule",
                                "frequencies_per_s": round(len(S.k) / dt, 2), "s": round(dt, 4)}
+        # the device side captured into a HIP graph (IA_GRAPH=1) and replayed
+        os.environ["IA_GRAPH"] = "1"
+        try:
+            dt = clock(lambda: ia.ibem_sweep(S.k, S.X, S.E, S.r_h, S.b, S.area, precision=prec))
+        finally:
+            del os.environ["IA_GRAPH"]
+        out[f"ibem_{prec}"]["graph_replay"] = {"frequencies_per_s": round(len(S.k) / dt, 2),
+                                               "s": round(dt, 4)}
     if "hels" in which:
         H = sc.hels_scenario()
         ia.set_air(H.c, H.rho)

@@ -161,10 +161,14 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
 template <typename T>
 ia_status getrs1_spread(int n, const cx<T> *A, int lda, size_t strideA, const int *ipiv,
                         cx<T> *B, size_t strideB, cx<T> *S, int batch, hipStream_t s);
+// work (optional): getrs_work_bytes<T>(n, nrhs, batch) bytes of device
+// scratch for nrhs > 1; without it the scratch is taken with hipMallocAsync
+template <typename T>
+size_t getrs_work_bytes(int n, int nrhs, int batch);
 template <typename T>
 ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_t strideA,
                         const int *ipiv, cx<T> *B, int ldb, size_t strideB, int batch,
-                        hipStream_t s);
+                        hipStream_t s, void *work = nullptr);
 
 // getrf's interchanges as a row index per matrix: (P B)[r] = B[idx[r]],
 // inv the inverse permutation (n ints each per matrix)

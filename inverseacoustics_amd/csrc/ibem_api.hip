@@ -36,6 +36,15 @@ ia_status regsolve_batched(int rows, int cols, int batch, const cx<double> *A, s
                            const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
                            size_t sx, double *lam, int *idx, int *kts, double *smax, hipStream_t s,
                            double *gap = nullptr);
+template <typename T>
+ia_status regsolve_async(int rows, int cols, int batch, const cx<double> *A, size_t sA,
+                         const cx<double> *b, size_t sb, const RegDev &prm, cx<double> *x,
+                         size_t sx, double *lam, int *idx, int *kts, double *smax,
+                         hipStream_t s);
+template <typename T>
+void regsolve_addresses(std::vector<uintptr_t> &v);
+template <typename T>
+ia_status regsolve_reserve(int rows, int cols, int batch);
 RegDev to_regdev(const ia_reg_params &p);
 ia_status check_reg(const ia_reg_params &p, int pmax);
 
@@ -211,19 +220,38 @@ ia_status build_mesh(int m, int n, const int *e_n, Mesh &M) {
 // IbemDev allocates nothing on a repeated call of the same size).
 struct IbemDev {
     DevBuf X, E, adj_ptr, adj, sing_ptr, sing_oc, sing_list, rule, pages, cp, rlh, k, Z, H;
-    DevBuf A, B, Ah, Bh, Ad, Bd, Ahd, Bhd, ipiv, info, tau;
+    DevBuf A, B, Ah, Bh, Ad, Bd, Ahd, Bhd, ipiv, info, tau, lw, gw;  // lw, gw: LU scratch
     void release() {
         for (DevBuf *b : {&X, &E, &adj_ptr, &adj, &sing_ptr, &sing_oc, &sing_list, &rule, &pages,
                           &cp, &rlh, &k, &Z, &H, &A, &B, &Ah, &Bh, &Ad, &Bd, &Ahd, &Bhd, &ipiv,
-                          &info, &tau})
+                          &info, &tau, &lw, &gw})
             b->reset();
+    }
+    void addresses(std::vector<uintptr_t> &v) const {
+        for (const DevBuf *b : {&X, &E, &adj_ptr, &adj, &sing_ptr, &sing_oc, &sing_list, &rule,
+                                &pages, &cp, &rlh, &k, &Z, &H, &A, &B, &Ah, &Bh, &Ad, &Bd, &Ahd,
+                                &Bhd, &ipiv, &info, &tau, &lw, &gw})
+            v.push_back((uintptr_t)b->p);
     }
 };
 
+// launch sizes of one IBEM build that come from the mesh and the rules
+struct IbemShape {
+    int nse = 0, nq = 0, nqs = 0;
+};
+
+__global__ void k_zero_ints(int *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Host side of an IBEM build for nf wavenumbers: the mesh adjacency, every
+// input uploaded and every device buffer sized (nothing of it is captured
+// into a graph).
 template <typename T>
-ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, const double *r_lh,
-                     int n_ch, const double *cp, const double *k, int nf, IbemDev &D,
-                     hipStream_t s) {
+ia_status ibem_upload(int m, int l, int n, const double *r_ms, const int *e_n,
+                      const double *r_lh, int n_ch, const double *cp, const double *k, int nf,
+                      IbemDev &D, hipStream_t s, IbemShape &sh) {
     Mesh M;
     IA_TRY(build_mesh(m, n, e_n, M));
     const Quad &Q = quad();
@@ -232,7 +260,9 @@ ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, co
     IA_TRY(to_device(D.E, M.E.data(), sizeof(int) * 3 * (size_t)n, s));
     IA_TRY(to_device(D.adj_ptr, M.adj_ptr.data(), sizeof(int) * (m + 1), s));
     IA_TRY(to_device(D.adj, M.adj.data(), sizeof(int) * M.adj.size(), s));
-    const int nse = (int)M.sing_oc.size() / 2;
+    sh.nse = (int)M.sing_oc.size() / 2;
+    sh.nq = Q.nq;
+    sh.nqs = Q.nqs;
     IA_TRY(to_device(D.sing_ptr, M.sing_ptr.data(), sizeof(int) * M.sing_ptr.size(), s));
     IA_TRY(to_device(D.sing_oc, M.sing_oc.data(), sizeof(int) * M.sing_oc.size(), s));
     IA_TRY(to_device(D.sing_list, M.sing_list.data(), sizeof(int) * M.sing_list.size(), s));
@@ -241,33 +271,59 @@ ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, co
     if (l > 0) IA_TRY(to_device(D.rlh, r_lh, sizeof(double) * 3 * l, s));
     if (n_ch > 0) IA_TRY(to_device(D.cp, cp, sizeof(double) * 3 * n_ch, s));
     IA_TRY(to_device(D.k, k, sizeof(double) * nf, s));
-    const double rho_c = air().rho * air().c;
-    DevBuf &bA = D.A, &bB = D.B, &bAh = D.Ah, &bBh = D.Bh, &bAd = D.Ad, &bBd = D.Bd,
-           &bAhd = D.Ahd, &bBhd = D.Bhd;
     const size_t sS = (size_t)mt * m, sH = (size_t)std::max(l, 1) * m;
-    IA_TRY(bA.alloc(sizeof(cx<T>) * sS * nf));
-    IA_TRY(bB.alloc(sizeof(cx<T>) * sS * nf));
-    IA_TRY(bAh.alloc(sizeof(cx<T>) * sH * nf));
-    IA_TRY(bBh.alloc(sizeof(cx<T>) * sH * nf));
+    IA_TRY(D.A.alloc(sizeof(cx<T>) * sS * nf));
+    IA_TRY(D.B.alloc(sizeof(cx<T>) * sS * nf));
+    IA_TRY(D.Ah.alloc(sizeof(cx<T>) * sH * nf));
+    IA_TRY(D.Bh.alloc(sizeof(cx<T>) * sH * nf));
+    if (sizeof(T) != sizeof(double)) {
+        IA_TRY(D.Ad.alloc(sizeof(cx<double>) * sS * nf));
+        IA_TRY(D.Bd.alloc(sizeof(cx<double>) * sS * nf));
+        IA_TRY(D.Ahd.alloc(sizeof(cx<double>) * sH * nf));
+        IA_TRY(D.Bhd.alloc(sizeof(cx<double>) * sH * nf));
+    }
+    IA_TRY(D.Z.alloc(sizeof(cx<double>) * (size_t)m * m * nf));
+    if (n_ch == 0) {
+        IA_TRY(D.ipiv.alloc(sizeof(int) * (size_t)m * nf));
+        IA_TRY(D.info.alloc(sizeof(int) * nf));
+        IA_TRY(D.lw.alloc(sizeof(int) * getrf_work_ints(m, nf)));
+        IA_TRY(D.gw.alloc(getrs_work_bytes<double>(m, m, nf)));
+    } else {
+        IA_TRY(D.tau.alloc(sizeof(cx<double>) * (size_t)m * nf));
+    }
+    if (l > 0) IA_TRY(D.H.alloc(sizeof(cx<double>) * sH * nf));
+    return IA_OK;
+}
+
+// Device side of the build, enqueued on s: assembly of Z and H_b, the solve
+// for Z (LU, or least squares with CHIEF points) and H.  check: wait for the
+// LU and fail on a singular system (without it, the caller reads D.info).
+template <typename T>
+ia_status ibem_enqueue(int m, int l, int n, int n_ch, int nf, IbemDev &D, const IbemShape &sh,
+                       hipStream_t s, bool check) {
+    const int mt = m + n_ch;
+    const double rho_c = air().rho * air().c;
+    DevBuf &bA = D.A, &bB = D.B, &bAh = D.Ah, &bBh = D.Bh;
+    const size_t sS = (size_t)mt * m, sH = (size_t)std::max(l, 1) * m;
     const int *dE = D.E.as<int>();
     launch_ibem_rows<T>(m, D.X.as<double>(), m, true, m, D.X.as<double>(), dE, n,
-                        D.adj_ptr.as<int>(), D.adj.as<int>(), D.rule.as<double>(), Q.nq,
+                        D.adj_ptr.as<int>(), D.adj.as<int>(), D.rule.as<double>(), sh.nq,
                         D.k.as<double>(), nf, rho_c, bA.as<cx<T>>(), bB.as<cx<T>>(), mt, sS, s);
-    launch_ibem_sing<T>(nse, D.sing_ptr.as<int>(), D.sing_oc.as<int>(), D.sing_list.as<int>(), m,
-                        D.X.as<double>(), dE, n, D.pages.as<double>(), Q.nqs, D.k.as<double>(),
-                        nf, rho_c, bA.as<cx<T>>(), bB.as<cx<T>>(), mt, sS, s);
+    launch_ibem_sing<T>(sh.nse, D.sing_ptr.as<int>(), D.sing_oc.as<int>(), D.sing_list.as<int>(),
+                        m, D.X.as<double>(), dE, n, D.pages.as<double>(), sh.nqs,
+                        D.k.as<double>(), nf, rho_c, bA.as<cx<T>>(), bB.as<cx<T>>(), mt, sS, s);
     if (n_ch > 0)
         launch_ibem_rows<T>(n_ch, D.cp.as<double>(), n_ch, false, m, D.X.as<double>(), dE, n,
-                            D.adj_ptr.as<int>(), D.adj.as<int>(), D.rule.as<double>(), Q.nq,
+                            D.adj_ptr.as<int>(), D.adj.as<int>(), D.rule.as<double>(), sh.nq,
                             D.k.as<double>(), nf, rho_c, bA.as<cx<T>>() + m, bB.as<cx<T>>() + m,
                             mt, sS, s);
     if (l > 0)
         launch_ibem_rows<T>(l, D.rlh.as<double>(), l, false, m, D.X.as<double>(), dE, n,
-                            D.adj_ptr.as<int>(), D.adj.as<int>(), D.rule.as<double>(), Q.nq,
+                            D.adj_ptr.as<int>(), D.adj.as<int>(), D.rule.as<double>(), sh.nq,
                             D.k.as<double>(), nf, rho_c, bAh.as<cx<T>>(), bBh.as<cx<T>>(), l, sH,
                             s);
-    launch_ibem_cdiag<T>(m, D.X.as<double>(), dE, n, D.rule.as<double>(), Q.nq,
-                         D.pages.as<double>(), Q.nqs, bA.as<cx<T>>(), mt, sS, nf, nullptr, s);
+    launch_ibem_cdiag<T>(m, D.X.as<double>(), dE, n, D.rule.as<double>(), sh.nq,
+                         D.pages.as<double>(), sh.nqs, bA.as<cx<T>>(), mt, sS, nf, nullptr, s);
     IA_HIP(hipGetLastError());
     // solve in FP64 (the mixed form assembles in complex64)
     cx<double> *A, *B, *Ah, *Bh;
@@ -277,39 +333,36 @@ ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, co
         Ah = reinterpret_cast<cx<double> *>(bAh.p);
         Bh = reinterpret_cast<cx<double> *>(bBh.p);
     } else {
-        IA_TRY(bAd.alloc(sizeof(cx<double>) * sS * nf));
-        IA_TRY(bBd.alloc(sizeof(cx<double>) * sS * nf));
-        IA_TRY(bAhd.alloc(sizeof(cx<double>) * sH * nf));
-        IA_TRY(bBhd.alloc(sizeof(cx<double>) * sH * nf));
-        A = bAd.as<cx<double>>();
-        B = bBd.as<cx<double>>();
-        Ah = bAhd.as<cx<double>>();
-        Bh = bBhd.as<cx<double>>();
+        A = D.Ad.as<cx<double>>();
+        B = D.Bd.as<cx<double>>();
+        Ah = D.Ahd.as<cx<double>>();
+        Bh = D.Bhd.as<cx<double>>();
         launch_convert<double, float>(sS * nf, reinterpret_cast<cx<float> *>(bA.p), A, s);
         launch_convert<double, float>(sS * nf, reinterpret_cast<cx<float> *>(bB.p), B, s);
         launch_convert<double, float>(sH * nf, reinterpret_cast<cx<float> *>(bAh.p), Ah, s);
         launch_convert<double, float>(sH * nf, reinterpret_cast<cx<float> *>(bBh.p), Bh, s);
     }
-    IA_TRY(D.Z.alloc(sizeof(cx<double>) * (size_t)m * m * nf));
     cx<double> *Z = D.Z.as<cx<double>>();
     if (n_ch == 0) {
-        DevBuf &ipiv = D.ipiv, &info = D.info;
-        IA_TRY(ipiv.alloc(sizeof(int) * (size_t)m * nf));
-        IA_TRY(info.alloc(sizeof(int) * nf));
-        IA_HIP(hipMemsetAsync(info.p, 0, sizeof(int) * nf, s));
-        IA_TRY(getrf_batched<double>(m, A, m, sS, ipiv.as<int>(), info.as<int>(), nf, s));
-        IA_TRY(getrs_batched<double>('N', m, m, A, m, sS, ipiv.as<int>(), B, m, sS, nf, s));
-        std::vector<int> hinfo(nf);
-        IA_HIP(hipMemcpy(hinfo.data(), info.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
-        for (int f = 0; f < nf; ++f)
-            if (hinfo[f]) return fail(IA_ERR_SINGULAR, "IBEM system singular at k[%d] (pivot %d)", f, hinfo[f]);
+        hipLaunchKernelGGL(k_zero_ints, dim3((nf + 255) / 256), dim3(256), 0, s, D.info.as<int>(),
+                           nf);
+        IA_TRY(getrf_batched<double>(m, A, m, sS, D.ipiv.as<int>(), D.info.as<int>(), nf, s,
+                                     D.lw.as<int>()));
+        IA_TRY(getrs_batched<double>('N', m, m, A, m, sS, D.ipiv.as<int>(), B, m, sS, nf, s,
+                                     D.gw.p));
+        if (check) {
+            std::vector<int> hinfo(nf);
+            IA_HIP(hipMemcpy(hinfo.data(), D.info.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
+            for (int f = 0; f < nf; ++f)
+                if (hinfo[f])
+                    return fail(IA_ERR_SINGULAR, "IBEM system singular at k[%d] (pivot %d)", f,
+                                hinfo[f]);
+        }
         IA_HIP(hipMemcpyAsync(Z, B, sizeof(cx<double>) * sS * nf, hipMemcpyDeviceToDevice, s));
     } else {
         // least squares through Householder QR (the reference's ZZgels)
-        DevBuf &tau = D.tau;
-        IA_TRY(tau.alloc(sizeof(cx<double>) * (size_t)m * nf));
-        IA_TRY(launch_geqr2<double>(mt, m, A, mt, sS, tau.as<cx<double>>(), nf, s));
-        launch_apply_left<double>(true, mt, m, A, mt, sS, tau.as<cx<double>>(), B, mt, m, nf, s);
+        IA_TRY(launch_geqr2<double>(mt, m, A, mt, sS, D.tau.as<cx<double>>(), nf, s));
+        launch_apply_left<double>(true, mt, m, A, mt, sS, D.tau.as<cx<double>>(), B, mt, m, nf, s);
         IA_TRY(trsm_upper_batched<double>(m, m, A, mt, sS, B, mt, sS, nf, s));
         IA_HIP(hipMemcpy2DAsync(Z, sizeof(cx<double>) * m, B, sizeof(cx<double>) * mt,
                                 sizeof(cx<double>) * m, (size_t)m * nf, hipMemcpyDeviceToDevice, s));
@@ -318,10 +371,19 @@ ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, co
         const cx<double> one{1, 0}, mone{-1, 0};
         launch_gemm<double>('N', 'N', l, m, m, one, Ah, l, sH, Z, m, (size_t)m * m, mone, Bh, l, sH,
                             nf, s);
-        IA_TRY(D.H.alloc(sizeof(cx<double>) * sH * nf));
         IA_HIP(hipMemcpyAsync(D.H.p, Bh, sizeof(cx<double>) * sH * nf, hipMemcpyDeviceToDevice, s));
     }
     IA_HIP(hipGetLastError());
+    return IA_OK;
+}
+
+template <typename T>
+ia_status ibem_build(int m, int l, int n, const double *r_ms, const int *e_n, const double *r_lh,
+                     int n_ch, const double *cp, const double *k, int nf, IbemDev &D,
+                     hipStream_t s) {
+    IbemShape sh;
+    IA_TRY(ibem_upload<T>(m, l, n, r_ms, e_n, r_lh, n_ch, cp, k, nf, D, s, sh));
+    IA_TRY(ibem_enqueue<T>(m, l, n, n_ch, nf, D, sh, s, true));
     IA_HIP(hipStreamSynchronize(s));
     return IA_OK;
 }
@@ -337,9 +399,18 @@ ia_status ibem_args(int m, int l, int n, const double *r_ms, const int *e_n, con
 
 // The IBEM sweep's device workspace, kept per host thread between calls;
 // ia_release() frees it.
+// With IA_GRAPH: the captured sweep, its key and the stream it replays on.
 struct IbemSweepCache {
     IbemDev D;
     DevBuf b, x, p, lam, idx, kt, sm, W;
+    hipStream_t gs = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uintptr_t> gkey;
+    void drop_graph() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        gkey.clear();
+    }
 };
 IbemSweepCache &ibem_cache() {
     static thread_local IbemSweepCache *c = new IbemSweepCache();
@@ -350,6 +421,7 @@ IbemSweepCache &ibem_cache() {
 
 void release_ibem_cache() {
     IbemSweepCache &c = ibem_cache();
+    c.drop_graph();
     c.D.release();
     for (DevBuf *b : {&c.b, &c.x, &c.p, &c.lam, &c.idx, &c.kt, &c.sm, &c.W}) b->reset();
 }
@@ -477,14 +549,15 @@ ia_status ia_ibem_sweep(const ia_ibem_desc *desc, const double *k, const double 
     const int chunk = desc->chunk > 0 ? std::min(desc->chunk, F) : std::min(F, 64);
     IbemSweepCache &C = ibem_cache();
     IbemDev &D = C.D;
-    for (int f0 = 0; f0 < F; f0 += chunk) {
-        const int nf = std::min(chunk, F - f0);
-        if (desc->precision == IA_PREC_MIXED)
-            IA_TRY(ibem_build<float>(m, l, n, r_ms, e_n, r_lh, n_ch, cp, k + f0, nf, D, s));
+    const bool mixed = desc->precision == IA_PREC_MIXED;
+    DevBuf &db = C.b, &bx = C.x, &bp = C.p, &bl = C.lam, &bi = C.idx, &bk = C.kt, &bs = C.sm,
+           &bW = C.W;
+    // host side of a chunk: inputs uploaded, buffers sized
+    auto upload = [&](int f0, int nf, IbemShape &sh) -> ia_status {
+        if (mixed)
+            IA_TRY(ibem_upload<float>(m, l, n, r_ms, e_n, r_lh, n_ch, cp, k + f0, nf, D, s, sh));
         else
-            IA_TRY(ibem_build<double>(m, l, n, r_ms, e_n, r_lh, n_ch, cp, k + f0, nf, D, s));
-        DevBuf &db = C.b, &bx = C.x, &bp = C.p, &bl = C.lam, &bi = C.idx, &bk = C.kt,
-               &bs = C.sm, &bW = C.W;
+            IA_TRY(ibem_upload<double>(m, l, n, r_ms, e_n, r_lh, n_ch, cp, k + f0, nf, D, s, sh));
         IA_TRY(to_device(db, b + (size_t)f0 * l, sizeof(ia_z) * (size_t)l * nf, s));
         IA_TRY(bx.alloc(sizeof(cx<double>) * (size_t)m * nf));
         IA_TRY(bp.alloc(sizeof(cx<double>) * (size_t)m * nf));
@@ -492,25 +565,42 @@ ia_status ia_ibem_sweep(const ia_ibem_desc *desc, const double *k, const double 
         IA_TRY(bi.alloc(sizeof(int) * nf));
         IA_TRY(bk.alloc(sizeof(int) * nf));
         IA_TRY(bs.alloc(sizeof(double) * nf));
-        IA_TRY(bW.alloc(sizeof(double) * nf));
-        if (desc->precision == IA_PREC_MIXED)
-            IA_TRY(regsolve_batched<float>(l, m, nf, D.H.as<cx<double>>(), (size_t)l * m,
-                                           db.as<cx<double>>(), l, prm, bx.as<cx<double>>(), m,
-                                           bl.as<double>(), bi.as<int>(), bk.as<int>(),
-                                           bs.as<double>(), s));
+        return bW.alloc(sizeof(double) * nf);
+    };
+    // device side: the build, the regularised solve, p_s = Z v and W =
+    // 1/2 Re(v^H p_s) S/m; graph: no host reads or synchronisation
+    auto enqueue = [&](int nf, const IbemShape &sh, hipStream_t st, bool graph) -> ia_status {
+        if (mixed)
+            IA_TRY(ibem_enqueue<float>(m, l, n, n_ch, nf, D, sh, st, !graph));
         else
-            IA_TRY(regsolve_batched<double>(l, m, nf, D.H.as<cx<double>>(), (size_t)l * m,
-                                            db.as<cx<double>>(), l, prm, bx.as<cx<double>>(), m,
-                                            bl.as<double>(), bi.as<int>(), bk.as<int>(),
-                                            bs.as<double>(), s));
-        // p_s = Z v, W = 1/2 Re(v^H p_s) S/m
+            IA_TRY(ibem_enqueue<double>(m, l, n, n_ch, nf, D, sh, st, !graph));
+        const cx<double> *H = D.H.as<cx<double>>();
+        if (mixed)
+            IA_TRY(graph ? regsolve_async<float>(l, m, nf, H, (size_t)l * m, db.as<cx<double>>(), l,
+                                                 prm, bx.as<cx<double>>(), m, bl.as<double>(),
+                                                 bi.as<int>(), bk.as<int>(), bs.as<double>(), st)
+                         : regsolve_batched<float>(l, m, nf, H, (size_t)l * m,
+                                                   db.as<cx<double>>(), l, prm,
+                                                   bx.as<cx<double>>(), m, bl.as<double>(),
+                                                   bi.as<int>(), bk.as<int>(), bs.as<double>(), st));
+        else
+            IA_TRY(graph ? regsolve_async<double>(l, m, nf, H, (size_t)l * m, db.as<cx<double>>(),
+                                                  l, prm, bx.as<cx<double>>(), m, bl.as<double>(),
+                                                  bi.as<int>(), bk.as<int>(), bs.as<double>(), st)
+                         : regsolve_batched<double>(l, m, nf, H, (size_t)l * m,
+                                                    db.as<cx<double>>(), l, prm,
+                                                    bx.as<cx<double>>(), m, bl.as<double>(),
+                                                    bi.as<int>(), bk.as<int>(), bs.as<double>(),
+                                                    st));
         const cx<double> one{1, 0}, zero{0, 0};
         launch_gemm<double>('N', 'N', m, 1, m, one, D.Z.as<cx<double>>(), m, (size_t)m * m,
-                            bx.as<cx<double>>(), m, m, zero, bp.as<cx<double>>(), m, m, nf, s);
+                            bx.as<cx<double>>(), m, m, zero, bp.as<cx<double>>(), m, m, nf, st);
         launch_power<double>(m, bx.as<cx<double>>(), m, bp.as<cx<double>>(), m, desc->area / m,
-                             bW.as<double>(), nf, s);
+                             bW.as<double>(), nf, st);
         IA_HIP(hipGetLastError());
-        IA_HIP(hipStreamSynchronize(s));
+        return IA_OK;
+    };
+    auto copy_out = [&](int f0, int nf) -> ia_status {
         if (o.lambda) IA_HIP(hipMemcpy(o.lambda + f0, bl.p, sizeof(double) * nf, hipMemcpyDeviceToHost));
         if (o.lambda_index) IA_HIP(hipMemcpy(o.lambda_index + f0, bi.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
         if (o.k_tsvd) IA_HIP(hipMemcpy(o.k_tsvd + f0, bk.p, sizeof(int) * nf, hipMemcpyDeviceToHost));
@@ -522,6 +612,67 @@ ia_status ia_ibem_sweep(const ia_ibem_desc *desc, const double *k, const double 
             IA_HIP(hipMemcpy(o.pressure + (size_t)f0 * m, bp.p, sizeof(ia_z) * (size_t)m * nf, hipMemcpyDeviceToHost));
         if (o.status)
             for (int f = 0; f < nf; ++f) o.status[f0 + f] = IA_OK;
+        return IA_OK;
+    };
+    if (graph_mode() && chunk >= F && n_ch == 0) {
+        // IA_GRAPH=1 (one chunk, no CHIEF points): the device side is
+        // captured once per key and replayed after each call's uploads; a
+        // singular system is reported from the LU's info after the replay
+        IbemShape sh;
+        IA_TRY(upload(0, F, sh));
+        if (mixed)
+            IA_TRY(regsolve_reserve<float>(l, m, F));
+        else
+            IA_TRY(regsolve_reserve<double>(l, m, F));
+        std::vector<uintptr_t> key = {
+            (uintptr_t)m, (uintptr_t)l, (uintptr_t)n, (uintptr_t)F, (uintptr_t)mixed,
+            (uintptr_t)prm.filter, (uintptr_t)prm.select, (uintptr_t)prm.n_lambda,
+            (uintptr_t)prm.k_fixed, dbits(prm.decades), dbits(prm.lambda_fixed),
+            dbits(desc->area), dbits(air().rho * air().c), (uintptr_t)sh.nse, (uintptr_t)sh.nq,
+            (uintptr_t)sh.nqs};
+        D.addresses(key);
+        for (const DevBuf *q : {&db, &bx, &bp, &bl, &bi, &bk, &bs, &bW}) key.push_back((uintptr_t)q->p);
+        if (mixed)
+            regsolve_addresses<float>(key);
+        else
+            regsolve_addresses<double>(key);
+        if (!C.gs) IA_HIP(hipStreamCreateWithFlags(&C.gs, hipStreamNonBlocking));
+        if (!C.gexec || C.gkey != key) {
+            C.drop_graph();
+            IA_HIP(hipStreamSynchronize(s));
+            IA_HIP(hipStreamBeginCapture(C.gs, hipStreamCaptureModeThreadLocal));
+            const ia_status st = enqueue(F, sh, C.gs, true);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(C.gs, &g);
+            if (st == IA_OK && ce == hipSuccess && g &&
+                hipGraphInstantiate(&C.gexec, g, nullptr, nullptr, 0) != hipSuccess)
+                C.gexec = nullptr;
+            if (g) (void)hipGraphDestroy(g);
+            IA_TRY(st);
+            if (!C.gexec) {
+                (void)hipGetLastError();
+                return fail(IA_ERR_DEVICE, "ia_ibem_sweep: capture into a graph failed");
+            }
+            C.gkey = key;
+        }
+        IA_HIP(hipStreamSynchronize(s));  // the uploads ran on the legacy stream
+        IA_HIP(hipGraphLaunch(C.gexec, C.gs));
+        IA_HIP(hipStreamSynchronize(C.gs));
+        std::vector<int> hinfo(F);
+        IA_HIP(hipMemcpy(hinfo.data(), D.info.p, sizeof(int) * F, hipMemcpyDeviceToHost));
+        for (int f = 0; f < F; ++f)
+            if (hinfo[f])
+                return fail(IA_ERR_SINGULAR, "IBEM system singular at k[%d] (pivot %d)", f,
+                            hinfo[f]);
+        return copy_out(0, F);
+    }
+    for (int f0 = 0; f0 < F; f0 += chunk) {
+        const int nf = std::min(chunk, F - f0);
+        IbemShape sh;
+        IA_TRY(upload(f0, nf, sh));
+        IA_TRY(enqueue(nf, sh, s, false));
+        IA_HIP(hipStreamSynchronize(s));
+        IA_TRY(copy_out(f0, nf));
     }
     return IA_OK;
 }

@@ -1169,9 +1169,16 @@ ia_status getrf_batched(int n, cx<T> *A, int lda, size_t strideA, int *ipiv, int
 }
 
 template <typename T>
+size_t getrs_work_bytes(int n, int nrhs, int batch) {
+    // the row index and its inverse, then the right-hand sides (16-byte aligned)
+    const size_t ib = (sizeof(int) * 2 * (size_t)n * batch + 15) / 16 * 16;
+    return ib + sizeof(cx<T>) * (size_t)n * nrhs * batch;
+}
+
+template <typename T>
 ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_t strideA,
                         const int *ipiv, cx<T> *B, int ldb, size_t strideB, int batch,
-                        hipStream_t s) {
+                        hipStream_t s, void *work) {
     if (n <= 0 || nrhs <= 0 || batch <= 0) return IA_OK;
     if (nrhs == 1 && getrs1_lds<T>(n) <= 160 * 1024) {
         const size_t lds = getrs1_lds<T>(n);
@@ -1188,13 +1195,21 @@ ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_
     }
     int *idx = nullptr;
     cx<T> *tmp = nullptr;
-    IA_HIP(hipMallocAsync((void **)&idx, sizeof(int) * 2 * (size_t)n * batch, s));
-    IA_HIP(hipMallocAsync((void **)&tmp, sizeof(cx<T>) * (size_t)n * nrhs * batch, s));
+    if (work) {
+        idx = static_cast<int *>(work);
+        tmp = reinterpret_cast<cx<T> *>(static_cast<char *>(work) +
+                                        (sizeof(int) * 2 * (size_t)n * batch + 15) / 16 * 16);
+    } else {
+        IA_HIP(hipMallocAsync((void **)&idx, sizeof(int) * 2 * (size_t)n * batch, s));
+        IA_HIP(hipMallocAsync((void **)&tmp, sizeof(cx<T>) * (size_t)n * nrhs * batch, s));
+    }
     int *inv = idx + (size_t)n * batch;
     const ia_status ps = lu_perm_index(n, ipiv, idx, inv, batch, s);
     if (ps != IA_OK) {
-        (void)hipFreeAsync(tmp, s);
-        (void)hipFreeAsync(idx, s);
+        if (!work) {
+            (void)hipFreeAsync(tmp, s);
+            (void)hipFreeAsync(idx, s);
+        }
         return ps;
     }
     dim3 pg((n + 255) / 256, nrhs, batch);
@@ -1217,8 +1232,10 @@ ia_status getrs_batched(char op, int n, int nrhs, const cx<T> *A, int lda, size_
         hipLaunchKernelGGL(k_row_gather<T>, pg, dim3(256), 0, s, n, nrhs, inv, (const cx<T> *)tmp,
                            n, sT, B, ldb, strideB);
     }
-    IA_HIP(hipFreeAsync(tmp, s));
-    IA_HIP(hipFreeAsync(idx, s));
+    if (!work) {
+        IA_HIP(hipFreeAsync(tmp, s));
+        IA_HIP(hipFreeAsync(idx, s));
+    }
     IA_HIP(hipGetLastError());
     return IA_OK;
 }
@@ -1410,8 +1427,12 @@ template ia_status getrf_batched<float>(int, cx<float> *, int, size_t, int *, in
 template ia_status getrs1_spread<double>(int, const cx<double> *, int, size_t, const int *,
                                          cx<double> *, size_t, cx<double> *, int, hipStream_t);
 template ia_status getrs_batched<double>(char, int, int, const cx<double> *, int, size_t,
-                                         const int *, cx<double> *, int, size_t, int, hipStream_t);
+                                         const int *, cx<double> *, int, size_t, int, hipStream_t,
+                                         void *);
 template ia_status getrs_batched<float>(char, int, int, const cx<float> *, int, size_t,
-                                        const int *, cx<float> *, int, size_t, int, hipStream_t);
+                                        const int *, cx<float> *, int, size_t, int, hipStream_t,
+                                        void *);
+template size_t getrs_work_bytes<double>(int, int, int);
+template size_t getrs_work_bytes<float>(int, int, int);
 
 }  // namespace ia

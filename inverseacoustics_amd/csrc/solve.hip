@@ -164,6 +164,12 @@ template <typename T>
 void regsolve_addresses(std::vector<uintptr_t> &v) {
     regsolve_work<T>(0).addresses(v);
 }
+template <typename T>
+ia_status regsolve_reserve(int rows, int cols, int batch) {
+    return regsolve_work<T>(0).reserve(rows, cols, batch);
+}
+template ia_status regsolve_reserve<double>(int, int, int);
+template ia_status regsolve_reserve<float>(int, int, int);
 template ia_status regsolve_async<double>(int, int, int, const cx<double> *, size_t,
                                           const cx<double> *, size_t, const RegDev &, cx<double> *,
                                           size_t, double *, int *, int *, double *, hipStream_t);

@@ -154,3 +154,37 @@ def test_radiation_graph_replays_match_eager(ia, monkeypatch, precision):
     np.testing.assert_allclose(got["amplitude"], 2.0 * ref["amplitude"], rtol=0,
                                atol=1e-9 * np.abs(ref["amplitude"]).max())
     monkeypatch.delenv("IA_GRAPH")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ibem_graph_replays_match_eager(ia, monkeypatch, precision):
+    """ia_ibem_sweep with IA_GRAPH=1 (one chunk, no CHIEF points): the inputs
+    are uploaded on every call and the device side (assembly, LU with a kept
+    workspace, regularised solve, surface pressure and power) is captured
+    once and replayed; three calls equal the eager sweep bit for bit with an
+    eager sweep of other frequencies between them, and doubled pressures give
+    doubled velocities."""
+    S = sc.ibem_scenario(n_freq=6, fmin=300.0, fmax=1800.0, subdiv=2, l=64)
+    ia.set_air(S.c, S.rho)
+    ia.set_quadrature()
+
+    def run(b):
+        return ia.ibem_sweep(S.k, S.X, S.E, S.r_h, b, S.area, precision=precision)
+
+    monkeypatch.delenv("IA_GRAPH", raising=False)
+    ref = run(S.b)
+    for rep in range(3):
+        monkeypatch.setenv("IA_GRAPH", "1")
+        got = run(S.b)
+        for q in ("lam_index", "k_tsvd", "lam", "power", "velocity", "pressure"):
+            np.testing.assert_array_equal(getattr(got, q), getattr(ref, q),
+                                          err_msg=f"call {rep}: {q}")
+        monkeypatch.delenv("IA_GRAPH")
+        other = ia.ibem_sweep(S.k[:2] * 1.1, S.X, S.E, S.r_h, S.b[:2], S.area)
+        assert np.all(np.isfinite(other.power))
+    monkeypatch.setenv("IA_GRAPH", "1")
+    got = run(2.0 * S.b)
+    np.testing.assert_array_equal(got.lam_index, ref.lam_index)
+    np.testing.assert_allclose(got.velocity, 2.0 * ref.velocity, rtol=0,
+                               atol=1e-9 * np.abs(ref.velocity).max())
+    monkeypatch.delenv("IA_GRAPH")
