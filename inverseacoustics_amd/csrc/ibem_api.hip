@@ -614,10 +614,10 @@ ia_status ia_ibem_sweep(const ia_ibem_desc *desc, const double *k, const double 
             for (int f = 0; f < nf; ++f) o.status[f0 + f] = IA_OK;
         return IA_OK;
     };
-    if (graph_mode() && chunk >= F && n_ch == 0) {
-        // IA_GRAPH=1 (one chunk, no CHIEF points): the device side is
-        // captured once per key and replayed after each call's uploads; a
-        // singular system is reported from the LU's info after the replay
+    if (graph_mode() && chunk >= F) {
+        // IA_GRAPH=1 (one chunk): the device side is captured once per key
+        // and replayed after each call's uploads; a singular system is
+        // reported from the LU's info after the replay
         IbemShape sh;
         IA_TRY(upload(0, F, sh));
         if (mixed)
@@ -625,7 +625,7 @@ ia_status ia_ibem_sweep(const ia_ibem_desc *desc, const double *k, const double 
         else
             IA_TRY(regsolve_reserve<double>(l, m, F));
         std::vector<uintptr_t> key = {
-            (uintptr_t)m, (uintptr_t)l, (uintptr_t)n, (uintptr_t)F, (uintptr_t)mixed,
+            (uintptr_t)m, (uintptr_t)l, (uintptr_t)n, (uintptr_t)n_ch, (uintptr_t)F, (uintptr_t)mixed,
             (uintptr_t)prm.filter, (uintptr_t)prm.select, (uintptr_t)prm.n_lambda,
             (uintptr_t)prm.k_fixed, dbits(prm.decades), dbits(prm.lambda_fixed),
             dbits(desc->area), dbits(air().rho * air().c), (uintptr_t)sh.nse, (uintptr_t)sh.nq,
@@ -658,12 +658,14 @@ ia_status ia_ibem_sweep(const ia_ibem_desc *desc, const double *k, const double 
         IA_HIP(hipStreamSynchronize(s));  // the uploads ran on the legacy stream
         IA_HIP(hipGraphLaunch(C.gexec, C.gs));
         IA_HIP(hipStreamSynchronize(C.gs));
-        std::vector<int> hinfo(F);
-        IA_HIP(hipMemcpy(hinfo.data(), D.info.p, sizeof(int) * F, hipMemcpyDeviceToHost));
-        for (int f = 0; f < F; ++f)
-            if (hinfo[f])
-                return fail(IA_ERR_SINGULAR, "IBEM system singular at k[%d] (pivot %d)", f,
-                            hinfo[f]);
+        if (n_ch == 0) {
+            std::vector<int> hinfo(F);
+            IA_HIP(hipMemcpy(hinfo.data(), D.info.p, sizeof(int) * F, hipMemcpyDeviceToHost));
+            for (int f = 0; f < F; ++f)
+                if (hinfo[f])
+                    return fail(IA_ERR_SINGULAR, "IBEM system singular at k[%d] (pivot %d)", f,
+                                hinfo[f]);
+        }
         return copy_out(0, F);
     }
     for (int f0 = 0; f0 < F; f0 += chunk) {

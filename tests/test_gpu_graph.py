@@ -156,20 +156,21 @@ def test_radiation_graph_replays_match_eager(ia, monkeypatch, precision):
     monkeypatch.delenv("IA_GRAPH")
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_ibem_graph_replays_match_eager(ia, monkeypatch, precision):
-    """ia_ibem_sweep with IA_GRAPH=1 (one chunk, no CHIEF points): the inputs
-    are uploaded on every call and the device side (assembly, LU with a kept
-    workspace, regularised solve, surface pressure and power) is captured
-    once and replayed; three calls equal the eager sweep bit for bit with an
-    eager sweep of other frequencies between them, and doubled pressures give
-    doubled velocities."""
-    S = sc.ibem_scenario(n_freq=6, fmin=300.0, fmax=1800.0, subdiv=2, l=64)
+@pytest.mark.parametrize("precision,n_chief", [("fp64", 0), ("mixed", 0), ("fp64", 6)])
+def test_ibem_graph_replays_match_eager(ia, monkeypatch, precision, n_chief):
+    """ia_ibem_sweep with IA_GRAPH=1 (one chunk): the inputs are uploaded on
+    every call and the device side (assembly, the LU with a kept workspace or
+    with CHIEF points the least-squares QR, regularised solve, surface
+    pressure and power) is captured once and replayed; three calls equal the
+    eager sweep bit for bit with an eager sweep of other frequencies between
+    them, and doubled pressures give doubled velocities."""
+    S = sc.ibem_scenario(n_freq=6, fmin=300.0, fmax=1800.0, subdiv=2, l=64, n_chief=n_chief)
     ia.set_air(S.c, S.rho)
     ia.set_quadrature()
+    cp = S.chief if n_chief else None
 
     def run(b):
-        return ia.ibem_sweep(S.k, S.X, S.E, S.r_h, b, S.area, precision=precision)
+        return ia.ibem_sweep(S.k, S.X, S.E, S.r_h, b, S.area, precision=precision, cp=cp)
 
     monkeypatch.delenv("IA_GRAPH", raising=False)
     ref = run(S.b)
